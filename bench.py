@@ -1,0 +1,184 @@
+"""Benchmark: Gpairs/s and s/scene of the discrete Kirchhoff engine on one MI355X.
+
+Workload (default): trihedral corner reflector, three right-isosceles rigid
+plates with 1 m legs, f = 20 kHz, c = 1480 m/s, fragment edge lambda/7.5
+(15,768 fragments), 64 monopole incidences at 15 m on a seeded jittered 8x8
+grid over the octant, 3 bounces with shadow rays, monostatic returns.
+
+One step = one whole scene for all 64 incidences with the geometry resident in
+HBM: zero the pressures, rebuild every visibility word, point->fragment, the
+3 bounces (rebuilding the transfer matrix in the mixed form), fragment->point.
+
+Two forms are timed back to back on the same scene:
+  plain_fp32   the pair kernel evaluated on the fly in FP32 every bounce;
+  mixed_fp16   the pair kernel stored once as a complex FP16 matrix, every
+               bounce an FP16 MFMA GEMM with FP32 accumulation.
+Each form's output is checked against the FP64 oracle on the incidences the
+CPU baseline runs. `value` is the Gpairs/s of the fastest form whose rel-L2
+is <= 1e-3.
+
+Prints one JSON line. --gpus > 1 is not supported (one GPU per process).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "sonar-render_amd")]
+
+import numpy as np  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3
+FP16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
+TOL = 1e-3
+
+
+def _configs():
+    import scenes
+    return {
+        "trihedral": lambda: scenes.config_trihedral(),
+        "sphere": lambda: scenes.config_sphere(),
+        "plate": lambda: scenes.config_plate(),
+    }
+
+
+def cpu_baseline(scene, sample_planes):
+    """The FP64 OpenMP oracle on this host's cores, on a bounded sample of incidences."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = os.cpu_count() or 1
+    oracle.set_threads(threads)
+    outs = []
+    t0 = time.perf_counter()
+    for p in sample_planes:
+        outs.append(oracle.run_scene(scene, plane_source=p if scene.get("batched") else None))
+    return time.perf_counter() - t0, threads, outs
+
+
+def time_form(scene, mode, steps, warmup):
+    import torch
+    import runner
+    import sonar_api as api
+
+    runner.load_scene(scene, transfer_mode=mode)
+    torch.cuda.synchronize()
+
+    def step():
+        api.reset_pressures()
+        api.clear_vis_cache()
+        runner.project_scene(scene)
+
+    for _ in range(warmup):
+        step()
+    api.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = {k: api.kernel_stats(k) for k in api.KERNEL_FAMILIES}
+    api.profile(False)
+    res = runner.read_results(scene)
+    api.TearDownCuda()
+    return dt, stats, res
+
+
+def roofline(stats, mode):
+    if mode == 0:
+        st = stats["f2f"]
+        ach = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
+        return {"bound": "mfma", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_gather<KIND_F2F> (FP32 VALU + transcendentals; the plain form uses no matrix cores)",
+                "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4), "launches": st["launches"]}
+    st = stats["gemm"]
+    ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] > 0 else 0.0
+    tf = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_transfer_gemm (FP16 MFMA 32x32x16, streams the FP16 transfer matrix once per bounce)",
+            "mfma_tflops": round(tf, 2), "mfma_frac": round(tf / FP16_PEAK_TFLOPS, 4),
+            "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4), "launches": st["launches"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="trihedral", choices=sorted(_configs()))
+    ap.add_argument("--forms", default="plain,mixed", help="comma list of plain, mixed")
+    ap.add_argument("--cpu-sample", type=int, default=1, help="incidences the CPU baseline runs")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs one GPU per process; multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import runner
+
+    scene = _configs()[args.config]()
+    n_planes = len(scene["sources"]) if scene.get("batched") else 1
+    pairs = runner.pair_count(scene)
+    forms = {"plain": ("plain_fp32", 0), "mixed": ("mixed_fp16_mfma", 1)}
+    chosen = [forms[f] for f in args.forms.split(",") if f in forms]
+
+    results = {}
+    for name, mode in chosen:
+        dt, stats, res = time_form(scene, mode, args.steps, args.warmup)
+        results[name] = dict(mode=mode, dt=dt, stats=stats, res=res)
+        print(f"[bench] {name}: {dt / args.steps * 1e3:.3f} ms/scene", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        sample = list(range(min(args.cpu_sample, n_planes)))
+        cdt, threads, outs = cpu_baseline(scene, sample)
+        frac = len(sample) / n_planes
+        cpu = {"value": round(pairs * frac / cdt / 1e9, 5), "unit": "Gpairs/s", "cores": threads, "kind": "port",
+               "sample": f"{len(sample)} of {n_planes} incidences of the same scene, FP64 OpenMP oracle; "
+                         f"{cdt:.2f} s -> {cdt / frac:.1f} s/scene extrapolated"}
+        for name, r in results.items():
+            errs = []
+            for p, o in zip(sample, outs):
+                a = np.concatenate([r["res"]["points"][p], r["res"]["targets"][0][p]])
+                b = np.concatenate([o["points"], o["targets"][0]])
+                errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+            r["err"] = max(errs)
+
+    per_form = {}
+    for name, r in results.items():
+        ms = r["dt"] / args.steps * 1e3
+        per_form[name] = {
+            "gpairs_per_s": round(pairs * args.steps / r["dt"] / 1e9, 3), "ms_per_scene": round(ms, 4),
+            "rel_l2_vs_fp64": r.get("err"),
+            "kernel_ms_per_scene": {k: round(v["ms"] / args.steps, 4) for k, v in r["stats"].items() if v["launches"]},
+            "roofline": roofline(r["stats"], r["mode"]),
+        }
+    valid = [n for n in per_form if per_form[n]["rel_l2_vs_fp64"] is None or per_form[n]["rel_l2_vs_fp64"] <= TOL]
+    primary = min(valid or list(per_form), key=lambda n: per_form[n]["ms_per_scene"])
+    P = per_form[primary]
+    speedup = None
+    if "plain_fp32" in per_form and "mixed_fp16_mfma" in per_form:
+        speedup = round(per_form["plain_fp32"]["ms_per_scene"] / per_form["mixed_fp16_mfma"]["ms_per_scene"], 2)
+
+    line = {
+        "metric": "Gpairs/s & s/scene at <=1e-3 rel-L2 vs FP64: mixed vs plain FP32",
+        "value": P["gpairs_per_s"], "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": P["ms_per_scene"], "s_per_scene": round(P["ms_per_scene"] / 1e3, 6),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16" if primary.startswith("mixed") else "f32", "data": "synthetic",
+        "config": {"workload": scene["name"], "form": primary, "incidences": n_planes,
+                   "bounces": scene.get("bounces", 0), "pairs_per_scene": pairs, "parallelism": "single GPU"},
+        "rel_l2_vs_fp64": P["rel_l2_vs_fp64"], "speedup_mixed_vs_plain": speedup,
+        "forms": per_form, "roofline": P["roofline"], "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

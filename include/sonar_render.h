@@ -1,0 +1,170 @@
+/*
+ * sonar_render.h: C ABI of the MI355X discrete Kirchhoff engine.
+ *
+ * Part 1 is the reference's own boundary, symbol for symbol, with the same
+ * argument types and meanings as src/python_interface.cpp, so a ctypes caller
+ * written for the reference (run_scripts/modules/api.py) binds unchanged.
+ * These entry points return void like the reference; instead of letting a
+ * C++ exception cross the boundary they record the error (sr_last_error) and
+ * print it to stderr.
+ *
+ * Part 2 adds status-returning twins and the batched-incidence, readback and
+ * visibility-mask entry points. Every sr_* function returns 0 on success or a
+ * negative status: -1 device/runtime failure, -2 bad argument, -3 wrong call
+ * order.
+ *
+ * One process-global scene, as in the reference (python_interface.cpp:11).
+ * Calls are synchronous and not re-entrant.
+ */
+#ifndef SONAR_RENDER_H
+#define SONAR_RENDER_H
+
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* complex128 as the reference lays it out: src/dcomplex.h:12-15 */
+typedef struct { double r, i; } sr_dcomplex;
+
+/* ---- Part 1: the reference's entry points ---------------------------- */
+
+/* python_interface.cpp:21-24 (Model.cpp:44-54). omega and k are rounded to
+ * FP32; attenuation becomes Im k unchanged; density is unused. */
+void set_initial_conditions(float cp, float frequency, float attenuation, float density);
+
+/* python_interface.cpp:26-41. v = num_facets x 9 floats (v1, v2, v3);
+ * objectType 0 target, 1 source (stored, unused), 2 field; resolution is the
+ * fragment edge delta. */
+void load_geometry(float *v, int num_facets, int objectType, float resolution);
+
+/* python_interface.cpp:43-50: n x 3 floats, appended. */
+void load_field_points(float *xyz, int num_field_points);
+
+/* python_interface.cpp:52-59: n x 3 floats, amplitude 1+0i (Model.cpp:18). */
+void load_source_points(float *xyz, int num_source_points);
+
+/* python_interface.cpp:61-65: fragment every object, build the LBVH over the
+ * target triangles, upload. Computes no pressures. */
+void render_cuda(void);
+
+/* python_interface.cpp:67-70: incident field onto targets (and onto field
+ * objects if projectFieldObjects). */
+void ProjectSrcPointsToObjects(bool projectFieldObjects);
+
+/* python_interface.cpp:72-75: targets -> field objects (FACET_NO_RESP). */
+void ProjectTargetToFieldObjects(void);
+
+/* python_interface.cpp:77-80: targets -> field points (FIELD_POINTS). */
+void ProjectTargetToFieldPoints(void);
+
+/* python_interface.cpp:82-85: number_reflections FACET_SELF bounces per
+ * target object. */
+void ProjectTargetToTargetObjects(int number_reflections);
+
+/* python_interface.cpp:87-90: copies min(NumPoints, #field points) complex128
+ * values (the reference ignores NumPoints, ModelCuda.cpp:206). */
+void GetFieldPointPressures(sr_dcomplex *field_points_pressure, int NumPoints);
+
+/* python_interface.cpp:92-96 and :103-107: the OpenGL render stays on the
+ * host and is not part of this library; these record an error and return. */
+void RenderOpenGL(int width, int height, char *filename, float viewSettings[9]);
+void SoundVisualisationInit(float maxDbValue, float minDbValue, bool renderPhaseTarget, bool renderPhaseField);
+
+/* python_interface.cpp:98-101: free the scene; the library can be loaded
+ * with a new scene afterwards. */
+void TearDownCuda(void);
+
+/* ---- Part 2: extensions (no counterpart in the reference) ------------- */
+
+/* Message of the last failed call ("" if none) and its status. */
+const char *sr_last_error(void);
+int sr_last_status(void);
+
+/* Status-returning twins of Part 1. */
+int sr_set_initial_conditions(float cp, float frequency, float attenuation, float density);
+int sr_load_geometry(const float *v, int num_facets, int objectType, float resolution);
+int sr_load_field_points(const float *xyz, int n);
+int sr_load_source_points(const float *xyz, int n);
+int sr_render(void);
+int sr_project_src_points_to_objects(int projectFieldObjects);
+int sr_project_target_to_field_objects(void);
+int sr_project_target_to_field_points(void);
+int sr_project_target_to_target_objects(int number_reflections);
+int sr_get_field_point_pressures(sr_dcomplex *out, int n);
+/* complex64 read-back: 2 floats per point. */
+int sr_get_field_point_pressures_c64(float *out, int n);
+int sr_teardown(void);
+
+/* Batched incidences: before render, make every source point its own
+ * incidence ("plane") instead of summing them coherently. Every projection
+ * then runs over all planes in one pass; plane p is what a separate
+ * reference run with only source p would produce. */
+int sr_set_batched_sources(int on);
+int sr_num_planes(void);
+int sr_get_field_point_pressures_plane(int plane, sr_dcomplex *out, int n);
+int sr_set_source_amplitude(int source, float re, float im);
+
+/* FACET_RESP between two target objects (OptiX.cu:427-430; the reference
+ * never dispatches it). sr_prepare_resp sets P_in <- P and P_out <- 0. */
+int sr_prepare_resp(int target);
+int sr_project_resp(int targetA, int targetB);
+
+/* Scene inspection. objectType 0 target, 2 field. which: 0 P, 1 P_in, 2 P_out.
+ * Pressures are per cell in facet order, cell (i,j) of a facet at j*nX+i. */
+int sr_object_count(int objectType);
+long sr_object_cell_count(int objectType, int index);
+int sr_object_active_count(int objectType, int index);
+int sr_object_active_cells(int objectType, int index, int *out, int n);
+int sr_get_object_pressures(int objectType, int index, int which, int plane, sr_dcomplex *out, long n);
+
+/* Visibility words. kind: 0 sources->cells of (tb,ib), 1 active cells of
+ * (ta,ia)->field points, 2 SELF of (ta,ia), 3 NO_RESP (ta,ia)->(tb,ib),
+ * 4 RESP forward, 5 RESP reciprocal. Word [rblock][emitter], bit = receiver
+ * rblock*64+bit contributes. */
+long sr_mask_words(int kind, int ta, int ia, int tb, int ib);
+int sr_get_mask(int kind, int ta, int ia, int tb, int ib, unsigned long long *out, long nwords);
+
+/* Visibility constants in use: scale, padTri, padNode, tminPoint, tminFrag. */
+int sr_vis_params(float *out5);
+
+/* Host-only fragmenter probe (no GPU): fragment one triangle with edge delta.
+ * Writes nX, nY, nNeg to grid3; if cap >= nX*nY also the cell areas, the
+ * emitter points (3 floats per cell) and the point->fragment receiver points.
+ * Returns the cell count, or a negative status. */
+long sr_fragment_facet(const float *v9, float delta, int *grid3, float *area, float *pe, float *pr, long cap);
+
+/* Zero every pressure buffer (P, P_in, P_out, field points) of the loaded
+ * scene, and drop the cached visibility words (rebuilt on next use). */
+int sr_reset_pressures(void);
+int sr_clear_vis_cache(void);
+
+/* Bounce arithmetic. 0 (default): plain FP32 gathers, the pair kernel
+ * evaluated on the fly for every bounce. 1: the pair kernel is stored once
+ * per geometry and frequency as a complex FP16 matrix and every bounce is a
+ * FP16 MFMA GEMM with FP32 accumulation over all incidence planes. */
+int sr_set_transfer_mode(int mode);
+int sr_transfer_mode(void);
+
+/* Per-launch profiling with HIP events on the library's stream. sr_profile(1)
+ * starts (and clears) the record. Kernel families: 0 point->fragment,
+ * 1 visibility words, 2 fragment->fragment, 3 fragment->field object,
+ * 4 fragment->field point, 5 split-K reductions, 6 transfer-matrix build,
+ * 7 transfer GEMM, 8 incidence-matrix build. Reports summed kernel time,
+ * launches, ordered pairs (x planes), flops (gathers: 37 FP32 flops per
+ * pair plus 8 per pair and plane; transfer GEMM: 2 M N K as executed) and
+ * algorithmic HBM bytes (transfer GEMM: S, B and the partial sums). */
+int sr_profile(int on);
+int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
+
+/* Wall time of the last projection entry point, milliseconds. */
+float sr_last_call_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SONAR_RENDER_H */

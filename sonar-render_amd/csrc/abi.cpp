@@ -1,0 +1,250 @@
+// The C-ABI: the reference's 13 entry points (src/python_interface.cpp) on one
+// process-global scene, plus status-returning sr_* extensions. No exception
+// crosses this boundary.
+#include <chrono>
+#include <cstdio>
+#include <exception>
+#include <mutex>
+#include <string>
+
+#include "../../include/sonar_render.h"
+#include "launch.h"
+#include "model_ops.h"
+
+namespace {
+
+sr::Model gModel;  // python_interface.cpp:11
+int gStatus = 0;
+std::string gMsg;
+float gLastMs = 0.f;
+
+template <class F>
+int guarded(const char *name, F &&f) {
+    int rc;
+    try {
+        sr::clear_error();
+        rc = f();
+    } catch (const std::exception &e) {
+        sr::set_error(std::string(name) + ": " + e.what());
+        rc = -1;
+    } catch (...) {
+        sr::set_error(std::string(name) + ": unknown exception");
+        rc = -1;
+    }
+    gStatus = rc < 0 ? rc : 0;
+    gMsg = rc < 0 ? sr::last_error() : std::string();
+    return rc;
+}
+
+template <class F>
+int timed(const char *name, F &&f) {
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = guarded(name, f);
+    gLastMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+void report(const char *name, int rc) {
+    if (rc < 0) std::fprintf(stderr, "sonar_render: %s failed (%d): %s\n", name, rc, gMsg.c_str());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sr_last_error(void) { return gMsg.c_str(); }
+int sr_last_status(void) { return gStatus; }
+float sr_last_call_ms(void) { return gLastMs; }
+
+int sr_set_initial_conditions(float cp, float f, float a, float d) {
+    return guarded("set_initial_conditions", [&] { return sr::model_set_initial_conditions(gModel, cp, f, a, d); });
+}
+int sr_load_geometry(const float *v, int nf, int type, float res) {
+    return guarded("load_geometry", [&] { return sr::model_load_geometry(gModel, v, nf, type, res); });
+}
+int sr_load_field_points(const float *p, int n) {
+    return guarded("load_field_points", [&] { return sr::model_load_points(gModel, p, n, false); });
+}
+int sr_load_source_points(const float *p, int n) {
+    return guarded("load_source_points", [&] { return sr::model_load_points(gModel, p, n, true); });
+}
+int sr_set_batched_sources(int on) {
+    return guarded("set_batched_sources", [&] { return sr::model_set_batched_sources(gModel, on); });
+}
+int sr_render(void) { return timed("render_cuda", [&] { return sr::model_render(gModel); }); }
+int sr_project_src_points_to_objects(int fo) {
+    return timed("ProjectSrcPointsToObjects", [&] { return sr::model_project_src(gModel, fo != 0); });
+}
+int sr_project_target_to_field_objects(void) {
+    return timed("ProjectTargetToFieldObjects", [&] { return sr::model_project_t2field_objects(gModel); });
+}
+int sr_project_target_to_field_points(void) {
+    return timed("ProjectTargetToFieldPoints", [&] { return sr::model_project_t2field_points(gModel); });
+}
+int sr_project_target_to_target_objects(int n) {
+    return timed("ProjectTargetToTargetObjects", [&] { return sr::model_project_t2t(gModel, n); });
+}
+int sr_prepare_resp(int t) { return guarded("prepare_resp", [&] { return sr::model_prepare_resp(gModel, t); }); }
+int sr_project_resp(int a, int b) {
+    return timed("project_resp", [&] { return sr::model_project_resp(gModel, a, b); });
+}
+int sr_get_field_point_pressures(sr_dcomplex *out, int n) {
+    return guarded("GetFieldPointPressures", [&] {
+        int rc = sr::model_get_field_points(gModel, (double *)out, n, 0);
+        return rc < 0 ? rc : 0;
+    });
+}
+int sr_get_field_point_pressures_plane(int plane, sr_dcomplex *out, int n) {
+    return guarded("get_field_point_pressures_plane", [&] {
+        int rc = sr::model_get_field_points(gModel, (double *)out, n, plane);
+        return rc < 0 ? rc : 0;
+    });
+}
+int sr_get_field_point_pressures_c64(float *out, int n) {
+    return guarded("get_field_point_pressures_c64", [&] {
+        if (n <= 0) return 0;
+        double *tmp = new double[2 * (size_t)n];
+        int rc = sr::model_get_field_points(gModel, tmp, n, 0);
+        for (int i = 0; rc > 0 && i < rc; i++) { out[2 * i] = (float)tmp[2 * i]; out[2 * i + 1] = (float)tmp[2 * i + 1]; }
+        delete[] tmp;
+        return rc < 0 ? rc : 0;
+    });
+}
+int sr_set_source_amplitude(int s, float re, float im) {
+    return guarded("set_source_amplitude", [&] { return sr::model_set_source_amplitude(gModel, s, re, im); });
+}
+int sr_num_planes(void) { return gModel.nplanes; }
+int sr_teardown(void) {
+    return guarded("TearDownCuda", [&] { sr::model_teardown(gModel); return 0; });
+}
+
+int sr_object_count(int type) {
+    if (type == sr::OBJ_TARGET) return (int)gModel.targets.size();
+    if (type == sr::OBJ_FIELD) return (int)gModel.fields.size();
+    if (type == sr::OBJ_SOURCE) return gModel.numSourceObjects;
+    return -2;
+}
+static sr::Object *obj(int type, int idx) {
+    auto &v = (type == sr::OBJ_FIELD) ? gModel.fields : gModel.targets;
+    if (type != sr::OBJ_FIELD && type != sr::OBJ_TARGET) return nullptr;
+    if (idx < 0 || idx >= (int)v.size()) return nullptr;
+    return &v[idx];
+}
+long sr_object_cell_count(int type, int idx) {
+    sr::Object *o = obj(type, idx);
+    if (!o || !gModel.rendered) return -2;
+    return o->ncell;
+}
+int sr_object_active_count(int type, int idx) {
+    sr::Object *o = obj(type, idx);
+    if (!o || !gModel.rendered) return -2;
+    return o->nact;
+}
+int sr_object_active_cells(int type, int idx, int *out, int n) {
+    sr::Object *o = obj(type, idx);
+    if (!o || !gModel.rendered || n < o->nact) return -2;
+    for (int k = 0; k < o->nact; k++) out[k] = o->actCellHost[k];
+    return 0;
+}
+int sr_get_object_pressures(int type, int idx, int which, int plane, sr_dcomplex *out, long n) {
+    return guarded("get_object_pressures",
+                   [&] { return sr::model_get_object_pressure(gModel, type, idx, which, plane, (double *)out, n); });
+}
+
+long sr_mask_words(int kind, int ta, int ia, int tb, int ib) {
+    if (!gModel.rendered) return -3;
+    const long ns = (long)gModel.srcHost.size() / 3, nq = (long)gModel.fldHost.size() / 3;
+    sr::Object *A = obj(ta, ia), *B = obj(tb, ib);
+    long ne, nr;
+    switch (kind) {
+    case 0: if (!B) return -2; ne = ns; nr = B->ncell; break;
+    case 1: if (!A) return -2; ne = A->nact; nr = nq; break;
+    case 2: if (!A) return -2; ne = nr = A->nact; break;
+    case 3: case 4: if (!A || !B) return -2; ne = A->nact; nr = B->nact; break;
+    case 5: if (!A || !B) return -2; ne = B->nact; nr = A->nact; break;
+    default: return -2;
+    }
+    return ((nr + 63) / 64) * ne;
+}
+int sr_get_mask(int kind, int ta, int ia, int tb, int ib, unsigned long long *out, long nwords) {
+    return guarded("get_mask", [&] {
+        long need = sr_mask_words(kind, ta, ia, tb, ib);
+        if (need < 0) { sr::set_error("bad mask request"); return (int)need; }
+        if (nwords < need) { sr::set_error("mask buffer too short"); return -2; }
+        return sr::model_get_mask(gModel, kind, ta, ia, tb, ib, (uint64_t *)out, need);
+    });
+}
+int sr_vis_params(float *out) {
+    if (!gModel.rendered) return -3;
+    out[0] = gModel.vc.scale; out[1] = gModel.vc.padTri; out[2] = gModel.vc.padNode;
+    out[3] = gModel.vc.tminPoint; out[4] = gModel.vc.tminFrag;
+    return 0;
+}
+
+long sr_fragment_facet(const float *v, float delta, int *grid3, float *area, float *pe, float *pr, long cap) {
+    if (!v || !grid3 || !(delta > 0.f)) return -2;
+    sr::FacetFrame F = sr::facet_frame(sr::Vec3{v[0], v[1], v[2]}, sr::Vec3{v[3], v[4], v[5]}, sr::Vec3{v[6], v[7], v[8]});
+    sr::FacetGrid G = sr::facet_grid(F, delta);
+    grid3[0] = G.nX; grid3[1] = G.nY; grid3[2] = G.nNeg;
+    const long n = (long)G.nX * G.nY;
+    if (cap >= n && area && pe && pr) {
+        for (int j = 0; j < G.nY; j++)
+            for (int i = 0; i < G.nX; i++) {
+                const long c = (long)j * G.nX + i;
+                area[c] = G.area[c];
+                sr::Vec3 a = sr::cell_point(F, G, delta, i, j, false), b = sr::cell_point(F, G, delta, i, j, true);
+                pe[3 * c] = a.x; pe[3 * c + 1] = a.y; pe[3 * c + 2] = a.z;
+                pr[3 * c] = b.x; pr[3 * c + 1] = b.y; pr[3 * c + 2] = b.z;
+            }
+    }
+    return n;
+}
+
+int sr_reset_pressures(void) {
+    return guarded("reset_pressures", [&] { return sr::model_reset_pressures(gModel); });
+}
+int sr_clear_vis_cache(void) {
+    return guarded("clear_vis_cache", [&] { return sr::model_clear_vis_cache(gModel); });
+}
+int sr_set_transfer_mode(int mode) {
+    return guarded("set_transfer_mode", [&] { return sr::model_set_transfer_mode(gModel, mode); });
+}
+int sr_transfer_mode(void) { return gModel.transferMode; }
+int sr_profile(int on) {
+    return guarded("profile", [&] { sr::model_profile(gModel, on); return 0; });
+}
+int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes) {
+    return guarded("kernel_stats", [&] {
+        if (kind < 0 || kind >= sr::PK_COUNT) { sr::set_error("unknown kernel family"); return -2; }
+        return sr::model_kernel_stats(gModel, kind, ms, launches, pairs, flops, bytes);
+    });
+}
+
+// ---- the reference's void entry points ----
+void set_initial_conditions(float cp, float f, float a, float d) { report("set_initial_conditions", sr_set_initial_conditions(cp, f, a, d)); }
+void load_geometry(float *v, int nf, int type, float res) { report("load_geometry", sr_load_geometry(v, nf, type, res)); }
+void load_field_points(float *p, int n) { report("load_field_points", sr_load_field_points(p, n)); }
+void load_source_points(float *p, int n) { report("load_source_points", sr_load_source_points(p, n)); }
+void render_cuda(void) { report("render_cuda", sr_render()); }
+void ProjectSrcPointsToObjects(bool fo) { report("ProjectSrcPointsToObjects", sr_project_src_points_to_objects(fo)); }
+void ProjectTargetToFieldObjects(void) { report("ProjectTargetToFieldObjects", sr_project_target_to_field_objects()); }
+void ProjectTargetToFieldPoints(void) { report("ProjectTargetToFieldPoints", sr_project_target_to_field_points()); }
+void ProjectTargetToTargetObjects(int n) { report("ProjectTargetToTargetObjects", sr_project_target_to_target_objects(n)); }
+void GetFieldPointPressures(sr_dcomplex *out, int n) { report("GetFieldPointPressures", sr_get_field_point_pressures(out, n)); }
+void RenderOpenGL(int, int, char *, float *) {
+    int rc = guarded("RenderOpenGL", [&] {
+        sr::set_error("the OpenGL/PNG render runs on the host outside this library");
+        return -2;
+    });
+    report("RenderOpenGL", rc);
+}
+void SoundVisualisationInit(float, float, bool, bool) {
+    int rc = guarded("SoundVisualisationInit", [&] {
+        sr::set_error("render colour settings belong to the host renderer");
+        return -2;
+    });
+    report("SoundVisualisationInit", rc);
+}
+void TearDownCuda(void) { report("TearDownCuda", sr_teardown()); }
+
+}  // extern "C"

@@ -1,0 +1,811 @@
+// Projection sequencing: the reference's render_cuda / Project* call sequence
+// (src/ModelCuda.cpp:242-356, src/Model.cpp:56-93) over SoA fragment tables.
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "launch.h"
+#include "model_ops.h"
+#include "transfer.h"
+
+namespace sr {
+
+// Kernel launchers from vis.hip
+int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 *dTrisSorted, float4 *dNodes,
+               hipStream_t st);
+int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st);
+
+namespace {
+
+std::mutex gErrMu;
+std::string gErr;
+
+template <class T>
+int dev_alloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return 0;
+    SR_HIP(hipMalloc((void **)p, n * sizeof(T)));
+    return 0;
+}
+template <class T>
+void dev_free(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+enum MaskKind { MK_P2F = 0, MK_F2P = 1, MK_SELF = 2, MK_NORESP = 3, MK_RESP_FWD = 4, MK_RESP_REC = 5 };
+
+Object *object_at(Model &m, int type, int idx) {
+    std::vector<Object> &v = (type == OBJ_FIELD) ? m.fields : m.targets;
+    if (idx < 0 || idx >= (int)v.size()) return nullptr;
+    return &v[idx];
+}
+
+Phys make_phys(const Model &m, float deltaReg, float damp) {
+    Phys p;
+    p.kr = m.kf;
+    const double krev = (double)m.kf / (2.0 * M_PI);
+    p.krevHi = (float)krev;
+    p.krevLo = (float)(krev - (double)p.krevHi);
+    p.alpha = m.atten;
+    p.alphaLog2e = (float)((double)m.atten * 1.4426950408889634);
+    p.deltaReg = deltaReg;
+    p.damp = damp;
+    return p;
+}
+
+int upload_object(Model &m, Object &o) {
+    const int nf = (int)o.frames.size();
+    o.grids.resize(nf);
+    o.cellOff.resize(nf);
+    long off = 0;
+    for (int f = 0; f < nf; f++) {
+        o.grids[f] = facet_grid(o.frames[f], o.delta);
+        o.cellOff[f] = off;
+        off += (long)o.grids[f].nX * o.grids[f].nY;
+    }
+    o.ncell = off;
+    std::vector<float4> cp(off), cn(off), ap, an;
+    o.actCellHost.clear();
+    for (int f = 0; f < nf; f++) {
+        const FacetFrame &F = o.frames[f];
+        const FacetGrid &G = o.grids[f];
+        const float4 nrm = make_float4(F.normal.x, F.normal.y, F.normal.z, 0.f);
+        float fid;
+        std::memcpy(&fid, &f, sizeof(int));
+        for (int j = 0; j < G.nY; j++)
+            for (int i = 0; i < G.nX; i++) {
+                const long c = o.cellOff[f] + (long)j * G.nX + i;
+                const float A = G.area[(size_t)j * G.nX + i];
+                Vec3 r = cell_point(F, G, o.delta, i, j, true);
+                cp[c] = make_float4(r.x, r.y, r.z, A);
+                cn[c] = make_float4(nrm.x, nrm.y, nrm.z, fid);
+                if (A > 1e-6f) {  // OptiX.cu:138-142,219-223,286-290
+                    Vec3 e = cell_point(F, G, o.delta, i, j, false);
+                    ap.push_back(make_float4(e.x, e.y, e.z, A));
+                    an.push_back(make_float4(nrm.x, nrm.y, nrm.z, fid));
+                    o.actCellHost.push_back((int)c);
+                }
+            }
+    }
+    o.nact = (int)ap.size();
+    o.actFacetHost.resize(o.nact);
+    for (int k = 0; k < o.nact; k++) std::memcpy(&o.actFacetHost[k], &an[k].w, sizeof(int));
+    {
+        std::vector<double> perFacet(nf, 0.0);
+        for (auto &q : an) { int f; std::memcpy(&f, &q.w, sizeof(int)); perFacet[f] += 1.0; }
+        o.sameFacetPairs = 0;
+        for (double k : perFacet) o.sameFacetPairs += k * k;
+    }
+    const size_t np = (size_t)m.nplanes;
+    if (dev_alloc(&o.cellP, off) || dev_alloc(&o.cellN, off) || dev_alloc(&o.actP, o.nact) ||
+        dev_alloc(&o.actN, o.nact) || dev_alloc(&o.actCell, o.nact) || dev_alloc(&o.P, np * off))
+        return -1;
+    if (off) {
+        SR_HIP(hipMemcpy(o.cellP, cp.data(), off * sizeof(float4), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(o.cellN, cn.data(), off * sizeof(float4), hipMemcpyHostToDevice));
+        SR_HIP(hipMemset(o.P, 0, np * off * sizeof(float2)));
+    }
+    if (o.nact) {
+        SR_HIP(hipMemcpy(o.actP, ap.data(), o.nact * sizeof(float4), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(o.actN, an.data(), o.nact * sizeof(float4), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(o.actCell, o.actCellHost.data(), o.nact * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (o.type == OBJ_TARGET && o.nact) {
+        if (dev_alloc(&o.Pin, np * o.nact) || dev_alloc(&o.Pout, np * o.nact)) return -1;
+        SR_HIP(hipMemset(o.Pin, 0, np * o.nact * sizeof(float2)));
+        SR_HIP(hipMemset(o.Pout, 0, np * o.nact * sizeof(float2)));
+    }
+    return 0;
+}
+
+void free_transfer(Object &o) {
+    auto &t = o.tc;
+    dev_free(t.A); dev_free(t.B); dev_free(t.facet); dev_free(t.rtFmin); dev_free(t.rtFmax);
+    dev_free(t.ksFmin); dev_free(t.ksFmax); dev_free(t.colScale); dev_free(t.colInv);
+    t = Object::Transfer{};
+}
+
+void free_object(Object &o) {
+    dev_free(o.cellP); dev_free(o.cellN); dev_free(o.actP); dev_free(o.actN); dev_free(o.actCell);
+    dev_free(o.P); dev_free(o.Pin); dev_free(o.Pout);
+    free_transfer(o);
+}
+
+int ensure_scratch(Model &m, size_t elems) {
+    if (elems <= m.scratchElems) return 0;
+    dev_free(m.scratchL);
+    dev_free(m.scratchU);
+    m.scratchElems = 0;
+    if (dev_alloc(&m.scratchL, elems) || dev_alloc(&m.scratchU, elems)) return -1;
+    m.scratchElems = elems;
+    return 0;
+}
+
+Bvh bvh_of(const Model &m) { return Bvh{m.trisSorted, m.nodes, m.ntri}; }
+
+// Brackets one launch with HIP events on the model's stream when profiling.
+struct ProfScope {
+    Model &m;
+    int kind;
+    double pairs, flops, bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(Model &mm, int k, double p, double f, double by = 0.0) : m(mm), kind(k), pairs(p), flops(f), bytes(by) {
+        if (!m.profiling) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+        (void)hipEventRecord(a, m.stream);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        (void)hipEventRecord(b, m.stream);
+        m.recs.push_back(Model::Rec{kind, a, b, pairs, flops, bytes});
+    }
+};
+
+}  // namespace
+
+void set_error(const std::string &msg) {
+    std::lock_guard<std::mutex> g(gErrMu);
+    gErr = msg;
+}
+std::string last_error() {
+    std::lock_guard<std::mutex> g(gErrMu);
+    return gErr;
+}
+void clear_error() { set_error(""); }
+
+// ---------------------------------------------------------------------------
+
+int model_set_initial_conditions(Model &m, float cp, float f, float att, float density) {
+    if (!(cp > 0.f)) { set_error("set_initial_conditions: sound speed must be > 0"); return -2; }
+    m.cp = cp; m.freq = f; m.atten = att; m.density = density;
+    const float omega = (float)(2 * M_PI * f);  // Model.cpp:50 (float member)
+    m.kf = omega / cp;                           // Model.cpp:51
+    for (auto &o : m.targets) o.tc.valid = false;  // S depends on k
+    return 0;
+}
+
+int model_load_geometry(Model &m, const float *v, int nf, int type, float res) {
+    if (type < OBJ_TARGET || type > OBJ_FIELD) { set_error("load_geometry: invalid object type"); return -2; }
+    if (m.rendered) { set_error("load_geometry: scene already rendered; call TearDownCuda first"); return -3; }
+    if (nf < 0 || (nf > 0 && !v)) { set_error("load_geometry: bad facet array"); return -2; }
+    if (!(res > 0.f)) { set_error("load_geometry: resolution must be > 0"); return -2; }
+    if (type == OBJ_SOURCE) { m.numSourceObjects++; return 0; }  // stored, never used (Model.cpp:33-35)
+    Object o;
+    o.type = type;
+    o.delta = res;
+    o.frames.reserve(nf);
+    for (int i = 0; i < nf; i++) {
+        const float *t = v + (size_t)9 * i;
+        o.frames.push_back(facet_frame(Vec3{t[0], t[1], t[2]}, Vec3{t[3], t[4], t[5]}, Vec3{t[6], t[7], t[8]}));
+    }
+    (type == OBJ_TARGET ? m.targets : m.fields).push_back(std::move(o));
+    return 0;
+}
+
+int model_load_points(Model &m, const float *p, int n, bool source) {
+    if (m.rendered) { set_error("points loaded after render_cuda (already allocated on the GPU)"); return -3; }
+    if (n < 0 || (n > 0 && !p)) { set_error("load points: bad array"); return -2; }
+    std::vector<float> &dst = source ? m.srcHost : m.fldHost;
+    dst.insert(dst.end(), p, p + (size_t)3 * n);
+    return 0;
+}
+
+int model_set_batched_sources(Model &m, int on) {
+    if (m.rendered) { set_error("batched sources must be chosen before render_cuda"); return -3; }
+    m.batchedSources = on != 0;
+    return 0;
+}
+
+int model_render(Model &m) {
+    if (m.rendered) { set_error("render_cuda called twice without TearDownCuda"); return -3; }
+    if (!m.stream) SR_HIP(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+    const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
+    m.nplanes = m.batchedSources ? std::max(1, ns) : 1;
+    for (auto &o : m.targets) if (upload_object(m, o)) return -1;
+    for (auto &o : m.fields) if (upload_object(m, o)) return -1;
+
+    // Occluders: every target triangle in load order (OptiX.cpp:273-292).
+    std::vector<float4> tris;
+    float scale = 1.0f;
+    float3 lo = make_float3(1e30f, 1e30f, 1e30f), hi = make_float3(-1e30f, -1e30f, -1e30f);
+    auto upd = [&](float x, float y, float z) {
+        scale = std::max(scale, std::max(std::fabs(x), std::max(std::fabs(y), std::fabs(z))));
+    };
+    for (auto &o : m.targets)
+        for (auto &F : o.frames) {
+            Vec3 a = F.v1, b = F.v2, c = F.v3;
+            Vec3 e1{b.x - a.x, b.y - a.y, b.z - a.z}, e2{c.x - a.x, c.y - a.y, c.z - a.z};
+            Vec3 mn{std::min(a.x, std::min(b.x, c.x)), std::min(a.y, std::min(b.y, c.y)), std::min(a.z, std::min(b.z, c.z))};
+            Vec3 mx{std::max(a.x, std::max(b.x, c.x)), std::max(a.y, std::max(b.y, c.y)), std::max(a.z, std::max(b.z, c.z))};
+            tris.push_back(make_float4(a.x, a.y, a.z, e1.x));
+            tris.push_back(make_float4(e1.y, e1.z, e2.x, e2.y));
+            tris.push_back(make_float4(e2.z, mn.x, mn.y, mn.z));
+            tris.push_back(make_float4(mx.x, mx.y, mx.z, 0.f));
+            lo = make_float3(std::min(lo.x, mn.x), std::min(lo.y, mn.y), std::min(lo.z, mn.z));
+            hi = make_float3(std::max(hi.x, mx.x), std::max(hi.y, mx.y), std::max(hi.z, mx.z));
+            upd(a.x, a.y, a.z); upd(b.x, b.y, b.z); upd(c.x, c.y, c.z);
+        }
+    for (auto &o : m.fields)
+        for (auto &F : o.frames) { upd(F.v1.x, F.v1.y, F.v1.z); upd(F.v2.x, F.v2.y, F.v2.z); upd(F.v3.x, F.v3.y, F.v3.z); }
+    for (size_t i = 0; i < m.srcHost.size(); i += 3) upd(m.srcHost[i], m.srcHost[i + 1], m.srcHost[i + 2]);
+    for (size_t i = 0; i < m.fldHost.size(); i += 3) upd(m.fldHost[i], m.fldHost[i + 1], m.fldHost[i + 2]);
+    m.vc.scale = scale;
+    m.vc.padTri = 1e-5f * scale;
+    m.vc.padNode = 4e-5f * scale;
+    m.vc.tminPoint = 0.0f;
+    m.vc.tminFrag = 1e-3f;
+
+    m.ntri = (int)(tris.size() / 4);
+    if (m.ntri > 0) {
+        float4 *unsorted = nullptr;
+        if (dev_alloc(&unsorted, tris.size()) || dev_alloc(&m.trisSorted, tris.size()) ||
+            dev_alloc(&m.nodes, (size_t)4 * std::max(1, m.ntri - 1)))
+            return -1;
+        SR_HIP(hipMemcpy(unsorted, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+        int rc = lbvh_build(unsorted, m.ntri, lo, hi, m.trisSorted, m.nodes, m.stream);
+        SR_HIP(hipStreamSynchronize(m.stream));
+        dev_free(unsorted);
+        if (rc) return rc;
+    }
+    if (dev_alloc(&m.errFlag, 2)) return -1;
+    SR_HIP(hipMemset(m.errFlag, 0, 2 * sizeof(int)));
+
+    // Points (ModelCuda.cpp:39-94); sources carry amplitude 1+0i (Model.cpp:18).
+    if (ns) {
+        std::vector<float4> sp(ns);
+        std::vector<float2> sa(ns, make_float2(1.f, 0.f));
+        for (int i = 0; i < ns; i++) sp[i] = make_float4(m.srcHost[3 * i], m.srcHost[3 * i + 1], m.srcHost[3 * i + 2], 0.f);
+        if (dev_alloc(&m.srcP, ns) || dev_alloc(&m.srcAmp, ns)) return -1;
+        SR_HIP(hipMemcpy(m.srcP, sp.data(), ns * sizeof(float4), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(m.srcAmp, sa.data(), ns * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    if (nq) {
+        std::vector<float4> fp(nq);
+        for (int i = 0; i < nq; i++) fp[i] = make_float4(m.fldHost[3 * i], m.fldHost[3 * i + 1], m.fldHost[3 * i + 2], 0.f);
+        if (dev_alloc(&m.fldP, nq) || dev_alloc(&m.fldVal, (size_t)nq * m.nplanes)) return -1;
+        SR_HIP(hipMemcpy(m.fldP, fp.data(), nq * sizeof(float4), hipMemcpyHostToDevice));
+        SR_HIP(hipMemset(m.fldVal, 0, (size_t)nq * m.nplanes * sizeof(double2)));
+    }
+    SR_HIP(hipDeviceSynchronize());
+    m.rendered = true;
+    return 0;
+}
+
+int model_set_source_amplitude(Model &m, int i, float re, float im) {
+    const int ns = (int)(m.srcHost.size() / 3);
+    if (!m.rendered) { set_error("source amplitudes live on the GPU: call render_cuda first"); return -3; }
+    if (i < 0 || i >= ns) { set_error("source index out of range"); return -2; }
+    float2 a = make_float2(re, im);
+    SR_HIP(hipMemcpy(m.srcAmp + i, &a, sizeof(float2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Visibility words for one (emitter set, receiver set), built on first use.
+int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out) {
+    auto key = std::make_tuple(kind, ta, ia, tb, ib);
+    auto it = m.masks.find(key);
+    if (it != m.masks.end() && it->second.valid) { *out = it->second.p; return 0; }
+    MaskTask tk{};
+    Object *A = (kind == MK_P2F) ? nullptr : object_at(m, ta, ia);
+    Object *B = (kind == MK_P2F || kind >= MK_NORESP) ? object_at(m, tb, ib) : nullptr;
+    const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
+    switch (kind) {
+    case MK_P2F:
+        tk.em = PointSet{m.srcP, nullptr, ns};
+        tk.rc = PointSet{B->cellP, B->cellN, (int)B->ncell};
+        tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 0; tk.useNormalRc = (B->type == OBJ_TARGET);
+        tk.tmin = m.vc.tminPoint;
+        break;
+    case MK_F2P:
+        tk.em = PointSet{A->actP, A->actN, A->nact};
+        tk.rc = PointSet{m.fldP, nullptr, nq};
+        tk.origin = ORIGIN_RECEIVER; tk.useNormalEm = 1; tk.useNormalRc = 0;
+        tk.tmin = m.vc.tminPoint;
+        break;
+    case MK_SELF:
+        tk.em = PointSet{A->actP, A->actN, A->nact};
+        tk.rc = tk.em;
+        tk.origin = ORIGIN_LOWER_FACET; tk.useNormalEm = 1; tk.useNormalRc = 1; tk.sameObject = 1;
+        tk.tmin = m.vc.tminFrag;
+        break;
+    case MK_NORESP:
+        tk.em = PointSet{A->actP, A->actN, A->nact};
+        tk.rc = PointSet{B->actP, B->actN, B->nact};
+        tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 1; tk.useNormalRc = (B->type == OBJ_TARGET);
+        tk.tmin = m.vc.tminFrag;
+        break;
+    case MK_RESP_FWD:
+        tk.em = PointSet{A->actP, A->actN, A->nact};
+        tk.rc = PointSet{B->actP, B->actN, B->nact};
+        tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 1; tk.useNormalRc = 1;
+        tk.tmin = m.vc.tminFrag;
+        break;
+    case MK_RESP_REC:
+        tk.em = PointSet{B->actP, B->actN, B->nact};
+        tk.rc = PointSet{A->actP, A->actN, A->nact};
+        tk.origin = ORIGIN_RECEIVER; tk.useNormalEm = 1; tk.useNormalRc = 1;
+        tk.tmin = m.vc.tminFrag;
+        break;
+    default:
+        set_error("unknown mask kind");
+        return -2;
+    }
+    const size_t nrb = (size_t)(tk.rc.count + 63) / 64;
+    const size_t words = std::max<size_t>(1, nrb * (size_t)tk.em.count);
+    Model::MaskBuf &mb = m.masks[key];
+    if (mb.words != words) {
+        dev_free(mb.p);
+        mb.words = 0;
+        if (dev_alloc(&mb.p, words)) return -1;
+        mb.words = words;
+    }
+    uint64_t *mk = mb.p;
+    tk.mask = mk;
+    tk.errFlag = m.errFlag;
+    {
+        ProfScope ps(m, PK_VIS, (double)tk.em.count * tk.rc.count, 0.0);
+        if (vis_mask_launch(tk, bvh_of(m), m.vc, m.stream)) return -1;
+    }
+    int herr = 0;
+    SR_HIP(hipMemcpyAsync(&herr, m.errFlag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    SR_HIP(hipStreamSynchronize(m.stream));
+    if (herr) { set_error("visibility traversal stack overflow"); return -1; }
+    mb.valid = true;
+    *out = mk;
+    return 0;
+}
+
+static int check_rendered(const Model &m) {
+    if (!m.rendered) { set_error("call render_cuda before projecting"); return -3; }
+    return 0;
+}
+
+// ProjectSrcPointsToObjects (ModelCuda.cpp:242-278)
+int model_project_src(Model &m, bool fieldObjects) {
+    if (int rc = check_rendered(m)) return rc;
+    const int ns = (int)(m.srcHost.size() / 3);
+    if (ns == 0) return 0;
+    Phys ph = make_phys(m, 0.f, 1.f);
+    auto run = [&](Object &o, int type, int idx) -> int {
+        if (o.ncell == 0) return 0;
+        uint64_t *mk;
+        if (get_mask(m, MK_P2F, 0, 0, type, idx, &mk)) return -1;
+        P2FTask t{o.cellP, o.cellN, (int)o.ncell, o.type == OBJ_TARGET, m.srcP, m.srcAmp, ns, mk,
+                  m.batchedSources ? 1 : 0, o.P, o.ncell};
+        ProfScope ps(m, PK_P2F, (double)ns * o.ncell, 30.0 * ns * o.ncell);
+        return p2f_launch(t, ph, m.stream);
+    };
+    for (int i = 0; i < (int)m.targets.size(); i++) if (run(m.targets[i], OBJ_TARGET, i)) return -1;
+    if (fieldObjects)
+        for (int i = 0; i < (int)m.fields.size(); i++) if (run(m.fields[i], OBJ_FIELD, i)) return -1;
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// One gather + reduce over all planes in groups of <= 16.
+// pairs: ordered pairs one plane of this gather evaluates (for the profiler's
+// flop count: 37 shared flops per pair plus 8 per plane for the complex MAC).
+static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, ReduceTask rt, int nplanes,
+                         long evalStride, long gvalStride, long outStride, long pcellStride, double pairs) {
+    const int nrb = (t.nr + 63) / 64;
+    const int minPer = (kind == KIND_F2P) ? 128 : 256;
+    t.nsplit = choose_splits(nrb, t.ne, minPer);
+    t.emPerSplit = (t.ne + t.nsplit - 1) / t.nsplit;
+    t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
+    for (int b0 = 0; b0 < nplanes; b0 += 16) {
+        const int nb = std::min(16, nplanes - b0);
+        const int nbs = gather_nb_stride(nb);
+        if (ensure_scratch(m, (size_t)t.nsplit * nbs * t.nr)) return -1;
+        GatherTask g = t;
+        g.nrhs = nb;
+        g.evalStride = evalStride;
+        g.gvalStride = gvalStride;
+        g.eval = t.eval + b0 * evalStride;
+        if (t.gval) g.gval = t.gval + b0 * gvalStride;
+        g.partL = m.scratchL;
+        g.partU = m.scratchU;
+        {
+            const int pk = kind == KIND_F2F ? PK_F2F : (kind == KIND_F2F_FIELD ? PK_F2F_FIELD : PK_F2P);
+            ProfScope ps(m, pk, pairs * nb, pairs * (37.0 + 8.0 * nb));
+            if (gather_launch(g, ph, kind, m.stream)) return -1;
+        }
+        ProfScope pr(m, PK_REDUCE, 0.0, 0.0);
+        if (kind == KIND_F2P) {
+            if (reduce_points_launch(m.scratchL, m.scratchU, g.nsplit, nbs, nb, t.nr, m.fldVal + (size_t)b0 * t.nr,
+                                     t.nr, m.stream))
+                return -1;
+        } else {
+            ReduceTask r = rt;
+            r.partL = m.scratchL;
+            r.partU = m.scratchU;
+            r.nsplit = g.nsplit;
+            r.nbStride = nbs;
+            r.nrhs = nb;
+            r.nr = t.nr;
+            r.gvalStride = gvalStride;
+            r.outStride = outStride;
+            r.pcellStride = pcellStride;
+            if (rt.gval) r.gval = rt.gval + b0 * gvalStride;
+            if (rt.out) r.out = rt.out + b0 * outStride;
+            if (rt.pcell) r.pcell = rt.pcell + b0 * pcellStride;
+            if (reduce_launch(r, m.stream)) return -1;
+        }
+    }
+    return 0;
+}
+
+// Builds (or rebuilds) the FP16 transfer matrix of target i.
+static int ensure_transfer(Model &m, int i) {
+    Object &o = m.targets[i];
+    auto &t = o.tc;
+    if (t.valid) return 0;
+    uint64_t *mk;
+    if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
+    const int n = o.nact;
+    const int nKs = (n + 7) / 8;
+    const int nRt = ((n + 127) / 128) * 4;
+    if (!t.A || t.nKs != nKs || t.nRt != nRt) {
+        free_transfer(o);
+        if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) || dev_alloc(&t.facet, n) ||
+            dev_alloc(&t.rtFmin, nRt) || dev_alloc(&t.rtFmax, nRt) || dev_alloc(&t.ksFmin, nKs) ||
+            dev_alloc(&t.ksFmax, nKs) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64))
+            return -1;
+        t.nKs = nKs;
+        t.nRt = nRt;
+        std::vector<int> rmin(nRt, 0), rmax(nRt, 0), kmin(nKs), kmax(nKs);
+        for (int r = 0; r < nRt; r++) {
+            const int a = std::min(n - 1, r * 32), b = std::min(n - 1, r * 32 + 31);
+            rmin[r] = o.actFacetHost[a];
+            rmax[r] = o.actFacetHost[b];
+        }
+        for (int k = 0; k < nKs; k++) {
+            const int a = k * 8, b = std::min(n - 1, k * 8 + 7);
+            kmin[k] = o.actFacetHost[a];
+            kmax[k] = o.actFacetHost[b];
+        }
+        SR_HIP(hipMemcpy(t.facet, o.actFacetHost.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(t.rtFmin, rmin.data(), nRt * sizeof(int), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(t.rtFmax, rmax.data(), nRt * sizeof(int), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(t.ksFmin, kmin.data(), nKs * sizeof(int), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(t.ksFmax, kmax.data(), nKs * sizeof(int), hipMemcpyHostToDevice));
+    }
+    // One FP16 scale for S: |cos_s cos_i| <= 1, |G| <= |k|/(4 pi delta) e^{max(0,-alpha) D} + 1/delta^2
+    // for r >= delta and |k|/(4 pi) below it.
+    const double kabs = std::hypot((double)m.kf, (double)m.atten);
+    const double D = 2.0 * m.vc.scale * std::sqrt(3.0);
+    const double grow = m.atten < 0 ? std::exp(-m.atten * D) : 1.0;
+    const double d = o.delta;
+    const double smax = 0.9 * std::max(kabs / (4 * M_PI), kabs * grow / (4 * M_PI * d) + 1.0 / (d * d));
+    t.sS = (float)(16384.0 / smax);
+    TransferGeom g{o.actP, o.actN, n, nKs, nRt};
+    TransferPhys ph{m.kf, m.atten, (float)((double)m.atten * 1.4426950408889634), o.delta, 0.9f, t.sS,
+                    (double)m.kf / (2.0 * M_PI)};
+    {
+        ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, 0.0);
+        if (transfer_build_s(g, ph, mk, t.A, m.stream)) return -1;
+    }
+    t.valid = true;
+    return 0;
+}
+
+// One bounce of target i in the transfer form: P_out = L x + [P_in != 0] (U x), P += P_out.
+static int bounce_mixed(Model &m, int i) {
+    Object &o = m.targets[i];
+    auto &t = o.tc;
+    const int n = o.nact;
+    int *flag = m.errFlag + 1;
+    SR_HIP(hipMemsetAsync(flag, 0, sizeof(int), m.stream));
+    if (any_zero_launch(o.Pin, (long)n * m.nplanes, flag, m.stream)) return -1;
+    int gated = 0;
+    SR_HIP(hipMemcpyAsync(&gated, flag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    SR_HIP(hipStreamSynchronize(m.stream));
+    const int wgRows = (t.nRt + 3) / 4;
+    int nsplit = std::max(1, std::min(1024 / std::max(1, wgRows), t.nKs / 64));
+    const int ksPer = (t.nKs + nsplit - 1) / nsplit;
+    nsplit = (t.nKs + ksPer - 1) / ksPer;
+    if (ensure_scratch(m, (size_t)nsplit * 64 * n)) return -1;
+    for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
+        const int np = std::min(64, m.nplanes - g0);
+        const float2 *pin = o.Pin + (size_t)g0 * n;
+        {
+            ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
+            if (transfer_col_scale(o.actP, pin, n, n, np, t.colScale, t.colInv, t.sS, m.stream)) return -1;
+            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, t.colScale, t.B, m.stream)) return -1;
+        }
+        TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.facet, t.facet, t.rtFmin, t.rtFmax,
+                        t.ksFmin, t.ksFmax, m.scratchL, m.scratchU};
+        {
+            const double pairs = ((double)n * n - o.sameFacetPairs) * np;
+            // flops of the GEMM as executed: 2 * rows * 128 columns * 2 * K (padded)
+            const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
+            // algorithmic HBM bytes: S read once, B read once, partial sums written
+            const double by = (double)t.nRt * t.nKs * 64 * 16 + (double)t.nKs * 256 * 16 +
+                              (double)nsplit * np * n * 8 * (gated ? 2 : 1);
+            ProfScope ps(m, PK_GEMM, pairs, fl, by);
+            if (transfer_gemm(gm, gated != 0, m.stream)) return -1;
+        }
+        ReduceTask r{};
+        r.partL = m.scratchL;
+        r.partU = gated ? m.scratchU : nullptr;
+        r.nsplit = nsplit;
+        r.nbStride = 64;
+        r.nrhs = np;
+        r.nr = n;
+        r.gval = pin;
+        r.gvalStride = n;
+        r.gateMode = gated ? GATE_UPPER : GATE_NONE;
+        r.out = o.Pout + (size_t)g0 * n;
+        r.outStride = n;
+        r.pcell = o.P + (size_t)g0 * o.ncell;
+        r.pcellStride = o.ncell;
+        r.rowCell = o.actCell;
+        r.colInvScale = t.colInv;
+        ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
+        if (reduce_launch(r, m.stream)) return -1;
+    }
+    return 0;
+}
+
+// FACET_SELF bounces (ModelCuda.cpp:323-356, FacetCuda.cpp:124-137, FacetCuda.cu:87-116).
+int model_project_t2t(Model &m, int nref) {
+    if (int rc = check_rendered(m)) return rc;
+    if (nref < 0) { set_error("number_reflections must be >= 0"); return -2; }
+    for (auto &o : m.targets)  // PrimeReflections: P_out <- P
+        if (o.nact && prime_launch(o.P, o.ncell, o.actCell, o.nact, m.nplanes, o.Pout, o.nact, m.stream)) return -1;
+    for (int it = 0; it < nref; it++)
+        for (int i = 0; i < (int)m.targets.size(); i++) {
+            Object &o = m.targets[i];
+            if (!o.nact) continue;
+            std::swap(o.Pin, o.Pout);  // SwapOutputToInputPressure; P_out is fully rewritten below
+            if (m.transferMode == 1) {
+                if (ensure_transfer(m, i) || bounce_mixed(m, i)) return -1;
+                continue;
+            }
+            uint64_t *mk;
+            if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
+            Phys ph = make_phys(m, o.delta, 0.9f);
+            GatherTask t{};
+            t.ep = o.actP; t.en = o.actN; t.ecell = nullptr; t.eval = o.Pin; t.ne = o.nact;
+            t.rp = o.actP; t.rn = o.actN; t.nr = o.nact; t.mask = mk;
+            t.gval = o.Pin; t.gateMode = GATE_UPPER;
+            ReduceTask r{};
+            r.gval = o.Pin; r.gateMode = GATE_UPPER;
+            r.out = o.Pout; r.pcell = o.P; r.rowCell = o.actCell;  // P += P_out (AccumulatePressure)
+            const double pairs = (double)o.nact * o.nact - o.sameFacetPairs;
+            if (gather_planes(m, t, ph, KIND_F2F, r, m.nplanes, o.nact, o.nact, o.nact, o.ncell, pairs)) return -1;
+        }
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// FACET_NO_RESP: every target into every field object (ModelCuda.cpp:280-301).
+int model_project_t2field_objects(Model &m) {
+    if (int rc = check_rendered(m)) return rc;
+    for (int i = 0; i < (int)m.targets.size(); i++)
+        for (int j = 0; j < (int)m.fields.size(); j++) {
+            Object &A = m.targets[i], &B = m.fields[j];
+            if (!A.nact || !B.nact) continue;
+            uint64_t *mk;
+            if (get_mask(m, MK_NORESP, OBJ_TARGET, i, OBJ_FIELD, j, &mk)) return -1;
+            Phys ph = make_phys(m, A.delta, 0.9f);
+            GatherTask t{};
+            t.ep = A.actP; t.en = A.actN; t.ecell = A.actCell; t.eval = A.P; t.ne = A.nact;
+            t.rp = B.actP; t.rn = B.actN; t.nr = B.nact; t.mask = mk;
+            t.gateMode = GATE_NONE;
+            ReduceTask r{};
+            r.gateMode = GATE_NONE;
+            r.pcell = B.P; r.rowCell = B.actCell;  // field objects alias P = P_in = P_out
+            if (gather_planes(m, t, ph, KIND_F2F_FIELD, r, m.nplanes, A.ncell, 0, 0, B.ncell, (double)A.nact * B.nact))
+                return -1;
+        }
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// FIELD_POINTS (ModelCuda.cpp:302-321): compensated FP32 gathers, FP64 totals.
+int model_project_t2field_points(Model &m) {
+    if (int rc = check_rendered(m)) return rc;
+    const int nq = (int)(m.fldHost.size() / 3);
+    if (!nq) return 0;
+    for (int i = 0; i < (int)m.targets.size(); i++) {
+        Object &A = m.targets[i];
+        if (!A.nact) continue;
+        uint64_t *mk;
+        if (get_mask(m, MK_F2P, OBJ_TARGET, i, 0, 0, &mk)) return -1;
+        Phys ph = make_phys(m, 0.f, 1.f);
+        GatherTask t{};
+        t.ep = A.actP; t.en = A.actN; t.ecell = A.actCell; t.eval = A.P; t.ne = A.nact;
+        t.rp = m.fldP; t.rn = nullptr; t.nr = nq; t.mask = mk;
+        t.gateMode = GATE_NONE;
+        ReduceTask r{};
+        if (gather_planes(m, t, ph, KIND_F2P, r, m.nplanes, A.ncell, 0, 0, 0, (double)A.nact * nq)) return -1;
+    }
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// FACET_RESP between two target objects (OptiX.cu:427-430; never dispatched by
+// the reference's orchestration). Accumulates into both objects' P_out.
+int model_project_resp(Model &m, int ia, int ib) {
+    if (int rc = check_rendered(m)) return rc;
+    Object *A = object_at(m, OBJ_TARGET, ia), *B = object_at(m, OBJ_TARGET, ib);
+    if (!A || !B || ia == ib) { set_error("RESP needs two distinct target objects"); return -2; }
+    if (!A->nact || !B->nact) return 0;
+    uint64_t *mf, *mr;
+    if (get_mask(m, MK_RESP_FWD, OBJ_TARGET, ia, OBJ_TARGET, ib, &mf)) return -1;
+    if (get_mask(m, MK_RESP_REC, OBJ_TARGET, ia, OBJ_TARGET, ib, &mr)) return -1;
+    Phys ph = make_phys(m, A->delta, 0.9f);
+    // forward: rows of B, emitters of A (skip when P_in_A == 0 is implicit: x = 0)
+    GatherTask t{};
+    t.ep = A->actP; t.en = A->actN; t.eval = A->Pin; t.ne = A->nact;
+    t.rp = B->actP; t.rn = B->actN; t.nr = B->nact; t.mask = mf; t.gateMode = GATE_NONE;
+    ReduceTask r{};
+    r.gateMode = GATE_NONE; r.out = B->Pout;
+    r.outAdd = 1;
+    if (gather_planes(m, t, ph, KIND_F2F, r, m.nplanes, A->nact, 0, B->nact, 0, (double)A->nact * B->nact)) return -1;
+    // reciprocal: rows of A gated by their own P_in, emitters of B
+    GatherTask u{};
+    u.ep = B->actP; u.en = B->actN; u.eval = B->Pin; u.ne = B->nact;
+    u.rp = A->actP; u.rn = A->actN; u.nr = A->nact; u.mask = mr; u.gateMode = GATE_NONE;
+    ReduceTask s{};
+    s.gateMode = GATE_ALL; s.gval = A->Pin; s.out = A->Pout; s.outAdd = 1;
+    if (gather_planes(m, u, ph, KIND_F2F, s, m.nplanes, B->nact, A->nact, A->nact, 0, (double)A->nact * B->nact))
+        return -1;
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// P_in <- P (active cells), P_out <- 0: the state a RESP call reads.
+int model_prepare_resp(Model &m, int it) {
+    if (int rc = check_rendered(m)) return rc;
+    Object *o = object_at(m, OBJ_TARGET, it);
+    if (!o) { set_error("target index out of range"); return -2; }
+    if (!o->nact) return 0;
+    if (prime_launch(o->P, o->ncell, o->actCell, o->nact, m.nplanes, o->Pin, o->nact, m.stream)) return -1;
+    SR_HIP(hipMemsetAsync(o->Pout, 0, (size_t)m.nplanes * o->nact * sizeof(float2), m.stream));
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int model_get_field_points(Model &m, double *out, long n, int plane) {
+    const int nq = (int)(m.fldHost.size() / 3);
+    if (!m.rendered) { set_error("no field points on the GPU: call render_cuda first"); return -3; }
+    if (!out && n > 0) { set_error("null output buffer"); return -2; }
+    if (plane < 0 || plane >= m.nplanes) { set_error("plane out of range"); return -2; }
+    const long cnt = std::min<long>(n, nq);
+    if (cnt > 0)
+        SR_HIP(hipMemcpy(out, m.fldVal + (size_t)plane * nq, cnt * sizeof(double2), hipMemcpyDeviceToHost));
+    return (int)cnt;
+}
+
+int model_get_object_pressure(Model &m, int type, int idx, int which, int plane, double *out, long n) {
+    if (!m.rendered) { set_error("call render_cuda first"); return -3; }
+    Object *o = object_at(m, type, idx);
+    if (!o) { set_error("object index out of range"); return -2; }
+    if (plane < 0 || plane >= m.nplanes) { set_error("plane out of range"); return -2; }
+    if (n < o->ncell) { set_error("output buffer shorter than the object's cell count"); return -2; }
+    std::vector<float2> h(o->ncell, make_float2(0.f, 0.f));
+    if (which == 0 || o->type == OBJ_FIELD) {
+        if (o->ncell)
+            SR_HIP(hipMemcpy(h.data(), o->P + (size_t)plane * o->ncell, o->ncell * sizeof(float2), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float2> a(o->nact);
+        const float2 *src = (which == 1) ? o->Pin : o->Pout;
+        if (o->nact)
+            SR_HIP(hipMemcpy(a.data(), src + (size_t)plane * o->nact, o->nact * sizeof(float2), hipMemcpyDeviceToHost));
+        for (int k = 0; k < o->nact; k++) h[o->actCellHost[k]] = a[k];
+    }
+    for (long c = 0; c < o->ncell; c++) { out[2 * c] = h[c].x; out[2 * c + 1] = h[c].y; }
+    return 0;
+}
+
+int model_get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t *out, long nwords) {
+    if (int rc = check_rendered(m)) return rc;
+    uint64_t *mk;
+    if (get_mask(m, kind, ta, ia, tb, ib, &mk)) return -1;
+    SR_HIP(hipMemcpy(out, mk, nwords * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int model_reset_pressures(Model &m) {
+    if (int rc = check_rendered(m)) return rc;
+    const size_t np = (size_t)m.nplanes;
+    for (auto *v : {&m.targets, &m.fields})
+        for (auto &o : *v) {
+            if (o.ncell) SR_HIP(hipMemsetAsync(o.P, 0, np * o.ncell * sizeof(float2), m.stream));
+            if (o.Pin) SR_HIP(hipMemsetAsync(o.Pin, 0, np * o.nact * sizeof(float2), m.stream));
+            if (o.Pout) SR_HIP(hipMemsetAsync(o.Pout, 0, np * o.nact * sizeof(float2), m.stream));
+        }
+    const size_t nq = m.fldHost.size() / 3;
+    if (nq) SR_HIP(hipMemsetAsync(m.fldVal, 0, np * nq * sizeof(double2), m.stream));
+    SR_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int model_clear_vis_cache(Model &m) {
+    if (m.stream) SR_HIP(hipStreamSynchronize(m.stream));
+    for (auto &kv : m.masks) kv.second.valid = false;  // buffers are kept and rewritten
+    for (auto &o : m.targets) o.tc.valid = false;
+    return 0;
+}
+
+int model_set_transfer_mode(Model &m, int mode) {
+    if (mode != 0 && mode != 1) { set_error("transfer mode: 0 plain FP32, 1 FP16 MFMA transfer matrix"); return -2; }
+    m.transferMode = mode;
+    return 0;
+}
+
+void model_profile(Model &m, int on) {
+    model_profile_reset(m);
+    m.profiling = on != 0;
+}
+
+void model_profile_reset(Model &m) {
+    if (m.stream) (void)hipStreamSynchronize(m.stream);
+    for (auto &r : m.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    m.recs.clear();
+}
+
+int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes) {
+    if (m.stream) SR_HIP(hipStreamSynchronize(m.stream));
+    double t = 0, p = 0, f = 0, b = 0;
+    long n = 0;
+    for (auto &r : m.recs) {
+        if (r.kind != kind) continue;
+        float e = 0.f;
+        SR_HIP(hipEventElapsedTime(&e, r.a, r.b));
+        t += e; p += r.pairs; f += r.flops; b += r.bytes; n++;
+    }
+    *ms = t; *launches = n; *pairs = p; *flops = f;
+    if (bytes) *bytes = b;
+    return 0;
+}
+
+void model_teardown(Model &m) {
+    model_profile_reset(m);
+    if (m.stream) (void)hipStreamSynchronize(m.stream);
+    for (auto &o : m.targets) free_object(o);
+    for (auto &o : m.fields) free_object(o);
+    m.targets.clear();
+    m.fields.clear();
+    m.numSourceObjects = 0;
+    for (auto &kv : m.masks) (void)hipFree(kv.second.p);
+    m.masks.clear();
+    m.transferMode = 0;
+    dev_free(m.srcP); dev_free(m.srcAmp); dev_free(m.fldP); dev_free(m.fldVal);
+    dev_free(m.trisSorted); dev_free(m.nodes); dev_free(m.errFlag);
+    dev_free(m.scratchL); dev_free(m.scratchU);
+    m.scratchElems = 0;
+    m.srcHost.clear();
+    m.fldHost.clear();
+    m.ntri = 0;
+    m.rendered = false;
+    m.batchedSources = false;
+    m.nplanes = 1;
+}
+
+}  // namespace sr

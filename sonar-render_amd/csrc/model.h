@@ -1,0 +1,92 @@
+// Scene model and projection sequencing on one GPU.
+//
+// Mirrors the reference's Model/ModelCuda/ObjectCuda/FacetCuda state machine
+// (src/Model.cpp, src/ModelCuda.cpp:242-356, src/FacetCuda.cpp:8-137) as plain
+// data: per object, one SoA fragment table in HBM instead of per-facet device
+// allocations, and one visibility bit-mask per (emitter set, receiver set).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fragmenter.h"
+#include "sr_types.h"
+#include "sums.h"
+
+namespace sr {
+
+struct Object {
+    int type = OBJ_TARGET;
+    float delta = 0.f;
+    std::vector<FacetFrame> frames;
+    std::vector<FacetGrid> grids;
+    std::vector<long> cellOff;
+    long ncell = 0;
+    int nact = 0;
+    std::vector<int> actCellHost;
+    // device tables
+    float4 *cellP = nullptr;   // point->fragment receiver point (+delta/2), area
+    float4 *cellN = nullptr;   // facet normal, facet id bits
+    float4 *actP = nullptr;    // active cell point (emitter form), area
+    float4 *actN = nullptr;    // facet normal, facet id bits
+    int *actCell = nullptr;    // active -> cell
+    double sameFacetPairs = 0; // sum over facets of (active cells)^2
+    std::vector<int> actFacetHost;
+    // mixed-precision transfer matrix (built once per geometry and frequency)
+    struct Transfer {
+        uint4 *A = nullptr, *B = nullptr;
+        int nKs = 0, nRt = 0;
+        int *facet = nullptr, *rtFmin = nullptr, *rtFmax = nullptr, *ksFmin = nullptr, *ksFmax = nullptr;
+        float *colScale = nullptr, *colInv = nullptr;
+        float sS = 1.f;
+        bool valid = false;
+    } tc;
+    // pressure planes (nplanes each)
+    float2 *P = nullptr;       // per cell
+    float2 *Pin = nullptr;     // per active cell (targets)
+    float2 *Pout = nullptr;    // per active cell (targets)
+};
+
+struct Model {
+    // initial conditions (Model.cpp:44-54)
+    float cp = 1480.f, freq = 0.f, atten = 0.f, density = 0.f;
+    float kf = 0.f;
+    std::vector<Object> targets, fields;
+    int numSourceObjects = 0;
+    std::vector<float> srcHost, fldHost;  // xyz triples
+    bool rendered = false;
+    bool batchedSources = false;
+    int nplanes = 1;
+
+    // device
+    hipStream_t stream = nullptr;
+    float4 *srcP = nullptr;
+    float2 *srcAmp = nullptr;
+    float4 *fldP = nullptr;
+    double2 *fldVal = nullptr;  // nplanes x Q
+    float4 *trisSorted = nullptr;
+    float4 *nodes = nullptr;
+    int ntri = 0;
+    VisConst vc{};
+    int *errFlag = nullptr;
+    struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; };
+    std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
+    int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
+    float2 *scratchL = nullptr, *scratchU = nullptr;
+    size_t scratchElems = 0;
+    // per-launch profiling (HIP events on the library's stream)
+    bool profiling = false;
+    struct Rec { int kind; hipEvent_t a, b; double pairs, flops, bytes; };
+    std::vector<Rec> recs;
+};
+
+// Kernel families reported by the profiler.
+enum ProfKind {
+    PK_P2F = 0, PK_VIS = 1, PK_F2F = 2, PK_F2F_FIELD = 3, PK_F2P = 4, PK_REDUCE = 5,
+    PK_BUILD_S = 6, PK_GEMM = 7, PK_BUILD_B = 8, PK_COUNT = 9
+};
+
+}  // namespace sr

@@ -1,0 +1,37 @@
+// Operations on the scene model; each returns 0 on success or a negative
+// status and records a message retrievable with last_error().
+#pragma once
+#include <cstdint>
+#include <string>
+
+#include "model.h"
+
+namespace sr {
+
+std::string last_error();
+void clear_error();
+
+int model_set_initial_conditions(Model &m, float cp, float f, float att, float density);
+int model_load_geometry(Model &m, const float *v, int nf, int type, float res);
+int model_load_points(Model &m, const float *p, int n, bool source);
+int model_set_batched_sources(Model &m, int on);
+int model_render(Model &m);
+int model_set_source_amplitude(Model &m, int i, float re, float im);
+int model_project_src(Model &m, bool fieldObjects);
+int model_project_t2t(Model &m, int nref);
+int model_project_t2field_objects(Model &m);
+int model_project_t2field_points(Model &m);
+int model_project_resp(Model &m, int ia, int ib);
+int model_prepare_resp(Model &m, int it);
+int model_get_field_points(Model &m, double *out, long n, int plane);
+int model_get_object_pressure(Model &m, int type, int idx, int which, int plane, double *out, long n);
+int model_get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t *out, long nwords);
+int model_reset_pressures(Model &m);
+int model_clear_vis_cache(Model &m);
+int model_set_transfer_mode(Model &m, int mode);
+void model_profile(Model &m, int on);
+void model_profile_reset(Model &m);
+int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
+void model_teardown(Model &m);
+
+}  // namespace sr

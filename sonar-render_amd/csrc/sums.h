@@ -1,0 +1,101 @@
+// Launch descriptors for the discrete Kirchhoff gathers (plain FP32 form).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sr {
+
+// Physical constants of one projection, all FP32 (k_r is rounded to FP32 by
+// the reference, src/Model.cpp:50-52).
+struct Phys {
+    float kr;         // Re k
+    float krevHi;     // kr / (2 pi), split hi + lo so r * kr/(2 pi) keeps ~40 bits
+    float krevLo;
+    float alpha;      // Im k (the attenuation argument, used as given)
+    float alphaLog2e; // alpha * log2(e)
+    float deltaReg;   // fragment->fragment: r < deltaReg uses ik/(4 pi) (OptiX.cu:340-346)
+    float damp;       // 0.9 for fragment->fragment (OptiX.cu:352), 1 otherwise
+};
+
+enum GatherKind {
+    KIND_F2F = 0,       // fragment->fragment, both cosines (target destination)
+    KIND_F2F_FIELD = 1, // fragment->fragment into a field object (cos_inc = 1)
+    KIND_F2P = 2        // fragment->field point (compensated FP32 sums)
+};
+
+enum GateMode {
+    GATE_NONE = 0,
+    GATE_UPPER = 1, // SELF: terms from higher facets need the receiver's own P_in != 0 (OptiX.cu:241-245)
+    GATE_ALL = 2    // RESP reciprocal: every term needs the receiver's own P_in != 0
+};
+
+struct GatherTask {
+    // emitters (columns)
+    const float4 *ep;       // position xyz, area w
+    const float4 *en;       // normal xyz, facet id bits w
+    const int *ecell;       // emitter -> index into eval (null: identity)
+    const float2 *eval;     // emitter pressure, plane b at eval + b * evalStride
+    long evalStride;
+    int ne;
+    // receivers (rows)
+    const float4 *rp;
+    const float4 *rn;
+    int nr;
+    const uint64_t *mask;   // [ceil(nr/64)][ne], bit = pair contributes
+    // gating on the receiver's own input pressure
+    const float2 *gval;
+    long gvalStride;
+    int gateMode;
+    // batching
+    int nrhs;               // planes handled by this launch
+    int nsplit;             // emitter ranges (split-K)
+    int emPerSplit;
+    // outputs: partial sums [split][NB][nr]
+    float2 *partL;          // lower / ungated terms (F2P: high part)
+    float2 *partU;          // gated terms (F2P: compensation)
+};
+
+struct ReduceTask {
+    const float2 *partL, *partU;
+    int nsplit, nbStride, nrhs, nr;
+    const float2 *gval;
+    long gvalStride;
+    int gateMode;
+    float2 *out;            // optional: P_out planes (compact rows)
+    long outStride;
+    int outAdd;             // 1: out += value (RESP), 0: out = value
+    float2 *pcell;          // optional: P += value, through rowCell
+    long pcellStride;
+    const int *rowCell;
+    const float *colInvScale; // optional: value *= colInvScale[plane]
+};
+
+struct P2FTask {
+    const float4 *cp;       // receiver (cell) positions
+    const float4 *cn;       // cell normals
+    int ncell;
+    int useNormal;          // 0 for field objects (cos_inc = 1)
+    const float4 *sp;       // sources
+    const float2 *samp;     // source amplitudes
+    int ns;
+    const uint64_t *mask;   // [ceil(ncell/64)][ns]
+    int batched;            // 1: source s writes plane s only
+    float2 *P;
+    long Pstride;
+};
+
+int gather_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st);
+int reduce_launch(const ReduceTask &t, hipStream_t st);
+int reduce_points_launch(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs, int nq,
+                         double2 *fld, long fldStride, hipStream_t st);
+int p2f_launch(const P2FTask &t, const Phys &ph, hipStream_t st);
+int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
+                 long outStride, hipStream_t st);
+int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st);
+
+// Split-K choice: enough waves to occupy every SIMD a few times.
+int choose_splits(int nrb, int ne, int minPerSplit);
+// Plane stride of the partial-sum buffers for a launch of nrhs planes.
+int gather_nb_stride(int nrhs);
+
+}  // namespace sr

@@ -1,0 +1,440 @@
+// Discrete Kirchhoff gathers, plain FP32 form, for gfx950.
+//
+// Reference sums (src/OptiX.cu):
+//   point->fragment   :35-110   P_B[j] += cos_inc * P_s * e^{ikr} / r
+//   fragment->point   :112-190  P_q    += A_i cos_s P_i (-e^{ikr}/(4 pi) (ik/r) + 1/r^2)
+//   fragment->fragment:192-406  P_out  += A_i cos_s cos_i P_in 0.9 G, G as above or ik/(4 pi) for r < delta
+// The reference runs one thread per facet and scatters with FP64 CAS atomics.
+// Here every receiver is owned by one lane (no atomics, deterministic order):
+// a wave holds 64 receivers, a workgroup 4 waves, and emitters stream through
+// LDS in tiles of 256 that every lane reads by broadcast. Which pairs
+// contribute (cosine culls and the shadow ray) is one bit per pair, computed
+// beforehand by the visibility kernel; a zero 64-bit word skips the emitter for
+// the whole wave. Long emitter lists are split across workgroups (split-K) and
+// the partial sums are reduced in a fixed order.
+//
+// Phase: sin/cos of k r are taken in revolutions, x = r * k/(2 pi), reduced to
+// [-1/2, 1/2] before v_sin_f32 / v_cos_f32. The split hi+lo product keeps the
+// fractional part accurate for r up to km.
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <algorithm>
+
+#include "sums.h"
+#include "launch.h"
+
+namespace sr {
+
+constexpr float kInv4Pi = 0.0795774715459476678f;
+constexpr int kTile = 256;
+
+__device__ inline float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline float hw_sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
+__device__ inline float hw_cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
+__device__ inline float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Fractional revolutions of r * (hi + lo), error ~1e-7 rev for r * k/(2pi) < 2^22.
+__device__ inline float phase_rev_split(float r, float hi, float lo) {
+    float p = r * hi;
+    float e = fmaf(r, hi, -p);
+    float fp = p - rintf(p);
+    return fp + fmaf(r, lo, e);
+}
+__device__ inline float phase_rev_fast(float r, float hi) {
+    float p = r * hi;
+    return p - rintf(p);
+}
+
+// TwoSum accumulation (Knuth): s + c exactly as hi + lo.
+__device__ inline void two_sum_acc(float &s, float &lo, float c) {
+    float t = s + c;
+    float bp = t - s;
+    lo += (s - (t - bp)) + (c - bp);
+    s = t;
+}
+
+template <int NB>
+struct Acc {
+    float2 v[NB];
+};
+
+// One pair's complex kernel value K (before multiplying by the emitter term).
+template <int KIND, bool ATT>
+__device__ inline void pair_kernel(const Phys &ph, float4 rp, float4 rn, float4 ep, float4 en, bool bit, float &Kre,
+                                   float &Kim) {
+    float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
+    float r2 = vx * vx + vy * vy + vz * vz;
+    float rinv = fast_rsq(r2);
+    float r = r2 * rinv;
+    float de = vx * en.x + vy * en.y + vz * en.z;
+    float w;
+    if (KIND == KIND_F2F) {
+        float dr = vx * rn.x + vy * rn.y + vz * rn.z;
+        w = -(de * dr) * (rinv * rinv);
+    } else {
+        w = de * rinv;
+    }
+    float x = (KIND == KIND_F2P) ? phase_rev_split(r, ph.krevHi, ph.krevLo) : phase_rev_fast(r, ph.krevHi);
+    float c = hw_cos_rev(x), s = hw_sin_rev(x);
+    float a = rinv * kInv4Pi;
+    if (ATT) a *= hw_exp2(-ph.alphaLog2e * r);
+    float Gre, Gim;
+    if (ATT) {
+        Gre = a * fmaf(ph.alpha, c, ph.kr * s) + rinv * rinv;
+        Gim = -a * fmaf(ph.kr, c, -ph.alpha * s);
+    } else {
+        Gre = (a * ph.kr) * s + rinv * rinv;
+        Gim = -(a * ph.kr) * c;
+    }
+    if (KIND != KIND_F2P) {
+        bool reg = r < ph.deltaReg;
+        Gre = reg ? -ph.alpha * kInv4Pi : Gre;
+        Gim = reg ? ph.kr * kInv4Pi : Gim;
+        w *= ph.damp;
+    }
+    Kre = bit ? w * Gre : 0.0f;
+    Kim = bit ? w * Gim : 0.0f;
+}
+
+template <int NB>
+__device__ inline void cmac(float2 &acc, float Kre, float Kim, float2 X) {
+    acc.x = fmaf(Kre, X.x, fmaf(-Kim, X.y, acc.x));
+    acc.y = fmaf(Kre, X.y, fmaf(Kim, X.x, acc.y));
+}
+
+template <int KIND, int NB, bool ATT, bool SPLITG>
+__device__ inline void tile_loop(const Phys &ph, int n, const float4 *s_p, const float4 *s_n, const float2 (*s_x)[kTile],
+                                 const uint64_t *s_m, int lane, float4 rp, float4 rn, int rfacet, float2 *accL,
+                                 float2 *accU, float2 *compL) {
+    for (int k = 0; k < n; k++) {
+        uint64_t word = s_m[k];
+        if (word == 0) continue;  // wave-uniform
+        bool bit = (word >> lane) & 1ull;
+        float4 ep = s_p[k], en = s_n[k];
+        float Kre, Kim;
+        pair_kernel<KIND, ATT>(ph, rp, rn, ep, en, bit, Kre, Kim);
+        if (KIND == KIND_F2P) {
+            // compensated: K * (A P) for one plane per lane register pair
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                float2 X = s_x[b][k];
+                float cre = Kre * X.x - Kim * X.y;
+                float cim = Kre * X.y + Kim * X.x;
+                two_sum_acc(accL[b].x, compL[b].x, cre);
+                two_sum_acc(accL[b].y, compL[b].y, cim);
+            }
+        } else if (SPLITG) {
+            bool up = __float_as_int(en.w) > rfacet;
+            float KUre = up ? Kre : 0.0f, KUim = up ? Kim : 0.0f;
+            float KLre = up ? 0.0f : Kre, KLim = up ? 0.0f : Kim;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                float2 X = s_x[b][k];
+                cmac<NB>(accL[b], KLre, KLim, X);
+                cmac<NB>(accU[b], KUre, KUim, X);
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++) cmac<NB>(accL[b], Kre, Kim, s_x[b][k]);
+        }
+    }
+}
+
+template <int KIND, int NB, bool ATT>
+__global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
+    __shared__ float4 s_p[kTile];
+    __shared__ float4 s_n[kTile];
+    __shared__ float2 s_x[NB][kTile];
+    __shared__ uint64_t s_m[4][kTile];
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nrb = (t.nr + 63) >> 6;
+    const int rb = blockIdx.x * 4 + wave;
+    const int split = blockIdx.y;
+    const int e0 = split * t.emPerSplit;
+    const int e1 = min(t.ne, e0 + t.emPerSplit);
+    const int r = rb * 64 + lane;
+    const bool rvalid = (rb < nrb) && (r < t.nr);
+
+    float4 rp = rvalid ? t.rp[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 rn = (rvalid && t.rn) ? t.rn[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int rfacet = __float_as_int(rn.w);
+
+    // Does this wave hold a receiver whose own input pressure is exactly zero?
+    bool needSplit = false;
+    if (KIND == KIND_F2F && t.gateMode == GATE_UPPER) {
+        bool z = false;
+        if (rvalid)
+            for (int b = 0; b < t.nrhs; b++) {
+                float2 g = t.gval[b * t.gvalStride + r];
+                z |= (g.x == 0.0f && g.y == 0.0f);
+            }
+        needSplit = __ballot(z) != 0;
+    }
+
+    float2 accL[NB], accU[NB], compL[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        accL[b] = make_float2(0.f, 0.f);
+        accU[b] = make_float2(0.f, 0.f);
+        compL[b] = make_float2(0.f, 0.f);
+    }
+
+    for (int base = e0; base < e1; base += kTile) {
+        const int n = min(kTile, e1 - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += 256) {
+            const int e = base + k;
+            float4 p = t.ep[e];
+            s_p[k] = p;
+            s_n[k] = t.en ? t.en[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const int ci = t.ecell ? t.ecell[e] : e;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                float2 v = (b < t.nrhs) ? t.eval[b * t.evalStride + ci] : make_float2(0.f, 0.f);
+                s_x[b][k] = make_float2(p.w * v.x, p.w * v.y);
+            }
+        }
+        for (int w = 0; w < 4; w++) {
+            const int rbw = blockIdx.x * 4 + w;
+            for (int k = threadIdx.x; k < n; k += 256)
+                s_m[w][k] = (rbw < nrb) ? t.mask[(size_t)rbw * t.ne + base + k] : 0ull;
+        }
+        __syncthreads();
+        if (rb < nrb) {
+            if (needSplit)
+                tile_loop<KIND, NB, ATT, true>(ph, n, s_p, s_n, s_x, s_m[wave], lane, rp, rn, rfacet, accL, accU,
+                                               compL);
+            else
+                tile_loop<KIND, NB, ATT, false>(ph, n, s_p, s_n, s_x, s_m[wave], lane, rp, rn, rfacet, accL, accU,
+                                                compL);
+        }
+    }
+
+    if (!rvalid) return;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        if (b >= t.nrhs) break;
+        const size_t o = ((size_t)split * NB + b) * t.nr + r;
+        if (KIND == KIND_F2P) {
+            t.partL[o] = accL[b];
+            t.partU[o] = compL[b];
+        } else {
+            t.partL[o] = accL[b];
+            if (t.partU) t.partU[o] = accU[b];
+        }
+    }
+}
+
+__global__ void k_reduce(ReduceTask t) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)t.nrhs * t.nr) return;
+    const int b = (int)(i / t.nr), r = (int)(i % t.nr);
+    float2 L = make_float2(0.f, 0.f), U = make_float2(0.f, 0.f);
+    for (int s = 0; s < t.nsplit; s++) {
+        const size_t o = ((size_t)s * t.nbStride + b) * t.nr + r;
+        float2 a = t.partL[o];
+        L.x += a.x; L.y += a.y;
+        if (t.partU) { float2 u = t.partU[o]; U.x += u.x; U.y += u.y; }
+    }
+    float g = 1.0f;
+    if (t.gateMode != GATE_NONE) {
+        float2 gv = t.gval[b * t.gvalStride + r];
+        g = (gv.x == 0.0f && gv.y == 0.0f) ? 0.0f : 1.0f;
+    }
+    float2 v;
+    if (t.gateMode == GATE_ALL) v = make_float2(g * L.x, g * L.y);
+    else v = make_float2(L.x + g * U.x, L.y + g * U.y);
+    if (t.colInvScale) {
+        const float sc = t.colInvScale[b];
+        v.x *= sc;
+        v.y *= sc;
+    }
+    if (t.out) {
+        float2 &o = t.out[b * t.outStride + r];
+        if (t.outAdd) { o.x += v.x; o.y += v.y; } else o = v;
+    }
+    if (t.pcell) {
+        float2 &P = t.pcell[b * t.pcellStride + t.rowCell[r]];
+        P.x += v.x; P.y += v.y;
+    }
+}
+
+__global__ void k_reduce_points(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs, int nq,
+                                double2 *fld, long fldStride) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nrhs * nq) return;
+    const int b = (int)(i / nq), q = (int)(i % nq);
+    double sr = 0.0, si = 0.0;
+    for (int s = 0; s < nsplit; s++) {
+        const size_t o = ((size_t)s * nbStride + b) * nq + q;
+        sr += (double)hi[o].x + (double)lo[o].x;
+        si += (double)hi[o].y + (double)lo[o].y;
+    }
+    double2 &f = fld[b * fldStride + q];
+    f.x += sr;
+    f.y += si;
+}
+
+// point->fragment: lanes own cells; sources are few.
+template <bool ATT>
+__global__ __launch_bounds__(256) void k_p2f(P2FTask t, Phys ph) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int cb = c >> 6;
+    if (cb * 64 >= t.ncell) return;
+    const bool valid = c < t.ncell;
+    float4 cp = valid ? t.cp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 cn = valid ? t.cn[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 sum = make_float2(0.f, 0.f);
+    for (int s = 0; s < t.ns; s++) {
+        uint64_t word = t.mask[(size_t)cb * t.ns + s];
+        bool bit = (word >> lane) & 1ull;
+        float2 contrib = make_float2(0.f, 0.f);
+        if (bit) {
+            float4 sp = t.sp[s];
+            // r and cos_inc exactly as the reference forms them (no contraction).
+            float vx = __fsub_rn(cp.x, sp.x), vy = __fsub_rn(cp.y, sp.y), vz = __fsub_rn(cp.z, sp.z);
+            float r = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(vx, vx), __fmul_rn(vy, vy)), __fmul_rn(vz, vz)));
+            float inv = __fdiv_rn(1.0f, r);
+            float ci = 1.0f;
+            if (t.useNormal) {
+                float ux = -__fmul_rn(vx, inv), uy = -__fmul_rn(vy, inv), uz = -__fmul_rn(vz, inv);
+                ci = __fadd_rn(__fadd_rn(__fmul_rn(ux, cn.x), __fmul_rn(uy, cn.y)), __fmul_rn(uz, cn.z));
+            }
+            float x = phase_rev_split(r, ph.krevHi, ph.krevLo);
+            float cc = hw_cos_rev(x), ss = hw_sin_rev(x);
+            float m = ci * inv;
+            if (ATT) m *= hw_exp2(-ph.alphaLog2e * r);
+            float2 A = t.samp[s];
+            // m * (cc + i ss) * A
+            contrib.x = m * (cc * A.x - ss * A.y);
+            contrib.y = m * (cc * A.y + ss * A.x);
+        }
+        if (t.batched) {
+            if (valid) {
+                float2 &P = t.P[s * t.Pstride + c];
+                P.x += contrib.x; P.y += contrib.y;
+            }
+        } else {
+            sum.x += contrib.x; sum.y += contrib.y;
+        }
+    }
+    if (!t.batched && valid) {
+        float2 &P = t.P[c];
+        P.x += sum.x; P.y += sum.y;
+    }
+}
+
+__global__ void k_prime(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
+                        long outStride) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)nact * nrhs) return;
+    const int b = (int)(i / nact), k = (int)(i % nact);
+    Pout[b * outStride + k] = P[b * pStride + actCell[k]];
+}
+
+__global__ void k_any_zero(const float2 *v, long n, int *flag) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 x = v[i];
+    if (x.x == 0.0f && x.y == 0.0f) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+
+int choose_splits(int nrb, int ne, int minPerSplit) {
+    const int targetWaves = 8192;  // 256 CUs x 4 SIMDs x 8 waves
+    int s = std::max(1, targetWaves / std::max(1, nrb));
+    s = std::min(s, std::max(1, ne / std::max(1, minPerSplit)));
+    return std::max(1, std::min(s, 65535));
+}
+
+template <int KIND, int NB>
+static int gather_dispatch_att(const GatherTask &t, const Phys &ph, hipStream_t st) {
+    const int nrb = (t.nr + 63) / 64;
+    dim3 grid((nrb + 3) / 4, t.nsplit);
+    if (ph.alpha != 0.0f) k_gather<KIND, NB, true><<<grid, 256, 0, st>>>(t, ph);
+    else k_gather<KIND, NB, false><<<grid, 256, 0, st>>>(t, ph);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int KIND>
+static int gather_dispatch_nb(const GatherTask &t, const Phys &ph, int nb, hipStream_t st) {
+    switch (nb) {
+    case 1: return gather_dispatch_att<KIND, 1>(t, ph, st);
+    case 2: return gather_dispatch_att<KIND, 2>(t, ph, st);
+    case 4: return gather_dispatch_att<KIND, 4>(t, ph, st);
+    case 8: return gather_dispatch_att<KIND, 8>(t, ph, st);
+    default: return gather_dispatch_att<KIND, 16>(t, ph, st);
+    }
+}
+
+static int nb_bucket(int nrhs) {
+    if (nrhs <= 1) return 1;
+    if (nrhs <= 2) return 2;
+    if (nrhs <= 4) return 4;
+    if (nrhs <= 8) return 8;
+    return 16;
+}
+
+int gather_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st) {
+    if (t.nr <= 0 || t.ne <= 0) return 0;
+    if (t.nrhs < 1 || t.nrhs > 16) { set_error("gather: 1..16 planes per launch"); return -1; }
+    const int nb = nb_bucket(t.nrhs);
+    switch (kind) {
+    case KIND_F2F: return gather_dispatch_nb<KIND_F2F>(t, ph, nb, st);
+    case KIND_F2F_FIELD: return gather_dispatch_nb<KIND_F2F_FIELD>(t, ph, nb, st);
+    case KIND_F2P: return gather_dispatch_nb<KIND_F2P>(t, ph, nb, st);
+    default: set_error("gather: unknown kind"); return -1;
+    }
+}
+
+int gather_nb_stride(int nrhs) { return nb_bucket(nrhs); }
+
+int reduce_launch(const ReduceTask &t, hipStream_t st) {
+    const long n = (long)t.nrhs * t.nr;
+    if (n <= 0) return 0;
+    k_reduce<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(t);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int reduce_points_launch(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs, int nq,
+                         double2 *fld, long fldStride, hipStream_t st) {
+    const long n = (long)nrhs * nq;
+    if (n <= 0) return 0;
+    k_reduce_points<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(hi, lo, nsplit, nbStride, nrhs, nq, fld, fldStride);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int p2f_launch(const P2FTask &t, const Phys &ph, hipStream_t st) {
+    if (t.ncell <= 0 || t.ns <= 0) return 0;
+    const unsigned g = (unsigned)((t.ncell + 255) / 256);
+    if (ph.alpha != 0.0f) k_p2f<true><<<g, 256, 0, st>>>(t, ph);
+    else k_p2f<false><<<g, 256, 0, st>>>(t, ph);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout, long outStride,
+                 hipStream_t st) {
+    const long n = (long)nact * nrhs;
+    if (n <= 0) return 0;
+    k_prime<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(P, pStride, actCell, nact, nrhs, Pout, outStride);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st) {
+    if (n <= 0) return 0;
+    k_any_zero<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(v, n, flag);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace sr

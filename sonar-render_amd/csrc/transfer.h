@@ -1,0 +1,40 @@
+// Launch descriptors of the mixed-precision (FP16 MFMA) transfer form.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace sr {
+
+struct TransferGeom {
+    const float4 *p;    // active cells: point, area
+    const float4 *nrm;  // normal, facet id bits
+    int n;              // active cells
+    int nKs;            // K steps of 8 emitters (ceil(n/8))
+    int nRt;            // row tiles of 32 (padded to a multiple of 4)
+};
+
+struct TransferPhys {
+    float kr, alpha, alphaLog2e, deltaReg, damp;
+    float scale;        // FP16 scale of S
+    double krevD;       // kr / (2 pi) in FP64
+};
+
+struct TransferGemm {
+    const uint4 *A, *B;
+    int n, nKs, nRt;
+    int nplanes;        // planes of this group (<= 64)
+    int nsplit, ksPerSplit;
+    const int *rowFacet, *emFacet;        // facet id per active cell
+    const int *rtFmin, *rtFmax;           // per row tile
+    const int *ksFmin, *ksFmax;           // per K step
+    float2 *partL, *partU;                // [split][64][n]
+};
+
+int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
+int transfer_col_scale(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, float *scale,
+                       float *invScale, float sS, hipStream_t st);
+int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes, const float *scale,
+                     uint4 *B, hipStream_t st);
+int transfer_gemm(const TransferGemm &t, bool gated, hipStream_t st);
+
+}  // namespace sr

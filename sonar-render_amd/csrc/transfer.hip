@@ -1,0 +1,300 @@
+// Mixed-precision transfer form of the fragment->fragment bounce for many
+// incidences sharing one geometry.
+//
+// The reference applies the same pair kernel K_ij = cos_s cos_i 0.9 G(r_ij)
+// (src/OptiX.cu:297-354) to every incidence separately. Here K is built once
+// per geometry and frequency as a complex FP16 matrix S (real and imaginary
+// parts interleaved along K, scaled by one factor so |S| stays below 2^14),
+// and every bounce is one real GEMM on the matrix cores:
+//     C[i, 0..63]   = sum_j Re S_ij X_re[j,b] - Im S_ij X_im[j,b]
+//     C[i, 64..127] = sum_j Re S_ij X_im[j,b] + Im S_ij X_re[j,b]
+// with A = S (M x 2N) and B = [[X_re, X_im], [-X_im, X_re]] (2N x 128),
+// X = A_j P_in[j,b] scaled per incidence, FP32 accumulation
+// (v_mfma_f32_32x32x16_f16). The reference's zero-input gate
+// (OptiX.cu:241-245: terms from higher facets need P_in[i] != 0) is kept exact
+// by accumulating lower-facet and higher-facet emitters separately.
+//
+// Layouts (both written by this file, read in fragment order so every load is
+// one coalesced 1 KiB per wave):
+//   A: [row tile of 32][K step of 8 emitters][lane][8 halves]
+//      lane l: row = tile*32 + l%32, emitters step*8 + (l/32)*4 + q, halves (Re, Im)
+//   B: [K step][column tile of 32][lane][8 halves], same K assignment, column l%32.
+// The phase k r is reduced modulo one revolution in FP64 before the FP32
+// sin/cos (the single FP64 step of the form).
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <algorithm>
+
+#include "launch.h"
+#include "sums.h"
+#include "transfer.h"
+
+namespace sr {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr float kInv4PiT = 0.0795774715459476678f;
+
+__device__ inline uint32_t pack_h2(float a, float b) {
+    _Float16 ha = (_Float16)a, hb = (_Float16)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+}
+
+// ---------------------------------------------------------------------------
+// S build: one lane = one row, 4 emitters of one K step.
+template <bool ATT>
+__global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
+                                                 uint4 *__restrict__ A, int ksPerWave) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int nKsChunks = (g.nKs + ksPerWave - 1) / ksPerWave;
+    const int rt = wave / nKsChunks;
+    const int kc = wave % nKsChunks;
+    if (rt >= g.nRt) return;
+    const int row = rt * 32 + (lane & 31);
+    const int h = lane >> 5;
+    const bool rvalid = row < g.n;
+    const float4 rp = rvalid ? g.p[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 rn = rvalid ? g.nrm[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint64_t *mrow = mask + (size_t)(rvalid ? (row >> 6) : 0) * g.n;
+    const int bit = row & 63;
+    const int ks0 = kc * ksPerWave, ks1 = min(g.nKs, ks0 + ksPerWave);
+    for (int ks = ks0; ks < ks1; ks++) {
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = ks * 8 + h * 4 + q;
+            float re = 0.f, im = 0.f;
+            if (rvalid && j < g.n && ((mrow[j] >> bit) & 1ull)) {
+                const float4 ep = g.p[j], en = g.nrm[j];
+                float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
+                float r2 = vx * vx + vy * vy + vz * vz;
+                float rinv = __builtin_amdgcn_rsqf(r2);
+                float r = r2 * rinv;
+                float de = vx * en.x + vy * en.y + vz * en.z;
+                float dr = vx * rn.x + vy * rn.y + vz * rn.z;
+                float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
+                if (r < ph.deltaReg) {
+                    re = wgt * (-ph.alpha * kInv4PiT);
+                    im = wgt * (ph.kr * kInv4PiT);
+                } else {
+                    // FP64 reduction of k r to one revolution, then FP32 sin/cos.
+                    double rev = (double)r * ph.krevD;
+                    float x = (float)(rev - rint(rev));
+                    float c = __builtin_amdgcn_cosf(x), s = __builtin_amdgcn_sinf(x);
+                    float a = rinv * kInv4PiT;
+                    if (ATT) a *= __builtin_amdgcn_exp2f(-ph.alphaLog2e * r);
+                    float Gre, Gim;
+                    if (ATT) {
+                        Gre = a * fmaf(ph.alpha, c, ph.kr * s) + rinv * rinv;
+                        Gim = -a * fmaf(ph.kr, c, -ph.alpha * s);
+                    } else {
+                        Gre = (a * ph.kr) * s + rinv * rinv;
+                        Gim = -(a * ph.kr) * c;
+                    }
+                    re = wgt * Gre;
+                    im = wgt * Gim;
+                }
+            }
+            w[q] = pack_h2(re, im);
+        }
+        A[((size_t)rt * g.nKs + ks) * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// Per-incidence scale: 2^13 / max(|Re X|, |Im X|) with X = A_j P_in[j,b].
+__global__ __launch_bounds__(256) void k_col_scale(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
+                                                   long stride, int n, float *__restrict__ scale,
+                                                   float *__restrict__ invScale, float sS) {
+    __shared__ float red[256];
+    const int b = blockIdx.x;
+    float m = 0.f;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        float2 v = Pin[b * stride + j];
+        float a = p[j].w;
+        m = fmaxf(m, fmaxf(fabsf(a * v.x), fabsf(a * v.y)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float mx = red[0];
+        float sc = mx > 0.f ? 8192.0f / mx : 1.0f;
+        scale[b] = sc;
+        invScale[b] = 1.0f / (sc * sS);
+    }
+}
+
+// B build for one group of up to 64 incidences.
+__global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
+                                                 long stride, int n, int nKs, int nplanes,
+                                                 const float *__restrict__ scale, uint4 *__restrict__ B) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nKs * 4 * 64) return;
+    const int lane = (int)(t & 63);
+    const int ct = (int)((t >> 6) & 3);
+    const int ks = (int)(t >> 8);
+    const int c = ct * 32 + (lane & 31);
+    const int h = lane >> 5;
+    const int b = c & 63;
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int j = ks * 8 + h * 4 + q;
+        float xr = 0.f, xi = 0.f;
+        if (j < n && b < nplanes) {
+            float2 v = Pin[b * stride + j];
+            float a = p[j].w * scale[b];
+            xr = a * v.x;
+            xi = a * v.y;
+        }
+        w[q] = (c < 64) ? pack_h2(xr, -xi) : pack_h2(xi, xr);
+    }
+    B[t] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: a workgroup = 4 waves = 128 rows x 128 columns; a wave = 32 rows.
+// B chunks of 8 K steps (32 KiB) are staged in LDS and shared by the 4 waves;
+// A fragments stream from HBM straight into registers.
+constexpr int kKC = 8;
+
+template <bool GATED>
+__global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
+    __shared__ uint4 sB[2][kKC * 4 * 64];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int rt = blockIdx.x * 4 + wave;
+    const int split = blockIdx.y;
+    const int ks0 = split * t.ksPerSplit;
+    const int ks1 = min(t.nKs, ks0 + t.ksPerSplit);
+    const bool rtValid = rt < t.nRt;
+    const int fr = (rtValid && rt * 32 + (lane & 31) < t.n) ? t.rowFacet[rt * 32 + (lane & 31)] : -1;
+    const int rtMin = rtValid ? t.rtFmin[rt] : 0, rtMax = rtValid ? t.rtFmax[rt] : 0;
+    const uint4 *Arow = t.A + (size_t)(rtValid ? rt : 0) * t.nKs * 64 + lane;
+
+    f32x16 accL[4], accU[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        accL[c] = f32x16{};
+        accU[c] = f32x16{};
+    }
+
+    int buf = 0;
+    auto stage = [&](int kb, int bb) {
+        const int n = min(kKC, ks1 - kb);
+        for (int idx = threadIdx.x; idx < n * 256; idx += 256) sB[bb][idx] = t.B[(size_t)kb * 256 + idx];
+    };
+    if (ks0 < ks1) stage(ks0, 0);
+    for (int kb = ks0; kb < ks1; kb += kKC) {
+        __syncthreads();
+        if (kb + kKC < ks1) stage(kb + kKC, buf ^ 1);
+        const int n = min(kKC, ks1 - kb);
+        uint4 a[kKC];
+#pragma unroll
+        for (int s = 0; s < kKC; s++)
+            if (s < n) a[s] = Arow[(size_t)(kb + s) * 64];
+#pragma unroll
+        for (int s = 0; s < kKC; s++) {
+            if (s >= n || !rtValid) continue;
+            const int ks = kb + s;
+            half8 av = __builtin_bit_cast(half8, a[s]);
+            int cls = 0;  // 0: lower facets, 1: higher facets, 2: mixed
+            if (GATED) {
+                const int kmin = t.ksFmin[ks], kmax = t.ksFmax[ks];
+                cls = (kmax <= rtMin) ? 0 : ((kmin >= rtMax) ? 1 : 2);
+            }
+            if (cls == 2) {
+                // split this step's emitters element by element
+                half8 aL = av, aU = av;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int j = ks * 8 + (lane >> 5) * 4 + q;
+                    const bool up = (j < t.n) && t.emFacet[j] > fr;
+                    if (up) { aL[2 * q] = 0; aL[2 * q + 1] = 0; }
+                    else { aU[2 * q] = 0; aU[2 * q + 1] = 0; }
+                }
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    half8 bv = __builtin_bit_cast(half8, sB[buf][(s * 4 + c) * 64 + lane]);
+                    accL[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aL, bv, accL[c], 0, 0, 0);
+                    accU[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aU, bv, accU[c], 0, 0, 0);
+                }
+            } else if (cls == 1) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    half8 bv = __builtin_bit_cast(half8, sB[buf][(s * 4 + c) * 64 + lane]);
+                    accU[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, accU[c], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    half8 bv = __builtin_bit_cast(half8, sB[buf][(s * 4 + c) * 64 + lane]);
+                    accL[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, accL[c], 0, 0, 0);
+                }
+            }
+        }
+        buf ^= 1;
+    }
+    if (!rtValid) return;
+    // C layout: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, column l%32).
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const int b = c * 32 + (lane & 31);
+        if (b >= t.nplanes) continue;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int row = rt * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+            if (row >= t.n) continue;
+            const size_t o = ((size_t)split * 64 + b) * t.n + row;
+            t.partL[o] = make_float2(accL[c][i], accL[c + 2][i]);
+            if (GATED) t.partU[o] = make_float2(accU[c][i], accU[c + 2][i]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st) {
+    const int ksPerWave = 16;
+    const long waves = (long)g.nRt * ((g.nKs + ksPerWave - 1) / ksPerWave);
+    const long blocks = (waves + 3) / 4;
+    if (blocks <= 0) return 0;
+    if (ph.alpha != 0.0f) k_build_s<true><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    else k_build_s<false><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int transfer_col_scale(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, float *scale,
+                       float *invScale, float sS, hipStream_t st) {
+    if (nplanes <= 0) return 0;
+    k_col_scale<<<nplanes, 256, 0, st>>>(p, Pin, stride, n, scale, invScale, sS);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes, const float *scale,
+                     uint4 *B, hipStream_t st) {
+    const long tot = (long)nKs * 256;
+    if (tot <= 0) return 0;
+    k_build_b<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, scale, B);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int transfer_gemm(const TransferGemm &t, bool gated, hipStream_t st) {
+    if (t.n <= 0) return 0;
+    dim3 grid((t.nRt + 3) / 4, t.nsplit);
+    if (gated) k_transfer_gemm<true><<<grid, 256, 0, st>>>(t);
+    else k_transfer_gemm<false><<<grid, 256, 0, st>>>(t);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace sr

@@ -1,0 +1,371 @@
+// Shadow-ray visibility on gfx950: LBVH build and wavefront-cooperative
+// any-hit traversal that writes one 64-bit visibility word per
+// (receiver block of 64, emitter).
+//
+// Replaces the reference's OptiX GAS build (src/OptiX.cpp:304-350) and its
+// optixTrace call sites (src/OptiX.cu:81-93,163-175,321-333). The cosine culls
+// of each call site (OptiX.cu:67-78,155-160,297-318) are folded into the same
+// word, so a set bit means "this pair contributes".
+//
+// Compiled with -ffp-contract=off: the ray set-up, culls and the triangle test
+// must round exactly as the host oracle does.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <cstdio>
+#include <vector>
+
+#include "sr_types.h"
+#include "vis_fp32.h"
+#include "launch.h"
+
+namespace sr {
+
+// ---------------------------------------------------------------------------
+// LBVH build (Karras 2012: Morton order, radix sort, hierarchy from key
+// prefixes, bottom-up box refit).
+
+__device__ inline unsigned expand_bits10(unsigned v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(const float4 *__restrict__ tris, int n, float3 lo, float3 invExt,
+                         unsigned *__restrict__ keys, int *__restrict__ ids) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    TriRec t = tri_unpack(tris[4 * i], tris[4 * i + 1], tris[4 * i + 2], tris[4 * i + 3]);
+    float cx = (t.bmin.x + t.bmax.x) * 0.5f, cy = (t.bmin.y + t.bmax.y) * 0.5f, cz = (t.bmin.z + t.bmax.z) * 0.5f;
+    float fx = fminf(fmaxf((cx - lo.x) * invExt.x, 0.0f), 1.0f);
+    float fy = fminf(fmaxf((cy - lo.y) * invExt.y, 0.0f), 1.0f);
+    float fz = fminf(fmaxf((cz - lo.z) * invExt.z, 0.0f), 1.0f);
+    unsigned ux = (unsigned)fminf(fx * 1024.0f, 1023.0f);
+    unsigned uy = (unsigned)fminf(fy * 1024.0f, 1023.0f);
+    unsigned uz = (unsigned)fminf(fz * 1024.0f, 1023.0f);
+    keys[i] = (expand_bits10(ux) << 2) | (expand_bits10(uy) << 1) | expand_bits10(uz);
+    ids[i] = i;
+}
+
+__global__ void k_gather_tris(const float4 *__restrict__ src, const int *__restrict__ order, int n,
+                              float4 *__restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = order[i];
+    dst[4 * i] = src[4 * s];
+    dst[4 * i + 1] = src[4 * s + 1];
+    dst[4 * i + 2] = src[4 * s + 2];
+    dst[4 * i + 3] = src[4 * s + 3];
+}
+
+// Common prefix of (key, index) pairs; -1 outside [0, n).
+__device__ inline int prefix_len(const unsigned *keys, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    unsigned a = keys[i], b = keys[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((unsigned)i ^ (unsigned)j);
+}
+
+// Internal node i -> children (Karras 2012, Alg. 4). Leaves are encoded ~leaf.
+__global__ void k_hierarchy(const unsigned *__restrict__ keys, int n, int *__restrict__ child,
+                            int *__restrict__ parentInt, int *__restrict__ parentLeaf) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (prefix_len(keys, n, i, i + 1) - prefix_len(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = prefix_len(keys, n, i, i - d);
+    int lmax = 2;
+    while (prefix_len(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (prefix_len(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = prefix_len(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (prefix_len(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + (d < 0 ? -1 : 0);
+    int lo = min(i, j), hi = max(i, j);
+    int L = (lo == gamma) ? ~gamma : gamma;
+    int R = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    child[2 * i] = L;
+    child[2 * i + 1] = R;
+    if (L < 0) parentLeaf[~L] = i; else parentInt[L] = i;
+    if (R < 0) parentLeaf[~R] = i; else parentInt[R] = i;
+}
+
+// Bottom-up refit: the second child to arrive at a node computes its box.
+__global__ void k_refit(const float4 *__restrict__ tris, int n, const int *__restrict__ child,
+                        const int *__restrict__ parentInt, const int *__restrict__ parentLeaf,
+                        float4 *boxes /* (n-1) x 2: min, max */, int *counters, int *depthErr) {
+    int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= n) return;
+    int node = parentLeaf[leaf];
+    int steps = 0;
+    while (node >= 0) {
+        __threadfence();
+        int prev = atomicAdd(&counters[node], 1);
+        if (prev == 0) return;  // the sibling finishes this node
+        __threadfence();
+        float3 mn = make_float3(1e30f, 1e30f, 1e30f), mx = make_float3(-1e30f, -1e30f, -1e30f);
+        for (int c = 0; c < 2; c++) {
+            int ch = child[2 * node + c];
+            float3 a, b;
+            if (ch < 0) {
+                TriRec t = tri_unpack(tris[4 * ~ch], tris[4 * ~ch + 1], tris[4 * ~ch + 2], tris[4 * ~ch + 3]);
+                a = make_float3(t.bmin.x, t.bmin.y, t.bmin.z);
+                b = make_float3(t.bmax.x, t.bmax.y, t.bmax.z);
+            } else {
+                float4 p = boxes[2 * ch];
+                float4 q = boxes[2 * ch + 1];
+                a = make_float3(p.x, p.y, p.z);
+                b = make_float3(q.x, q.y, q.z);
+            }
+            mn = make_float3(fminf(mn.x, a.x), fminf(mn.y, a.y), fminf(mn.z, a.z));
+            mx = make_float3(fmaxf(mx.x, b.x), fmaxf(mx.y, b.y), fmaxf(mx.z, b.z));
+        }
+        boxes[2 * node] = make_float4(mn.x, mn.y, mn.z, 0.f);
+        boxes[2 * node + 1] = make_float4(mx.x, mx.y, mx.z, 0.f);
+        node = parentInt[node];
+        if (++steps > 4 * 64 + 64) { atomicExch(depthErr, 1); return; }
+    }
+}
+
+__global__ void k_pack_nodes(const float4 *__restrict__ tris, int n, const int *__restrict__ child,
+                             const float4 *__restrict__ boxes, float4 *__restrict__ nodes) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    float4 cb[2][2];
+    for (int c = 0; c < 2; c++) {
+        int ch = child[2 * i + c];
+        if (ch < 0) {
+            TriRec t = tri_unpack(tris[4 * ~ch], tris[4 * ~ch + 1], tris[4 * ~ch + 2], tris[4 * ~ch + 3]);
+            cb[c][0] = make_float4(t.bmin.x, t.bmin.y, t.bmin.z, 0.f);
+            cb[c][1] = make_float4(t.bmax.x, t.bmax.y, t.bmax.z, 0.f);
+        } else {
+            cb[c][0] = boxes[2 * ch];
+            cb[c][1] = boxes[2 * ch + 1];
+        }
+    }
+    nodes[4 * i + 0] = make_float4(cb[0][0].x, cb[0][0].y, cb[0][0].z, __int_as_float(child[2 * i]));
+    nodes[4 * i + 1] = make_float4(cb[0][1].x, cb[0][1].y, cb[0][1].z, __int_as_float(child[2 * i + 1]));
+    nodes[4 * i + 2] = make_float4(cb[1][0].x, cb[1][0].y, cb[1][0].z, 0.f);
+    nodes[4 * i + 3] = make_float4(cb[1][1].x, cb[1][1].y, cb[1][1].z, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-cooperative any-hit.
+//
+// The 64 lanes of a wave carry 64 rays with (usually) one shared end point.
+// The wave walks the tree together: node and triangle records are read with
+// wave-uniform addresses (scalar loads), each lane tests its own ray, and a
+// child is entered when any still-unoccluded lane's ray touches it. The stack
+// is wave-uniform and lives in LDS. A lane retires on its first hit; the walk
+// ends when every lane has retired or the stack is empty.
+
+constexpr int kStack = 128;
+
+__device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
+
+__device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Returns this lane's "occluded" flag. active: lane has a ray to trace.
+__device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk, bool active, F3 o, F3 d,
+                                   float tmin, float tmax, int *errFlag) {
+    if (bvh.ntri <= 0) return false;
+    bool hit = false;
+    unsigned long long live = ballot64(active);
+    if (live == 0) return false;
+    F3 inv{safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    if (bvh.ntri == 1) {
+        TriRec t = tri_unpack(bvh.tris[0], bvh.tris[1], bvh.tris[2], bvh.tris[3]);
+        if (active) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
+        return hit;
+    }
+    const int lane = __lane_id();
+    int sp = 0;
+    int node = 0;
+    for (;;) {
+        // node is an internal node index (wave-uniform)
+        const float4 *np = bvh.nodes + 4 * node;
+        float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        int cl = __float_as_int(n0.w), cr = __float_as_int(n1.w);
+        bool want = active && !hit;
+        bool hl = want && slab_hit(F3{n0.x, n0.y, n0.z}, F3{n1.x, n1.y, n1.z}, vc.padNode, o, inv, tmin, tmax);
+        bool hr = want && slab_hit(F3{n2.x, n2.y, n2.z}, F3{n3.x, n3.y, n3.z}, vc.padNode, o, inv, tmin, tmax);
+        unsigned long long bl = ballot64(hl), br = ballot64(hr);
+        // leaves are resolved immediately
+        if (cl < 0 && bl) {
+            int ti = ~cl;
+            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], bvh.tris[4 * ti + 3]);
+            if (hl && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
+            bl = 0;
+        }
+        if (cr < 0 && br) {
+            int ti = ~cr;
+            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], bvh.tris[4 * ti + 3]);
+            if (hr && !hit && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
+            br = 0;
+        }
+        unsigned long long remaining = ballot64(active && !hit);
+        if (remaining == 0) break;
+        int next = -1;
+        if (bl && br) {
+            if (sp >= kStack) { if (lane == 0) atomicExch(errFlag, 2); break; }
+            if (lane == 0) stk[sp] = cr;
+            sp++;
+            next = cl;
+        } else if (bl) {
+            next = cl;
+        } else if (br) {
+            next = cr;
+        } else {
+            if (sp == 0) break;
+            sp--;
+            __builtin_amdgcn_wave_barrier();
+            next = uniform_int(stk[sp]);
+        }
+        node = uniform_int(next);
+    }
+    return hit;
+}
+
+// One wave = one receiver block of 64 and a run of emitters.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, VisConst vc, int emPerWave) {
+    __shared__ int stkAll[WAVES][kStack];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    int *stk = stkAll[wave];
+    const int nrb = (tk.rc.count + 63) >> 6;
+    const int nEmChunks = (tk.em.count + emPerWave - 1) / emPerWave;
+    const long gw = (long)blockIdx.x * WAVES + wave;
+    if (gw >= (long)nrb * nEmChunks) return;
+    const int rb = (int)(gw / nEmChunks);
+    const int ec = (int)(gw % nEmChunks);
+    const int r = rb * 64 + lane;
+    const bool rvalid = r < tk.rc.count;
+    float4 rp = rvalid ? tk.rc.p[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 rn = (rvalid && tk.rc.n) ? tk.rc.n[r] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int rfacet = __float_as_int(rn.w);
+    const int e0 = ec * emPerWave;
+    const int e1 = min(tk.em.count, e0 + emPerWave);
+    for (int e = e0; e < e1; e++) {
+        float4 ep = tk.em.p[e];
+        float4 en = tk.em.n ? tk.em.n[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        bool pair = rvalid;
+        // Orientation: which end the ray leaves from and which normal culls where.
+        bool fromEm = true;
+        if (tk.origin == ORIGIN_RECEIVER) fromEm = false;
+        if (tk.origin == ORIGIN_LOWER_FACET) {
+            int ef = __float_as_int(en.w);
+            if (ef == rfacet) pair = false;
+            fromEm = ef < rfacet;
+        }
+        F3 o, t, no, nt;
+        bool useNo, useNt;
+        if (fromEm) {
+            o = F3{ep.x, ep.y, ep.z}; t = F3{rp.x, rp.y, rp.z};
+            no = F3{en.x, en.y, en.z}; nt = F3{rn.x, rn.y, rn.z};
+            useNo = tk.useNormalEm; useNt = tk.useNormalRc;
+        } else {
+            o = F3{rp.x, rp.y, rp.z}; t = F3{ep.x, ep.y, ep.z};
+            no = F3{rn.x, rn.y, rn.z}; nt = F3{en.x, en.y, en.z};
+            useNo = tk.useNormalRc; useNt = tk.useNormalEm;
+        }
+        F3 uv; float rr;
+        ray_setup(o, t, uv, rr);
+        if (useNo) {
+            float c = f3dot(uv, no);
+            if (!(c >= 1e-6f)) pair = false;
+        }
+        if (useNt) {
+            float c = f3dot(F3{-uv.x, -uv.y, -uv.z}, nt);
+            if (!(c >= 1e-6f)) pair = false;
+        }
+        bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag);
+        unsigned long long word = ballot64(pair && !occ);
+        if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 *dTrisSorted, float4 *dNodes,
+               hipStream_t st) {
+    if (n <= 0) return 0;
+    if (n == 1) {
+        SR_HIP(hipMemcpyAsync(dTrisSorted, dTrisUnsorted, 4 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        return 0;
+    }
+    float3 ext = make_float3(hi.x - lo.x, hi.y - lo.y, hi.z - lo.z);
+    float3 invExt = make_float3(ext.x > 0 ? 1.0f / ext.x : 0.f, ext.y > 0 ? 1.0f / ext.y : 0.f,
+                                ext.z > 0 ? 1.0f / ext.z : 0.f);
+    unsigned *keys = nullptr, *keys2 = nullptr;
+    int *ids = nullptr, *ids2 = nullptr, *child = nullptr, *pInt = nullptr, *pLeaf = nullptr, *cnt = nullptr;
+    int *err = nullptr;
+    float4 *boxes = nullptr;
+    void *tmp = nullptr;
+    size_t tmpBytes = 0;
+    SR_HIP(hipMalloc(&keys, n * sizeof(unsigned)));
+    SR_HIP(hipMalloc(&keys2, n * sizeof(unsigned)));
+    SR_HIP(hipMalloc(&ids, n * sizeof(int)));
+    SR_HIP(hipMalloc(&ids2, n * sizeof(int)));
+    SR_HIP(hipMalloc(&child, 2 * (n - 1) * sizeof(int)));
+    SR_HIP(hipMalloc(&pInt, (n - 1) * sizeof(int)));
+    SR_HIP(hipMalloc(&pLeaf, n * sizeof(int)));
+    SR_HIP(hipMalloc(&cnt, (n - 1) * sizeof(int)));
+    SR_HIP(hipMalloc(&err, sizeof(int)));
+    SR_HIP(hipMalloc(&boxes, 2 * (n - 1) * sizeof(float4)));
+    const int B = 256, G = (n + B - 1) / B;
+    k_morton<<<G, B, 0, st>>>(dTrisUnsorted, n, lo, invExt, keys, ids);
+    SR_HIP(hipGetLastError());
+    SR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, keys, keys2, ids, ids2, n, 0, 30, st));
+    SR_HIP(hipMalloc(&tmp, tmpBytes));
+    SR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, keys2, ids, ids2, n, 0, 30, st));
+    k_gather_tris<<<G, B, 0, st>>>(dTrisUnsorted, ids2, n, dTrisSorted);
+    SR_HIP(hipGetLastError());
+    SR_HIP(hipMemsetAsync(pInt, 0xff, (n - 1) * sizeof(int), st));
+    SR_HIP(hipMemsetAsync(cnt, 0, (n - 1) * sizeof(int), st));
+    SR_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    const int Gi = (n - 1 + B - 1) / B;
+    k_hierarchy<<<Gi, B, 0, st>>>(keys2, n, child, pInt, pLeaf);
+    SR_HIP(hipGetLastError());
+    k_refit<<<G, B, 0, st>>>(dTrisSorted, n, child, pInt, pLeaf, boxes, cnt, err);
+    SR_HIP(hipGetLastError());
+    k_pack_nodes<<<Gi, B, 0, st>>>(dTrisSorted, n, child, boxes, dNodes);
+    SR_HIP(hipGetLastError());
+    int herr = 0;
+    SR_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    SR_HIP(hipStreamSynchronize(st));
+    void *bufs[] = {keys, keys2, ids, ids2, child, pInt, pLeaf, cnt, err, boxes, tmp};
+    for (void *b : bufs) (void)hipFree(b);
+    if (herr) { set_error("LBVH refit exceeded the depth limit"); return -1; }
+    return 0;
+}
+
+int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st) {
+    if (tk.rc.count <= 0 || tk.em.count <= 0) return 0;
+    constexpr int W = 4;
+    const long nrb = (tk.rc.count + 63) / 64;
+    // Enough waves to fill 256 CUs several times over, but runs of emitters long
+    // enough that the receiver block's data is reused.
+    int emPerWave = 16;
+    long waves = nrb * ((tk.em.count + emPerWave - 1) / emPerWave);
+    while (waves > (1L << 22) && emPerWave < 1024) {
+        emPerWave *= 2;
+        waves = nrb * ((tk.em.count + emPerWave - 1) / emPerWave);
+    }
+    long blocks = (waves + W - 1) / W;
+    if (blocks > 0x7fffffffL) { set_error("visibility grid too large"); return -1; }
+    k_vis_mask<W><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace sr

@@ -1,0 +1,83 @@
+// FP32 shadow-ray semantics shared by the visibility kernels.
+//
+// The reference gates every contribution with an OptiX any-hit ray over all
+// target triangles, double sided, t in [0, r - 1e-3] (src/OptiX.cu:81-93,
+// 163-175, 321-333; geometry from src/OptiX.cpp:264-350). This build defines
+// that test as FP32 Moller-Trumbore, evaluated exactly in the order written
+// (the translation unit is compiled with -ffp-contract=off):
+//   hit iff det != 0, 0 <= u <= 1, v >= 0, u + v <= 1, tmin < t < tmax and the
+//   computed hit point lies inside the triangle's box padded by padTri.
+// Ray set-up follows src/OptiX/vec_math.h:493-497,549-552:
+//   r = sqrtf(dot(v, v)); uv = v * (1.0f / r).
+// Node culling uses boxes padded by padNode > padTri, so the any-hit answer is
+// the OR of the triangle tests, independent of the tree.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sr {
+
+struct F3 { float x, y, z; };
+
+__host__ __device__ inline F3 f3sub(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline float f3dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline F3 f3cross(F3 a, F3 b) {
+    return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Ray from o towards t: unit direction and length exactly as the reference forms them.
+__host__ __device__ inline void ray_setup(F3 o, F3 t, F3 &uv, float &r) {
+    F3 v = f3sub(t, o);
+    r = sqrtf(f3dot(v, v));
+    float inv = 1.0f / r;
+    uv = F3{v.x * inv, v.y * inv, v.z * inv};
+}
+
+struct TriRec {
+    F3 v0, e1, e2, bmin, bmax;
+};
+
+__host__ __device__ inline TriRec tri_unpack(float4 q0, float4 q1, float4 q2, float4 q3) {
+    TriRec t;
+    t.v0 = F3{q0.x, q0.y, q0.z};
+    t.e1 = F3{q0.w, q1.x, q1.y};
+    t.e2 = F3{q1.z, q1.w, q2.x};
+    t.bmin = F3{q2.y, q2.z, q2.w};
+    t.bmax = F3{q3.x, q3.y, q3.z};
+    return t;
+}
+
+__host__ __device__ inline bool mt_hit(const TriRec &T, float padTri, F3 o, F3 d, float tmin, float tmax) {
+    F3 p = f3cross(d, T.e2);
+    float det = f3dot(T.e1, p);
+    if (det == 0.0f) return false;
+    float inv = 1.0f / det;
+    F3 s = f3sub(o, T.v0);
+    float u = f3dot(s, p) * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    F3 q = f3cross(s, T.e1);
+    float v = f3dot(d, q) * inv;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+    float t = f3dot(T.e2, q) * inv;
+    if (!(t > tmin && t < tmax)) return false;
+    F3 h{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+    return h.x >= T.bmin.x - padTri && h.x <= T.bmax.x + padTri && h.y >= T.bmin.y - padTri &&
+           h.y <= T.bmax.y + padTri && h.z >= T.bmin.z - padTri && h.z <= T.bmax.z + padTri;
+}
+
+__host__ __device__ inline float safe_inv(float d) {
+    float dd = (fabsf(d) < 1e-20f) ? (d < 0.0f ? -1e-20f : 1e-20f) : d;
+    return 1.0f / dd;
+}
+
+// Conservative slab test against a box padded by pad (any rounding here only
+// widens the accepted set by far less than the padding margin).
+__host__ __device__ inline bool slab_hit(F3 bmin, F3 bmax, float pad, F3 o, F3 inv, float tmin, float tmax) {
+    float tx0 = (bmin.x - pad - o.x) * inv.x, tx1 = (bmax.x + pad - o.x) * inv.x;
+    float ty0 = (bmin.y - pad - o.y) * inv.y, ty1 = (bmax.y + pad - o.y) * inv.y;
+    float tz0 = (bmin.z - pad - o.z) * inv.z, tz1 = (bmax.z + pad - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    return tn <= tf && tf >= tmin - pad && tn <= tmax + pad;
+}
+
+}  // namespace sr

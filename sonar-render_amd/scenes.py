@@ -1,0 +1,170 @@
+"""Scene builders (numpy only) for the workloads the engine is measured on.
+
+Mesh conventions follow the reference's helpers (run_scripts/modules/geo.py):
+``make_rectangle`` (geo.py:75-113), ``generate_field_points`` (geo.py:64-73,
+x = R sin(theta), z = R cos(theta)) and ``halve_facets`` (geo.py:158-196). A
+facet's normal is unit((v1-v2) x (v1-v3)) (src/Facet.cpp:102-105), so every
+builder orders vertices counter-clockwise seen from the lit side.
+
+Every mesh is returned as a float32 array of shape (N, 3, 3).
+"""
+import math
+
+import numpy as np
+
+C_WATER = 1480.0
+
+
+def wavelength(f, c=C_WATER):
+    return c / f
+
+
+def make_rectangle(length, width, xy_plane=True):
+    """Two-triangle plate centred on the origin (geo.py:75-113)."""
+    if xy_plane:
+        v = np.array([[-length / 2, -width / 2, 0.0], [length / 2, -width / 2, 0.0],
+                      [length / 2, width / 2, 0.0], [-length / 2, width / 2, 0.0]])
+    else:
+        v = np.array([[-length / 2, 0.0, -width / 2], [length / 2, 0.0, -width / 2],
+                      [length / 2, 0.0, width / 2], [-length / 2, 0.0, width / 2]])
+    return np.array([[v[0], v[1], v[2]], [v[0], v[2], v[3]]], dtype=np.float32)
+
+
+def generate_field_points(radius, angles_deg):
+    """Points on an arc in the x-z plane (geo.py:64-73)."""
+    a = np.radians(np.asarray(angles_deg, dtype=np.float64))
+    return np.stack([radius * np.sin(a), np.zeros_like(a), radius * np.cos(a)], axis=1).astype(np.float32)
+
+
+def halve_facets(tris):
+    """Split each triangle into four at its edge midpoints (geo.py:158-196)."""
+    t = np.asarray(tris, dtype=np.float32)
+    v1, v2, v3 = t[:, 0], t[:, 1], t[:, 2]
+    m12, m23, m31 = (v1 + v2) / 2, (v2 + v3) / 2, (v3 + v1) / 2
+    out = np.stack([np.stack([v1, m12, m31], 1), np.stack([m12, v2, m23], 1),
+                    np.stack([m23, v3, m31], 1), np.stack([m12, m23, m31], 1)], 1)
+    return out.reshape(-1, 3, 3)
+
+
+def icosphere(subdivisions=4, radius=1.0):
+    """Geodesic sphere: icosahedron split `subdivisions` times, vertices on the sphere.
+
+    subdivisions=4 gives 5120 triangles and 2562 vertices, the mesh of the
+    reference's run_scripts/sphere_1m_radius.stl. Outward normals.
+    """
+    p = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, p, 0), (1, p, 0), (-1, -p, 0), (1, -p, 0), (0, -1, p), (0, 1, p),
+             (0, -1, -p), (0, 1, -p), (p, 0, -1), (p, 0, 1), (-p, 0, -1), (-p, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius
+    F = np.array(faces)
+    tris = V[F]
+    # orient outward: (v2-v1) x (v3-v1) . centroid > 0
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    flip = np.einsum("ij,ij->i", n, tris.mean(axis=1)) < 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    return tris.astype(np.float32), len(V)
+
+
+def trihedral(leg=1.0):
+    """Three mutually perpendicular right-isosceles plates, normals into the octant."""
+    o = [0.0, 0.0, 0.0]
+    return np.array([[o, [leg, 0, 0], [0, leg, 0]],
+                     [o, [0, leg, 0], [0, 0, leg]],
+                     [o, [0, 0, leg], [leg, 0, 0]]], dtype=np.float32)
+
+
+def octant_incidences(n_side=8, radius=15.0, seed=0):
+    """Seeded jittered n_side x n_side (theta, phi) grid over the octant interior."""
+    rng = np.random.default_rng(seed)
+    i, j = np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")
+    th = (i + rng.uniform(0.05, 0.95, i.shape)) / n_side * (np.pi / 2)
+    ph = (j + rng.uniform(0.05, 0.95, j.shape)) / n_side * (np.pi / 2)
+    th, ph = th.ravel(), ph.ravel()
+    return (radius * np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Workload definitions. Each returns a dict consumed by run_scene() / tests.
+
+def config_plate(f=10e3, c=C_WATER, frac=7.5, n_points=181):
+    """1x1 m plate, monopole 5 m away 20 deg off normal, 181-point 10 m arc."""
+    lam = c / f
+    a = math.radians(20.0)
+    return dict(
+        name="plate_1m_10kHz", f=f, c=c, atten=0.0,
+        targets=[(make_rectangle(1.0, 1.0), lam / frac)], fields=[],
+        sources=np.array([[5 * math.sin(a), 0.0, 5 * math.cos(a)]], np.float32),
+        points=generate_field_points(10.0, np.linspace(-90, 90, n_points)),
+        bounces=0, project_field_objects=False)
+
+
+def config_sphere(f=10e3, c=C_WATER, frac=7.5, subdiv=4, n_points=360, surface_cells=256, field_surface=True):
+    """Rigid r=1 m icosphere, monopole 20 m away, 360-point 30 m ring, field plane."""
+    lam = c / f
+    tris, _ = icosphere(subdiv)
+    dfld = lam / frac
+    fields = []
+    if field_surface:
+        L = surface_cells * dfld
+        fields = [(make_rectangle(L, L, xy_plane=False), dfld)]
+    return dict(
+        name="sphere_r1_10kHz", f=f, c=c, atten=0.0,
+        targets=[(tris, lam / frac)], fields=fields,
+        sources=np.array([[0.0, 0.0, 20.0]], np.float32),
+        points=generate_field_points(30.0, np.arange(n_points) * (360.0 / n_points)),
+        bounces=0, project_field_objects=False)
+
+
+def config_trihedral(f=20e3, c=C_WATER, frac=7.5, n_side=8, bounces=3, radius=15.0):
+    """Trihedral corner reflector, 1 m legs, batched octant incidences, monostatic."""
+    lam = c / f
+    src = octant_incidences(n_side, radius)
+    return dict(
+        name="trihedral_1m_20kHz", f=f, c=c, atten=0.0,
+        targets=[(trihedral(1.0), lam / frac)], fields=[],
+        sources=src, points=src.copy(), bounces=bounces, project_field_objects=False,
+        batched=True)
+
+
+def scene_pairs(scene, n_act, n_cells_field=None):
+    """Ordered pair count of one scene run (the Gpairs/s unit of work).
+
+    One pair is one ordered (emitter fragment with A > 1e-6, receiver)
+    contribution counted before culling; reciprocity counts 2 per unordered
+    pair; sources and field points count as single emitters/receivers.
+    ``n_act`` = per-target list of (active fragments, cells, per-facet active counts);
+    ``n_cells_field`` = active cells of each field object.
+    """
+    ns = len(scene["sources"])
+    nq = len(scene["points"])
+    planes = ns if scene.get("batched") else 1
+    per_plane_src = 1 if scene.get("batched") else ns
+    total = 0
+    for nact, ncell, per_facet in n_act:
+        total += per_plane_src * nact  # point->fragment
+        self_pairs = nact * nact - sum(k * k for k in per_facet)
+        total += scene["bounces"] * self_pairs
+        total += nact * nq
+        if n_cells_field:
+            total += nact * sum(n_cells_field)
+    return total * planes

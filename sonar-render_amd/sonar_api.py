@@ -1,0 +1,298 @@
+"""ctypes binding of libsonar_render.so.
+
+The first group of functions keeps the names, arguments and behaviour of the
+reference's Python wrapper (run_scripts/modules/api.py:6-112) so scene scripts
+written for the reference run unchanged. Two reference slips are fixed and
+documented: ``set_initial_conditions`` passes the fourth (density) argument the
+C function declares (api.py:11-14 passes three), and
+``project_target_to_field_points`` calls ProjectTargetToFieldPoints
+(api.py:96-98 calls ProjectTargetToFieldObjects).
+
+The second group exposes the library's status-returning extensions; every one
+raises :class:`SonarRenderError` with the library's message on failure.
+
+The library is loaded from ``lib/libsonar_render.so`` beside this file (or the
+path in ``SONAR_RENDER_LIB``). A missing library raises at import: there is no
+fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("SONAR_RENDER_LIB", os.path.join(_HERE, "lib", "libsonar_render.so"))
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libsonar_render.so not built: {LIB_PATH} (run __graft_entry__.build())")
+
+cpp_lib = ctypes.CDLL(LIB_PATH)
+
+_f = ctypes.c_float
+_i = ctypes.c_int
+_l = ctypes.c_long
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u64p = ctypes.POINTER(ctypes.c_ulonglong)
+
+
+def _sig(name, args, res=ctypes.c_int):
+    fn = getattr(cpp_lib, name)
+    fn.argtypes = args
+    fn.restype = res
+    return fn
+
+
+_sig("sr_last_error", [], ctypes.c_char_p)
+_sig("sr_last_status", [])
+for _n, _a in [
+    ("sr_set_initial_conditions", [_f, _f, _f, _f]),
+    ("sr_load_geometry", [_fp, _i, _i, _f]),
+    ("sr_load_field_points", [_fp, _i]),
+    ("sr_load_source_points", [_fp, _i]),
+    ("sr_render", []),
+    ("sr_project_src_points_to_objects", [_i]),
+    ("sr_project_target_to_field_objects", []),
+    ("sr_project_target_to_field_points", []),
+    ("sr_project_target_to_target_objects", [_i]),
+    ("sr_get_field_point_pressures", [_dp, _i]),
+    ("sr_get_field_point_pressures_c64", [_fp, _i]),
+    ("sr_teardown", []),
+    ("sr_set_batched_sources", [_i]),
+    ("sr_num_planes", []),
+    ("sr_get_field_point_pressures_plane", [_i, _dp, _i]),
+    ("sr_set_source_amplitude", [_i, _f, _f]),
+    ("sr_prepare_resp", [_i]),
+    ("sr_project_resp", [_i, _i]),
+    ("sr_object_count", [_i]),
+    ("sr_object_active_count", [_i, _i]),
+    ("sr_object_active_cells", [_i, _i, _ip, _i]),
+    ("sr_get_object_pressures", [_i, _i, _i, _i, _dp, _l]),
+    ("sr_get_mask", [_i, _i, _i, _i, _i, _u64p, _l]),
+    ("sr_vis_params", [_fp]),
+]:
+    _sig(_n, _a)
+_sig("sr_object_cell_count", [_i, _i], _l)
+_sig("sr_mask_words", [_i, _i, _i, _i, _i], _l)
+_sig("sr_fragment_facet", [_fp, _f, _ip, _fp, _fp, _fp, _l], _l)
+_sig("sr_last_call_ms", [], ctypes.c_float)
+
+
+class SonarRenderError(RuntimeError):
+    pass
+
+
+def _check(rc, what):
+    if rc < 0:
+        msg = cpp_lib.sr_last_error().decode("utf-8", "replace")
+        raise SonarRenderError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _ptr(a, t=_fp):
+    return a.ctypes.data_as(t)
+
+
+# ---------------------------------------------------------------------------
+# Reference-compatible wrapper (run_scripts/modules/api.py)
+
+def set_initial_conditions(cp, frequency, attenuation, density=0.0):
+    """Sets the medium: sound speed, frequency, attenuation (used as Im k)."""
+    _check(cpp_lib.sr_set_initial_conditions(cp, frequency, attenuation, density), "set_initial_conditions")
+
+
+def load_points_to_cuda(points, isSource=False):
+    """Loads source or field points (N x 3)."""
+    pts = _f32(points).reshape(-1, 3)
+    fn = cpp_lib.sr_load_source_points if isSource else cpp_lib.sr_load_field_points
+    _check(fn(_ptr(pts), len(pts)), "load points")
+
+
+def load_stl_mesh_to_cuda(stl_mesh, object_type, resolution=2e-3):
+    """Loads a mesh (anything with ``.vectors`` of shape (N,3,3), or such an array)."""
+    vec = getattr(stl_mesh, "vectors", stl_mesh)
+    v = _f32(vec).reshape(-1, 9)
+    _check(cpp_lib.sr_load_geometry(_ptr(v), len(v), int(object_type), resolution), "load_geometry")
+
+
+def render_cuda():
+    """Fragments the facets, builds the LBVH and uploads the scene."""
+    _check(cpp_lib.sr_render(), "render_cuda")
+
+
+def GetFieldPoints(NumFieldPnts):
+    """Field-point pressures as an (N, 2) float64 array of (real, imag)."""
+    out = np.zeros((NumFieldPnts, 2), dtype=np.float64)
+    _check(cpp_lib.sr_get_field_point_pressures(_ptr(out, _dp), NumFieldPnts), "GetFieldPointPressures")
+    return out
+
+
+def render_openGL(width, height, viewSettings, filename="output"):
+    """The OpenGL/PNG render is host-side and outside this library."""
+    raise NotImplementedError("render_openGL: the OpenGL/PNG render is not part of the GPU engine")
+
+
+def TearDownCuda():
+    _check(cpp_lib.sr_teardown(), "TearDownCuda")
+
+
+def project_source_points_to_objects(project_to_field_surface=False):
+    _check(cpp_lib.sr_project_src_points_to_objects(1 if project_to_field_surface else 0), "ProjectSrcPointsToObjects")
+
+
+def project_target_to_field_objects():
+    _check(cpp_lib.sr_project_target_to_field_objects(), "ProjectTargetToFieldObjects")
+
+
+def project_target_to_field_points():
+    _check(cpp_lib.sr_project_target_to_field_points(), "ProjectTargetToFieldPoints")
+
+
+def project_target_to_target_objects(number_reflections):
+    _check(cpp_lib.sr_project_target_to_target_objects(int(number_reflections)), "ProjectTargetToTargetObjects")
+
+
+def sound_visualisation_init(max_db, min_db, render_phase_target, render_phase_field):
+    raise NotImplementedError("sound_visualisation_init: render settings belong to the host renderer")
+
+
+# ---------------------------------------------------------------------------
+# Extensions
+
+def set_batched_sources(on=True):
+    _check(cpp_lib.sr_set_batched_sources(1 if on else 0), "set_batched_sources")
+
+
+def num_planes():
+    return cpp_lib.sr_num_planes()
+
+
+def field_points(n, plane=0):
+    """complex128 field-point pressures of one incidence plane."""
+    out = np.zeros((n, 2), dtype=np.float64)
+    _check(cpp_lib.sr_get_field_point_pressures_plane(plane, _ptr(out, _dp), n), "field_points")
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def field_points_c64(n):
+    out = np.zeros((n, 2), dtype=np.float32)
+    _check(cpp_lib.sr_get_field_point_pressures_c64(_ptr(out), n), "field_points_c64")
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def set_source_amplitude(i, value):
+    _check(cpp_lib.sr_set_source_amplitude(i, float(np.real(value)), float(np.imag(value))), "set_source_amplitude")
+
+
+def prepare_resp(target):
+    _check(cpp_lib.sr_prepare_resp(target), "prepare_resp")
+
+
+def project_resp(a, b):
+    _check(cpp_lib.sr_project_resp(a, b), "project_resp")
+
+
+def object_count(object_type):
+    return cpp_lib.sr_object_count(object_type)
+
+
+def cell_count(object_type, index):
+    return _check(cpp_lib.sr_object_cell_count(object_type, index), "cell_count")
+
+
+def active_cells(object_type, index):
+    n = _check(cpp_lib.sr_object_active_count(object_type, index), "active_count")
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    _check(cpp_lib.sr_object_active_cells(object_type, index, _ptr(out, _ip), n), "active_cells")
+    return out[:n]
+
+
+def object_pressures(object_type, index, which=0, plane=0):
+    """complex128 per-cell pressure (which: 0 P, 1 P_in, 2 P_out)."""
+    n = cell_count(object_type, index)
+    out = np.zeros((max(n, 1), 2), dtype=np.float64)
+    _check(cpp_lib.sr_get_object_pressures(object_type, index, which, plane, _ptr(out, _dp), n), "object_pressures")
+    return out[:n, 0] + 1j * out[:n, 1]
+
+
+def mask_words(kind, ta=0, ia=0, tb=0, ib=0):
+    """Visibility words [rblock][emitter] as uint64."""
+    n = _check(cpp_lib.sr_mask_words(kind, ta, ia, tb, ib), "mask_words")
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    _check(cpp_lib.sr_get_mask(kind, ta, ia, tb, ib, _ptr(out, _u64p), n), "get_mask")
+    return out[:n]
+
+
+def mask_bits(kind, n_emit, n_recv, ta=0, ia=0, tb=0, ib=0):
+    """Visibility as a bool array [receiver, emitter]."""
+    w = mask_words(kind, ta, ia, tb, ib).reshape(-1, n_emit)
+    bits = np.unpackbits(w.view(np.uint8).reshape(w.shape[0], n_emit, 8), axis=2, bitorder="little")
+    bits = bits.transpose(0, 2, 1).reshape(-1, n_emit)
+    return bits[:n_recv].astype(bool)
+
+
+def vis_params():
+    out = np.zeros(5, dtype=np.float32)
+    _check(cpp_lib.sr_vis_params(_ptr(out)), "vis_params")
+    return out
+
+
+def last_call_ms():
+    return float(cpp_lib.sr_last_call_ms())
+
+
+def fragment_facet(tri, delta):
+    """Host fragmenter probe: (grid (nX,nY,nNeg), area[nY,nX], emitter pts, receiver pts)."""
+    v = _f32(tri).reshape(9)
+    g = np.zeros(3, dtype=np.int32)
+    n = _check(cpp_lib.sr_fragment_facet(_ptr(v), delta, _ptr(g, _ip), None, None, None, 0), "fragment_facet")
+    area = np.zeros(max(n, 1), np.float32)
+    pe = np.zeros((max(n, 1), 3), np.float32)
+    pr = np.zeros((max(n, 1), 3), np.float32)
+    _check(cpp_lib.sr_fragment_facet(_ptr(v), delta, _ptr(g, _ip), _ptr(area), _ptr(pe), _ptr(pr), n), "fragment_facet")
+    nx, ny = int(g[0]), int(g[1])
+    return (nx, ny, int(g[2])), area[:n].reshape(ny, nx), pe[:n], pr[:n]
+
+
+_sig("sr_reset_pressures", [])
+_sig("sr_clear_vis_cache", [])
+_sig("sr_profile", [_i])
+_sig("sr_kernel_stats", [_i, _dp, ctypes.POINTER(ctypes.c_long), _dp, _dp, _dp])
+
+KERNEL_FAMILIES = {"p2f": 0, "vis": 1, "f2f": 2, "f2f_field": 3, "f2p": 4, "reduce": 5}
+
+
+def reset_pressures():
+    _check(cpp_lib.sr_reset_pressures(), "reset_pressures")
+
+
+def clear_vis_cache():
+    _check(cpp_lib.sr_clear_vis_cache(), "clear_vis_cache")
+
+
+def profile(on=True):
+    _check(cpp_lib.sr_profile(1 if on else 0), "profile")
+
+
+def kernel_stats(family):
+    """dict(ms, launches, pairs, flops, bytes) summed over the recorded launches of one kernel family."""
+    ms, pairs, flops, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_long()
+    _check(cpp_lib.sr_kernel_stats(KERNEL_FAMILIES[family], ctypes.byref(ms), ctypes.byref(n), ctypes.byref(pairs),
+                                   ctypes.byref(flops), ctypes.byref(by)), "kernel_stats")
+    return dict(ms=ms.value, launches=n.value, pairs=pairs.value, flops=flops.value, bytes=by.value)
+
+
+_sig("sr_set_transfer_mode", [_i])
+_sig("sr_transfer_mode", [])
+KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8})
+
+
+def set_transfer_mode(mode):
+    """0: plain FP32 gathers; 1: FP16 MFMA transfer matrix (FP32 accumulation)."""
+    _check(cpp_lib.sr_set_transfer_mode(int(mode)), "set_transfer_mode")

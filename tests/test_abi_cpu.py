@@ -1,0 +1,69 @@
+"""C-ABI checks that need no GPU: every declared symbol is exported, and the
+error paths the reference turned into std::terminate return status codes."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sonar_render.h")
+
+REFERENCE_SYMBOLS = [  # src/python_interface.cpp:14-107
+    "set_initial_conditions", "load_geometry", "load_field_points", "load_source_points", "render_cuda",
+    "ProjectSrcPointsToObjects", "ProjectTargetToFieldObjects", "ProjectTargetToFieldPoints",
+    "ProjectTargetToTargetObjects", "GetFieldPointPressures", "RenderOpenGL", "TearDownCuda",
+    "SoundVisualisationInit",
+]
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", text)))
+
+
+def test_header_declares_reference_symbols():
+    decl = _declared()
+    for s in REFERENCE_SYMBOLS:
+        assert s in decl
+
+
+def test_library_exports_every_declared_symbol():
+    import sonar_api
+    lib = ctypes.CDLL(sonar_api.LIB_PATH)
+    missing = [s for s in _declared() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_error_paths_return_status(api):
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    rc = api.cpp_lib.sr_load_geometry(tri.ctypes.data_as(api._fp), 1, 7, 0.1)
+    assert rc == -2 and "invalid object type" in api.cpp_lib.sr_last_error().decode()
+    assert api.cpp_lib.sr_project_target_to_target_objects(1) == -3
+    assert api.cpp_lib.sr_project_src_points_to_objects(0) == -3
+    out = np.zeros(4, np.float64)
+    assert api.cpp_lib.sr_get_field_point_pressures(out.ctypes.data_as(api._dp), 2) == -3
+    assert api.cpp_lib.sr_load_geometry(tri.ctypes.data_as(api._fp), 1, 0, 0.0) == -2
+    # the reference's void entry point with a bad type must not abort the process
+    api.cpp_lib.load_geometry.argtypes = [api._fp, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    api.cpp_lib.load_geometry(tri.ctypes.data_as(api._fp), 1, 9, 0.1)
+    assert api.cpp_lib.sr_last_status() == -2
+    with pytest.raises(api.SonarRenderError):
+        api.load_stl_mesh_to_cuda(tri[None], 5, 0.1)
+    with pytest.raises(NotImplementedError):
+        api.render_openGL(8, 8, [0.0] * 9)
+
+
+def test_source_objects_are_stored_not_used(api):
+    tri = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
+    api.load_stl_mesh_to_cuda(tri, 1, 0.1)
+    assert api.object_count(1) == 1 and api.object_count(0) == 0
+
+
+def test_fragment_probe_shapes(api):
+    (nx, ny, nneg), area, pe, pr = api.fragment_facet(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), 0.1)
+    assert area.shape == (ny, nx) and pe.shape == (nx * ny, 3) and pr.shape == (nx * ny, 3)
+    assert 0 <= nneg <= nx
+    assert abs(area.sum() - 0.5) < 1e-5

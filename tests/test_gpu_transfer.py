@@ -1,0 +1,63 @@
+"""GPU: the FP16 MFMA transfer form of the bounce against the FP64 oracle and
+against the plain FP32 form. Bound: rel-L2 <= 1e-3 (the metric's accuracy bar)."""
+import numpy as np
+import pytest
+
+import oracle
+import runner
+import scenes
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_MIXED = 1e-3
+
+
+def _banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def test_mixed_trihedral_vs_oracle_gated(api):
+    sc = scenes.config_trihedral(f=5e3, n_side=2, bounces=3)
+    _banner("mixed form: trihedral 5 kHz, 4 incidences, 3 bounces")
+    out = runner.run_scene(sc, transfer_mode=1)
+    worst = 0.0
+    for p in range(len(sc["sources"])):
+        ref = oracle.run_scene(sc, plane_source=p)
+        e = rel_l2(np.concatenate([out["points"][p], out["targets"][0][p]]),
+                   np.concatenate([ref["points"], ref["targets"][0]]))
+        print("plane", p, "rel-L2", e)
+        worst = max(worst, e)
+    assert worst < TOL_MIXED
+
+
+def test_mixed_many_small_facets(api):
+    """Facets with fewer than 8 fragments force K steps that straddle facets."""
+    tris = scenes.trihedral(1.0)
+    for _ in range(3):
+        tris = scenes.halve_facets(tris)
+    sc = dict(name="trihedral_192_facets", f=3e3, c=1480.0, atten=0.1, targets=[(tris, 1480 / 3e3 / 7.5)], fields=[],
+              sources=scenes.octant_incidences(2, 15.0), points=scenes.octant_incidences(2, 15.0), bounces=3,
+              batched=True)
+    _banner("mixed form: 192-facet trihedral, attenuation, 3 bounces")
+    out = runner.run_scene(sc, transfer_mode=1)
+    plain = runner.run_scene(sc, transfer_mode=0)
+    for p in range(len(sc["sources"])):
+        ref = oracle.run_scene(sc, plane_source=p)
+        a = np.concatenate([out["points"][p], out["targets"][0][p]])
+        b = np.concatenate([ref["points"], ref["targets"][0]])
+        c = np.concatenate([plain["points"][p], plain["targets"][0][p]])
+        print("plane", p, "mixed", rel_l2(a, b), "plain", rel_l2(c, b))
+        assert rel_l2(a, b) < TOL_MIXED
+        assert rel_l2(c, b) < 2e-4
+
+
+def test_mixed_equals_plain_at_20khz_subset(api):
+    sc = scenes.config_trihedral(f=12e3, n_side=2, bounces=3)
+    _banner("mixed vs plain: trihedral 12 kHz, 4 incidences")
+    a = runner.run_scene(sc, transfer_mode=1)
+    b = runner.run_scene(sc, transfer_mode=0)
+    e = rel_l2(np.concatenate([a["points"].ravel(), a["targets"][0].ravel()]),
+               np.concatenate([b["points"].ravel(), b["targets"][0].ravel()]))
+    print("mixed vs plain rel-L2", e)
+    assert e < TOL_MIXED
