@@ -69,9 +69,13 @@ void ProjectTargetToTargetObjects(int number_reflections);
  * values (the reference ignores NumPoints, ModelCuda.cpp:206). */
 void GetFieldPointPressures(sr_dcomplex *field_points_pressure, int NumPoints);
 
-/* python_interface.cpp:92-96 and :103-107: the OpenGL render stays on the
- * host and is not part of this library; these record an error and return. */
+/* python_interface.cpp:92-96: the OpenGL/PNG render stays on the host and
+ * is not part of this library; this records an error and returns.
+ * sr_surface_rgba gives the per-cell colours that render would draw. */
 void RenderOpenGL(int width, int height, char *filename, float viewSettings[9]);
+
+/* python_interface.cpp:103-107: colour range and phase flags used by
+ * sr_surface_rgba (defaults -20 / -90 dB as ModelCuda.hpp:45-48). */
 void SoundVisualisationInit(float maxDbValue, float minDbValue, bool renderPhaseTarget, bool renderPhaseField);
 
 /* python_interface.cpp:98-101: free the scene; the library can be loaded
@@ -149,6 +153,27 @@ int sr_clear_vis_cache(void);
 int sr_set_transfer_mode(int mode);
 int sr_transfer_mode(void);
 
+/* Near-field terms of the transfer matrix (mode 1 transfer only; pairs of
+ * fragments closer than radiusWavelengths): 0 the reference's point rule,
+ * 1 16x16 sub-fragment quadrature of the same kernel over the emitter's
+ * cell, 2 the learned correction (an 8-64-64-2 MLP, BF16 inference). */
+int sr_set_near_mode(int mode, float radiusWavelengths);
+
+/* The learned correction's network: trained on the GPU with the library's own
+ * forward/backward/Adam kernels on pairs labelled by the quadrature.
+ * report2: relative RMS error on training and held-out pairs. Parameters are
+ * 4866 floats (W1[64][8] b1 W2[64][64] b2 W3[2][64] b3). sr_nf_samples
+ * returns labelled pairs (8 features, 2 targets each); sr_nf_forward the FP32
+ * network output; sr_nf_gradient the loss gradient of a given parameter set
+ * on given pairs (L = sum |out - y|^2 / n). */
+int sr_nf_init(unsigned seed);
+int sr_nf_train(int nsamples, int steps, int batch, float lr, unsigned seed, float *report2);
+int sr_nf_get_params(float *out, int n);
+int sr_nf_set_params(const float *in, int n);
+int sr_nf_samples(unsigned seed, int n, float *X, float *Y);
+int sr_nf_forward(const float *X, int n, float *out);
+int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, float *grad);
+
 /* Per-launch profiling with HIP events on the library's stream. sr_profile(1)
  * starts (and clears) the record. Kernel families: 0 point->fragment,
  * 1 visibility words, 2 fragment->fragment, 3 fragment->field object,
@@ -159,6 +184,17 @@ int sr_transfer_mode(void);
  * algorithmic HBM bytes (transfer GEMM: S, B and the partial sums). */
 int sr_profile(int on);
 int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
+
+/* STL meshes (binary or ASCII in, binary out): 9 floats per triangle. */
+long sr_read_stl(const char *path, float *out9, long cap);
+int sr_write_stl(const char *path, const float *v9, int n);
+int sr_load_stl(const char *path, int objectType, float resolution);
+
+/* Surface colour map of src/FacetCuda.cu:43-72: brightness from
+ * 20 log10(|p| + 1e-8) between min_dB and max_dB, grey with saturation 0.8 or
+ * (phase) hue (arg p + pi) / 2 pi on an HSV wheel. rgba: 4 floats per value. */
+int sr_colour_map(const double *p2, long n, float minDb, float maxDb, int phase, float *rgba);
+int sr_surface_rgba(int objectType, int index, int plane, float *rgba, long n);
 
 /* Wall time of the last projection entry point, milliseconds. */
 float sr_last_call_ms(void);

@@ -213,7 +213,41 @@ struct Scene {
     VisParams vp;
     bool rendered = false;
     bool bruteForce = false;
+    // near-field quadrature (0 = the reference's point rule everywhere)
+    int nearSub = 0;
+    float nearRadius = 0.f;
 };
+
+// Sub-fragment quadrature of one fragment->fragment term (test infrastructure's
+// definition of the near-field judge): the delta x delta square centred on the
+// emitter point is split into nsub x nsub cells; each cell centre is an
+// emitter with area A/nsub^2, cosine culls and the r < delta/nsub
+// regularisation of the point rule, in FP64. Returns the replacement for
+// A_e cos_s cos_i 0.9 G(r) of OptiX.cu:340-354.
+static cd near_quadrature(const Scene &s, const Facet &FA, float dA, float Ae, V3 pe, const Facet *FB, V3 pr,
+                          bool fieldDst) {
+    const int n = s.nearSub;
+    const double d = dA, h = d / n;
+    const cd ik = cd(0, 1) * s.k;
+    cd acc(0, 0);
+    for (int sy = 0; sy < n; sy++)
+        for (int sx = 0; sx < n; sx++) {
+            const double ox = ((sx + 0.5) * h - 0.5 * d), oy = ((sy + 0.5) * h - 0.5 * d);
+            const double px = pe.x + FA.xAxis.x * ox + FA.yAxis.x * oy;
+            const double py = pe.y + FA.xAxis.y * ox + FA.yAxis.y * oy;
+            const double pz = pe.z + FA.xAxis.z * ox + FA.yAxis.z * oy;
+            const double vx = pr.x - px, vy = pr.y - py, vz = pr.z - pz;
+            const double r = std::sqrt(vx * vx + vy * vy + vz * vz);
+            const double cs = (vx * FA.normal.x + vy * FA.normal.y + vz * FA.normal.z) / r;
+            const double ci = fieldDst ? 1.0 : -(vx * FB->normal.x + vy * FB->normal.y + vz * FB->normal.z) / r;
+            if (!(cs >= 1e-6) || !(ci >= 1e-6)) continue;
+            cd G;
+            if (r < h) G = ik / (4.0 * M_PI);
+            else G = -std::exp(cd(0, 1) * s.k * r) / (4 * M_PI) * (ik / r) + 1.0 / (r * r);
+            acc += cs * ci * 0.9 * G;
+        }
+    return acc * ((double)Ae / (double)(n * n));
+}
 
 static cd expik(const cd &k, float r) {
     // thrust::exp(i1 * k * r) with r promoted to double (OptiX.cu:102,182,350)
@@ -350,6 +384,14 @@ static void project_facets(Scene &s, Object &A, Object &B, int mode) {
                                 if (B.type == OBJ_FIELD) cinc = 1;
                                 if (cinc < 1e-6f) continue;
                                 if (occluded(s, pi, uv, s.vp.tminFrag, r - eps)) continue;
+                                if (s.nearSub > 0 && r < s.nearRadius) {
+                                    // near-field judge: quadrature over each emitter's own cell
+                                    const bool fld = (B.type == OBJ_FIELD);
+                                    oB[cj] += near_quadrature(s, FA, A.delta, Ai, pi, &FB, pj, fld) * Pri;
+                                    if (recip)
+                                        oA[ci] += near_quadrature(s, FB, B.delta, Bj, pj, &FA, pi, false) * dstVal[cj];
+                                    continue;
+                                }
                                 cd G;
                                 if (r < dI) {
                                     G = ik / (4.0 * M_PI);
@@ -394,6 +436,10 @@ void orc_destroy(void *h) { delete (Scene *)h; }
 void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 void orc_set_bruteforce(void *h, int on) { ((Scene *)h)->bruteForce = on != 0; }
+void orc_set_near_quadrature(void *h, int nsub, float radius) {
+    ((Scene *)h)->nearSub = nsub;
+    ((Scene *)h)->nearRadius = radius;
+}
 
 // Model::set_inital_conditions (Model.cpp:44-54): omega and k stored as float.
 void orc_set_initial_conditions(void *h, float cp, float f, float att, float dens) {

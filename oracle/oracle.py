@@ -49,6 +49,7 @@ for name, args, res in [
     ("orc_set_object_pressure", [_v, _i, _i, _i, _dp], None),
     ("orc_anyhit", [_v, _fp, _fp, _fp, _fp, ctypes.c_long, _u8p, _i], None),
     ("orc_vis_params", [_v, _fp], None), ("orc_kf", [_v], _f),
+    ("orc_set_near_quadrature", [_v, _i, _f], None),
 ]:
     fn = getattr(_lib, name)
     fn.argtypes = args
@@ -101,6 +102,10 @@ class Scene:
 
     def set_bruteforce(self, on):
         _lib.orc_set_bruteforce(self.h, 1 if on else 0)
+
+    def set_near_quadrature(self, nsub, radius):
+        """Near-field judge: pairs closer than radius use nsub x nsub sub-fragment quadrature."""
+        _lib.orc_set_near_quadrature(self.h, int(nsub), float(radius))
 
     def render(self):
         assert _lib.orc_render(self.h) == 0
@@ -183,7 +188,7 @@ class Scene:
         return _lib.orc_kf(self.h)
 
 
-def run_scene(scene, plane_source=None, bruteforce=False):
+def run_scene(scene, plane_source=None, bruteforce=False, near=None):
     """Runs a scene dict (see scenes.py) through the oracle.
 
     plane_source: for batched scenes, the single source index to run.
@@ -191,6 +196,8 @@ def run_scene(scene, plane_source=None, bruteforce=False):
     """
     s = Scene()
     s.set_bruteforce(bruteforce)
+    if near:
+        s.set_near_quadrature(*near)
     s.set_initial_conditions(scene["c"], scene["f"], scene.get("atten", 0.0))
     for tris, d in scene["targets"]:
         s.load_geometry(tris, 0, d)

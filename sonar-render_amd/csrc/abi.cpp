@@ -10,10 +10,15 @@
 #include "../../include/sonar_render.h"
 #include "launch.h"
 #include "model_ops.h"
+#include "hostio.h"
+#include "nearfield.h"
 
 namespace {
 
 sr::Model gModel;  // python_interface.cpp:11
+// Render colour settings (ModelCuda.hpp:45-48 defaults), set by SoundVisualisationInit.
+float gMaxDb = -20.f, gMinDb = -90.f;
+bool gPhaseTarget = false, gPhaseField = false;
 int gStatus = 0;
 std::string gMsg;
 float gLastMs = 0.f;
@@ -210,6 +215,26 @@ int sr_set_transfer_mode(int mode) {
     return guarded("set_transfer_mode", [&] { return sr::model_set_transfer_mode(gModel, mode); });
 }
 int sr_transfer_mode(void) { return gModel.transferMode; }
+int sr_set_near_mode(int mode, float radiusWavelengths) {
+    return guarded("set_near_mode", [&] { return sr::model_set_near_mode(gModel, mode, radiusWavelengths); });
+}
+int sr_nf_init(unsigned seed) { return guarded("nf_init", [&] { return sr::nf_init(seed); }); }
+int sr_nf_train(int nsamples, int steps, int batch, float lr, unsigned seed, float *report2) {
+    return guarded("nf_train", [&] { return sr::nf_train(nsamples, steps, batch, lr, seed, report2); });
+}
+int sr_nf_get_params(float *out, int n) { return guarded("nf_get_params", [&] { return sr::nf_get_params(out, n); }); }
+int sr_nf_set_params(const float *in, int n) {
+    return guarded("nf_set_params", [&] { return sr::nf_set_params(in, n); });
+}
+int sr_nf_samples(unsigned seed, int n, float *X, float *Y) {
+    return guarded("nf_samples", [&] { return sr::nf_samples_host(seed, n, X, Y); });
+}
+int sr_nf_forward(const float *X, int n, float *out) {
+    return guarded("nf_forward", [&] { return sr::nf_eval(X, nullptr, n, out, nullptr, nullptr); });
+}
+int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, float *grad) {
+    return guarded("nf_gradient", [&] { return sr::nf_eval(X, Y, n, nullptr, grad, params); });
+}
 int sr_profile(int on) {
     return guarded("profile", [&] { sr::model_profile(gModel, on); return 0; });
 }
@@ -217,6 +242,52 @@ int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double 
     return guarded("kernel_stats", [&] {
         if (kind < 0 || kind >= sr::PK_COUNT) { sr::set_error("unknown kernel family"); return -2; }
         return sr::model_kernel_stats(gModel, kind, ms, launches, pairs, flops, bytes);
+    });
+}
+
+long sr_read_stl(const char *path, float *out, long cap) {
+    long rc = -1;
+    guarded("read_stl", [&] {
+        std::vector<float> t;
+        std::string err;
+        int n = sr::stl_read(path ? path : "", t, err);
+        if (n < 0) { sr::set_error(err); rc = n; return n; }
+        if (out && cap >= n) std::copy(t.begin(), t.end(), out);
+        rc = n;
+        return 0;
+    });
+    return rc;
+}
+int sr_write_stl(const char *path, const float *v, int n) {
+    return guarded("write_stl", [&] {
+        std::string err;
+        int rc = sr::stl_write(path ? path : "", v, n, err);
+        if (rc) sr::set_error(err);
+        return rc;
+    });
+}
+int sr_load_stl(const char *path, int type, float res) {
+    return guarded("load_stl", [&] {
+        std::vector<float> t;
+        std::string err;
+        int n = sr::stl_read(path ? path : "", t, err);
+        if (n < 0) { sr::set_error(err); return n; }
+        return sr::model_load_geometry(gModel, t.data(), n, type, res);
+    });
+}
+int sr_colour_map(const double *p, long n, float minDb, float maxDb, int phase, float *rgba) {
+    if (n < 0 || (n > 0 && (!p || !rgba))) return -2;
+    sr::colour_map(p, n, minDb, maxDb, phase != 0, rgba);
+    return 0;
+}
+int sr_surface_rgba(int type, int idx, int plane, float *rgba, long n) {
+    return guarded("surface_rgba", [&] {
+        std::vector<double> p((size_t)2 * std::max<long>(n, 0));
+        int rc = sr::model_get_object_pressure(gModel, type, idx, 0, plane, p.data(), n);
+        if (rc) return rc;
+        const bool ph = (type == sr::OBJ_FIELD) ? gPhaseField : gPhaseTarget;
+        sr::colour_map(p.data(), n, gMinDb, gMaxDb, ph, rgba);
+        return 0;
     });
 }
 
@@ -238,12 +309,11 @@ void RenderOpenGL(int, int, char *, float *) {
     });
     report("RenderOpenGL", rc);
 }
-void SoundVisualisationInit(float, float, bool, bool) {
-    int rc = guarded("SoundVisualisationInit", [&] {
-        sr::set_error("render colour settings belong to the host renderer");
-        return -2;
-    });
-    report("SoundVisualisationInit", rc);
+void SoundVisualisationInit(float maxDb, float minDb, bool phaseTarget, bool phaseField) {
+    gMaxDb = maxDb;
+    gMinDb = minDb;
+    gPhaseTarget = phaseTarget;
+    gPhaseField = phaseField;
 }
 void TearDownCuda(void) { report("TearDownCuda", sr_teardown()); }
 

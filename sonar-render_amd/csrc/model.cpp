@@ -10,6 +10,7 @@
 #include "launch.h"
 #include "model_ops.h"
 #include "transfer.h"
+#include "nearfield.h"
 
 namespace sr {
 
@@ -109,6 +110,15 @@ int upload_object(Model &m, Object &o) {
         SR_HIP(hipMemcpy(o.cellN, cn.data(), off * sizeof(float4), hipMemcpyHostToDevice));
         SR_HIP(hipMemset(o.P, 0, np * off * sizeof(float2)));
     }
+    {
+        std::vector<float4> fr(2 * std::max(1, nf));
+        for (int f = 0; f < nf; f++) {
+            fr[2 * f] = make_float4(o.frames[f].xAxis.x, o.frames[f].xAxis.y, o.frames[f].xAxis.z, 0.f);
+            fr[2 * f + 1] = make_float4(o.frames[f].yAxis.x, o.frames[f].yAxis.y, o.frames[f].yAxis.z, 0.f);
+        }
+        if (dev_alloc(&o.frameTab, fr.size())) return -1;
+        SR_HIP(hipMemcpy(o.frameTab, fr.data(), fr.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
     if (o.nact) {
         SR_HIP(hipMemcpy(o.actP, ap.data(), o.nact * sizeof(float4), hipMemcpyHostToDevice));
         SR_HIP(hipMemcpy(o.actN, an.data(), o.nact * sizeof(float4), hipMemcpyHostToDevice));
@@ -131,7 +141,7 @@ void free_transfer(Object &o) {
 
 void free_object(Object &o) {
     dev_free(o.cellP); dev_free(o.cellN); dev_free(o.actP); dev_free(o.actN); dev_free(o.actCell);
-    dev_free(o.P); dev_free(o.Pin); dev_free(o.Pout);
+    dev_free(o.P); dev_free(o.Pin); dev_free(o.Pout); dev_free(o.frameTab);
     free_transfer(o);
 }
 
@@ -503,7 +513,13 @@ static int ensure_transfer(Model &m, int i) {
     t.sS = (float)(16384.0 / smax);
     TransferGeom g{o.actP, o.actN, n, nKs, nRt};
     TransferPhys ph{m.kf, m.atten, (float)((double)m.atten * 1.4426950408889634), o.delta, 0.9f, t.sS,
-                    (double)m.kf / (2.0 * M_PI)};
+                    (double)m.kf / (2.0 * M_PI), m.nearMode,
+                    (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr};
+    if (m.nearMode == 2) {
+        NearField &nf = nearfield();
+        if (!nf.trained) { set_error("learned near-field mode needs a trained network (sr_nf_train)"); return -3; }
+        ph.mlp = nf.P;
+    }
     {
         ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, 0.0);
         if (transfer_build_s(g, ph, mk, t.A, m.stream)) return -1;
@@ -753,6 +769,15 @@ int model_clear_vis_cache(Model &m) {
     return 0;
 }
 
+int model_set_near_mode(Model &m, int mode, float radiusWl) {
+    if (mode < 0 || mode > 2) { set_error("near mode: 0 point rule, 1 quadrature, 2 learned"); return -2; }
+    if (!(radiusWl > 0.f)) { set_error("near radius must be > 0 wavelengths"); return -2; }
+    m.nearMode = mode;
+    m.nearRadiusWl = radiusWl;
+    for (auto &o : m.targets) o.tc.valid = false;
+    return 0;
+}
+
 int model_set_transfer_mode(Model &m, int mode) {
     if (mode != 0 && mode != 1) { set_error("transfer mode: 0 plain FP32, 1 FP16 MFMA transfer matrix"); return -2; }
     m.transferMode = mode;
@@ -796,6 +821,7 @@ void model_teardown(Model &m) {
     for (auto &kv : m.masks) (void)hipFree(kv.second.p);
     m.masks.clear();
     m.transferMode = 0;
+    m.nearMode = 0;
     dev_free(m.srcP); dev_free(m.srcAmp); dev_free(m.fldP); dev_free(m.fldVal);
     dev_free(m.trisSorted); dev_free(m.nodes); dev_free(m.errFlag);
     dev_free(m.scratchL); dev_free(m.scratchU);

@@ -33,6 +33,7 @@ struct Object {
     float4 *actP = nullptr;    // active cell point (emitter form), area
     float4 *actN = nullptr;    // facet normal, facet id bits
     int *actCell = nullptr;    // active -> cell
+    float4 *frameTab = nullptr;  // per facet: xAxis, yAxis
     double sameFacetPairs = 0; // sum over facets of (active cells)^2
     std::vector<int> actFacetHost;
     // mixed-precision transfer matrix (built once per geometry and frequency)
@@ -75,6 +76,8 @@ struct Model {
     struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; };
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
+    int nearMode = 0;      // transfer matrix near field: 0 point rule, 1 16x16 quadrature, 2 learned
+    float nearRadiusWl = 2.0f;
     float2 *scratchL = nullptr, *scratchU = nullptr;
     size_t scratchElems = 0;
     // per-launch profiling (HIP events on the library's stream)

@@ -61,7 +61,7 @@ struct Acc {
 
 // One pair's complex kernel value K (before multiplying by the emitter term).
 template <int KIND, bool ATT>
-__device__ inline void pair_kernel(const Phys &ph, float4 rp, float4 rn, float4 ep, float4 en, bool bit, float &Kre,
+__device__ __attribute__((always_inline)) inline void pair_kernel(const Phys &ph, float4 rp, float4 rn, float4 ep, float4 en, bool bit, float &Kre,
                                    float &Kim) {
     float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
     float r2 = vx * vx + vy * vy + vz * vz;
@@ -104,7 +104,7 @@ __device__ inline void cmac(float2 &acc, float Kre, float Kim, float2 X) {
 }
 
 template <int KIND, int NB, bool ATT, bool SPLITG>
-__device__ inline void tile_loop(const Phys &ph, int n, const float4 *s_p, const float4 *s_n, const float2 (*s_x)[kTile],
+__device__ __attribute__((always_inline)) inline void tile_loop(const Phys &ph, int n, const float4 *s_p, const float4 *s_n, const float2 (*s_x)[kTile],
                                  const uint64_t *s_m, int lane, float4 rp, float4 rn, int rfacet, float2 *accL,
                                  float2 *accU, float2 *compL) {
     for (int k = 0; k < n; k++) {
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
     if (!rvalid) return;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        if (b >= t.nrhs) break;
+        if (b >= t.nrhs) continue;
         const size_t o = ((size_t)split * NB + b) * t.nr + r;
         if (KIND == KIND_F2P) {
             t.partL[o] = accL[b];

@@ -17,6 +17,11 @@ struct TransferPhys {
     float kr, alpha, alphaLog2e, deltaReg, damp;
     float scale;        // FP16 scale of S
     double krevD;       // kr / (2 pi) in FP64
+    // near field: 0 point rule (the reference), 1 16x16 quadrature, 2 learned MLP
+    int nearMode;
+    float nearRadius;   // metres
+    const float4 *frames;  // per facet: xAxis (w unused), yAxis
+    const float *mlp;      // kMlpParams FP32 weights (rounded to BF16 on load)
 };
 
 struct TransferGemm {

@@ -29,6 +29,7 @@
 #include "launch.h"
 #include "sums.h"
 #include "transfer.h"
+#include "learned.h"
 
 namespace sr {
 
@@ -43,10 +44,97 @@ __device__ inline uint32_t pack_h2(float a, float b) {
 }
 
 // ---------------------------------------------------------------------------
+// Near-field values (per unit emitter area) for a pair closer than nearRadius.
+// rv, nr: receiver point (relative to the emitter point) and normal, in the
+// emitter's frame (x = xAxis, y = yAxis, z = normal).
+
+__device__ inline float bf16_round(float x) {
+    uint32_t u = __float_as_uint(x);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    return __uint_as_float(u);
+}
+
+__device__ inline float2 g_point(float r, float cs, float ci, float k, float reg) {
+    float re, im;
+    if (r < reg) {
+        re = 0.f;
+        im = k * kInv4PiT;
+    } else {
+        float s, c;
+        sincosf(k * r, &s, &c);
+        const float a = k * kInv4PiT / r;
+        re = a * s + 1.0f / (r * r);
+        im = -a * c;
+    }
+    const float w = cs * ci * 0.9f;
+    return make_float2(w * re, w * im);
+}
+
+__device__ float2 quad16(float3 rv, float3 nr, float d, float k) {
+    const int n = 16;
+    const float h = d / n;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int sy = 0; sy < n; sy++)
+        for (int sx = 0; sx < n; sx++) {
+            const float vx = rv.x - ((sx + 0.5f) * h - 0.5f * d), vy = rv.y - ((sy + 0.5f) * h - 0.5f * d);
+            const float vz = rv.z;
+            const float r = sqrtf(vx * vx + vy * vy + vz * vz);
+            const float cs = vz / r, ci = -(vx * nr.x + vy * nr.y + vz * nr.z) / r;
+            if (!(cs >= 1e-6f) || !(ci >= 1e-6f)) continue;
+            const float2 v = g_point(r, cs, ci, k, h);
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+    return make_float2(acc.x * (1.0f / 256.0f), acc.y * (1.0f / 256.0f));
+}
+
+// MLP in BF16 (weights pre-rounded in LDS, activations rounded per layer).
+__device__ float2 mlp_bf16(const float *w, const float *f) {
+    const float *W1 = w, *b1 = w + kOffB1, *W2 = w + kOffW2, *b2 = w + kOffB2, *W3 = w + kOffW3, *b3 = w + kOffB3;
+    float h1[64];
+    for (int o = 0; o < 64; o++) {
+        float a = b1[o];
+        for (int q = 0; q < kMlpIn; q++) a = fmaf(W1[o * kMlpIn + q], bf16_round(f[q]), a);
+        h1[o] = bf16_round(tanhf(a));
+    }
+    float o0 = b3[0], o1 = b3[1];
+    for (int o = 0; o < 64; o++) {
+        float a = b2[o];
+        for (int q = 0; q < 64; q++) a = fmaf(W2[o * 64 + q], h1[q], a);
+        const float h = bf16_round(tanhf(a));
+        o0 = fmaf(W3[o], h, o0);
+        o1 = fmaf(W3[64 + o], h, o1);
+    }
+    return make_float2(o0, o1);
+}
+
+template <int NEAR>
+__device__ float2 near_value(float3 rv, float3 nr, float d, float k, const float *w) {
+    const float r = sqrtf(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
+    if (NEAR == 1) return quad16(rv, nr, d, k);
+    const float lam = 2.0f * 3.14159265358979f / k;
+    const float cs = rv.z / r, ci = -(rv.x * nr.x + rv.y * nr.y + rv.z * nr.z) / r;
+    const float2 K = g_point(r, cs, ci, k, d);
+    float f[kMlpIn] = {r / lam - 1.0f, (d / lam - 0.19f) * 14.0f, rv.x / r, rv.y / r, 2.0f * cs - 1.0f,
+                       nr.x, nr.y, nr.z};
+    const float2 t = mlp_bf16(w, f);
+    const float M = 0.9f * (k * kInv4PiT / r + 1.0f / (r * r));
+    float s, c;
+    sincosf(k * r, &s, &c);
+    // K + M t e^{ikr}
+    return make_float2(K.x + M * (t.x * c - t.y * s), K.y + M * (t.x * s + t.y * c));
+}
+
+// ---------------------------------------------------------------------------
 // S build: one lane = one row, 4 emitters of one K step.
-template <bool ATT>
+template <bool ATT, int NEAR>
 __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
                                                  uint4 *__restrict__ A, int ksPerWave) {
+    __shared__ float sw[NEAR == 2 ? kMlpParams : 1];
+    if (NEAR == 2) {
+        for (int q = threadIdx.x; q < kMlpParams; q += blockDim.x) sw[q] = bf16_round(ph.mlp[q]);
+        __syncthreads();
+    }
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int nKsChunks = (g.nKs + ksPerWave - 1) / ksPerWave;
@@ -76,7 +164,19 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 float de = vx * en.x + vy * en.y + vz * en.z;
                 float dr = vx * rn.x + vy * rn.y + vz * rn.z;
                 float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
-                if (r < ph.deltaReg) {
+                if (NEAR != 0 && r < ph.nearRadius) {
+                    // emitter frame of the emitter's facet
+                    const int fe = __float_as_int(en.w);
+                    const float4 xa = ph.frames[2 * fe], ya = ph.frames[2 * fe + 1];
+                    const float3 rv = make_float3(vx * xa.x + vy * xa.y + vz * xa.z, vx * ya.x + vy * ya.y + vz * ya.z,
+                                                  vx * en.x + vy * en.y + vz * en.z);
+                    const float3 nr = make_float3(rn.x * xa.x + rn.y * xa.y + rn.z * xa.z,
+                                                  rn.x * ya.x + rn.y * ya.y + rn.z * ya.z,
+                                                  rn.x * en.x + rn.y * en.y + rn.z * en.z);
+                    const float2 v = near_value<NEAR>(rv, nr, ph.deltaReg, ph.kr, sw);
+                    re = v.x * ph.scale;
+                    im = v.y * ph.scale;
+                } else if (r < ph.deltaReg) {
                     re = wgt * (-ph.alpha * kInv4PiT);
                     im = wgt * (ph.kr * kInv4PiT);
                 } else {
@@ -265,8 +365,14 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
     const long waves = (long)g.nRt * ((g.nKs + ksPerWave - 1) / ksPerWave);
     const long blocks = (waves + 3) / 4;
     if (blocks <= 0) return 0;
-    if (ph.alpha != 0.0f) k_build_s<true><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    else k_build_s<false><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    if (ph.nearMode != 0 && ph.alpha != 0.0f) {
+        set_error("near-field correction is defined for attenuation-free media only");
+        return -2;
+    }
+    if (ph.nearMode == 1) k_build_s<false, 1><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    else if (ph.nearMode == 2) k_build_s<false, 2><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    else if (ph.alpha != 0.0f) k_build_s<true, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    else k_build_s<false, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
     SR_HIP(hipGetLastError());
     return 0;
 }

@@ -296,3 +296,102 @@ KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8})
 def set_transfer_mode(mode):
     """0: plain FP32 gathers; 1: FP16 MFMA transfer matrix (FP32 accumulation)."""
     _check(cpp_lib.sr_set_transfer_mode(int(mode)), "set_transfer_mode")
+
+
+_sig("sr_read_stl", [ctypes.c_char_p, _fp, _l], _l)
+_sig("sr_write_stl", [ctypes.c_char_p, _fp, _i])
+_sig("sr_load_stl", [ctypes.c_char_p, _i, _f])
+_sig("sr_colour_map", [_dp, _l, _f, _f, _i, _fp])
+_sig("sr_surface_rgba", [_i, _i, _i, _fp, _l])
+
+
+def read_stl(path):
+    """(N, 3, 3) float32 triangles of a binary or ASCII STL file."""
+    n = _check(cpp_lib.sr_read_stl(str(path).encode(), None, 0), "read_stl")
+    out = np.zeros((max(n, 1), 9), np.float32)
+    _check(cpp_lib.sr_read_stl(str(path).encode(), _ptr(out), n), "read_stl")
+    return out[:n].reshape(-1, 3, 3)
+
+
+def write_stl(path, tris):
+    v = _f32(tris).reshape(-1, 9)
+    _check(cpp_lib.sr_write_stl(str(path).encode(), _ptr(v), len(v)), "write_stl")
+
+
+def load_stl(path, object_type, resolution):
+    _check(cpp_lib.sr_load_stl(str(path).encode(), int(object_type), resolution), "load_stl")
+
+
+def colour_map(p, min_db=-90.0, max_db=-20.0, phase=False):
+    """RGBA (N, 4) of complex pressures, as the reference's surface render colours them."""
+    p = np.ascontiguousarray(np.stack([np.real(p), np.imag(p)], -1).astype(np.float64).reshape(-1, 2))
+    out = np.zeros((len(p), 4), np.float32)
+    _check(cpp_lib.sr_colour_map(_ptr(p, _dp), len(p), min_db, max_db, 1 if phase else 0, _ptr(out)), "colour_map")
+    return out
+
+
+def surface_rgba(object_type, index, plane=0):
+    n = cell_count(object_type, index)
+    out = np.zeros((max(n, 1), 4), np.float32)
+    _check(cpp_lib.sr_surface_rgba(object_type, index, plane, _ptr(out), n), "surface_rgba")
+    return out[:n]
+
+
+_sig("sr_set_near_mode", [_i, _f])
+_sig("sr_nf_init", [ctypes.c_uint])
+_sig("sr_nf_train", [_i, _i, _i, _f, ctypes.c_uint, _fp])
+_sig("sr_nf_get_params", [_fp, _i])
+_sig("sr_nf_set_params", [_fp, _i])
+_sig("sr_nf_samples", [ctypes.c_uint, _i, _fp, _fp])
+_sig("sr_nf_forward", [_fp, _i, _fp])
+_sig("sr_nf_gradient", [_fp, _fp, _fp, _i, _fp])
+
+NF_PARAMS = 4866
+NF_IN = 8
+NEAR_POINT, NEAR_QUADRATURE, NEAR_LEARNED = 0, 1, 2
+
+
+def set_near_mode(mode, radius_wavelengths=2.0):
+    _check(cpp_lib.sr_set_near_mode(int(mode), float(radius_wavelengths)), "set_near_mode")
+
+
+def nf_init(seed=1):
+    _check(cpp_lib.sr_nf_init(seed), "nf_init")
+
+
+def nf_train(nsamples=262144, steps=3000, batch=4096, lr=3e-3, seed=7):
+    rep = np.zeros(2, np.float32)
+    _check(cpp_lib.sr_nf_train(nsamples, steps, batch, lr, seed, _ptr(rep)), "nf_train")
+    return float(rep[0]), float(rep[1])
+
+
+def nf_params():
+    out = np.zeros(NF_PARAMS, np.float32)
+    _check(cpp_lib.sr_nf_get_params(_ptr(out), NF_PARAMS), "nf_get_params")
+    return out
+
+
+def nf_set_params(p):
+    p = _f32(p).ravel()
+    _check(cpp_lib.sr_nf_set_params(_ptr(p), len(p)), "nf_set_params")
+
+
+def nf_samples(seed, n):
+    X = np.zeros((n, NF_IN), np.float32)
+    Y = np.zeros((n, 2), np.float32)
+    _check(cpp_lib.sr_nf_samples(seed, n, _ptr(X), _ptr(Y)), "nf_samples")
+    return X, Y
+
+
+def nf_forward(X):
+    X = _f32(X).reshape(-1, NF_IN)
+    out = np.zeros((len(X), 2), np.float32)
+    _check(cpp_lib.sr_nf_forward(_ptr(X), len(X), _ptr(out)), "nf_forward")
+    return out
+
+
+def nf_gradient(params, X, Y):
+    p, X, Y = _f32(params).ravel(), _f32(X).reshape(-1, NF_IN), _f32(Y).reshape(-1, 2)
+    g = np.zeros(NF_PARAMS, np.float32)
+    _check(cpp_lib.sr_nf_gradient(_ptr(p), _ptr(X), _ptr(Y), len(X), _ptr(g)), "nf_gradient")
+    return g
