@@ -18,6 +18,9 @@ CPU baseline runs. `value` is the Gpairs/s of the fastest form whose rel-L2
 is <= 1e-3.
 
 Prints one JSON line. --gpus > 1 is not supported (one GPU per process).
+The engine is HIP code behind a C-ABI; timing brackets are device-wide
+synchronisations on the engine's own HIP runtime (sr_device_synchronize).
+PyTorch is not loaded: its wheel carries a second copy of the HIP runtime.
 """
 import argparse
 import json
@@ -34,6 +37,11 @@ FP32_PEAK_TFLOPS = 157.3
 FP16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-3
+
+# HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (FETCH_SIZE x 2 + WRITE_SIZE,
+# the factor 2 from a 1 GiB copy calibration), per workload; the figures and commands are in
+# profiles/transfer_gemm_hbm_bytes.md.
+PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 879e6}
 
 
 def _configs():
@@ -59,12 +67,11 @@ def cpu_baseline(scene, sample_planes):
 
 
 def time_form(scene, mode, steps, warmup):
-    import torch
     import runner
     import sonar_api as api
 
     runner.load_scene(scene, transfer_mode=mode)
-    torch.cuda.synchronize()
+    api.device_synchronize()
 
     def step():
         api.reset_pressures()
@@ -74,11 +81,11 @@ def time_form(scene, mode, steps, warmup):
     for _ in range(warmup):
         step()
     api.profile(True)
-    torch.cuda.synchronize()
+    api.device_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
-    torch.cuda.synchronize()
+    api.device_synchronize()
     dt = time.perf_counter() - t0
     stats = {k: api.kernel_stats(k) for k in api.KERNEL_FAMILIES}
     api.profile(False)
@@ -87,7 +94,7 @@ def time_form(scene, mode, steps, warmup):
     return dt, stats, res
 
 
-def roofline(stats, mode):
+def roofline(stats, mode, workload):
     if mode == 0:
         st = stats["f2f"]
         ach = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
@@ -98,8 +105,11 @@ def roofline(stats, mode):
     st = stats["gemm"]
     ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9 if st["ms"] > 0 else 0.0
     tf = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
+    traffic = PMC_TRAFFIC.get((workload, "k_transfer_gemm"))
+    algo = st["bytes"] / max(1, st["launches"])
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(algo),
             "kernel": "k_transfer_gemm (FP16 MFMA 32x32x16, streams the FP16 transfer matrix once per bounce)",
             "mfma_tflops": round(tf, 2), "mfma_frac": round(tf / FP16_PEAK_TFLOPS, 4),
             "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4), "launches": st["launches"]}
@@ -114,7 +124,18 @@ def main():
     ap.add_argument("--forms", default="plain,mixed", help="comma list of plain, mixed")
     ap.add_argument("--cpu-sample", type=int, default=1, help="incidences the CPU baseline runs")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (the PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs one GPU per process; multi-GPU runs are not part of this build",
               file=sys.stderr)
@@ -157,7 +178,7 @@ def main():
             "gpairs_per_s": round(pairs * args.steps / r["dt"] / 1e9, 3), "ms_per_scene": round(ms, 4),
             "rel_l2_vs_fp64": r.get("err"),
             "kernel_ms_per_scene": {k: round(v["ms"] / args.steps, 4) for k, v in r["stats"].items() if v["launches"]},
-            "roofline": roofline(r["stats"], r["mode"]),
+            "roofline": roofline(r["stats"], r["mode"], scene["name"]),
         }
     valid = [n for n in per_form if per_form[n]["rel_l2_vs_fp64"] is None or per_form[n]["rel_l2_vs_fp64"] <= TOL]
     primary = min(valid or list(per_form), key=lambda n: per_form[n]["ms_per_scene"])

@@ -156,14 +156,14 @@ int sr_transfer_mode(void);
 /* Near-field terms of the transfer matrix (mode 1 transfer only; pairs of
  * fragments closer than radiusWavelengths): 0 the reference's point rule,
  * 1 16x16 sub-fragment quadrature of the same kernel over the emitter's
- * cell, 2 the learned correction (an 8-64-64-2 MLP, BF16 inference). */
+ * cell, 2 the learned correction (a 9-64-64-2 MLP, BF16 inference). */
 int sr_set_near_mode(int mode, float radiusWavelengths);
 
 /* The learned correction's network: trained on the GPU with the library's own
  * forward/backward/Adam kernels on pairs labelled by the quadrature.
  * report2: relative RMS error on training and held-out pairs. Parameters are
- * 4866 floats (W1[64][8] b1 W2[64][64] b2 W3[2][64] b3). sr_nf_samples
- * returns labelled pairs (8 features, 2 targets each); sr_nf_forward the FP32
+ * 4930 floats (W1[64][9] b1 W2[64][64] b2 W3[2][64] b3). sr_nf_samples
+ * returns labelled pairs (9 features, 2 targets each); sr_nf_forward the FP32
  * network output; sr_nf_gradient the loss gradient of a given parameter set
  * on given pairs (L = sum |out - y|^2 / n). */
 int sr_nf_init(unsigned seed);
@@ -178,7 +178,8 @@ int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, f
  * starts (and clears) the record. Kernel families: 0 point->fragment,
  * 1 visibility words, 2 fragment->fragment, 3 fragment->field object,
  * 4 fragment->field point, 5 split-K reductions, 6 transfer-matrix build,
- * 7 transfer GEMM, 8 incidence-matrix build. Reports summed kernel time,
+ * 7 transfer GEMM, 8 incidence-matrix build, 9 exact recomputation of the
+ * rows a zero input gates in the transfer form. Reports summed kernel time,
  * launches, ordered pairs (x planes), flops (gathers: 37 FP32 flops per
  * pair plus 8 per pair and plane; transfer GEMM: 2 M N K as executed) and
  * algorithmic HBM bytes (transfer GEMM: S, B and the partial sums). */
@@ -195,6 +196,30 @@ int sr_load_stl(const char *path, int objectType, float resolution);
  * (phase) hue (arg p + pi) / 2 pi on an HSV wheel. rgba: 4 floats per value. */
 int sr_colour_map(const double *p2, long n, float minDb, float maxDb, int phase, float *rgba);
 int sr_surface_rgba(int objectType, int index, int plane, float *rgba, long n);
+
+/* Batched independent scenes: each its own mesh (target), fragment edge,
+ * sound speed, frequency, attenuation, sources (summed) and field points.
+ * sr_batch_prepare fragments and uploads them; sr_batch_run projects all of
+ * them (sources -> fragments, n bounces, fragments -> field points) in one
+ * sequence of persistent multi-scene launches, replayed as one hipGraph when
+ * useGraph is 1. sr_batch_autotune times candidate tilings (persistent grid,
+ * emitters per visibility item, emitters per gather item) and keeps the
+ * fastest; report gets 5 floats per candidate (ms, vis grid, gather grid,
+ * emitters per vis item, emitters per gather item). Tune: the same 4 ints. */
+int sr_batch_clear(void);
+int sr_batch_add_scene(const float *tris9, int ntri, float delta, float cp, float frequency, float attenuation,
+                       const float *src, int nsrc, const float *pts, int npts);
+int sr_batch_set_bounces(int n);
+int sr_batch_prepare(void);
+int sr_batch_run(int useGraph, float *ms);
+int sr_batch_autotune(float *report, int nmax);
+int sr_batch_field_points(int scene, sr_dcomplex *out, int n);
+double sr_batch_pairs(void);
+int sr_batch_get_tune(int *out4);
+int sr_batch_set_tune(const int *in4);
+
+/* hipDeviceSynchronize on the library's runtime. */
+int sr_device_synchronize(void);
 
 /* Wall time of the last projection entry point, milliseconds. */
 float sr_last_call_ms(void);

@@ -12,6 +12,7 @@
 #include "model_ops.h"
 #include "hostio.h"
 #include "nearfield.h"
+#include "batch.h"
 
 namespace {
 
@@ -119,6 +120,12 @@ int sr_set_source_amplitude(int s, float re, float im) {
     return guarded("set_source_amplitude", [&] { return sr::model_set_source_amplitude(gModel, s, re, im); });
 }
 int sr_num_planes(void) { return gModel.nplanes; }
+int sr_device_synchronize(void) {
+    return guarded("device_synchronize", [&] {
+        SR_HIP(hipDeviceSynchronize());
+        return 0;
+    });
+}
 int sr_teardown(void) {
     return guarded("TearDownCuda", [&] { sr::model_teardown(gModel); return 0; });
 }
@@ -290,6 +297,37 @@ int sr_surface_rgba(int type, int idx, int plane, float *rgba, long n) {
         return 0;
     });
 }
+
+int sr_batch_clear(void) { return guarded("batch_clear", [&] { return sr::batch_clear(); }); }
+int sr_batch_add_scene(const float *tris, int ntri, float delta, float cp, float f, float atten, const float *src,
+                       int nsrc, const float *pts, int npts) {
+    int idx = -1;
+    int rc = guarded("batch_add_scene", [&] {
+        idx = sr::batch_add_scene(tris, ntri, delta, cp, f, atten, src, nsrc, pts, npts);
+        return idx < 0 ? idx : 0;
+    });
+    return rc < 0 ? rc : idx;
+}
+int sr_batch_set_bounces(int n) { return guarded("batch_set_bounces", [&] { return sr::batch_set_bounces(n); }); }
+int sr_batch_prepare(void) { return guarded("batch_prepare", [&] { return sr::batch_prepare(); }); }
+int sr_batch_run(int useGraph, float *ms) { return guarded("batch_run", [&] { return sr::batch_run(useGraph, ms); }); }
+int sr_batch_autotune(float *report, int n) {
+    int k = 0;
+    int rc = guarded("batch_autotune", [&] {
+        k = sr::batch_autotune(report, n);
+        return k < 0 ? k : 0;
+    });
+    return rc < 0 ? rc : k;
+}
+int sr_batch_field_points(int scene, sr_dcomplex *out, int n) {
+    return guarded("batch_field_points", [&] {
+        int rc = sr::batch_field_points(scene, (double *)out, n);
+        return rc < 0 ? rc : 0;
+    });
+}
+double sr_batch_pairs(void) { return sr::batch_pairs(); }
+int sr_batch_get_tune(int *out4) { return sr::batch_get_tune(out4); }
+int sr_batch_set_tune(const int *in4) { return guarded("batch_set_tune", [&] { return sr::batch_set_tune(in4); }); }
 
 // ---- the reference's void entry points ----
 void set_initial_conditions(float cp, float f, float a, float d) { report("set_initial_conditions", sr_set_initial_conditions(cp, f, a, d)); }

@@ -5,9 +5,9 @@
 
 namespace sr {
 
-// 8 -> 64 -> 64 -> 2 MLP, parameters packed W1[64][8] b1 W2[64][64] b2 W3[2][64] b3.
-constexpr int kMlpIn = 8;
-constexpr int kMlpParams = 64 * kMlpIn + 64 + 64 * 64 + 64 + 2 * 64 + 2;  // 4866
+// 9 -> 64 -> 64 -> 2 MLP, parameters packed W1[64][9] b1 W2[64][64] b2 W3[2][64] b3.
+constexpr int kMlpIn = 9;
+constexpr int kMlpParams = 64 * kMlpIn + 64 + 64 * 64 + 64 + 2 * 64 + 2;  // 4930
 constexpr int kOffB1 = 64 * kMlpIn, kOffW2 = kOffB1 + 64, kOffB2 = kOffW2 + 4096, kOffW3 = kOffB2 + 64,
               kOffB3 = kOffW3 + 128;
 

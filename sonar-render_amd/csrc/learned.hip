@@ -7,12 +7,15 @@
 // square centred on the emitter point (cosine culls and the r < delta/16
 // regularisation applied at every sub-point).
 //
-// A small MLP (8 -> 64 -> 64 -> 2, tanh) learns that correction from pairs
+// A small MLP (9 -> 64 -> 64 -> 2, tanh) learns that correction from pairs
 // labelled by the quadrature, in units of the wavelength, so one network
 // serves every frequency. In the emitter's frame (x = xAxis, y = yAxis,
 // z = normal), with u the unit vector emitter -> receiver:
-//   features: r/lambda - 1, 14 (delta/lambda - 0.19), u.x, u.y, 2 u.z - 1, n_r.x, n_r.y, n_r.z
-//   target:   t = (Q - K_point) e^{-ikr} / M,  M = A_e 0.9 (k/(4 pi r) + 1/r^2)
+//   features: r/lambda - 1, 14 (delta/lambda - 0.19), u.x, u.y, 2 u.z - 1,
+//             n_r.x, n_r.y, n_r.z, ln(r/delta)/2
+//   target:   t = (Q - K_true) e^{-ikr} / M,  M = 0.9 (k/(4 pi r) + 1/r^2),
+// with K_true = cos_s cos_i 0.9 G(r) (per unit area), for delta <= r < 2 lambda.
+// Pairs closer than delta keep the quadrature as fallback.
 // Training (forward, backward, weight gradients, Adam) and inference are the
 // kernels below; inference rounds weights and activations to BF16.
 
@@ -85,6 +88,7 @@ __device__ inline void features(float r, float lam, float d, float3 u, float3 nr
     f[5] = nr.x;
     f[6] = nr.y;
     f[7] = nr.z;
+    f[8] = 0.5f * logf(r / d);
 }
 
 // Training pairs in the emitter's frame, lengths in wavelengths (k = 2 pi).
@@ -93,9 +97,11 @@ __global__ void k_nf_samples(uint32_t seed, int n, float dMin, float dMax, float
     if (i >= n) return;
     const float k = 2.0f * kPi, lam = 1.0f;
     const float d = dMin + (dMax - dMin) * urand(seed, i, 0);
-    // distance: more samples close in, where the correction is large
+    // distance in [delta, rMax], denser close in where the correction is large;
+    // closer pairs are left to the quadrature (the point rule's r < delta
+    // regularisation makes them ill-conditioned for a correction)
     const float u0 = urand(seed, i, 1);
-    const float r = fmaxf(0.02f, rMax * u0 * u0);
+    const float r = d + (rMax - d) * u0 * u0;
     float3 uv, nr;
     float cs, ci;
     for (int tr = 0;; tr++) {
@@ -116,7 +122,7 @@ __global__ void k_nf_samples(uint32_t seed, int n, float dMin, float dMax, float
     features(r, lam, d, uv, nr, f);
     for (int q = 0; q < kMlpIn; q++) X[(size_t)i * kMlpIn + q] = f[q];
     const float2 Q = near_quad16(rv, nr, false, d, k);
-    const float2 K = point_kernel(r, cs, ci, k, d);
+    const float2 K = point_kernel(r, cs, ci, k, 0.0f);
     const float M = 0.9f * (k * kInv4PiL / r + 1.0f / (r * r));
     float s, c;
     sincosf(k * r, &s, &c);

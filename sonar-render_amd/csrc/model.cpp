@@ -134,8 +134,8 @@ int upload_object(Model &m, Object &o) {
 
 void free_transfer(Object &o) {
     auto &t = o.tc;
-    dev_free(t.A); dev_free(t.B); dev_free(t.facet); dev_free(t.rtFmin); dev_free(t.rtFmax);
-    dev_free(t.ksFmin); dev_free(t.ksFmax); dev_free(t.colScale); dev_free(t.colInv);
+    dev_free(t.A); dev_free(t.B); dev_free(t.facet); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
+    dev_free(t.gatedList); dev_free(t.gatedCount);
     t = Object::Transfer{};
 }
 
@@ -314,12 +314,9 @@ int model_set_source_amplitude(Model &m, int i, float re, float im) {
     return 0;
 }
 
-// Visibility words for one (emitter set, receiver set), built on first use.
-int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out) {
-    auto key = std::make_tuple(kind, ta, ia, tb, ib);
-    auto it = m.masks.find(key);
-    if (it != m.masks.end() && it->second.valid) { *out = it->second.p; return 0; }
-    MaskTask tk{};
+// Describes the visibility words of one (emitter set, receiver set); no launch.
+int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, size_t &words) {
+    tk = MaskTask{};
     Object *A = (kind == MK_P2F) ? nullptr : object_at(m, ta, ia);
     Object *B = (kind == MK_P2F || kind >= MK_NORESP) ? object_at(m, tb, ib) : nullptr;
     const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
@@ -340,6 +337,7 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
         tk.em = PointSet{A->actP, A->actN, A->nact};
         tk.rc = tk.em;
         tk.origin = ORIGIN_LOWER_FACET; tk.useNormalEm = 1; tk.useNormalRc = 1; tk.sameObject = 1;
+        tk.upperOnly = 1;  // the set is symmetric: half the rays, the rest by bit transposition
         tk.tmin = m.vc.tminFrag;
         break;
     case MK_NORESP:
@@ -365,7 +363,19 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
         return -2;
     }
     const size_t nrb = (size_t)(tk.rc.count + 63) / 64;
-    const size_t words = std::max<size_t>(1, nrb * (size_t)tk.em.count);
+    words = std::max<size_t>(1, nrb * (size_t)tk.em.count);
+    tk.errFlag = m.errFlag;
+    return 0;
+}
+
+// Visibility words for one (emitter set, receiver set), built on first use.
+int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out) {
+    auto key = std::make_tuple(kind, ta, ia, tb, ib);
+    auto it = m.masks.find(key);
+    if (it != m.masks.end() && it->second.valid) { *out = it->second.p; return 0; }
+    MaskTask tk{};
+    size_t words = 0;
+    if (mask_task(m, kind, ta, ia, tb, ib, tk, words)) return -1;
     Model::MaskBuf &mb = m.masks[key];
     if (mb.words != words) {
         dev_free(mb.p);
@@ -375,7 +385,6 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
     }
     uint64_t *mk = mb.p;
     tk.mask = mk;
-    tk.errFlag = m.errFlag;
     {
         ProfScope ps(m, PK_VIS, (double)tk.em.count * tk.rc.count, 0.0);
         if (vis_mask_launch(tk, bvh_of(m), m.vc, m.stream)) return -1;
@@ -422,7 +431,8 @@ int model_project_src(Model &m, bool fieldObjects) {
 static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, ReduceTask rt, int nplanes,
                          long evalStride, long gvalStride, long outStride, long pcellStride, double pairs) {
     const int nrb = (t.nr + 63) / 64;
-    const int minPer = (kind == KIND_F2P) ? 128 : 256;
+    // few receivers (field points): spread the emitters thinly across many workgroups
+    const int minPer = (t.nr <= 256) ? 16 : 256;
     t.nsplit = choose_splits(nrb, t.ne, minPer);
     t.emPerSplit = (t.ne + t.nsplit - 1) / t.nsplit;
     t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
@@ -476,32 +486,36 @@ static int ensure_transfer(Model &m, int i) {
     uint64_t *mk;
     if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
     const int n = o.nact;
-    const int nKs = (n + 7) / 8;
+    const int nKs = ((n + 7) / 8 + 7) / 8 * 8;  // K steps of 8 emitters, whole chunks of 8 steps
     const int nRt = ((n + 127) / 128) * 4;
     if (!t.A || t.nKs != nKs || t.nRt != nRt) {
         free_transfer(o);
+        const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) || dev_alloc(&t.facet, n) ||
-            dev_alloc(&t.rtFmin, nRt) || dev_alloc(&t.rtFmax, nRt) || dev_alloc(&t.ksFmin, nKs) ||
-            dev_alloc(&t.ksFmax, nKs) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64))
+            dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
+            dev_alloc(&t.gatedList, n) || dev_alloc(&t.gatedCount, 1))
             return -1;
         t.nKs = nKs;
         t.nRt = nRt;
-        std::vector<int> rmin(nRt, 0), rmax(nRt, 0), kmin(nKs), kmax(nKs);
-        for (int r = 0; r < nRt; r++) {
-            const int a = std::min(n - 1, r * 32), b = std::min(n - 1, r * 32 + 31);
-            rmin[r] = o.actFacetHost[a];
-            rmax[r] = o.actFacetHost[b];
+        // A group of 128 rows on one facet f never reads the K chunks (64 emitters)
+        // that lie wholly inside f's emitter range: same-facet pairs are zero.
+        const auto &fa = o.actFacetHost;
+        std::vector<int2> skip(nWg, make_int2(0, 0));
+        double steps = 0;
+        for (int w = 0; w < nWg; w++) {
+            const int r0 = std::min(n - 1, w * 128), r1 = std::min(n - 1, w * 128 + 127);
+            if (fa[r0] == fa[r1]) {
+                const int f = fa[r0];
+                const int fs = (int)(std::lower_bound(fa.begin(), fa.end(), f) - fa.begin());
+                const int fe = (int)(std::upper_bound(fa.begin(), fa.end(), f) - fa.begin());
+                const int cLo = ((fs + 7) / 8 + 7) / 8, cHi = (fe / 8) / 8;  // chunks wholly inside [fs, fe)
+                if (cHi > cLo) skip[w] = make_int2(cLo * 8, cHi * 8);
+            }
+            steps += 4.0 * (nKs - (skip[w].y - skip[w].x));
         }
-        for (int k = 0; k < nKs; k++) {
-            const int a = k * 8, b = std::min(n - 1, k * 8 + 7);
-            kmin[k] = o.actFacetHost[a];
-            kmax[k] = o.actFacetHost[b];
-        }
-        SR_HIP(hipMemcpy(t.facet, o.actFacetHost.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        SR_HIP(hipMemcpy(t.rtFmin, rmin.data(), nRt * sizeof(int), hipMemcpyHostToDevice));
-        SR_HIP(hipMemcpy(t.rtFmax, rmax.data(), nRt * sizeof(int), hipMemcpyHostToDevice));
-        SR_HIP(hipMemcpy(t.ksFmin, kmin.data(), nKs * sizeof(int), hipMemcpyHostToDevice));
-        SR_HIP(hipMemcpy(t.ksFmax, kmax.data(), nKs * sizeof(int), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(t.facet, fa.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
+        t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
     }
     // One FP16 scale for S: |cos_s cos_i| <= 1, |G| <= |k|/(4 pi delta) e^{max(0,-alpha) D} + 1/delta^2
     // for r >= delta and |k|/(4 pi) below it.
@@ -528,22 +542,30 @@ static int ensure_transfer(Model &m, int i) {
     return 0;
 }
 
-// One bounce of target i in the transfer form: P_out = L x + [P_in != 0] (U x), P += P_out.
+// One bounce of target i in the transfer form: P_out = S x for every row, then
+// the rows whose own P_in is zero in some plane are recomputed exactly in FP32
+// with the lower-facet emitters only (the zero-input gate), P += P_out.
 static int bounce_mixed(Model &m, int i) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     const int n = o.nact;
-    int *flag = m.errFlag + 1;
-    SR_HIP(hipMemsetAsync(flag, 0, sizeof(int), m.stream));
-    if (any_zero_launch(o.Pin, (long)n * m.nplanes, flag, m.stream)) return -1;
-    int gated = 0;
-    SR_HIP(hipMemcpyAsync(&gated, flag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
-    SR_HIP(hipStreamSynchronize(m.stream));
-    const int wgRows = (t.nRt + 3) / 4;
+    const int wgRows = t.nRt / 4;
+    // split K (in whole chunks of 8 steps) so that the grid holds ~1024 workgroups
     int nsplit = std::max(1, std::min(1024 / std::max(1, wgRows), t.nKs / 64));
-    const int ksPer = (t.nKs + nsplit - 1) / nsplit;
+    const int ksPer = ((t.nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
     nsplit = (t.nKs + ksPer - 1) / ksPer;
-    if (ensure_scratch(m, (size_t)nsplit * 64 * n)) return -1;
+    // rows with a zero input in some plane (the gate), counted before the product
+    if (counters_zero_launch(t.gatedCount, 1, m.stream)) return -1;
+    if (transfer_gated_rows(o.Pin, n, m.nplanes, t.gatedList, t.gatedCount, m.stream)) return -1;
+    int gated = 0;
+    SR_HIP(hipMemcpyAsync(&gated, t.gatedCount, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    SR_HIP(hipStreamSynchronize(m.stream));
+    // gated-row recomputation: enough waves to fill the chip, emitter ranges of >= 64
+    const int fixWaves = ((gated + 63) / 64) * ((m.nplanes + 15) / 16);
+    int fixSplit = std::max(1, std::min((2048 + fixWaves - 1) / std::max(1, fixWaves), n / 64));
+    const int fixPer = (((n + fixSplit - 1) / fixSplit) + 7) / 8 * 8;
+    fixSplit = (n + fixPer - 1) / fixPer;
+    if (ensure_scratch(m, std::max((size_t)nsplit * 64 * n, (size_t)fixSplit * m.nplanes * gated))) return -1;
     for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
         const int np = std::min(64, m.nplanes - g0);
         const float2 *pin = o.Pin + (size_t)g0 * n;
@@ -552,28 +574,24 @@ static int bounce_mixed(Model &m, int i) {
             if (transfer_col_scale(o.actP, pin, n, n, np, t.colScale, t.colInv, t.sS, m.stream)) return -1;
             if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, t.colScale, t.B, m.stream)) return -1;
         }
-        TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.facet, t.facet, t.rtFmin, t.rtFmax,
-                        t.ksFmin, t.ksFmax, m.scratchL, m.scratchU};
+        TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL};
         {
             const double pairs = ((double)n * n - o.sameFacetPairs) * np;
             // flops of the GEMM as executed: 2 * rows * 128 columns * 2 * K (padded)
             const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
             // algorithmic HBM bytes: S read once, B read once, partial sums written
-            const double by = (double)t.nRt * t.nKs * 64 * 16 + (double)t.nKs * 256 * 16 +
-                              (double)nsplit * np * n * 8 * (gated ? 2 : 1);
+            const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + (double)nsplit * np * n * 8;
             ProfScope ps(m, PK_GEMM, pairs, fl, by);
-            if (transfer_gemm(gm, gated != 0, m.stream)) return -1;
+            if (transfer_gemm(gm, m.stream)) return -1;
         }
         ReduceTask r{};
         r.partL = m.scratchL;
-        r.partU = gated ? m.scratchU : nullptr;
+        r.partU = nullptr;
         r.nsplit = nsplit;
         r.nbStride = 64;
         r.nrhs = np;
         r.nr = n;
-        r.gval = pin;
-        r.gvalStride = n;
-        r.gateMode = gated ? GATE_UPPER : GATE_NONE;
+        r.gateMode = GATE_NONE;
         r.out = o.Pout + (size_t)g0 * n;
         r.outStride = n;
         r.pcell = o.P + (size_t)g0 * o.ncell;
@@ -583,7 +601,15 @@ static int bounce_mixed(Model &m, int i) {
         ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
         if (reduce_launch(r, m.stream)) return -1;
     }
-    return 0;
+    if (!gated) return 0;
+    GatedFix f{};
+    f.list = t.gatedList; f.count = gated; f.facet = t.facet; f.n = n; f.p = o.actP; f.nrm = o.actN;
+    f.Pin = o.Pin; f.stride = n; f.nplanes = m.nplanes;
+    f.A = t.A; f.nKs = t.nKs; f.mask = nullptr; f.valueScale = 1.0f / t.sS;
+    f.emPerSplit = fixPer; f.nsplit = fixSplit; f.part = m.scratchL;
+    f.Pout = o.Pout; f.P = o.P; f.pStride = o.ncell; f.rowCell = o.actCell;
+    ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
+    return gated_fix_launch(f, make_phys(m, o.delta, 0.9f), m.stream);
 }
 
 // FACET_SELF bounces (ModelCuda.cpp:323-356, FacetCuda.cpp:124-137, FacetCuda.cu:87-116).

@@ -40,9 +40,12 @@ struct Object {
     struct Transfer {
         uint4 *A = nullptr, *B = nullptr;
         int nKs = 0, nRt = 0;
-        int *facet = nullptr, *rtFmin = nullptr, *rtFmax = nullptr, *ksFmin = nullptr, *ksFmax = nullptr;
+        int *facet = nullptr;      // facet per active index
+        int2 *wgSkip = nullptr;    // per 128-row group: zero K-step range (its own single facet)
+        int *gatedList = nullptr, *gatedCount = nullptr;  // rows with a zero input in some plane
         float *colScale = nullptr, *colInv = nullptr;
         float sS = 1.f;
+        double aBytesRead = 0;  // bytes of S one GEMM pass reads (same-facet steps are skipped)
         bool valid = false;
     } tc;
     // pressure planes (nplanes each)
@@ -89,7 +92,7 @@ struct Model {
 // Kernel families reported by the profiler.
 enum ProfKind {
     PK_P2F = 0, PK_VIS = 1, PK_F2F = 2, PK_F2F_FIELD = 3, PK_F2P = 4, PK_REDUCE = 5,
-    PK_BUILD_S = 6, PK_GEMM = 7, PK_BUILD_B = 8, PK_COUNT = 9
+    PK_BUILD_S = 6, PK_GEMM = 7, PK_BUILD_B = 8, PK_GATED_FIX = 9, PK_COUNT = 10
 };
 
 }  // namespace sr

@@ -35,4 +35,7 @@ void model_profile_reset(Model &m);
 int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
 void model_teardown(Model &m);
 
+// Visibility-word task of one pair set (kinds as sr_get_mask), without launching it.
+int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, size_t &words);
+
 }  // namespace sr

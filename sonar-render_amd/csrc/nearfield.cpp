@@ -68,7 +68,7 @@ int nf_init(uint32_t seed) {
 }
 
 int nf_set_params(const float *in, int n) {
-    if (n != kMlpParams) { set_error("near-field MLP has 4866 parameters"); return -2; }
+    if (n != kMlpParams) { set_error("near-field MLP has 4930 parameters"); return -2; }
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
     SR_HIP(hipMemcpy(nf.P, in, n * sizeof(float), hipMemcpyHostToDevice));
@@ -77,7 +77,7 @@ int nf_set_params(const float *in, int n) {
 }
 
 int nf_get_params(float *out, int n) {
-    if (n < kMlpParams) { set_error("buffer shorter than 4866 parameters"); return -2; }
+    if (n < kMlpParams) { set_error("buffer shorter than 4930 parameters"); return -2; }
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
     SR_HIP(hipMemcpy(out, nf.P, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
@@ -171,6 +171,11 @@ int nf_train(int nsamples, int steps, int batch, float lr, uint32_t seed, float 
                     (void *)D2, (void *)D1, (void *)g, (void *)m, (void *)v, (void *)acc})
         dfree(p);
     if (rc) return rc;
+    std::vector<float> chk(kMlpParams);
+    SR_HIP(hipMemcpy(chk.data(), nf.P, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+    for (float x : chk)
+        if (!std::isfinite(x)) { set_error("near-field training diverged (non-finite parameters)"); return -1; }
+    if (!std::isfinite(tr) || !std::isfinite(va)) { set_error("near-field training: non-finite error"); return -1; }
     nf.trained = true;
     nf.trainErr = tr;
     nf.valErr = va;

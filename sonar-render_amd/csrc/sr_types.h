@@ -53,6 +53,7 @@ struct MaskTask {
     int useNormalEm;      // cull with the emitter-side normal
     int useNormalRc;      // cull with the receiver-side normal (0 for field destinations)
     int sameObject;       // SELF: pairs on one facet never interact
+    int upperOnly;        // symmetric set: compute emitters >= 64*rb only (the rest by transposition)
     float tmin;
     uint64_t *mask;       // [ceil(rc.count/64)][em.count]
     int *errFlag;

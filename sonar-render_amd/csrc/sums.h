@@ -93,6 +93,36 @@ int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, in
                  long outStride, hipStream_t st);
 int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st);
 
+// ---- multi-task (many scenes in one launch) ----
+struct PointsReduceTask {
+    const float2 *hi, *lo;
+    int nsplit, nbStride, nrhs, nq;
+    double2 *fld;
+    long fldStride;
+};
+struct PrimeTask {
+    const float2 *P;
+    long pStride;
+    const int *actCell;
+    int nact, nrhs;
+    float2 *Pout;
+    long outStride;
+};
+struct ZeroTask {
+    float *p;   // float count n, zeroed with a kernel (graph capture records no memset nodes)
+    long n;
+};
+
+int gather_multi_launch(const GatherTask *tasks, const Phys *phys, const int4 *items, int nitems, int *counter,
+                        int kind, int nb, bool att, int grid, hipStream_t st);
+// items: (task, first element) for 256-element blocks
+int p2f_multi_launch(const P2FTask *tasks, const Phys *phys, const int2 *items, int nitems, bool att, hipStream_t st);
+int reduce_multi_launch(const ReduceTask *tasks, const int2 *items, int nitems, hipStream_t st);
+int reduce_points_multi_launch(const PointsReduceTask *tasks, const int2 *items, int nitems, hipStream_t st);
+int prime_multi_launch(const PrimeTask *tasks, const int2 *items, int nitems, hipStream_t st);
+int zero_multi_launch(const ZeroTask *tasks, const int2 *items, int nitems, hipStream_t st);
+int counters_zero_launch(int *counters, int n, hipStream_t st);
+
 // Split-K choice: enough waves to occupy every SIMD a few times.
 int choose_splits(int nrb, int ne, int minPerSplit);
 // Plane stride of the partial-sum buffers for a launch of nrhs planes.

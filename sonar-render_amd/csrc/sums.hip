@@ -22,86 +22,12 @@
 #include <algorithm>
 
 #include "sums.h"
+#include "pair_kernel.h"
 #include "launch.h"
 
 namespace sr {
 
-constexpr float kInv4Pi = 0.0795774715459476678f;
 constexpr int kTile = 256;
-
-__device__ inline float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ inline float hw_sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
-__device__ inline float hw_cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
-__device__ inline float hw_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
-
-// Fractional revolutions of r * (hi + lo), error ~1e-7 rev for r * k/(2pi) < 2^22.
-__device__ inline float phase_rev_split(float r, float hi, float lo) {
-    float p = r * hi;
-    float e = fmaf(r, hi, -p);
-    float fp = p - rintf(p);
-    return fp + fmaf(r, lo, e);
-}
-__device__ inline float phase_rev_fast(float r, float hi) {
-    float p = r * hi;
-    return p - rintf(p);
-}
-
-// TwoSum accumulation (Knuth): s + c exactly as hi + lo.
-__device__ inline void two_sum_acc(float &s, float &lo, float c) {
-    float t = s + c;
-    float bp = t - s;
-    lo += (s - (t - bp)) + (c - bp);
-    s = t;
-}
-
-template <int NB>
-struct Acc {
-    float2 v[NB];
-};
-
-// One pair's complex kernel value K (before multiplying by the emitter term).
-template <int KIND, bool ATT>
-__device__ __attribute__((always_inline)) inline void pair_kernel(const Phys &ph, float4 rp, float4 rn, float4 ep, float4 en, bool bit, float &Kre,
-                                   float &Kim) {
-    float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
-    float r2 = vx * vx + vy * vy + vz * vz;
-    float rinv = fast_rsq(r2);
-    float r = r2 * rinv;
-    float de = vx * en.x + vy * en.y + vz * en.z;
-    float w;
-    if (KIND == KIND_F2F) {
-        float dr = vx * rn.x + vy * rn.y + vz * rn.z;
-        w = -(de * dr) * (rinv * rinv);
-    } else {
-        w = de * rinv;
-    }
-    float x = (KIND == KIND_F2P) ? phase_rev_split(r, ph.krevHi, ph.krevLo) : phase_rev_fast(r, ph.krevHi);
-    float c = hw_cos_rev(x), s = hw_sin_rev(x);
-    float a = rinv * kInv4Pi;
-    if (ATT) a *= hw_exp2(-ph.alphaLog2e * r);
-    float Gre, Gim;
-    if (ATT) {
-        Gre = a * fmaf(ph.alpha, c, ph.kr * s) + rinv * rinv;
-        Gim = -a * fmaf(ph.kr, c, -ph.alpha * s);
-    } else {
-        Gre = (a * ph.kr) * s + rinv * rinv;
-        Gim = -(a * ph.kr) * c;
-    }
-    if (KIND != KIND_F2P) {
-        bool reg = r < ph.deltaReg;
-        Gre = reg ? -ph.alpha * kInv4Pi : Gre;
-        Gim = reg ? ph.kr * kInv4Pi : Gim;
-        w *= ph.damp;
-    }
-    Kre = bit ? w * Gre : 0.0f;
-    Kim = bit ? w * Gim : 0.0f;
-}
-
-template <int NB>
-__device__ inline void cmac(float2 &acc, float Kre, float Kim, float2 X) {
-    acc.x = fmaf(Kre, X.x, fmaf(-Kim, X.y, acc.x));
-    acc.y = fmaf(Kre, X.y, fmaf(Kim, X.x, acc.y));
-}
 
 template <int KIND, int NB, bool ATT, bool SPLITG>
 __device__ __attribute__((always_inline)) inline void tile_loop(const Phys &ph, int n, const float4 *s_p, const float4 *s_n, const float2 (*s_x)[kTile],
@@ -141,18 +67,28 @@ __device__ __attribute__((always_inline)) inline void tile_loop(const Phys &ph, 
     }
 }
 
+template <int NB>
+struct GatherLds {
+    float4 p[kTile];
+    float4 n[kTile];
+    float2 x[NB][kTile];
+    uint64_t m[4][kTile];
+};
+
+// One workgroup's share of a gather: receiver blocks 4*wg .. 4*wg+3, emitter range `split`.
+// Called by all 256 threads of the workgroup (it synchronises internally).
 template <int KIND, int NB, bool ATT>
-__global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
-    __shared__ float4 s_p[kTile];
-    __shared__ float4 s_n[kTile];
-    __shared__ float2 s_x[NB][kTile];
-    __shared__ uint64_t s_m[4][kTile];
+__device__ __attribute__((always_inline)) inline void gather_block(const GatherTask &t, const Phys &ph, int wg,
+                                                                   int split, GatherLds<NB> &L) {
+    float4 *s_p = L.p;
+    float4 *s_n = L.n;
+    float2 (*s_x)[kTile] = L.x;
+    uint64_t (*s_m)[kTile] = L.m;
 
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int nrb = (t.nr + 63) >> 6;
-    const int rb = blockIdx.x * 4 + wave;
-    const int split = blockIdx.y;
+    const int rb = wg * 4 + wave;
     const int e0 = split * t.emPerSplit;
     const int e1 = min(t.ne, e0 + t.emPerSplit);
     const int r = rb * 64 + lane;
@@ -198,7 +134,7 @@ __global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
             }
         }
         for (int w = 0; w < 4; w++) {
-            const int rbw = blockIdx.x * 4 + w;
+            const int rbw = wg * 4 + w;
             for (int k = threadIdx.x; k < n; k += 256)
                 s_m[w][k] = (rbw < nrb) ? t.mask[(size_t)rbw * t.ne + base + k] : 0ull;
         }
@@ -228,8 +164,35 @@ __global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
     }
 }
 
-__global__ void k_reduce(ReduceTask t) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <int KIND, int NB, bool ATT>
+__global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
+    __shared__ GatherLds<NB> L;
+    gather_block<KIND, NB, ATT>(t, ph, blockIdx.x, blockIdx.y, L);
+}
+
+// Persistent form: workgroups pull (task, receiver-block group, emitter range)
+// items from a shared counter until the list is exhausted; every task may be a
+// different scene with its own sizes and wavenumber.
+template <int KIND, int NB, bool ATT>
+__global__ __launch_bounds__(256) void k_gather_multi(const GatherTask *__restrict__ tasks,
+                                                      const Phys *__restrict__ phys, const int4 *__restrict__ items,
+                                                      int nitems, int *counter) {
+    __shared__ GatherLds<NB> L;
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const int4 w = items[it];
+        const GatherTask t = tasks[w.x];
+        const Phys ph = phys[w.x];
+        gather_block<KIND, NB, ATT>(t, ph, w.y, w.z, L);
+    }
+}
+
+__device__ inline void reduce_elem(const ReduceTask &t, long i) {
     if (i >= (long)t.nrhs * t.nr) return;
     const int b = (int)(i / t.nr), r = (int)(i % t.nr);
     float2 L = make_float2(0.f, 0.f), U = make_float2(0.f, 0.f);
@@ -262,9 +225,8 @@ __global__ void k_reduce(ReduceTask t) {
     }
 }
 
-__global__ void k_reduce_points(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs, int nq,
-                                double2 *fld, long fldStride) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void reduce_points_elem(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs,
+                                          int nq, double2 *fld, long fldStride, long i) {
     if (i >= (long)nrhs * nq) return;
     const int b = (int)(i / nq), q = (int)(i % nq);
     double sr = 0.0, si = 0.0;
@@ -280,8 +242,7 @@ __global__ void k_reduce_points(const float2 *hi, const float2 *lo, int nsplit, 
 
 // point->fragment: lanes own cells; sources are few.
 template <bool ATT>
-__global__ __launch_bounds__(256) void k_p2f(P2FTask t, Phys ph) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __attribute__((always_inline)) inline void p2f_elem(const P2FTask &t, const Phys &ph, int c) {
     const int lane = threadIdx.x & 63;
     const int cb = c >> 6;
     if (cb * 64 >= t.ncell) return;
@@ -328,9 +289,8 @@ __global__ __launch_bounds__(256) void k_p2f(P2FTask t, Phys ph) {
     }
 }
 
-__global__ void k_prime(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
-                        long outStride) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void prime_elem(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
+                                  long outStride, long i) {
     if (i >= (long)nact * nrhs) return;
     const int b = (int)(i / nact), k = (int)(i % nact);
     Pout[b * outStride + k] = P[b * pStride + actCell[k]];
@@ -338,9 +298,82 @@ __global__ void k_prime(const float2 *P, long pStride, const int *actCell, int n
 
 __global__ void k_any_zero(const float2 *v, long n, int *flag) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float2 x = v[i];
-    if (x.x == 0.0f && x.y == 0.0f) atomicOr(flag, 1);
+    bool z = false;
+    if (i < n) {
+        const float2 x = v[i];
+        z = (x.x == 0.0f && x.y == 0.0f);
+    }
+    if (__ballot(z) && __lane_id() == 0) atomicOr(flag, 1);  // one atomic per wave
+}
+
+__global__ void k_reduce(ReduceTask t) { reduce_elem(t, (long)blockIdx.x * blockDim.x + threadIdx.x); }
+
+// One wave per field-point value: lanes take every 64th split, then a fixed
+// butterfly adds the 64 lane sums (the order depends only on nsplit).
+__global__ __launch_bounds__(256) void k_reduce_points(const float2 *hi, const float2 *lo, int nsplit, int nbStride,
+                                                       int nrhs, int nq, double2 *fld, long fldStride) {
+    const long i = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= (long)nrhs * nq) return;  // wave-uniform
+    const int b = (int)(i / nq), q = (int)(i % nq);
+    double sr = 0.0, si = 0.0;
+    for (int s = lane; s < nsplit; s += 64) {
+        const size_t o = ((size_t)s * nbStride + b) * nq + q;
+        sr += (double)hi[o].x + (double)lo[o].x;
+        si += (double)hi[o].y + (double)lo[o].y;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        sr += __shfl_xor(sr, m);
+        si += __shfl_xor(si, m);
+    }
+    if (lane == 0) {
+        double2 &f = fld[b * fldStride + q];
+        f.x += sr;
+        f.y += si;
+    }
+}
+
+template <bool ATT>
+__global__ __launch_bounds__(256) void k_p2f(P2FTask t, Phys ph) {
+    p2f_elem<ATT>(t, ph, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_prime(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
+                        long outStride) {
+    prime_elem(P, pStride, actCell, nact, nrhs, Pout, outStride, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- multi-task kernels: block = (task, 256 consecutive elements of it) ----
+__global__ void k_reduce_multi(const ReduceTask *__restrict__ tasks, const int2 *__restrict__ items) {
+    const int2 w = items[blockIdx.x];
+    reduce_elem(tasks[w.x], (long)w.y + threadIdx.x);
+}
+__global__ void k_reduce_points_multi(const PointsReduceTask *__restrict__ tasks, const int2 *__restrict__ items) {
+    const int2 w = items[blockIdx.x];
+    const PointsReduceTask t = tasks[w.x];
+    reduce_points_elem(t.hi, t.lo, t.nsplit, t.nbStride, t.nrhs, t.nq, t.fld, t.fldStride, (long)w.y + threadIdx.x);
+}
+template <bool ATT>
+__global__ __launch_bounds__(256) void k_p2f_multi(const P2FTask *__restrict__ tasks, const Phys *__restrict__ phys,
+                                                   const int2 *__restrict__ items) {
+    const int2 w = items[blockIdx.x];
+    p2f_elem<ATT>(tasks[w.x], phys[w.x], w.y + threadIdx.x);
+}
+__global__ void k_prime_multi(const PrimeTask *__restrict__ tasks, const int2 *__restrict__ items) {
+    const int2 w = items[blockIdx.x];
+    const PrimeTask t = tasks[w.x];
+    prime_elem(t.P, t.pStride, t.actCell, t.nact, t.nrhs, t.Pout, t.outStride, (long)w.y + threadIdx.x);
+}
+__global__ void k_zero_multi(const ZeroTask *__restrict__ tasks, const int2 *__restrict__ items) {
+    const int2 w = items[blockIdx.x];
+    const ZeroTask t = tasks[w.x];
+    const long i = (long)w.y + threadIdx.x;
+    if (i < t.n) t.p[i] = 0.0f;
+}
+__global__ void k_counters_zero(int *c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -349,7 +382,7 @@ int choose_splits(int nrb, int ne, int minPerSplit) {
     const int targetWaves = 8192;  // 256 CUs x 4 SIMDs x 8 waves
     int s = std::max(1, targetWaves / std::max(1, nrb));
     s = std::min(s, std::max(1, ne / std::max(1, minPerSplit)));
-    return std::max(1, std::min(s, 65535));
+    return std::max(1, std::min(s, 4096));
 }
 
 template <int KIND, int NB>
@@ -407,7 +440,7 @@ int reduce_points_launch(const float2 *hi, const float2 *lo, int nsplit, int nbS
                          double2 *fld, long fldStride, hipStream_t st) {
     const long n = (long)nrhs * nq;
     if (n <= 0) return 0;
-    k_reduce_points<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(hi, lo, nsplit, nbStride, nrhs, nq, fld, fldStride);
+    k_reduce_points<<<(unsigned)((n + 3) / 4), 256, 0, st>>>(hi, lo, nsplit, nbStride, nrhs, nq, fld, fldStride);
     SR_HIP(hipGetLastError());
     return 0;
 }
@@ -426,6 +459,64 @@ int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, in
     const long n = (long)nact * nrhs;
     if (n <= 0) return 0;
     k_prime<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(P, pStride, actCell, nact, nrhs, Pout, outStride);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int gather_multi_launch(const GatherTask *tasks, const Phys *phys, const int4 *items, int nitems, int *counter,
+                        int kind, int nb, bool att, int grid, hipStream_t st) {
+    if (nitems <= 0) return 0;
+#define SR_GM(K, N, A) k_gather_multi<K, N, A><<<grid, 256, 0, st>>>(tasks, phys, items, nitems, counter)
+#define SR_GM_NB(K, A)                   \
+    switch (nb) {                        \
+    case 1: SR_GM(K, 1, A); break;       \
+    case 2: SR_GM(K, 2, A); break;       \
+    case 4: SR_GM(K, 4, A); break;       \
+    case 8: SR_GM(K, 8, A); break;       \
+    default: SR_GM(K, 16, A); break;     \
+    }
+    if (kind == KIND_F2F) { if (att) { SR_GM_NB(KIND_F2F, true) } else { SR_GM_NB(KIND_F2F, false) } }
+    else if (kind == KIND_F2F_FIELD) { if (att) { SR_GM_NB(KIND_F2F_FIELD, true) } else { SR_GM_NB(KIND_F2F_FIELD, false) } }
+    else { if (att) { SR_GM_NB(KIND_F2P, true) } else { SR_GM_NB(KIND_F2P, false) } }
+#undef SR_GM_NB
+#undef SR_GM
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int p2f_multi_launch(const P2FTask *tasks, const Phys *phys, const int2 *items, int nitems, bool att, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    if (att) k_p2f_multi<true><<<nitems, 256, 0, st>>>(tasks, phys, items);
+    else k_p2f_multi<false><<<nitems, 256, 0, st>>>(tasks, phys, items);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+int reduce_multi_launch(const ReduceTask *tasks, const int2 *items, int nitems, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    k_reduce_multi<<<nitems, 256, 0, st>>>(tasks, items);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+int reduce_points_multi_launch(const PointsReduceTask *tasks, const int2 *items, int nitems, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    k_reduce_points_multi<<<nitems, 256, 0, st>>>(tasks, items);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+int prime_multi_launch(const PrimeTask *tasks, const int2 *items, int nitems, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    k_prime_multi<<<nitems, 256, 0, st>>>(tasks, items);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+int zero_multi_launch(const ZeroTask *tasks, const int2 *items, int nitems, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    k_zero_multi<<<nitems, 256, 0, st>>>(tasks, items);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+int counters_zero_launch(int *counters, int n, hipStream_t st) {
+    k_counters_zero<<<(n + 255) / 256, 256, 0, st>>>(counters, n);
     SR_HIP(hipGetLastError());
     return 0;
 }

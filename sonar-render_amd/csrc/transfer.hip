@@ -28,6 +28,7 @@
 
 #include "launch.h"
 #include "sums.h"
+#include "pair_kernel.h"
 #include "transfer.h"
 #include "learned.h"
 
@@ -39,6 +40,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr float kInv4PiT = 0.0795774715459476678f;
 
 __device__ inline uint32_t pack_h2(float a, float b) {
+    // saturate to the FP16 range (near-field terms can exceed the |S| bound)
+    a = fminf(fmaxf(a, -65000.f), 65000.f);
+    b = fminf(fmaxf(b, -65000.f), 65000.f);
     _Float16 ha = (_Float16)a, hb = (_Float16)b;
     return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
 }
@@ -111,12 +115,12 @@ __device__ float2 mlp_bf16(const float *w, const float *f) {
 template <int NEAR>
 __device__ float2 near_value(float3 rv, float3 nr, float d, float k, const float *w) {
     const float r = sqrtf(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
-    if (NEAR == 1) return quad16(rv, nr, d, k);
+    if (NEAR == 1 || r < d) return quad16(rv, nr, d, k);  // the learned range starts at r = delta
     const float lam = 2.0f * 3.14159265358979f / k;
     const float cs = rv.z / r, ci = -(rv.x * nr.x + rv.y * nr.y + rv.z * nr.z) / r;
-    const float2 K = g_point(r, cs, ci, k, d);
+    const float2 K = g_point(r, cs, ci, k, 0.0f);  // unregularised point value
     float f[kMlpIn] = {r / lam - 1.0f, (d / lam - 0.19f) * 14.0f, rv.x / r, rv.y / r, 2.0f * cs - 1.0f,
-                       nr.x, nr.y, nr.z};
+                       nr.x, nr.y, nr.z, 0.5f * logf(r / d)};
     const float2 t = mlp_bf16(w, f);
     const float M = 0.9f * (k * kInv4PiT / r + 1.0f / (r * r));
     float s, c;
@@ -260,11 +264,17 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
 
 // ---------------------------------------------------------------------------
 // GEMM: a workgroup = 4 waves = 128 rows x 128 columns; a wave = 32 rows.
-// B chunks of 8 K steps (32 KiB) are staged in LDS and shared by the 4 waves;
-// A fragments stream from HBM straight into registers.
+// K advances in chunks of 8 steps (8 emitters each). B chunks (32 KiB) are
+// copied global -> LDS without passing through registers and are shared by
+// the 4 waves; A fragments stream from HBM straight into registers. Both are
+// requested one chunk ahead of the MFMAs that use them (two LDS buffers, two
+// A register sets). Chunks whose emitters all lie on the workgroup's single
+// facet hold only zeros (same-facet pairs never interact) and are skipped.
+// One FP32 accumulator: rows that the zero-input gate affects are recomputed
+// afterwards (k_gated_part).
 constexpr int kKC = 8;
+static_assert(kKC == 8, "K steps and split sizes are padded to multiples of 8");
 
-template <bool GATED>
 __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     __shared__ uint4 sB[2][kKC * 4 * 64];
     const int wave = threadIdx.x >> 6;
@@ -274,73 +284,56 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     const int ks0 = split * t.ksPerSplit;
     const int ks1 = min(t.nKs, ks0 + t.ksPerSplit);
     const bool rtValid = rt < t.nRt;
-    const int fr = (rtValid && rt * 32 + (lane & 31) < t.n) ? t.rowFacet[rt * 32 + (lane & 31)] : -1;
-    const int rtMin = rtValid ? t.rtFmin[rt] : 0, rtMax = rtValid ? t.rtFmax[rt] : 0;
+    const int2 skip = t.wgSkip[blockIdx.x];  // chunk-aligned K steps [x, y) that hold only zeros
     const uint4 *Arow = t.A + (size_t)(rtValid ? rt : 0) * t.nKs * 64 + lane;
+    const uint4 *Bt = t.B + threadIdx.x;     // thread x stages element x of every K step
 
-    f32x16 accL[4], accU[4];
+    f32x16 acc[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        accL[c] = f32x16{};
-        accU[c] = f32x16{};
-    }
+    for (int c = 0; c < 4; c++) acc[c] = f32x16{};
 
-    int buf = 0;
-    auto stage = [&](int kb, int bb) {
-        const int n = min(kKC, ks1 - kb);
-        for (int idx = threadIdx.x; idx < n * 256; idx += 256) sB[bb][idx] = t.B[(size_t)kb * 256 + idx];
-    };
-    if (ks0 < ks1) stage(ks0, 0);
-    for (int kb = ks0; kb < ks1; kb += kKC) {
-        __syncthreads();
-        if (kb + kKC < ks1) stage(kb + kKC, buf ^ 1);
-        const int n = min(kKC, ks1 - kb);
-        uint4 a[kKC];
+    auto next_chunk = [&](int kb) { return (kb >= skip.x && kb < skip.y) ? skip.y : kb; };
+    // B goes global -> LDS directly (global_load_lds_dwordx4: lane l of wave w
+    // writes element 64 w + l of a K step); A alternates between two register
+    // sets with the two LDS buffers. Both are requested one chunk ahead.
+    typedef __attribute__((address_space(3))) void lds_void;
+    auto stage_b = [&](int k, int bb) {
 #pragma unroll
         for (int s = 0; s < kKC; s++)
-            if (s < n) a[s] = Arow[(size_t)(kb + s) * 64];
+            __builtin_amdgcn_global_load_lds((const void *)(Bt + (size_t)(k + s) * 256),
+                                             (lds_void *)&sB[bb][s * 256 + wave * 64], 16, 0, 0);
+    };
+    uint4 a0[kKC], a1[kKC];
+    int kb = next_chunk(ks0);
+    if (kb < ks1) {
+        stage_b(kb, 0);
 #pragma unroll
-        for (int s = 0; s < kKC; s++) {
-            if (s >= n || !rtValid) continue;
-            const int ks = kb + s;
-            half8 av = __builtin_bit_cast(half8, a[s]);
-            int cls = 0;  // 0: lower facets, 1: higher facets, 2: mixed
-            if (GATED) {
-                const int kmin = t.ksFmin[ks], kmax = t.ksFmax[ks];
-                cls = (kmax <= rtMin) ? 0 : ((kmin >= rtMax) ? 1 : 2);
-            }
-            if (cls == 2) {
-                // split this step's emitters element by element
-                half8 aL = av, aU = av;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int j = ks * 8 + (lane >> 5) * 4 + q;
-                    const bool up = (j < t.n) && t.emFacet[j] > fr;
-                    if (up) { aL[2 * q] = 0; aL[2 * q + 1] = 0; }
-                    else { aU[2 * q] = 0; aU[2 * q + 1] = 0; }
-                }
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    half8 bv = __builtin_bit_cast(half8, sB[buf][(s * 4 + c) * 64 + lane]);
-                    accL[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aL, bv, accL[c], 0, 0, 0);
-                    accU[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(aU, bv, accU[c], 0, 0, 0);
-                }
-            } else if (cls == 1) {
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    half8 bv = __builtin_bit_cast(half8, sB[buf][(s * 4 + c) * 64 + lane]);
-                    accU[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, accU[c], 0, 0, 0);
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    half8 bv = __builtin_bit_cast(half8, sB[buf][(s * 4 + c) * 64 + lane]);
-                    accL[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, accL[c], 0, 0, 0);
-                }
-            }
-        }
-        buf ^= 1;
+        for (int s = 0; s < kKC; s++) a0[s] = Arow[(size_t)(kb + s) * 64];
     }
+#define SR_GEMM_PHASE(ACUR, ANEXT, BUF)                                                              \
+    {                                                                                                \
+        const int kn = next_chunk(kb + kKC);                                                         \
+        __builtin_amdgcn_s_waitcnt(0); /* this wave's share of sB[BUF] has landed */                 \
+        __syncthreads(); /* sB[BUF] complete; every wave is done reading sB[BUF ^ 1] */              \
+        if (kn < ks1) {                                                                              \
+            stage_b(kn, BUF ^ 1);                                                                    \
+            _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = Arow[(size_t)(kn + s) * 64];  \
+        }                                                                                            \
+        _Pragma("unroll") for (int s = 0; s < kKC; s++) {                                            \
+            const half8 av = __builtin_bit_cast(half8, ACUR[s]);                                     \
+            _Pragma("unroll") for (int c = 0; c < 4; c++) {                                          \
+                const half8 bv = __builtin_bit_cast(half8, sB[BUF][(s * 4 + c) * 64 + lane]);        \
+                acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[c], 0, 0, 0);           \
+            }                                                                                        \
+        }                                                                                            \
+        kb = kn;                                                                                     \
+    }
+    while (kb < ks1) {
+        SR_GEMM_PHASE(a0, a1, 0)
+        if (kb >= ks1) break;
+        SR_GEMM_PHASE(a1, a0, 1)
+    }
+#undef SR_GEMM_PHASE
     if (!rtValid) return;
     // C layout: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, column l%32).
 #pragma unroll
@@ -351,11 +344,149 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
         for (int i = 0; i < 16; i++) {
             const int row = rt * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
             if (row >= t.n) continue;
-            const size_t o = ((size_t)split * 64 + b) * t.n + row;
-            t.partL[o] = make_float2(accL[c][i], accL[c + 2][i]);
-            if (GATED) t.partU[o] = make_float2(accU[c][i], accU[c + 2][i]);
+            t.partL[((size_t)split * 64 + b) * t.n + row] = make_float2(acc[c][i], acc[c + 2][i]);
         }
     }
+}
+
+// Rows whose own input pressure is exactly zero in some plane (the zero-input
+// gate of OptiX.cu:241-245 drops their higher-facet terms there).
+__global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes, int *__restrict__ list,
+                             int *__restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool z = false;
+    if (i < n)
+        for (int b = 0; b < nplanes; b++) {
+            const float2 v = Pin[(size_t)b * n + i];
+            z |= (v.x == 0.0f && v.y == 0.0f);
+        }
+    const unsigned long long m = __ballot(z);
+    if (!m) return;
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0);
+    if (z) list[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
+}
+
+// Exact zero-input gate. For each listed row r with P_in[b][r] == 0 in plane
+// b, OptiX.cu:241-245 keeps only the emitters on lower facets; active cells
+// ascend by facet, so these are the emitters [0, fs(r)). The sum is redone for
+// those (row, plane) entries only and replaces what the full product gave.
+// The pair values are read back from the FP16 transfer matrix (FROM_S: the
+// gated rows see exactly the operator the other rows see, near-field terms
+// included) or evaluated in FP32 (plain form). A wave owns 64 listed rows, an
+// emitter range (split) and 16 planes; emitters are wave-uniform, so their
+// position and pressures are broadcast loads.
+constexpr int kGT = 64;  // emitters per LDS tile of the gated-row kernel
+
+template <bool FROM_S, bool ATT>
+__global__ __launch_bounds__(64) void k_gated_part(GatedFix t, Phys ph) {
+    __shared__ float2 s_x[16][kGT];  // A_e P_in[b][e] for this wave's 16 planes
+    __shared__ float4 s_p[kGT], s_n[kGT];
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * 64 + lane;
+    const int split = blockIdx.y, b0 = blockIdx.z * 16;
+    const bool valid = i < t.count;
+    const int r = t.list[valid ? i : 0];
+    const int fr = t.facet[r];
+    int lo = 0, hi = r;  // first active index on facet fr
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (t.facet[mid] < fr) lo = mid + 1; else hi = mid;
+    }
+    const int e0 = split * t.emPerSplit;
+    const int e1 = min(t.n, e0 + t.emPerSplit);
+    const int eEnd = valid ? min(e1, lo) : e0;  // this lane's emitters: [e0, eEnd)
+    int eMax = eEnd;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) eMax = max(eMax, __shfl_xor(eMax, m));
+
+    // Every plane of the chunk is accumulated; only the entries whose input is
+    // zero are used (k_gated_apply checks).
+    float2 acc[16];
+#pragma unroll
+    for (int b = 0; b < 16; b++) acc[b] = make_float2(0.f, 0.f);
+    const uint4 *Arow = FROM_S ? t.A + (size_t)(r >> 5) * t.nKs * 64 + (r & 31) : nullptr;
+    const float4 rp = FROM_S ? make_float4(0.f, 0.f, 0.f, 0.f) : t.p[r];
+    const float4 rn = FROM_S ? make_float4(0.f, 0.f, 0.f, 0.f) : t.nrm[r];
+    const uint64_t *mrow = FROM_S ? nullptr : t.mask + (size_t)(r >> 6) * t.n;
+
+    for (int base = e0; base < eMax; base += kGT) {  // base is a multiple of 8
+        const int ne = min(kGT, eMax - base);
+        __syncthreads();
+        if (lane < ne) {
+            const int e = base + lane;
+            const float4 pe = t.p[e];
+            if (!FROM_S) {
+                s_p[lane] = pe;
+                s_n[lane] = t.nrm[e];
+            }
+#pragma unroll
+            for (int b = 0; b < 16; b++) {
+                const float2 v = (b0 + b < t.nplanes) ? t.Pin[(size_t)(b0 + b) * t.stride + e] : make_float2(0.f, 0.f);
+                s_x[b][lane] = make_float2(pe.w * v.x, pe.w * v.y);
+            }
+        }
+        __syncthreads();
+        if (FROM_S) {
+            // A fragment order: lane (row & 31) + 32 * ((e >> 2) & 1) of K step e >> 3
+            // holds emitters 4 * ((e >> 2) & 1) .. +3 of the step as (re, im) halves.
+#pragma unroll 2
+            for (int k = 0; k < ne; k += 4) {
+                const int e = base + k;
+                uint4 q = make_uint4(0, 0, 0, 0);
+                if (e < eEnd) q = Arow[(size_t)(e >> 3) * 64 + ((e >> 2) & 1) * 32];
+                const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const bool in = e + j < eEnd;  // also false past ne
+                    const float Kre = in ? (float)__builtin_bit_cast(_Float16, (uint16_t)(w[j] & 0xffffu)) : 0.0f;
+                    const float Kim = in ? (float)__builtin_bit_cast(_Float16, (uint16_t)(w[j] >> 16)) : 0.0f;
+#pragma unroll
+                    for (int b = 0; b < 16; b++) cmac<16>(acc[b], Kre, Kim, s_x[b][min(k + j, kGT - 1)]);
+                }
+            }
+        } else {
+            const int sh = r & 63;
+            for (int k = 0; k < ne; k++) {
+                const int e = base + k;
+                const bool bit = e < eEnd && ((mrow[e] >> sh) & 1ull);
+                float Kre, Kim;
+                pair_kernel<KIND_F2F, ATT>(ph, rp, rn, s_p[k], s_n[k], bit, Kre, Kim);
+#pragma unroll
+                for (int b = 0; b < 16; b++) cmac<16>(acc[b], Kre, Kim, s_x[b][k]);
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+        if (b0 + b >= t.nplanes) continue;
+        t.part[((size_t)split * t.nplanes + b0 + b) * t.count + i] = acc[b];
+    }
+}
+
+// P_out[b][r] <- L, P[b][cell(r)] += L - (old P_out) for the zero-input entries.
+__global__ void k_gated_apply(GatedFix t) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)t.count * t.nplanes) return;
+    const int b = (int)(k / t.count), i = (int)(k % t.count);
+    const int r = t.list[i];
+    const float2 g = t.Pin[(size_t)b * t.stride + r];
+    if (!(g.x == 0.0f && g.y == 0.0f)) return;
+    float2 L = make_float2(0.f, 0.f);
+    for (int s = 0; s < t.nsplit; s++) {
+        const float2 v = t.part[((size_t)s * t.nplanes + b) * t.count + i];
+        L.x += v.x;
+        L.y += v.y;
+    }
+    L.x *= t.valueScale;
+    L.y *= t.valueScale;
+    float2 &o = t.Pout[(size_t)b * t.stride + r];
+    float2 &P = t.P[(size_t)b * t.pStride + t.rowCell[r]];
+    P.x += L.x - o.x;
+    P.y += L.y - o.y;
+    o = L;
 }
 
 // ---------------------------------------------------------------------------
@@ -394,11 +525,44 @@ int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int
     return 0;
 }
 
-int transfer_gemm(const TransferGemm &t, bool gated, hipStream_t st) {
+int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     if (t.n <= 0) return 0;
+    if (t.nKs % kKC || t.ksPerSplit % kKC || t.nRt % 4 || (long)t.nsplit * t.ksPerSplit < t.nKs) {
+        set_error("transfer GEMM: K steps, split size and row tiles must be padded (8, 8, 4)");
+        return -2;
+    }
     dim3 grid((t.nRt + 3) / 4, t.nsplit);
-    if (gated) k_transfer_gemm<true><<<grid, 256, 0, st>>>(t);
-    else k_transfer_gemm<false><<<grid, 256, 0, st>>>(t);
+    k_transfer_gemm<<<grid, 256, 0, st>>>(t);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int gated_fix_launch(const GatedFix &t, const Phys &ph, hipStream_t st) {
+    if (t.count <= 0 || t.nplanes <= 0) return 0;
+    if (t.emPerSplit <= 0 || t.emPerSplit % 8 || (long)t.nsplit * t.emPerSplit < t.n) {
+        set_error("gated fix: bad emitter split");
+        return -2;
+    }
+    if (t.A && t.nKs * 8 < t.n) { set_error("gated fix: transfer matrix too small"); return -2; }
+    dim3 grid((t.count + 63) / 64, t.nsplit, (t.nplanes + 15) / 16);
+    const bool att = ph.alpha != 0.0f;
+    if (t.A) {
+        if (att) k_gated_part<true, true><<<grid, 64, 0, st>>>(t, ph);
+        else k_gated_part<true, false><<<grid, 64, 0, st>>>(t, ph);
+    } else {
+        if (att) k_gated_part<false, true><<<grid, 64, 0, st>>>(t, ph);
+        else k_gated_part<false, false><<<grid, 64, 0, st>>>(t, ph);
+    }
+    SR_HIP(hipGetLastError());
+    const long m = (long)t.count * t.nplanes;
+    k_gated_apply<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(t);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st) {
+    if (n <= 0) return 0;
+    k_gated_rows<<<(n + 255) / 256, 256, 0, st>>>(Pin, n, nplanes, list, count);
     SR_HIP(hipGetLastError());
     return 0;
 }

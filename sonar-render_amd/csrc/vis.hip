@@ -168,6 +168,7 @@ __global__ void k_pack_nodes(const float4 *__restrict__ tris, int n, const int *
 // ends when every lane has retired or the stack is empty.
 
 constexpr int kStack = 128;
+constexpr int kBruteTris = 8;
 
 __device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
 
@@ -180,12 +181,16 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
     bool hit = false;
     unsigned long long live = ballot64(active);
     if (live == 0) return false;
-    F3 inv{safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-    if (bvh.ntri == 1) {
-        TriRec t = tri_unpack(bvh.tris[0], bvh.tris[1], bvh.tris[2], bvh.tris[3]);
-        if (active) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
+    if (bvh.ntri <= kBruteTris) {
+        // a handful of occluders: test every triangle, no tree
+        for (int ti = 0; ti < bvh.ntri; ti++) {
+            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], bvh.tris[4 * ti + 3]);
+            if (active && !hit) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
+            if (ballot64(active && !hit) == 0) break;
+        }
         return hit;
     }
+    F3 inv{fast_safe_inv(d.x), fast_safe_inv(d.y), fast_safe_inv(d.z)};
     const int lane = __lane_id();
     int sp = 0;
     int node = 0;
@@ -234,26 +239,19 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
     return hit;
 }
 
-// One wave = one receiver block of 64 and a run of emitters.
-template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, VisConst vc, int emPerWave) {
-    __shared__ int stkAll[WAVES][kStack];
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    int *stk = stkAll[wave];
-    const int nrb = (tk.rc.count + 63) >> 6;
-    const int nEmChunks = (tk.em.count + emPerWave - 1) / emPerWave;
-    const long gw = (long)blockIdx.x * WAVES + wave;
-    if (gw >= (long)nrb * nEmChunks) return;
-    const int rb = (int)(gw / nEmChunks);
-    const int ec = (int)(gw % nEmChunks);
+// One wave's share: receiver block rb, emitters [ec*emPerWave, (ec+1)*emPerWave).
+__device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &tk, const Bvh &bvh,
+                                                               const VisConst &vc, int rb, int ec, int emPerWave,
+                                                               int *stk) {
+    const int lane = __lane_id();
     const int r = rb * 64 + lane;
     const bool rvalid = r < tk.rc.count;
     float4 rp = rvalid ? tk.rc.p[r] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 rn = (rvalid && tk.rc.n) ? tk.rc.n[r] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int rfacet = __float_as_int(rn.w);
-    const int e0 = ec * emPerWave;
+    int e0 = ec * emPerWave;
     const int e1 = min(tk.em.count, e0 + emPerWave);
+    if (tk.upperOnly) e0 = max(e0, rb * 64);
     for (int e = e0; e < e1; e++) {
         float4 ep = tk.em.p[e];
         float4 en = tk.em.n ? tk.em.n[e] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -290,6 +288,60 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
         bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag);
         unsigned long long word = ballot64(pair && !occ);
         if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
+    }
+}
+
+// One wave = one receiver block of 64 and a run of emitters.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, VisConst vc, int emPerWave) {
+    __shared__ int stkAll[WAVES][kStack];
+    const int wave = threadIdx.x >> 6;
+    const int nrb = (tk.rc.count + 63) >> 6;
+    const int nEmChunks = (tk.em.count + emPerWave - 1) / emPerWave;
+    const long gw = (long)blockIdx.x * WAVES + wave;
+    if (gw >= (long)nrb * nEmChunks) return;
+    vis_wave(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave]);
+}
+
+// Persistent form over many tasks (scenes): each wave pulls (task, receiver
+// block, emitter chunk) items from a shared counter until the list is empty.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *__restrict__ tasks,
+                                                               const Bvh *__restrict__ bvhs,
+                                                               const VisConst *__restrict__ vcs,
+                                                               const int4 *__restrict__ items, int nitems,
+                                                               int emPerWave, int *counter) {
+    __shared__ int stkAll[WAVES][kStack];
+    const int wave = threadIdx.x >> 6;
+    for (;;) {
+        int it = 0;
+        if (__lane_id() == 0) it = atomicAdd(counter, 1);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= nitems) break;
+        const int4 w = items[it];
+        vis_wave(tasks[w.x], bvhs[w.x], vcs[w.x], w.y, w.z, emPerWave, stkAll[wave]);
+    }
+}
+
+// Fills the words with emitter e < 64*rb of a symmetric set from the computed
+// ones: word (rb, 64*eb + i) bit j = word (eb, 64*rb + j) bit i, for eb < rb.
+__global__ __launch_bounds__(256) void k_mask_transpose(uint64_t *mask, int n) {
+    const int nrb = (n + 63) >> 6;
+    const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = __lane_id();
+    // pair -> (rb, eb) with 0 <= eb < rb < nrb, row-major over the strictly lower triangle
+    const long npairs = (long)nrb * (nrb - 1) / 2;
+    if (pair >= npairs) return;
+    int rb = (int)((1.0 + sqrt(1.0 + 8.0 * (double)pair)) / 2.0);
+    while ((long)rb * (rb - 1) / 2 > pair) rb--;
+    while ((long)(rb + 1) * rb / 2 <= pair) rb++;
+    const int eb = (int)(pair - (long)rb * (rb - 1) / 2);
+    const int r = rb * 64 + lane;
+    const uint64_t w = (r < n) ? mask[(size_t)eb * n + r] : 0ull;
+    for (int i = 0; i < 64; i++) {
+        const uint64_t col = __ballot((w >> i) & 1ull);
+        const int e = eb * 64 + i;
+        if (lane == i && e < n) mask[(size_t)rb * n + e] = col;
     }
 }
 
@@ -364,6 +416,49 @@ int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipS
     long blocks = (waves + W - 1) / W;
     if (blocks > 0x7fffffffL) { set_error("visibility grid too large"); return -1; }
     k_vis_mask<W><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
+    SR_HIP(hipGetLastError());
+    if (tk.upperOnly && nrb > 1) {
+        const long pairs = nrb * (nrb - 1) / 2;
+        k_mask_transpose<<<(unsigned)((pairs * 64 + 255) / 256), 256, 0, st>>>(tk.mask, tk.rc.count);
+        SR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// Multi-task transposition: item (task, lower-triangle block pair index), one wave each.
+__global__ __launch_bounds__(256) void k_mask_transpose_multi(const MaskTask *__restrict__ tasks,
+                                                              const int2 *__restrict__ items, int nitems) {
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (w >= nitems) return;
+    const int2 it = items[w];
+    const MaskTask &tk = tasks[it.x];
+    const int n = tk.rc.count;
+    const long pair = it.y;
+    const int lane = __lane_id();
+    int rb = (int)((1.0 + sqrt(1.0 + 8.0 * (double)pair)) / 2.0);
+    while ((long)rb * (rb - 1) / 2 > pair) rb--;
+    while ((long)(rb + 1) * rb / 2 <= pair) rb++;
+    const int eb = (int)(pair - (long)rb * (rb - 1) / 2);
+    const int r = rb * 64 + lane;
+    const uint64_t wd = (r < n) ? tk.mask[(size_t)eb * n + r] : 0ull;
+    for (int i = 0; i < 64; i++) {
+        const uint64_t col = __ballot((wd >> i) & 1ull);
+        const int e = eb * 64 + i;
+        if (lane == i && e < n) tk.mask[(size_t)rb * n + e] = col;
+    }
+}
+
+int mask_transpose_multi_launch(const MaskTask *tasks, const int2 *items, int nitems, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    k_mask_transpose_multi<<<(unsigned)(((long)nitems * 64 + 255) / 256), 256, 0, st>>>(tasks, items, nitems);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int vis_mask_multi_launch(const MaskTask *tasks, const Bvh *bvhs, const VisConst *vcs, const int4 *items, int nitems,
+                          int emPerWave, int *counter, int grid, hipStream_t st) {
+    if (nitems <= 0) return 0;
+    k_vis_mask_multi<4><<<grid, 256, 0, st>>>(tasks, bvhs, vcs, items, nitems, emPerWave, counter);
     SR_HIP(hipGetLastError());
     return 0;
 }

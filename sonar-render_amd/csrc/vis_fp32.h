@@ -69,6 +69,12 @@ __host__ __device__ inline float safe_inv(float d) {
     return 1.0f / dd;
 }
 
+// Reciprocal for the slab test only: any rounding is far inside the box padding.
+__device__ inline float fast_safe_inv(float d) {
+    float dd = (fabsf(d) < 1e-20f) ? (d < 0.0f ? -1e-20f : 1e-20f) : d;
+    return __builtin_amdgcn_rcpf(dd);
+}
+
 // Conservative slab test against a box padded by pad (any rounding here only
 // widens the accepted set by far less than the padding margin).
 __host__ __device__ inline bool slab_hit(F3 bmin, F3 bmax, float pad, F3 o, F3 inv, float tmin, float tmax) {

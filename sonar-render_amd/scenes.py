@@ -146,6 +146,50 @@ def config_trihedral(f=20e3, c=C_WATER, frac=7.5, n_side=8, bounces=3, radius=15
         batched=True)
 
 
+def random_convex_polyhedron(rng, ntri_min=8, ntri_max=40, extent=(0.3, 1.5)):
+    """Convex hull of random points: 8-40 outward-facing triangles, seeded."""
+    from scipy.spatial import ConvexHull
+    while True:
+        npts = int(rng.integers(6, 22))
+        pts = rng.normal(size=(npts, 3))
+        pts /= np.linalg.norm(pts, axis=1, keepdims=True)
+        pts *= rng.uniform(0.6, 1.0, (npts, 1))
+        hull = ConvexHull(pts)
+        if ntri_min <= len(hull.simplices) <= ntri_max:
+            break
+    size = rng.uniform(*extent)
+    span = (pts.max(0) - pts.min(0)).max()
+    pts = pts * (size / span)
+    tris = pts[hull.simplices]
+    c = pts.mean(0)
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    flip = np.einsum("ij,ij->i", n, tris.mean(1) - c) < 0
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    return tris.astype(np.float32)
+
+
+def config_batch(n_scenes=256, seed=0, frac=4.0, n_points=32, bounces=2, c=C_WATER):
+    """Batched uneven scenes: convex polyhedra or plates, f in 2-20 kHz, fragments lambda/frac."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_scenes):
+        f = float(rng.uniform(2e3, 20e3))
+        if rng.random() < 0.25:
+            a, b = rng.uniform(0.3, 1.5, 2)
+            tris = make_rectangle(a, b)
+        else:
+            tris = random_convex_polyhedron(rng)
+        u = rng.normal(size=3)
+        u[2] = abs(u[2]) + 0.3
+        u /= np.linalg.norm(u)
+        src = (10.0 * u)[None].astype(np.float32)
+        ang = np.sort(rng.uniform(-90, 90, n_points))
+        pts = generate_field_points(10.0, ang)
+        out.append(dict(name=f"batch_{s}", f=f, c=c, atten=0.0, targets=[(tris, c / f / frac)], fields=[],
+                        sources=src, points=pts, bounces=bounces))
+    return out
+
+
 def scene_pairs(scene, n_act, n_cells_field=None):
     """Ordered pair count of one scene run (the Gpairs/s unit of work).
 

@@ -290,7 +290,7 @@ def kernel_stats(family):
 
 _sig("sr_set_transfer_mode", [_i])
 _sig("sr_transfer_mode", [])
-KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8})
+KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9})
 
 
 def set_transfer_mode(mode):
@@ -346,8 +346,8 @@ _sig("sr_nf_samples", [ctypes.c_uint, _i, _fp, _fp])
 _sig("sr_nf_forward", [_fp, _i, _fp])
 _sig("sr_nf_gradient", [_fp, _fp, _fp, _i, _fp])
 
-NF_PARAMS = 4866
-NF_IN = 8
+NF_PARAMS = 4930
+NF_IN = 9
 NEAR_POINT, NEAR_QUADRATURE, NEAR_LEARNED = 0, 1, 2
 
 
@@ -395,3 +395,70 @@ def nf_gradient(params, X, Y):
     g = np.zeros(NF_PARAMS, np.float32)
     _check(cpp_lib.sr_nf_gradient(_ptr(p), _ptr(X), _ptr(Y), len(X), _ptr(g)), "nf_gradient")
     return g
+
+
+_sig("sr_device_synchronize", [])
+
+
+def device_synchronize():
+    """Waits for all work of the library's HIP runtime on the current device."""
+    _check(cpp_lib.sr_device_synchronize(), "device_synchronize")
+
+
+for _n, _a in [("sr_batch_clear", []), ("sr_batch_add_scene", [_fp, _i, _f, _f, _f, _f, _fp, _i, _fp, _i]),
+               ("sr_batch_set_bounces", [_i]), ("sr_batch_prepare", []),
+               ("sr_batch_run", [_i, _fp]), ("sr_batch_autotune", [_fp, _i]),
+               ("sr_batch_field_points", [_i, _dp, _i]), ("sr_batch_get_tune", [_ip]), ("sr_batch_set_tune", [_ip])]:
+    _sig(_n, _a)
+_sig("sr_batch_pairs", [], ctypes.c_double)
+
+
+def batch_clear():
+    _check(cpp_lib.sr_batch_clear(), "batch_clear")
+
+
+def batch_add_scene(tris, delta, f, sources, points, c=1480.0, atten=0.0):
+    t = _f32(tris).reshape(-1, 9)
+    s = _f32(sources).reshape(-1, 3)
+    p = _f32(points).reshape(-1, 3)
+    return _check(cpp_lib.sr_batch_add_scene(_ptr(t), len(t), delta, c, f, atten, _ptr(s), len(s), _ptr(p), len(p)),
+                  "batch_add_scene")
+
+
+def batch_set_bounces(n):
+    _check(cpp_lib.sr_batch_set_bounces(int(n)), "batch_set_bounces")
+
+
+def batch_prepare():
+    _check(cpp_lib.sr_batch_prepare(), "batch_prepare")
+
+
+def batch_run(use_graph=True):
+    ms = np.zeros(1, np.float32)
+    _check(cpp_lib.sr_batch_run(1 if use_graph else 0, _ptr(ms)), "batch_run")
+    return float(ms[0])
+
+
+def batch_autotune(nmax=64):
+    rep = np.zeros((nmax, 5), np.float32)
+    k = _check(cpp_lib.sr_batch_autotune(_ptr(rep), nmax), "batch_autotune")
+    return rep[:k]
+
+
+def batch_field_points(scene, n):
+    out = np.zeros((n, 2), np.float64)
+    _check(cpp_lib.sr_batch_field_points(scene, _ptr(out, _dp), n), "batch_field_points")
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def batch_pairs():
+    return float(cpp_lib.sr_batch_pairs())
+
+
+def batch_tune(values=None):
+    v = np.zeros(4, np.int32)
+    if values is not None:
+        v[:] = values
+        _check(cpp_lib.sr_batch_set_tune(_ptr(v, _ip)), "batch_set_tune")
+    _check(cpp_lib.sr_batch_get_tune(_ptr(v, _ip)), "batch_get_tune")
+    return v.tolist()

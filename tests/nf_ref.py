@@ -1,19 +1,19 @@
 """FP64 numpy restatement of the near-field learning problem (test infrastructure).
 
 * label(x): the 16 x 16 sub-fragment quadrature of the pair kernel over the
-  emitter's cell, minus the point rule, times e^{-ikr} / M, rebuilt from the
-  features alone (lengths in wavelengths, emitter frame).
-* forward / gradient: the 8-64-64-2 tanh network and the gradient of
+  emitter's cell, minus the unregularised point value, times e^{-ikr} / M,
+  rebuilt from the features alone (lengths in wavelengths, emitter frame).
+* forward / gradient: the 9-64-64-2 tanh network and the gradient of
   L = sum_i |out_i - y_i|^2 / n, by explicit back-propagation.
 """
 import numpy as np
 
-OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, NPAR = 512, 576, 4672, 4736, 4864, 4866
+OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, NPAR = 576, 640, 4736, 4800, 4928, 4930
 
 
 def unpack(p):
     p = np.asarray(p, np.float64)
-    return (p[:OFF_B1].reshape(64, 8), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(64, 64), p[OFF_B2:OFF_W3],
+    return (p[:OFF_B1].reshape(64, 9), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(64, 64), p[OFF_B2:OFF_W3],
             p[OFF_W3:OFF_B3].reshape(2, 64), p[OFF_B3:NPAR])
 
 
@@ -59,7 +59,7 @@ def label(x):
     css, cis = v[..., 2] / rs, -(v @ nr) / rs
     ok = (css >= 1e-6) & (cis >= 1e-6)
     Q = np.where(ok, _g(rs, css, cis, k, h), 0).sum() / 256
-    K = _g(r, cs, ci, k, d)
+    K = _g(r, cs, ci, k, 0.0)
     M = 0.9 * (k / (4 * np.pi * r) + 1 / r ** 2)
     t = (Q - K) * np.exp(-1j * k * r) / M
     return np.array([t.real, t.imag])

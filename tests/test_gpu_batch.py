@@ -1,0 +1,96 @@
+"""GPU: batched uneven scenes in persistent multi-scene launches and one hipGraph.
+
+Every scene of a batch must equal the FP64 oracle run of that scene alone;
+graph replays (with eager work between them) must equal the eager run bit for
+bit; the autotuner must leave results unchanged.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def _small_batch():
+    b = scenes.config_batch(n_scenes=10, seed=3, n_points=16)
+    for sc in b:  # keep the oracle quick
+        sc["f"] = min(sc["f"], 6e3)
+        sc["targets"] = [(sc["targets"][0][0], sc["c"] / sc["f"] / 4.0)]
+    tri = scenes.config_trihedral(f=4e3, frac=5.0, n_side=1, bounces=2)
+    tri["points"] = scenes.generate_field_points(10.0, np.linspace(-60, 60, 16))
+    tri["batched"] = False
+    plate = scenes.make_rectangle(0.8, 0.8)
+    wall = plate.copy()
+    wall[:, :, [1, 2]] = wall[:, :, [2, 1]]
+    wall[:, :, 1] += 0.4
+    wall[:, :, 2] += 0.4
+    vee = dict(name="vee", f=5e3, c=1480.0, atten=0.2, targets=[(np.concatenate([plate, wall]), 1480 / 5e3 / 5)],
+               fields=[], sources=np.array([[0.5, -3.0, 4.0]], np.float32),
+               points=scenes.generate_field_points(8.0, np.linspace(-50, 50, 16)), bounces=2)
+    return b + [tri, vee]
+
+
+def _load(api, batch):
+    api.batch_clear()
+    api.batch_set_bounces(2)
+    for sc in batch:
+        tris, d = sc["targets"][0]
+        api.batch_add_scene(tris, d, sc["f"], sc["sources"], sc["points"], c=sc["c"], atten=sc.get("atten", 0.0))
+    api.batch_prepare()
+
+
+def test_batch_scenes_match_oracle(api):
+    batch = _small_batch()
+    _banner(f"batch of {len(batch)} uneven scenes vs oracle")
+    _load(api, batch)
+    api.batch_run(use_graph=False)
+    worst = 0.0
+    for i, sc in enumerate(batch):
+        sc = dict(sc, bounces=2)
+        ref = oracle.run_scene(sc)["points"]
+        got = api.batch_field_points(i, len(sc["points"]))
+        e = rel_l2(got, ref)
+        print(sc["name"], "rel-L2", e, "|p| max", np.abs(ref).max())
+        worst = max(worst, e)
+    api.batch_clear()
+    assert worst < 2e-4
+
+
+def test_batch_graph_replays_match_eager(api):
+    batch = _small_batch()
+    _banner("batch: hipGraph replays vs eager")
+    _load(api, batch)
+    api.batch_run(use_graph=False)
+    eager = [api.batch_field_points(i, len(sc["points"])) for i, sc in enumerate(batch)]
+    for rep in range(3):
+        api.batch_run(use_graph=True)
+        got = [api.batch_field_points(i, len(sc["points"])) for i, sc in enumerate(batch)]
+        for a, b in zip(got, eager):
+            assert np.array_equal(a, b)
+        api.batch_run(use_graph=False)  # eager work between replays
+    api.batch_clear()
+
+
+def test_batch_autotune_keeps_results(api):
+    batch = _small_batch()
+    _banner("batch autotune")
+    _load(api, batch)
+    api.batch_run(use_graph=False)
+    before = [api.batch_field_points(i, len(sc["points"])) for i, sc in enumerate(batch)]
+    rep = api.batch_autotune()
+    print("candidates (ms, visGrid, gatherGrid, emPerWave, minPerSplit):")
+    print(rep)
+    print("chosen", api.batch_tune())
+    assert len(rep) >= 8
+    api.batch_run(use_graph=True)
+    after = [api.batch_field_points(i, len(sc["points"])) for i, sc in enumerate(batch)]
+    for a, b in zip(after, before):
+        assert rel_l2(a, b) < 1e-6
+    api.batch_clear()

@@ -61,3 +61,29 @@ def test_mixed_equals_plain_at_20khz_subset(api):
                np.concatenate([b["points"].ravel(), b["targets"][0].ravel()]))
     print("mixed vs plain rel-L2", e)
     assert e < TOL_MIXED
+
+
+def test_mixed_gated_rows_exact(api):
+    """Incidences from outside the octant leave whole plates unlit: their rows have
+    P_in == 0 in some planes only, and the transfer form recomputes them exactly."""
+    sc = scenes.config_trihedral(f=5e3, n_side=2, bounces=3)
+    sc["sources"] = np.array([[-6.0, 4.0, 13.0], [5.0, -3.0, 12.0], [4.0, 5.0, 13.0], [3.0, 4.0, -0.5]], np.float32)
+    sc["points"] = sc["sources"].copy()
+    _banner("mixed form: trihedral 5 kHz, incidences leaving plates unlit, 3 bounces")
+    runner.load_scene(sc, transfer_mode=1)
+    api.profile(True)
+    runner.project_scene(sc)
+    fix = api.kernel_stats("gated_fix")
+    api.profile(False)
+    out = runner.read_results(sc)
+    api.TearDownCuda()
+    assert fix["launches"] == 3
+    plain = runner.run_scene(sc, transfer_mode=0)
+    for p in range(len(sc["sources"])):
+        ref = oracle.run_scene(sc, plane_source=p)
+        a = np.concatenate([out["points"][p], out["targets"][0][p]])
+        b = np.concatenate([ref["points"], ref["targets"][0]])
+        c = np.concatenate([plain["points"][p], plain["targets"][0][p]])
+        print("plane", p, "mixed", rel_l2(a, b), "plain", rel_l2(c, b))
+        assert rel_l2(a, b) < TOL_MIXED
+        assert rel_l2(c, b) < 2e-4
