@@ -153,6 +153,14 @@ int sr_clear_vis_cache(void);
 int sr_set_transfer_mode(int mode);
 int sr_transfer_mode(void);
 
+/* Plain fragment->fragment sums (transfer mode 0). 0 (default): with 16 or
+ * more incidence planes the pair kernel is evaluated in FP32 on the vector
+ * units and the complex products accumulate on the FP32 matrix cores, 64
+ * planes per pass; 1: vector units only (16 planes per pass). Both are FP32
+ * throughout. Reset to 0 by TearDownCuda. */
+int sr_set_gather_mode(int mode);
+int sr_gather_mode(void);
+
 /* Near-field terms of the transfer matrix (mode 1 transfer only; pairs of
  * fragments closer than radiusWavelengths): 0 the reference's point rule,
  * 1 16x16 sub-fragment quadrature of the same kernel over the emitter's
@@ -179,9 +187,10 @@ int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, f
  * 1 visibility words, 2 fragment->fragment, 3 fragment->field object,
  * 4 fragment->field point, 5 split-K reductions, 6 transfer-matrix build,
  * 7 transfer GEMM, 8 incidence-matrix build, 9 exact recomputation of the
- * rows a zero input gates in the transfer form. Reports summed kernel time,
- * launches, ordered pairs (x planes), flops (gathers: 37 FP32 flops per
- * pair plus 8 per pair and plane; transfer GEMM: 2 M N K as executed) and
+ * rows a zero input gates (matrix-core forms). Reports summed kernel time,
+ * launches, ordered pairs (x planes), flops (vector gathers: 37 FP32
+ * flops per pair plus 8 per pair and plane; matrix-core fragment->fragment:
+ * the 8 per pair and plane that run on the matrix cores; transfer GEMM: 2 M N K as executed) and
  * algorithmic HBM bytes (transfer GEMM: S, B and the partial sums). */
 int sr_profile(int on);
 int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);

@@ -222,6 +222,10 @@ int sr_set_transfer_mode(int mode) {
     return guarded("set_transfer_mode", [&] { return sr::model_set_transfer_mode(gModel, mode); });
 }
 int sr_transfer_mode(void) { return gModel.transferMode; }
+int sr_set_gather_mode(int mode) {
+    return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(gModel, mode); });
+}
+int sr_gather_mode(void) { return gModel.gatherMode; }
 int sr_set_near_mode(int mode, float radiusWavelengths) {
     return guarded("set_near_mode", [&] { return sr::model_set_near_mode(gModel, mode, radiusWavelengths); });
 }

@@ -134,7 +134,7 @@ int upload_object(Model &m, Object &o) {
 
 void free_transfer(Object &o) {
     auto &t = o.tc;
-    dev_free(t.A); dev_free(t.B); dev_free(t.facet); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
+    dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
     dev_free(t.gatedList); dev_free(t.gatedCount);
     t = Object::Transfer{};
 }
@@ -428,8 +428,89 @@ int model_project_src(Model &m, bool fieldObjects) {
 // One gather + reduce over all planes in groups of <= 16.
 // pairs: ordered pairs one plane of this gather evaluates (for the profiler's
 // flop count: 37 shared flops per pair plus 8 per plane for the complex MAC).
+// Fragment->fragment with the complex products on the FP32 matrix cores, in
+// groups of up to 64 planes; a GATE_UPPER task then has its gated rows
+// recomputed exactly (lower-facet emitters only).
+static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask rt, int nplanes, long evalStride,
+                              long gvalStride, long outStride, long pcellStride, double pairs) {
+    const bool gated = t.gateMode == GATE_UPPER;
+    int ngated = 0;
+    if (gated) {
+        if (t.ep != t.rp || t.en != t.rn || !rt.out || !rt.pcell) {
+            set_error("gated MFMA gather needs one object on both sides");
+            return -2;
+        }
+        if (m.gatedCap < t.nr) {
+            dev_free(m.gatedList);
+            m.gatedCap = 0;
+            if (dev_alloc(&m.gatedList, t.nr)) return -1;
+            m.gatedCap = t.nr;
+        }
+        if (!m.gatedCount && dev_alloc(&m.gatedCount, 1)) return -1;
+        if (counters_zero_launch(m.gatedCount, 1, m.stream)) return -1;
+        if (transfer_gated_rows(t.gval, t.nr, nplanes, m.gatedList, m.gatedCount, m.stream)) return -1;
+        SR_HIP(hipMemcpyAsync(&ngated, m.gatedCount, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+        SR_HIP(hipStreamSynchronize(m.stream));
+    }
+    // ~2048 workgroups of 128 receivers; emitter ranges in whole LDS tiles of 32
+    const int groups = (t.nr + 127) / 128;
+    int nsplit = std::max(1, std::min((2048 + groups - 1) / groups, t.ne / 64));
+    t.emPerSplit = ((t.ne + nsplit - 1) / nsplit + 31) / 32 * 32;
+    t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
+    if (ensure_scratch(m, (size_t)t.nsplit * 64 * t.nr)) return -1;
+    for (int b0 = 0; b0 < nplanes; b0 += 64) {
+        const int nb = std::min(64, nplanes - b0);
+        GatherTask g = t;
+        g.nrhs = nb;
+        g.evalStride = evalStride;
+        g.eval = t.eval + b0 * evalStride;
+        g.gval = nullptr;
+        g.gateMode = GATE_NONE;
+        g.partL = m.scratchL;
+        g.partU = nullptr;
+        {
+            // flops: the complex multiply-accumulates, 8 per pair and plane, on the matrix cores
+            ProfScope ps(m, PK_F2F, pairs * nb, pairs * 8.0 * nb);
+            if (f2f_mfma_launch(g, ph, m.stream)) return -1;
+        }
+        ReduceTask r = rt;
+        r.partL = m.scratchL;
+        r.partU = nullptr;
+        r.nsplit = g.nsplit;
+        r.nbStride = 64;
+        r.nrhs = nb;
+        r.nr = t.nr;
+        r.gvalStride = gvalStride;
+        r.gateMode = gated ? GATE_NONE : rt.gateMode;
+        r.outStride = outStride;
+        r.pcellStride = pcellStride;
+        if (rt.gval) r.gval = rt.gval + b0 * gvalStride;
+        if (rt.out) r.out = rt.out + b0 * outStride;
+        if (rt.pcell) r.pcell = rt.pcell + b0 * pcellStride;
+        ProfScope pr(m, PK_REDUCE, 0.0, 0.0);
+        if (reduce_launch(r, m.stream)) return -1;
+    }
+    if (!ngated) return 0;
+    const int fixWaves = ((ngated + 63) / 64) * ((nplanes + 15) / 16);
+    int fixSplit = std::max(1, std::min((2048 + fixWaves - 1) / std::max(1, fixWaves), t.ne / 64));
+    const int fixPer = (((t.ne + fixSplit - 1) / fixSplit) + 7) / 8 * 8;
+    fixSplit = (t.ne + fixPer - 1) / fixPer;
+    if (ensure_scratch(m, (size_t)fixSplit * nplanes * ngated)) return -1;
+    GatedFix f{};
+    f.list = m.gatedList; f.count = ngated; f.n = t.nr; f.p = t.ep; f.nrm = t.en;
+    f.Pin = t.gval; f.stride = gvalStride; f.nplanes = nplanes;
+    f.A = nullptr; f.nKs = 0; f.mask = t.mask; f.valueScale = 1.0f;
+    f.emPerSplit = fixPer; f.nsplit = fixSplit; f.part = m.scratchL;
+    f.Pout = rt.out; f.P = rt.pcell; f.pStride = pcellStride; f.rowCell = rt.rowCell;
+    ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
+    return gated_fix_launch(f, ph, m.stream);
+}
+
 static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, ReduceTask rt, int nplanes,
                          long evalStride, long gvalStride, long outStride, long pcellStride, double pairs) {
+    if (kind == KIND_F2F && m.gatherMode == 0 && nplanes >= 16 && !t.ecell &&
+        (t.gateMode == GATE_NONE || (t.gateMode == GATE_UPPER && t.gval == t.eval && gvalStride == evalStride)))
+        return gather_planes_mfma(m, t, ph, rt, nplanes, evalStride, gvalStride, outStride, pcellStride, pairs);
     const int nrb = (t.nr + 63) / 64;
     // few receivers (field points): spread the emitters thinly across many workgroups
     const int minPer = (t.nr <= 256) ? 16 : 256;
@@ -491,7 +572,7 @@ static int ensure_transfer(Model &m, int i) {
     if (!t.A || t.nKs != nKs || t.nRt != nRt) {
         free_transfer(o);
         const int nWg = nRt / 4;
-        if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) || dev_alloc(&t.facet, n) ||
+        if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
             dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
             dev_alloc(&t.gatedList, n) || dev_alloc(&t.gatedCount, 1))
             return -1;
@@ -513,7 +594,6 @@ static int ensure_transfer(Model &m, int i) {
             }
             steps += 4.0 * (nKs - (skip[w].y - skip[w].x));
         }
-        SR_HIP(hipMemcpy(t.facet, fa.data(), n * sizeof(int), hipMemcpyHostToDevice));
         SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
         t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
     }
@@ -603,7 +683,7 @@ static int bounce_mixed(Model &m, int i) {
     }
     if (!gated) return 0;
     GatedFix f{};
-    f.list = t.gatedList; f.count = gated; f.facet = t.facet; f.n = n; f.p = o.actP; f.nrm = o.actN;
+    f.list = t.gatedList; f.count = gated; f.n = n; f.p = o.actP; f.nrm = o.actN;
     f.Pin = o.Pin; f.stride = n; f.nplanes = m.nplanes;
     f.A = t.A; f.nKs = t.nKs; f.mask = nullptr; f.valueScale = 1.0f / t.sS;
     f.emPerSplit = fixPer; f.nsplit = fixSplit; f.part = m.scratchL;
@@ -804,6 +884,12 @@ int model_set_near_mode(Model &m, int mode, float radiusWl) {
     return 0;
 }
 
+int model_set_gather_mode(Model &m, int mode) {
+    if (mode != 0 && mode != 1) { set_error("gather mode: 0 FP32 MFMA products from 16 planes on, 1 VALU only"); return -2; }
+    m.gatherMode = mode;
+    return 0;
+}
+
 int model_set_transfer_mode(Model &m, int mode) {
     if (mode != 0 && mode != 1) { set_error("transfer mode: 0 plain FP32, 1 FP16 MFMA transfer matrix"); return -2; }
     m.transferMode = mode;
@@ -848,6 +934,9 @@ void model_teardown(Model &m) {
     m.masks.clear();
     m.transferMode = 0;
     m.nearMode = 0;
+    m.gatherMode = 0;
+    dev_free(m.gatedList); dev_free(m.gatedCount);
+    m.gatedCap = 0;
     dev_free(m.srcP); dev_free(m.srcAmp); dev_free(m.fldP); dev_free(m.fldVal);
     dev_free(m.trisSorted); dev_free(m.nodes); dev_free(m.errFlag);
     dev_free(m.scratchL); dev_free(m.scratchU);

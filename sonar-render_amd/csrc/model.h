@@ -40,7 +40,6 @@ struct Object {
     struct Transfer {
         uint4 *A = nullptr, *B = nullptr;
         int nKs = 0, nRt = 0;
-        int *facet = nullptr;      // facet per active index
         int2 *wgSkip = nullptr;    // per 128-row group: zero K-step range (its own single facet)
         int *gatedList = nullptr, *gatedCount = nullptr;  // rows with a zero input in some plane
         float *colScale = nullptr, *colInv = nullptr;
@@ -79,6 +78,9 @@ struct Model {
     struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; };
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
+    int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
+    int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
+    int gatedCap = 0;
     int nearMode = 0;      // transfer matrix near field: 0 point rule, 1 16x16 quadrature, 2 learned
     float nearRadiusWl = 2.0f;
     float2 *scratchL = nullptr, *scratchU = nullptr;

@@ -92,6 +92,10 @@ int p2f_launch(const P2FTask &t, const Phys &ph, hipStream_t st);
 int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
                  long outStride, hipStream_t st);
 int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st);
+// Fragment->fragment for 1..64 planes with the complex products on the FP32
+// matrix cores. Partial sums [split][64][nr] (plane stride 64); the gate is
+// not applied (gateMode is ignored).
+int f2f_mfma_launch(const GatherTask &t, const Phys &ph, hipStream_t st);
 
 // ---- multi-task (many scenes in one launch) ----
 struct PointsReduceTask {

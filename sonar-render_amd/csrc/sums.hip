@@ -192,6 +192,165 @@ __global__ __launch_bounds__(256) void k_gather_multi(const GatherTask *__restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fragment->fragment, 16..64 incidences: FP32 pair kernel on the VALU, complex
+// multiply-accumulate on the FP32 matrix cores (v_mfma_f32_32x32x2_f32).
+//
+// As a real product, P_out[r][b] = sum_e [Kr Ki](r,e) x [[Xr Xi], [-Xi Xr]](e,b)
+// with X = A_e P_in[b][e]: K = 2 per emitter (re, im), 128 columns (Re of 64
+// planes, then Im of 64 planes). A wave owns 32 receivers and all 128 columns
+// (4 accumulator tiles of 32 x 32); its 64 lanes evaluate 2 emitters x 32
+// receivers at once, and one v_permlane32_swap turns the lanes' (Kr, Ki)
+// into the A operands of both emitters:
+//   A operand: lane l holds A[row l % 32][k l / 32]; B: lane l holds B[k l / 32][col l % 32];
+//   C: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, col l % 32).
+// Emitter tiles (pressures of all planes, positions, normals, visibility
+// words) are copied global -> LDS with global_load_lds (no registers), double
+// buffered one tile ahead; each lane picks its B values from the raw pressure
+// rows, and the emitter area scales K. A pair of emitters whose visibility
+// bits are all zero for the wave's 32 receivers is skipped. The zero-input
+// gate is not applied here: the rows it affects are recomputed afterwards with
+// the lower-facet emitters only (k_gated_part).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int kMT = 32;       // emitters per LDS tile
+constexpr int kXPitch = 66;   // floats per plane row of a tile (32 float2 + 2: conflict-free columns)
+
+// One tile of emitter data, copied global -> LDS by global_load_lds_dword
+// (no registers involved): lane l of a wave-instruction writes dword l.
+struct MfmaTile {
+    float x[64][kXPitch];  // P_in[plane][emitter] as (re, im) pairs
+    float4 p[kMT], n[kMT];
+    uint64_t m[2][kMT];    // visibility words of the workgroup's two 64-receiver blocks
+};
+
+// Requests tile [base, base + kMT) into T. Emitter indices are clamped into
+// range (the values past the tile's end are never used: their bits read as 0).
+__device__ __attribute__((always_inline)) inline void mfma_tile_dma(const GatherTask &t, int base, int rb0, int nrb,
+                                                                    int wave, int lane, MfmaTile &T) {
+    const int last = t.ne - 1;
+    // x: plane q = 16 w + i, one dword per lane: emitter lane / 2, component lane & 1
+    {
+        const int e = min(base + (lane >> 1), last);
+        for (int i = 0; i < 16; i++) {
+            const int q = wave * 16 + i;
+            if (q >= t.nrhs) break;  // wave-uniform; the columns of absent planes are never written out
+            const float *g = reinterpret_cast<const float *>(t.eval + (size_t)q * t.evalStride + e) + (lane & 1);
+            __builtin_amdgcn_global_load_lds(g, (lds_void *)&T.x[q][0], 4, 0, 0);
+        }
+    }
+    // positions (wave 0, 1) and normals (wave 2, 3): 16 float4 = 64 dwords per instruction
+    {
+        const float4 *src = wave < 2 ? t.ep : t.en;
+        const int k = (wave & 1) * 16 + (lane >> 2);
+        const float *g = reinterpret_cast<const float *>(src + min(base + k, last)) + (lane & 3);
+        float4 *dst = wave < 2 ? &T.p[(wave & 1) * 16] : &T.n[(wave & 1) * 16];
+        __builtin_amdgcn_global_load_lds(g, (lds_void *)dst, 4, 0, 0);
+    }
+    // visibility words: block w of the two, 32 words = 64 dwords, by waves 0 and 1
+    if (wave < 2) {
+        const int rb = min(rb0 + wave, nrb - 1);  // rows past the end: the wave is inactive
+        const int k = lane >> 1;
+        const uint32_t *g =
+            reinterpret_cast<const uint32_t *>(t.mask + (size_t)rb * t.ne + min(base + k, last)) + (lane & 1);
+        __builtin_amdgcn_global_load_lds(g, (lds_void *)&T.m[wave][0], 4, 0, 0);
+    }
+}
+
+template <bool ATT>
+__global__ __launch_bounds__(256) void k_f2f_mfma(GatherTask t, Phys ph) {
+    __shared__ MfmaTile TA, TB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h = lane >> 5, j = lane & 31;
+    const int rt = blockIdx.x * 4 + wave;  // 32-receiver tile of this wave
+    const int split = blockIdx.y;
+    const int row = rt * 32 + j;
+    const bool rvalid = row < t.nr;
+    const bool waveActive = rt * 32 < t.nr;
+    const int nrb = (t.nr + 63) >> 6;
+    const int rb0 = blockIdx.x * 2;
+    const float4 rp = rvalid ? t.rp[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 rn = rvalid ? t.rn[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int mslot = wave >> 1;
+    const int mshift = (wave & 1) * 32 + j;
+    const int e0 = split * t.emPerSplit;
+    const int e1 = min(t.ne, e0 + t.emPerSplit);
+
+    f32x16 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = f32x16{};
+
+    // The two tile buffers are separate LDS objects and every phase names them
+    // statically, so the compiler sees that reads of one never wait on copies
+    // into the other.
+    auto compute = [&](const MfmaTile &T, int ne) {
+        // emitter pairs with any visible receiver of this wave: lane q < kMT/2 tests pair q
+        bool act = false;
+        if (waveActive && lane < kMT / 2 && 2 * lane < ne) {
+            uint64_t w = T.m[mslot][2 * lane];
+            if (2 * lane + 1 < ne) w |= T.m[mslot][2 * lane + 1];
+            act = ((w >> ((wave & 1) * 32)) & 0xffffffffull) != 0ull;
+        }
+        uint64_t pairs = __ballot(act);
+        while (pairs) {  // a straight-line body keeps the accumulators in place
+            const int k = 2 * __builtin_ctzll(pairs);
+            pairs &= pairs - 1;
+            const int ke = k + h;  // this lane's emitter (kMT is even: ke < kMT)
+            const bool bit = ke < ne && ((T.m[mslot][ke] >> mshift) & 1ull);
+            const float4 ep = T.p[ke];
+            float Kre, Kim;
+            pair_kernel<KIND_F2F, ATT>(ph, rp, rn, ep, T.n[ke], bit, Kre, Kim);
+            Kre *= ep.w;  // emitter area
+            Kim *= ep.w;
+            const auto sw =
+                __builtin_amdgcn_permlane32_swap(__float_as_uint(Kre), __float_as_uint(Kim), false, false);
+            const float a0 = __uint_as_float(sw[0]), a1 = __uint_as_float(sw[1]);
+            // B of this lane: h = 0 (multiplies Kr): Xr(j), Xr(32+j), Xi(j), Xi(32+j);
+            //                 h = 1 (multiplies Ki): -Xi(j), -Xi(32+j), Xr(j), Xr(32+j)
+            const float2 x00 = *reinterpret_cast<const float2 *>(&T.x[j][2 * k]);
+            const float2 x01 = *reinterpret_cast<const float2 *>(&T.x[32 + j][2 * k]);
+            const float2 x10 = *reinterpret_cast<const float2 *>(&T.x[j][2 * k + 2]);
+            const float2 x11 = *reinterpret_cast<const float2 *>(&T.x[32 + j][2 * k + 2]);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? -x00.y : x00.x, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? -x01.y : x01.x, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? x00.x : x00.y, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? x01.x : x01.y, acc[3], 0, 0, 0);
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? -x10.y : x10.x, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? -x11.y : x11.x, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? x10.x : x10.y, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? x11.x : x11.y, acc[3], 0, 0, 0);
+        }
+    };
+    if (e0 < e1) mfma_tile_dma(t, e0, rb0, nrb, wave, lane, TA);
+    int base = e0;
+    while (base < e1) {
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's share of TA has landed
+        __syncthreads();                // TA complete; every wave is done with TB
+        if (base + kMT < e1) mfma_tile_dma(t, base + kMT, rb0, nrb, wave, lane, TB);
+        compute(TA, min(kMT, e1 - base));
+        base += kMT;
+        if (base >= e1) break;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (base + kMT < e1) mfma_tile_dma(t, base + kMT, rb0, nrb, wave, lane, TA);
+        compute(TB, min(kMT, e1 - base));
+        base += kMT;
+    }
+    if (!waveActive) return;
+    // column j of tiles 0/2 is plane j, of tiles 1/3 plane 32 + j
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const int b = c * 32 + j;
+        if (b >= t.nrhs) continue;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = rt * 32 + (i >> 2) * 8 + h * 4 + (i & 3);
+            if (r >= t.nr) continue;
+            t.partL[((size_t)split * 64 + b) * t.nr + r] = make_float2(acc[c][i], acc[c + 2][i]);
+        }
+    }
+}
+
 __device__ inline void reduce_elem(const ReduceTask &t, long i) {
     if (i >= (long)t.nrhs * t.nr) return;
     const int b = (int)(i / t.nr), r = (int)(i % t.nr);
@@ -427,6 +586,19 @@ int gather_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st)
 }
 
 int gather_nb_stride(int nrhs) { return nb_bucket(nrhs); }
+
+int f2f_mfma_launch(const GatherTask &t, const Phys &ph, hipStream_t st) {
+    if (t.nr <= 0 || t.ne <= 0) return 0;
+    if (t.ecell || t.nrhs < 1 || t.nrhs > 64 || t.emPerSplit <= 0 || (long)t.nsplit * t.emPerSplit < t.ne) {
+        set_error("f2f_mfma: needs compact emitters, 1..64 planes and a covering emitter split");
+        return -2;
+    }
+    dim3 grid((t.nr + 127) / 128, t.nsplit);
+    if (ph.alpha != 0.0f) k_f2f_mfma<true><<<grid, 256, 0, st>>>(t, ph);
+    else k_f2f_mfma<false><<<grid, 256, 0, st>>>(t, ph);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
 
 int reduce_launch(const ReduceTask &t, hipStream_t st) {
     const long n = (long)t.nrhs * t.nr;

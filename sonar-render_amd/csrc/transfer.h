@@ -45,9 +45,8 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st);
 struct GatedFix {
     const int *list;        // gated rows (active indices)
     int count;
-    const int *facet;       // facet per active index (ascending)
     int n;
-    const float4 *p, *nrm;  // active points (area in w), normals (facet bits in w)
+    const float4 *p, *nrm;  // active points (area in w), normals (facet id bits in w, ascending)
     const float2 *Pin;      // [plane][stride]
     long stride;
     int nplanes;

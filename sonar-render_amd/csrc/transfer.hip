@@ -388,11 +388,11 @@ __global__ __launch_bounds__(64) void k_gated_part(GatedFix t, Phys ph) {
     const int split = blockIdx.y, b0 = blockIdx.z * 16;
     const bool valid = i < t.count;
     const int r = t.list[valid ? i : 0];
-    const int fr = t.facet[r];
+    const int fr = __float_as_int(t.nrm[r].w);
     int lo = 0, hi = r;  // first active index on facet fr
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (t.facet[mid] < fr) lo = mid + 1; else hi = mid;
+        if (__float_as_int(t.nrm[mid].w) < fr) lo = mid + 1; else hi = mid;
     }
     const int e0 = split * t.emPerSplit;
     const int e1 = min(t.n, e0 + t.emPerSplit);

@@ -293,6 +293,15 @@ _sig("sr_transfer_mode", [])
 KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9})
 
 
+_sig("sr_set_gather_mode", [_i])
+_sig("sr_gather_mode", [])
+
+
+def set_gather_mode(mode):
+    """Plain fragment->fragment: 0 FP32 matrix-core products from 16 planes on, 1 vector units only."""
+    _check(cpp_lib.sr_set_gather_mode(int(mode)), "set_gather_mode")
+
+
 def set_transfer_mode(mode):
     """0: plain FP32 gathers; 1: FP16 MFMA transfer matrix (FP32 accumulation)."""
     _check(cpp_lib.sr_set_transfer_mode(int(mode)), "set_transfer_mode")
