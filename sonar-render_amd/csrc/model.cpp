@@ -135,7 +135,7 @@ int upload_object(Model &m, Object &o) {
 void free_transfer(Object &o) {
     auto &t = o.tc;
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
-    dev_free(t.gatedList); dev_free(t.gatedCount);
+    dev_free(t.gatedList); dev_free(t.gatedCount); dev_free(t.Ac); dev_free(t.skipZero);
     t = Object::Transfer{};
 }
 
@@ -491,19 +491,49 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         if (reduce_launch(r, m.stream)) return -1;
     }
     if (!ngated) return 0;
-    const int fixWaves = ((ngated + 63) / 64) * ((nplanes + 15) / 16);
-    int fixSplit = std::max(1, std::min((2048 + fixWaves - 1) / std::max(1, fixWaves), t.ne / 64));
-    const int fixPer = (((t.ne + fixSplit - 1) / fixSplit) + 7) / 8 * 8;
-    fixSplit = (t.ne + fixPer - 1) / fixPer;
-    if (ensure_scratch(m, (size_t)fixSplit * nplanes * ngated)) return -1;
-    GatedFix f{};
-    f.list = m.gatedList; f.count = ngated; f.n = t.nr; f.p = t.ep; f.nrm = t.en;
-    f.Pin = t.gval; f.stride = gvalStride; f.nplanes = nplanes;
-    f.A = nullptr; f.nKs = 0; f.mask = t.mask; f.valueScale = 1.0f;
-    f.emPerSplit = fixPer; f.nsplit = fixSplit; f.part = m.scratchL;
-    f.Pout = rt.out; f.P = rt.pcell; f.pStride = pcellStride; f.rowCell = rt.rowCell;
-    ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-    return gated_fix_launch(f, ph, m.stream);
+    if (outStride != gvalStride) { set_error("gated MFMA gather: P_in and P_out strides differ"); return -2; }
+    // the gated rows as a compact receiver set whose words hold only lower-facet emitters
+    const size_t words = (size_t)((ngated + 63) / 64) * t.ne;
+    if (m.gCap < ngated) {
+        dev_free(m.gPc); dev_free(m.gNc); dev_free(m.gFs);
+        m.gCap = 0;
+        if (dev_alloc(&m.gPc, ngated) || dev_alloc(&m.gNc, ngated) || dev_alloc(&m.gFs, ngated)) return -1;
+        m.gCap = ngated;
+    }
+    if (m.gMcCap < words) {
+        dev_free(m.gMc);
+        m.gMcCap = 0;
+        if (dev_alloc(&m.gMc, words)) return -1;
+        m.gMcCap = words;
+    }
+    {
+        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
+        if (gated_compact_launch(m.gatedList, ngated, t.rp, t.rn, t.mask, t.ne, m.gPc, m.gNc, m.gFs, m.gMc, m.stream))
+            return -1;
+    }
+    GatherTask c = t;
+    c.evalStride = evalStride;
+    c.rp = m.gPc; c.rn = m.gNc; c.nr = ngated; c.mask = m.gMc;
+    c.gval = nullptr; c.gateMode = GATE_NONE; c.partL = m.scratchL; c.partU = nullptr;
+    // ~512 workgroups: enough to fill the chip, few partial sums per entry
+    const int cgroups = (ngated + 127) / 128;
+    int cs = std::max(1, std::min((512 + cgroups - 1) / cgroups, t.ne / 64));
+    c.emPerSplit = ((t.ne + cs - 1) / cs + 31) / 32 * 32;
+    c.nsplit = (t.ne + c.emPerSplit - 1) / c.emPerSplit;
+    if (ensure_scratch(m, (size_t)c.nsplit * 64 * ngated)) return -1;
+    c.partL = m.scratchL;
+    for (int b0 = 0; b0 < nplanes; b0 += 64) {
+        const int nb = std::min(64, nplanes - b0);
+        c.nrhs = nb;
+        c.eval = t.eval + b0 * evalStride;
+        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
+        if (f2f_mfma_launch(c, ph, m.stream)) return -1;
+        if (gated_replace_launch(m.scratchL, c.nsplit, ngated, m.gatedList, t.gval + b0 * gvalStride, gvalStride, nb,
+                                 rt.out + b0 * outStride, rt.pcell + b0 * pcellStride, pcellStride, rt.rowCell,
+                                 nullptr, m.stream))
+            return -1;
+    }
+    return 0;
 }
 
 static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, ReduceTask rt, int nplanes,
@@ -623,8 +653,8 @@ static int ensure_transfer(Model &m, int i) {
 }
 
 // One bounce of target i in the transfer form: P_out = S x for every row, then
-// the rows whose own P_in is zero in some plane are recomputed exactly in FP32
-// with the lower-facet emitters only (the zero-input gate), P += P_out.
+// the rows whose own P_in is zero in some plane get the product of their
+// lower-facet part of S (the zero-input gate) for those planes, P += P_out.
 static int bounce_mixed(Model &m, int i) {
     Object &o = m.targets[i];
     auto &t = o.tc;
@@ -640,12 +670,34 @@ static int bounce_mixed(Model &m, int i) {
     int gated = 0;
     SR_HIP(hipMemcpyAsync(&gated, t.gatedCount, sizeof(int), hipMemcpyDeviceToHost, m.stream));
     SR_HIP(hipStreamSynchronize(m.stream));
-    // gated-row recomputation: enough waves to fill the chip, emitter ranges of >= 64
-    const int fixWaves = ((gated + 63) / 64) * ((m.nplanes + 15) / 16);
-    int fixSplit = std::max(1, std::min((2048 + fixWaves - 1) / std::max(1, fixWaves), n / 64));
-    const int fixPer = (((n + fixSplit - 1) / fixSplit) + 7) / 8 * 8;
-    fixSplit = (n + fixPer - 1) / fixPer;
-    if (ensure_scratch(m, std::max((size_t)nsplit * 64 * n, (size_t)fixSplit * m.nplanes * gated))) return -1;
+    // the gated rows as compact row tiles of S (lower-facet emitters only)
+    const int nRtC = ((gated + 127) / 128) * 4;
+    int nsplitC = 1, ksPerC = t.nKs;
+    if (gated) {
+        if (m.gCap < gated) {
+            dev_free(m.gPc); dev_free(m.gNc); dev_free(m.gFs);
+            m.gCap = 0;
+            if (dev_alloc(&m.gPc, gated) || dev_alloc(&m.gNc, gated) || dev_alloc(&m.gFs, gated)) return -1;
+            m.gCap = gated;
+        }
+        const size_t need = (size_t)nRtC * t.nKs * 64;
+        if (t.acCap < need || t.skipZeroCap < nRtC / 4) {
+            dev_free(t.Ac); dev_free(t.skipZero);
+            t.acCap = 0; t.skipZeroCap = 0;
+            if (dev_alloc(&t.Ac, need) || dev_alloc(&t.skipZero, nRtC / 4)) return -1;
+            SR_HIP(hipMemset(t.skipZero, 0, (nRtC / 4) * sizeof(int2)));
+            t.acCap = need; t.skipZeroCap = nRtC / 4;
+        }
+        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
+        if (gated_compact_launch(t.gatedList, gated, o.actP, o.actN, nullptr, n, m.gPc, m.gNc, m.gFs, nullptr,
+                                 m.stream) ||
+            transfer_gather_rows(t.A, t.nKs, t.gatedList, m.gFs, gated, nRtC, t.Ac, m.stream))
+            return -1;
+        nsplitC = std::max(1, std::min(1024 / (nRtC / 4), t.nKs / 64));
+        ksPerC = ((t.nKs + nsplitC - 1) / nsplitC + 7) / 8 * 8;
+        nsplitC = (t.nKs + ksPerC - 1) / ksPerC;
+    }
+    if (ensure_scratch(m, std::max((size_t)nsplit * 64 * n, (size_t)nsplitC * 64 * gated))) return -1;
     for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
         const int np = std::min(64, m.nplanes - g0);
         const float2 *pin = o.Pin + (size_t)g0 * n;
@@ -678,18 +730,19 @@ static int bounce_mixed(Model &m, int i) {
         r.pcellStride = o.ncell;
         r.rowCell = o.actCell;
         r.colInvScale = t.colInv;
-        ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
-        if (reduce_launch(r, m.stream)) return -1;
+        {
+            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
+            if (reduce_launch(r, m.stream)) return -1;
+        }
+        if (!gated) continue;
+        TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, m.scratchL};
+        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
+        if (transfer_gemm(gc, m.stream) ||
+            gated_replace_launch(m.scratchL, nsplitC, gated, t.gatedList, pin, n, np, o.Pout + (size_t)g0 * n,
+                                 o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv, m.stream))
+            return -1;
     }
-    if (!gated) return 0;
-    GatedFix f{};
-    f.list = t.gatedList; f.count = gated; f.n = n; f.p = o.actP; f.nrm = o.actN;
-    f.Pin = o.Pin; f.stride = n; f.nplanes = m.nplanes;
-    f.A = t.A; f.nKs = t.nKs; f.mask = nullptr; f.valueScale = 1.0f / t.sS;
-    f.emPerSplit = fixPer; f.nsplit = fixSplit; f.part = m.scratchL;
-    f.Pout = o.Pout; f.P = o.P; f.pStride = o.ncell; f.rowCell = o.actCell;
-    ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-    return gated_fix_launch(f, make_phys(m, o.delta, 0.9f), m.stream);
+    return 0;
 }
 
 // FACET_SELF bounces (ModelCuda.cpp:323-356, FacetCuda.cpp:124-137, FacetCuda.cu:87-116).
@@ -937,6 +990,9 @@ void model_teardown(Model &m) {
     m.gatherMode = 0;
     dev_free(m.gatedList); dev_free(m.gatedCount);
     m.gatedCap = 0;
+    dev_free(m.gPc); dev_free(m.gNc); dev_free(m.gFs); dev_free(m.gMc);
+    m.gCap = 0;
+    m.gMcCap = 0;
     dev_free(m.srcP); dev_free(m.srcAmp); dev_free(m.fldP); dev_free(m.fldVal);
     dev_free(m.trisSorted); dev_free(m.nodes); dev_free(m.errFlag);
     dev_free(m.scratchL); dev_free(m.scratchU);

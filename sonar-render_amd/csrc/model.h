@@ -42,6 +42,10 @@ struct Object {
         int nKs = 0, nRt = 0;
         int2 *wgSkip = nullptr;    // per 128-row group: zero K-step range (its own single facet)
         int *gatedList = nullptr, *gatedCount = nullptr;  // rows with a zero input in some plane
+        uint4 *Ac = nullptr;       // ... as compact row tiles of A (lower-facet emitters only)
+        int2 *skipZero = nullptr;  // empty skip ranges for those tiles
+        size_t acCap = 0;
+        int skipZeroCap = 0;
         float *colScale = nullptr, *colInv = nullptr;
         float sS = 1.f;
         double aBytesRead = 0;  // bytes of S one GEMM pass reads (same-facet steps are skipped)
@@ -81,6 +85,11 @@ struct Model {
     int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
     int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
     int gatedCap = 0;
+    float4 *gPc = nullptr, *gNc = nullptr;  // ... as a compact receiver set
+    int *gFs = nullptr;
+    uint64_t *gMc = nullptr;
+    int gCap = 0;
+    size_t gMcCap = 0;
     int nearMode = 0;      // transfer matrix near field: 0 point rule, 1 16x16 quadrature, 2 learned
     float nearRadiusWl = 2.0f;
     float2 *scratchL = nullptr, *scratchU = nullptr;

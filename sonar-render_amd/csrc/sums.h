@@ -96,6 +96,17 @@ int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st);
 // matrix cores. Partial sums [split][64][nr] (plane stride 64); the gate is
 // not applied (gateMode is ignored).
 int f2f_mfma_launch(const GatherTask &t, const Phys &ph, hipStream_t st);
+// Gated rows (list of active indices) as a compact receiver set: positions,
+// normals, first lower-facet bound and (when mask is given) visibility words
+// [ceil(count/64)][ne] restricted to the lower-facet emitters.
+int gated_compact_launch(const int *list, int count, const float4 *p, const float4 *nrm, const uint64_t *mask, int ne,
+                         float4 *pc, float4 *nc, int *fsc, uint64_t *mc, hipStream_t st);
+// Replaces the zero-input entries of the listed rows with the compact set's
+// sums (partials [split][64][count], planes of one group of <= 64), each
+// plane's sum times colScale[b] when colScale is given.
+int gated_replace_launch(const float2 *part, int nsplit, int count, const int *list, const float2 *Pin, long stride,
+                         int np, float2 *Pout, float2 *P, long pStride, const int *rowCell, const float *colScale,
+                         hipStream_t st);
 
 // ---- multi-task (many scenes in one launch) ----
 struct PointsReduceTask {

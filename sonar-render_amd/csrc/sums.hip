@@ -209,8 +209,8 @@ __global__ __launch_bounds__(256) void k_gather_multi(const GatherTask *__restri
 // buffered one tile ahead; each lane picks its B values from the raw pressure
 // rows, and the emitter area scales K. A pair of emitters whose visibility
 // bits are all zero for the wave's 32 receivers is skipped. The zero-input
-// gate is not applied here: the rows it affects are recomputed afterwards with
-// the lower-facet emitters only (k_gated_part).
+// gate is not applied here: the rows it affects are recomputed afterwards as a
+// compact receiver set with the lower-facet emitters only.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void;
 constexpr int kMT = 32;       // emitters per LDS tile
@@ -258,7 +258,7 @@ __device__ __attribute__((always_inline)) inline void mfma_tile_dma(const Gather
 }
 
 template <bool ATT>
-__global__ __launch_bounds__(256) void k_f2f_mfma(GatherTask t, Phys ph) {
+__global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
     __shared__ MfmaTile TA, TB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int h = lane >> 5, j = lane & 31;
@@ -349,6 +349,74 @@ __global__ __launch_bounds__(256) void k_f2f_mfma(GatherTask t, Phys ph) {
             t.partL[((size_t)split * 64 + b) * t.nr + r] = make_float2(acc[c][i], acc[c + 2][i]);
         }
     }
+}
+
+// ---- gated rows of a matrix-core bounce, as a compact receiver set ----
+// Row i of the set is active row list[i]; its visibility words keep only the
+// emitters on lower facets (e < first index of the row's facet), which is
+// the zero-input gate of OptiX.cu:241-245. The set then goes through
+// k_f2f_mfma unchanged, and k_gated_replace swaps the result in for the
+// (row, plane) entries whose input is zero.
+__global__ void k_gated_compact_rows(const int *__restrict__ list, int count, const float4 *__restrict__ p,
+                                     const float4 *__restrict__ nrm, float4 *__restrict__ pc, float4 *__restrict__ nc,
+                                     int *__restrict__ fsc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int r = list[i];
+    const float4 rn = nrm[r];
+    const int fr = __float_as_int(rn.w);
+    int lo = 0, hi = r;  // first active index on facet fr (facets ascend)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (__float_as_int(nrm[mid].w) < fr) lo = mid + 1; else hi = mid;
+    }
+    pc[i] = p[r];
+    nc[i] = rn;
+    fsc[i] = lo;
+}
+
+// word (g, e) of the compact set: bit i = visibility bit of (list[64 g + i], e) and e < fs
+__global__ void k_gated_compact_mask(const int *__restrict__ list, const int *__restrict__ fsc, int count,
+                                     const uint64_t *__restrict__ mask, int ne, uint64_t *__restrict__ mc) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ng = (count + 63) / 64;
+    if (t >= (long)ng * ne) return;
+    const int g = (int)(t / ne), e = (int)(t % ne);
+    uint64_t w = 0;
+    const int n = min(64, count - 64 * g);
+    for (int i = 0; i < n; i++) {
+        const int r = list[64 * g + i];
+        if (e < fsc[64 * g + i]) w |= ((mask[(size_t)(r >> 6) * ne + e] >> (r & 63)) & 1ull) << i;
+    }
+    mc[t] = w;
+}
+
+// P_out[b][r] <- L, P[b][cell(r)] += L - (old P_out) where P_in[b][r] == 0; L from the partials [split][64][count].
+__global__ void k_gated_replace(const float2 *__restrict__ part, int nsplit, int count, const int *__restrict__ list,
+                                const float2 *__restrict__ Pin, long stride, int np, float2 *__restrict__ Pout,
+                                float2 *__restrict__ P, long pStride, const int *__restrict__ rowCell,
+                                const float *__restrict__ colScale) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)count * np) return;
+    const int b = (int)(k / count), i = (int)(k % count);
+    const int r = list[i];
+    const float2 g = Pin[(size_t)b * stride + r];
+    if (!(g.x == 0.0f && g.y == 0.0f)) return;
+    float2 L = make_float2(0.f, 0.f);
+    for (int s = 0; s < nsplit; s++) {
+        const float2 v = part[((size_t)s * 64 + b) * count + i];
+        L.x += v.x;
+        L.y += v.y;
+    }
+    if (colScale) {
+        L.x *= colScale[b];
+        L.y *= colScale[b];
+    }
+    float2 &o = Pout[(size_t)b * stride + r];
+    float2 &c = P[(size_t)b * pStride + rowCell[r]];
+    c.x += L.x - o.x;
+    c.y += L.y - o.y;
+    o = L;
 }
 
 __device__ inline void reduce_elem(const ReduceTask &t, long i) {
@@ -586,6 +654,29 @@ int gather_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st)
 }
 
 int gather_nb_stride(int nrhs) { return nb_bucket(nrhs); }
+
+int gated_compact_launch(const int *list, int count, const float4 *p, const float4 *nrm, const uint64_t *mask, int ne,
+                         float4 *pc, float4 *nc, int *fsc, uint64_t *mc, hipStream_t st) {
+    if (count <= 0) return 0;
+    k_gated_compact_rows<<<(count + 255) / 256, 256, 0, st>>>(list, count, p, nrm, pc, nc, fsc);
+    SR_HIP(hipGetLastError());
+    if (!mask) return 0;  // row bounds only
+    const long words = (long)((count + 63) / 64) * ne;
+    k_gated_compact_mask<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(list, fsc, count, mask, ne, mc);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int gated_replace_launch(const float2 *part, int nsplit, int count, const int *list, const float2 *Pin, long stride,
+                         int np, float2 *Pout, float2 *P, long pStride, const int *rowCell, const float *colScale,
+                         hipStream_t st) {
+    const long m = (long)count * np;
+    if (m <= 0) return 0;
+    k_gated_replace<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(part, nsplit, count, list, Pin, stride, np, Pout, P,
+                                                                 pStride, rowCell, colScale);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
 
 int f2f_mfma_launch(const GatherTask &t, const Phys &ph, hipStream_t st) {
     if (t.nr <= 0 || t.ne <= 0) return 0;

@@ -39,29 +39,10 @@ int transfer_col_scale(const float4 *p, const float2 *Pin, long stride, int n, i
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes, const float *scale,
                      uint4 *B, hipStream_t st);
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
-// Rows of one object's SELF bounce whose own input pressure is exactly zero in
-// some plane, recomputed with the lower-facet emitters only after a product
-// that summed every emitter. Active cells ascend by facet.
-struct GatedFix {
-    const int *list;        // gated rows (active indices)
-    int count;
-    int n;
-    const float4 *p, *nrm;  // active points (area in w), normals (facet id bits in w, ascending)
-    const float2 *Pin;      // [plane][stride]
-    long stride;
-    int nplanes;
-    const uint4 *A;         // FP16 transfer matrix (pair values read back), or null: evaluate in FP32
-    int nKs;
-    const uint64_t *mask;   // SELF visibility words (FP32 evaluation only)
-    float valueScale;       // 1 / (S scale) for A, 1 otherwise
-    int emPerSplit, nsplit; // emitter ranges, emPerSplit a multiple of 8
-    float2 *part;           // [split][nplanes][count]
-    float2 *Pout;           // [plane][stride]
-    float2 *P;              // [plane][pStride] through rowCell
-    long pStride;
-    const int *rowCell;
-};
-int gated_fix_launch(const GatedFix &t, const Phys &ph, hipStream_t st);
+// Gated rows as compact row tiles of the transfer matrix (nRtC tiles of 32,
+// a multiple of 4): emitters at or above each row's facet bound fsc[i] zeroed.
+int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *fsc, int count, int nRtC, uint4 *Ac,
+                         hipStream_t st);
 // Lists the rows with an exactly zero input pressure in some plane (count: device int, zeroed by the caller).
 int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st);
 

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
 // A register sets). Chunks whose emitters all lie on the workgroup's single
 // facet hold only zeros (same-facet pairs never interact) and are skipped.
 // One FP32 accumulator: rows that the zero-input gate affects are recomputed
-// afterwards (k_gated_part).
+// afterwards as compact row tiles (k_gather_a_rows).
 constexpr int kKC = 8;
 static_assert(kKC == 8, "K steps and split sizes are padded to multiples of 8");
 
@@ -368,125 +368,32 @@ __global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes,
     if (z) list[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
 }
 
-// Exact zero-input gate. For each listed row r with P_in[b][r] == 0 in plane
-// b, OptiX.cu:241-245 keeps only the emitters on lower facets; active cells
-// ascend by facet, so these are the emitters [0, fs(r)). The sum is redone for
-// those (row, plane) entries only and replaces what the full product gave.
-// The pair values are read back from the FP16 transfer matrix (FROM_S: the
-// gated rows see exactly the operator the other rows see, near-field terms
-// included) or evaluated in FP32 (plain form). A wave owns 64 listed rows, an
-// emitter range (split) and 16 planes; emitters are wave-uniform, so their
-// position and pressures are broadcast loads.
-constexpr int kGT = 64;  // emitters per LDS tile of the gated-row kernel
-
-template <bool FROM_S, bool ATT>
-__global__ __launch_bounds__(64) void k_gated_part(GatedFix t, Phys ph) {
-    __shared__ float2 s_x[16][kGT];  // A_e P_in[b][e] for this wave's 16 planes
-    __shared__ float4 s_p[kGT], s_n[kGT];
-    const int lane = threadIdx.x;
-    const int i = blockIdx.x * 64 + lane;
-    const int split = blockIdx.y, b0 = blockIdx.z * 16;
-    const bool valid = i < t.count;
-    const int r = t.list[valid ? i : 0];
-    const int fr = __float_as_int(t.nrm[r].w);
-    int lo = 0, hi = r;  // first active index on facet fr
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (__float_as_int(t.nrm[mid].w) < fr) lo = mid + 1; else hi = mid;
-    }
-    const int e0 = split * t.emPerSplit;
-    const int e1 = min(t.n, e0 + t.emPerSplit);
-    const int eEnd = valid ? min(e1, lo) : e0;  // this lane's emitters: [e0, eEnd)
-    int eMax = eEnd;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) eMax = max(eMax, __shfl_xor(eMax, m));
-
-    // Every plane of the chunk is accumulated; only the entries whose input is
-    // zero are used (k_gated_apply checks).
-    float2 acc[16];
-#pragma unroll
-    for (int b = 0; b < 16; b++) acc[b] = make_float2(0.f, 0.f);
-    const uint4 *Arow = FROM_S ? t.A + (size_t)(r >> 5) * t.nKs * 64 + (r & 31) : nullptr;
-    const float4 rp = FROM_S ? make_float4(0.f, 0.f, 0.f, 0.f) : t.p[r];
-    const float4 rn = FROM_S ? make_float4(0.f, 0.f, 0.f, 0.f) : t.nrm[r];
-    const uint64_t *mrow = FROM_S ? nullptr : t.mask + (size_t)(r >> 6) * t.n;
-
-    for (int base = e0; base < eMax; base += kGT) {  // base is a multiple of 8
-        const int ne = min(kGT, eMax - base);
-        __syncthreads();
-        if (lane < ne) {
-            const int e = base + lane;
-            const float4 pe = t.p[e];
-            if (!FROM_S) {
-                s_p[lane] = pe;
-                s_n[lane] = t.nrm[e];
-            }
-#pragma unroll
-            for (int b = 0; b < 16; b++) {
-                const float2 v = (b0 + b < t.nplanes) ? t.Pin[(size_t)(b0 + b) * t.stride + e] : make_float2(0.f, 0.f);
-                s_x[b][lane] = make_float2(pe.w * v.x, pe.w * v.y);
-            }
-        }
-        __syncthreads();
-        if (FROM_S) {
-            // A fragment order: lane (row & 31) + 32 * ((e >> 2) & 1) of K step e >> 3
-            // holds emitters 4 * ((e >> 2) & 1) .. +3 of the step as (re, im) halves.
-#pragma unroll 2
-            for (int k = 0; k < ne; k += 4) {
-                const int e = base + k;
-                uint4 q = make_uint4(0, 0, 0, 0);
-                if (e < eEnd) q = Arow[(size_t)(e >> 3) * 64 + ((e >> 2) & 1) * 32];
-                const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const bool in = e + j < eEnd;  // also false past ne
-                    const float Kre = in ? (float)__builtin_bit_cast(_Float16, (uint16_t)(w[j] & 0xffffu)) : 0.0f;
-                    const float Kim = in ? (float)__builtin_bit_cast(_Float16, (uint16_t)(w[j] >> 16)) : 0.0f;
-#pragma unroll
-                    for (int b = 0; b < 16; b++) cmac<16>(acc[b], Kre, Kim, s_x[b][min(k + j, kGT - 1)]);
-                }
-            }
-        } else {
-            const int sh = r & 63;
-            for (int k = 0; k < ne; k++) {
-                const int e = base + k;
-                const bool bit = e < eEnd && ((mrow[e] >> sh) & 1ull);
-                float Kre, Kim;
-                pair_kernel<KIND_F2F, ATT>(ph, rp, rn, s_p[k], s_n[k], bit, Kre, Kim);
-#pragma unroll
-                for (int b = 0; b < 16; b++) cmac<16>(acc[b], Kre, Kim, s_x[b][k]);
-            }
+// Gated rows of a transfer-form bounce as a compact set of row tiles: row i
+// of the set is active row list[i], its FP16 pair values copied from the
+// transfer matrix with the emitters at or above its own facet zeroed (the
+// zero-input gate of OptiX.cu:241-245 keeps lower facets only). The set then
+// goes through k_transfer_gemm with the bounce's B: the gated rows see exactly
+// the operator the other rows see, near-field terms included.
+__global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int *__restrict__ list,
+                                const int *__restrict__ fsc, int count, int nRtC, uint4 *__restrict__ Ac) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nRtC * nKs * 64) return;
+    const int lane = (int)(t & 63);
+    const long tk = t >> 6;
+    const int ks = (int)(tk % nKs), ct = (int)(tk / nKs);
+    const int i = ct * 32 + (lane & 31), h = lane >> 5;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (i < count) {
+        const int r = list[i], fs = fsc[i];
+        const int j0 = ks * 8 + h * 4;  // first emitter of this fragment
+        if (j0 < fs) {
+            q = A[((size_t)(r >> 5) * nKs + ks) * 64 + (r & 31) + 32 * h];
+            if (j0 + 1 >= fs) q.y = 0u;
+            if (j0 + 2 >= fs) q.z = 0u;
+            if (j0 + 3 >= fs) q.w = 0u;
         }
     }
-    if (!valid) return;
-#pragma unroll
-    for (int b = 0; b < 16; b++) {
-        if (b0 + b >= t.nplanes) continue;
-        t.part[((size_t)split * t.nplanes + b0 + b) * t.count + i] = acc[b];
-    }
-}
-
-// P_out[b][r] <- L, P[b][cell(r)] += L - (old P_out) for the zero-input entries.
-__global__ void k_gated_apply(GatedFix t) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (long)t.count * t.nplanes) return;
-    const int b = (int)(k / t.count), i = (int)(k % t.count);
-    const int r = t.list[i];
-    const float2 g = t.Pin[(size_t)b * t.stride + r];
-    if (!(g.x == 0.0f && g.y == 0.0f)) return;
-    float2 L = make_float2(0.f, 0.f);
-    for (int s = 0; s < t.nsplit; s++) {
-        const float2 v = t.part[((size_t)s * t.nplanes + b) * t.count + i];
-        L.x += v.x;
-        L.y += v.y;
-    }
-    L.x *= t.valueScale;
-    L.y *= t.valueScale;
-    float2 &o = t.Pout[(size_t)b * t.stride + r];
-    float2 &P = t.P[(size_t)b * t.pStride + t.rowCell[r]];
-    P.x += L.x - o.x;
-    P.y += L.y - o.y;
-    o = L;
+    Ac[t] = q;
 }
 
 // ---------------------------------------------------------------------------
@@ -537,25 +444,11 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     return 0;
 }
 
-int gated_fix_launch(const GatedFix &t, const Phys &ph, hipStream_t st) {
-    if (t.count <= 0 || t.nplanes <= 0) return 0;
-    if (t.emPerSplit <= 0 || t.emPerSplit % 8 || (long)t.nsplit * t.emPerSplit < t.n) {
-        set_error("gated fix: bad emitter split");
-        return -2;
-    }
-    if (t.A && t.nKs * 8 < t.n) { set_error("gated fix: transfer matrix too small"); return -2; }
-    dim3 grid((t.count + 63) / 64, t.nsplit, (t.nplanes + 15) / 16);
-    const bool att = ph.alpha != 0.0f;
-    if (t.A) {
-        if (att) k_gated_part<true, true><<<grid, 64, 0, st>>>(t, ph);
-        else k_gated_part<true, false><<<grid, 64, 0, st>>>(t, ph);
-    } else {
-        if (att) k_gated_part<false, true><<<grid, 64, 0, st>>>(t, ph);
-        else k_gated_part<false, false><<<grid, 64, 0, st>>>(t, ph);
-    }
-    SR_HIP(hipGetLastError());
-    const long m = (long)t.count * t.nplanes;
-    k_gated_apply<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(t);
+int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *fsc, int count, int nRtC, uint4 *Ac,
+                         hipStream_t st) {
+    const long tot = (long)nRtC * nKs * 64;
+    if (tot <= 0) return 0;
+    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, fsc, count, nRtC, Ac);
     SR_HIP(hipGetLastError());
     return 0;
 }
