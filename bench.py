@@ -9,13 +9,19 @@ One step = one whole scene for all 64 incidences with the geometry resident in
 HBM: zero the pressures, rebuild every visibility word, point->fragment, the
 3 bounces (rebuilding the transfer matrix in the mixed form), fragment->point.
 
-Two forms are timed back to back on the same scene:
-  plain_fp32   the pair kernel evaluated on the fly in FP32 every bounce;
+Three forms are timed back to back:
+  plain_fp32   the pair kernel evaluated on the fly in FP32 every bounce (its
+               64-incidence complex products on the FP32 matrix cores);
   mixed_fp16   the pair kernel stored once as a complex FP16 matrix, every
-               bounce an FP16 MFMA GEMM with FP32 accumulation.
-Each form's output is checked against the FP64 oracle on the incidences the
-CPU baseline runs. `value` is the Gpairs/s of the fastest form whose rel-L2
-is <= 1e-3.
+               bounce an FP16 MFMA GEMM with FP32 accumulation;
+  learned      the same scene fragmented at lambda/4 (3.5x fewer fragments),
+               pairs closer than 2 lambda corrected by the learned near-field
+               network (trained once on the GPU before timing), mixed form.
+plain and mixed are checked against the FP64 oracle (point rule, lambda/7.5)
+on the incidences the CPU baseline runs; learned against the FP64 oracle with
+the 16x16 sub-fragment quadrature judge at lambda/4, and its field points
+against the lambda/7.5 oracle. `value` is the Gpairs/s of the fastest form
+whose rel-L2 is <= 1e-3.
 
 Prints one JSON line. --gpus > 1 is not supported (one GPU per process).
 The engine is HIP code behind a C-ABI; timing brackets are device-wide
@@ -66,11 +72,13 @@ def cpu_baseline(scene, sample_planes):
     return time.perf_counter() - t0, threads, outs
 
 
-def time_form(scene, mode, steps, warmup):
+def time_form(scene, mode, steps, warmup, near=None):
     import runner
     import sonar_api as api
 
     runner.load_scene(scene, transfer_mode=mode)
+    if near:
+        api.set_near_mode(*near)
     api.device_synchronize()
 
     def step():
@@ -121,7 +129,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="trihedral", choices=sorted(_configs()))
-    ap.add_argument("--forms", default="plain,mixed", help="comma list of plain, mixed")
+    ap.add_argument("--forms", default="plain,mixed,learned", help="comma list of plain, mixed, learned")
     ap.add_argument("--cpu-sample", type=int, default=1, help="incidences the CPU baseline runs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -146,13 +154,28 @@ def main():
     scene = _configs()[args.config]()
     n_planes = len(scene["sources"]) if scene.get("batched") else 1
     pairs = runner.pair_count(scene)
-    forms = {"plain": ("plain_fp32", 0), "mixed": ("mixed_fp16_mfma", 1)}
-    chosen = [forms[f] for f in args.forms.split(",") if f in forms]
+    want = args.forms.split(",")
+    forms = [("plain_fp32", 0, None, scene, "plain"), ("mixed_fp16_mfma", 1, None, scene, "mixed")]
+    learned = None
+    if "learned" in want and scene.get("bounces", 0):
+        import scenes
+        import sonar_api as api
+        lam = scene["c"] / scene["f"]
+        coarse = dict(scene, name=scene["name"] + "_lambda4",
+                      targets=[(tris, lam / 4.0) for tris, _ in scene["targets"]])
+        api.nf_init(1)
+        t0 = time.perf_counter()
+        tr, va = api.nf_train(nsamples=262144, steps=4000, batch=4096, lr=3e-3, seed=7)
+        api.device_synchronize()
+        learned = dict(scene=coarse, lam=lam, train_s=time.perf_counter() - t0, train_err=tr, heldout_err=va)
+        forms.append(("learned_mixed_fp16_lambda4", 1, (2, 2.0), coarse, "learned"))
+    chosen = [f for f in forms if f[4] in want]
 
     results = {}
-    for name, mode in chosen:
-        dt, stats, res = time_form(scene, mode, args.steps, args.warmup)
-        results[name] = dict(mode=mode, dt=dt, stats=stats, res=res)
+    for name, mode, near, sc, _ in chosen:
+        dt, stats, res = time_form(sc, mode, args.steps, args.warmup, near)
+        results[name] = dict(mode=mode, dt=dt, stats=stats, res=res, scene=sc,
+                             pairs=runner.pair_count(sc) if sc is not scene else pairs)
         print(f"[bench] {name}: {dt / args.steps * 1e3:.3f} ms/scene", file=sys.stderr, flush=True)
 
     cpu = None
@@ -163,32 +186,52 @@ def main():
         cpu = {"value": round(pairs * frac / cdt / 1e9, 5), "unit": "Gpairs/s", "cores": threads, "kind": "port",
                "sample": f"{len(sample)} of {n_planes} incidences of the same scene, FP64 OpenMP oracle; "
                          f"{cdt:.2f} s -> {cdt / frac:.1f} s/scene extrapolated"}
+        import oracle
         for name, r in results.items():
-            errs = []
+            errs, perrs = [], []
             for p, o in zip(sample, outs):
-                a = np.concatenate([r["res"]["points"][p], r["res"]["targets"][0][p]])
-                b = np.concatenate([o["points"], o["targets"][0]])
+                if r["scene"] is scene:
+                    a = np.concatenate([r["res"]["points"][p], r["res"]["targets"][0][p]])
+                    b = np.concatenate([o["points"], o["targets"][0]])
+                else:  # learned: the judge at lambda/4, and the field points of the lambda/7.5 oracle
+                    j = oracle.run_scene(r["scene"], plane_source=p if scene.get("batched") else None,
+                                         near=(16, 2.0 * learned["lam"]))
+                    a = np.concatenate([r["res"]["points"][p], r["res"]["targets"][0][p]])
+                    b = np.concatenate([j["points"], j["targets"][0]])
+                    q = r["res"]["points"][p]
+                    perrs.append(float(np.linalg.norm(q - o["points"]) / np.linalg.norm(o["points"])))
                 errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
             r["err"] = max(errs)
+            if perrs:
+                r["err_points_vs_lambda7p5"] = max(perrs)
 
     per_form = {}
     for name, r in results.items():
         ms = r["dt"] / args.steps * 1e3
         per_form[name] = {
-            "gpairs_per_s": round(pairs * args.steps / r["dt"] / 1e9, 3), "ms_per_scene": round(ms, 4),
-            "rel_l2_vs_fp64": r.get("err"),
+            "gpairs_per_s": round(r["pairs"] * args.steps / r["dt"] / 1e9, 3), "ms_per_scene": round(ms, 4),
+            "pairs_per_scene": r["pairs"], "rel_l2_vs_fp64": r.get("err"),
             "kernel_ms_per_scene": {k: round(v["ms"] / args.steps, 4) for k, v in r["stats"].items() if v["launches"]},
-            "roofline": roofline(r["stats"], r["mode"], scene["name"]),
+            "roofline": roofline(r["stats"], r["mode"], r["scene"]["name"]),
         }
-    valid = [n for n in per_form if per_form[n]["rel_l2_vs_fp64"] is None or per_form[n]["rel_l2_vs_fp64"] <= TOL]
-    primary = min(valid or list(per_form), key=lambda n: per_form[n]["ms_per_scene"])
+    valid = [n for n in per_form if per_form[n]["rel_l2_vs_fp64"] is not None and per_form[n]["rel_l2_vs_fp64"] <= TOL]
+    # without a CPU check, the fastest form at the reference's own fragment size
+    primary = min(valid or [n for n in per_form if results[n]["scene"] is scene] or list(per_form),
+                  key=lambda n: per_form[n]["ms_per_scene"])
     P = per_form[primary]
-    speedup = None
+    speedup = speedup_l = None
     if "plain_fp32" in per_form and "mixed_fp16_mfma" in per_form:
         speedup = round(per_form["plain_fp32"]["ms_per_scene"] / per_form["mixed_fp16_mfma"]["ms_per_scene"], 2)
+    if "learned_mixed_fp16_lambda4" in per_form:
+        L = per_form["learned_mixed_fp16_lambda4"]
+        L["rel_l2_vs"] = "FP64 oracle with the 16x16 quadrature judge, same lambda/4 fragments"
+        L["rel_l2_points_vs_lambda7p5_fp64"] = results["learned_mixed_fp16_lambda4"].get("err_points_vs_lambda7p5")
+        L["training"] = {k: round(float(learned[k]), 4) for k in ("train_s", "train_err", "heldout_err")}
+        if "plain_fp32" in per_form:
+            speedup_l = round(per_form["plain_fp32"]["ms_per_scene"] / L["ms_per_scene"], 2)
 
     line = {
-        "metric": "Gpairs/s & s/scene at <=1e-3 rel-L2 vs FP64: mixed vs plain FP32",
+        "metric": "Gpairs/s & s/scene at <=1e-3 rel-L2 vs FP64: learned+mixed vs plain FP32",
         "value": P["gpairs_per_s"], "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": P["ms_per_scene"], "s_per_scene": round(P["ms_per_scene"] / 1e3, 6),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -196,6 +239,7 @@ def main():
         "config": {"workload": scene["name"], "form": primary, "incidences": n_planes,
                    "bounces": scene.get("bounces", 0), "pairs_per_scene": pairs, "parallelism": "single GPU"},
         "rel_l2_vs_fp64": P["rel_l2_vs_fp64"], "speedup_mixed_vs_plain": speedup,
+        "speedup_learned_vs_plain": speedup_l,
         "forms": per_form, "roofline": P["roofline"], "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

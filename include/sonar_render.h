@@ -187,7 +187,8 @@ int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, f
  * 1 visibility words, 2 fragment->fragment, 3 fragment->field object,
  * 4 fragment->field point, 5 split-K reductions, 6 transfer-matrix build,
  * 7 transfer GEMM, 8 incidence-matrix build, 9 exact recomputation of the
- * rows a zero input gates (matrix-core forms). Reports summed kernel time,
+ * rows a zero input gates (matrix-core forms), 10 near-field terms of the
+ * transfer matrix (quadrature / learned network; pairs = near pairs). Reports summed kernel time,
  * launches, ordered pairs (x planes), flops (vector gathers: 37 FP32
  * flops per pair plus 8 per pair and plane; matrix-core fragment->fragment:
  * the 8 per pair and plane that run on the matrix cores; transfer GEMM: 2 M N K as executed) and

@@ -136,6 +136,7 @@ void free_transfer(Object &o) {
     auto &t = o.tc;
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
     dev_free(t.gatedList); dev_free(t.gatedCount); dev_free(t.Ac); dev_free(t.skipZero);
+    dev_free(t.nearQ); dev_free(t.nearM); dev_free(t.nearCount);
     t = Object::Transfer{};
 }
 
@@ -638,15 +639,40 @@ static int ensure_transfer(Model &m, int i) {
     TransferGeom g{o.actP, o.actN, n, nKs, nRt};
     TransferPhys ph{m.kf, m.atten, (float)((double)m.atten * 1.4426950408889634), o.delta, 0.9f, t.sS,
                     (double)m.kf / (2.0 * M_PI), m.nearMode,
-                    (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr};
+                    (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr,
+                    nullptr, nullptr, nullptr, 0};
     if (m.nearMode == 2) {
         NearField &nf = nearfield();
         if (!nf.trained) { set_error("learned near-field mode needs a trained network (sr_nf_train)"); return -3; }
         ph.mlp = nf.P;
     }
-    {
-        ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, 0.0);
-        if (transfer_build_s(g, ph, mk, t.A, m.stream)) return -1;
+    int counts[2] = {0, 0};
+    for (int pass = 0; pass < 2; pass++) {
+        if (m.nearMode != 0) {
+            if (!t.nearCount && dev_alloc(&t.nearCount, 2)) return -1;
+            if (t.nearCap == 0) {
+                // first guess: 64 near neighbours per fragment; grown to the count if it overflows
+                t.nearCap = std::max(1024, 64 * n);
+                if (dev_alloc(&t.nearQ, t.nearCap) || dev_alloc(&t.nearM, t.nearCap)) return -1;
+            }
+            ph.nearQ = t.nearQ; ph.nearM = t.nearM; ph.nearCount = t.nearCount; ph.nearCap = t.nearCap;
+            if (counters_zero_launch(t.nearCount, 2, m.stream)) return -1;
+        }
+        {
+            ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, 0.0);
+            if (transfer_build_s(g, ph, mk, t.A, m.stream)) return -1;
+        }
+        if (m.nearMode == 0) break;
+        SR_HIP(hipMemcpyAsync(counts, t.nearCount, 2 * sizeof(int), hipMemcpyDeviceToHost, m.stream));
+        SR_HIP(hipStreamSynchronize(m.stream));
+        if (counts[0] <= t.nearCap && counts[1] <= t.nearCap) break;
+        dev_free(t.nearQ); dev_free(t.nearM);
+        t.nearCap = std::max(counts[0], counts[1]) + std::max(counts[0], counts[1]) / 4;
+        if (dev_alloc(&t.nearQ, t.nearCap) || dev_alloc(&t.nearM, t.nearCap)) return -1;
+    }
+    if (m.nearMode != 0) {
+        ProfScope ps(m, PK_NEAR, (double)counts[0] + counts[1], 0.0);
+        if (transfer_near(g, ph, counts[0], counts[1], t.A, m.stream)) return -1;
     }
     t.valid = true;
     return 0;

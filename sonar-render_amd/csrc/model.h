@@ -46,6 +46,9 @@ struct Object {
         int2 *skipZero = nullptr;  // empty skip ranges for those tiles
         size_t acCap = 0;
         int skipZeroCap = 0;
+        int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
+        int *nearCount = nullptr;
+        int nearCap = 0;
         float *colScale = nullptr, *colInv = nullptr;
         float sS = 1.f;
         double aBytesRead = 0;  // bytes of S one GEMM pass reads (same-facet steps are skipped)
@@ -103,7 +106,7 @@ struct Model {
 // Kernel families reported by the profiler.
 enum ProfKind {
     PK_P2F = 0, PK_VIS = 1, PK_F2F = 2, PK_F2F_FIELD = 3, PK_F2P = 4, PK_REDUCE = 5,
-    PK_BUILD_S = 6, PK_GEMM = 7, PK_BUILD_B = 8, PK_GATED_FIX = 9, PK_COUNT = 10
+    PK_BUILD_S = 6, PK_GEMM = 7, PK_BUILD_B = 8, PK_GATED_FIX = 9, PK_NEAR = 10, PK_COUNT = 11
 };
 
 }  // namespace sr

@@ -22,6 +22,10 @@ struct TransferPhys {
     float nearRadius;   // metres
     const float4 *frames;  // per facet: xAxis (w unused), yAxis
     const float *mlp;      // kMlpParams FP32 weights (rounded to BF16 on load)
+    // near pairs found by the S build: by quadrature (nearQ) and learned (nearM)
+    int2 *nearQ, *nearM;   // (row, emitter)
+    int *nearCount;        // [2] device counters, zeroed by the caller
+    int nearCap;           // capacity of each list
 };
 
 struct TransferGemm {
@@ -34,6 +38,8 @@ struct TransferGemm {
 };
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
+// Writes the listed near pairs' values into S (counts read back from nearCount).
+int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st);
 int transfer_col_scale(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, float *scale,
                        float *invScale, float sS, hipStream_t st);
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes, const float *scale,

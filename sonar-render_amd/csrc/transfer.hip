@@ -92,53 +92,185 @@ __device__ float2 quad16(float3 rv, float3 nr, float d, float k) {
     return make_float2(acc.x * (1.0f / 256.0f), acc.y * (1.0f / 256.0f));
 }
 
-// MLP in BF16 (weights pre-rounded in LDS, activations rounded per layer).
-__device__ float2 mlp_bf16(const float *w, const float *f) {
-    const float *W1 = w, *b1 = w + kOffB1, *W2 = w + kOffW2, *b2 = w + kOffB2, *W3 = w + kOffW3, *b3 = w + kOffB3;
-    float h1[64];
-    for (int o = 0; o < 64; o++) {
-        float a = b1[o];
-        for (int q = 0; q < kMlpIn; q++) a = fmaf(W1[o * kMlpIn + q], bf16_round(f[q]), a);
-        h1[o] = bf16_round(tanhf(a));
-    }
-    float o0 = b3[0], o1 = b3[1];
-    for (int o = 0; o < 64; o++) {
-        float a = b2[o];
-        for (int q = 0; q < 64; q++) a = fmaf(W2[o * 64 + q], h1[q], a);
-        const float h = bf16_round(tanhf(a));
-        o0 = fmaf(W3[o], h, o0);
-        o1 = fmaf(W3[64 + o], h, o1);
-    }
-    return make_float2(o0, o1);
+// Geometry of one near pair in the emitter's frame (x = xAxis, y = yAxis,
+// z = emitter normal): receiver point relative to the emitter point, and the
+// receiver normal.
+struct NearGeom {
+    float3 rv, nr;
+    float r;
+};
+
+__device__ inline NearGeom near_geom(const TransferGeom &g, const TransferPhys &ph, int row, int j) {
+    const float4 rp = g.p[row], rn = g.nrm[row], ep = g.p[j], en = g.nrm[j];
+    const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
+    const int fe = __float_as_int(en.w);
+    const float4 xa = ph.frames[2 * fe], ya = ph.frames[2 * fe + 1];
+    NearGeom q;
+    q.rv = make_float3(vx * xa.x + vy * xa.y + vz * xa.z, vx * ya.x + vy * ya.y + vz * ya.z,
+                       vx * en.x + vy * en.y + vz * en.z);
+    q.nr = make_float3(rn.x * xa.x + rn.y * xa.y + rn.z * xa.z, rn.x * ya.x + rn.y * ya.y + rn.z * ya.z,
+                       rn.x * en.x + rn.y * en.y + rn.z * en.z);
+    q.r = sqrtf(q.rv.x * q.rv.x + q.rv.y * q.rv.y + q.rv.z * q.rv.z);
+    return q;
 }
 
-template <int NEAR>
-__device__ float2 near_value(float3 rv, float3 nr, float d, float k, const float *w) {
-    const float r = sqrtf(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
-    if (NEAR == 1 || r < d) return quad16(rv, nr, d, k);  // the learned range starts at r = delta
-    const float lam = 2.0f * 3.14159265358979f / k;
-    const float cs = rv.z / r, ci = -(rv.x * nr.x + rv.y * nr.y + rv.z * nr.z) / r;
-    const float2 K = g_point(r, cs, ci, k, 0.0f);  // unregularised point value
-    float f[kMlpIn] = {r / lam - 1.0f, (d / lam - 0.19f) * 14.0f, rv.x / r, rv.y / r, 2.0f * cs - 1.0f,
-                       nr.x, nr.y, nr.z, 0.5f * logf(r / d)};
-    const float2 t = mlp_bf16(w, f);
-    const float M = 0.9f * (k * kInv4PiT / r + 1.0f / (r * r));
-    float s, c;
-    sincosf(k * r, &s, &c);
-    // K + M t e^{ikr}
-    return make_float2(K.x + M * (t.x * c - t.y * s), K.y + M * (t.x * s + t.y * c));
+// The FP16 word of pair (row, emitter j) in the tiled transfer matrix.
+__device__ inline void store_pair(uint4 *A, int nKs, int row, int j, uint32_t w) {
+    const size_t lane = (size_t)(row & 31) + 32 * ((j >> 2) & 1);
+    uint32_t *q = reinterpret_cast<uint32_t *>(A + ((size_t)(row >> 5) * nKs + (j >> 3)) * 64 + lane);
+    q[j & 3] = w;
+}
+
+// Near pairs by 16x16 sub-fragment quadrature: one lane per pair.
+__global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
+                                                   int count, uint4 *__restrict__ A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int2 pr = list[i];
+    const NearGeom q = near_geom(g, ph, pr.x, pr.y);
+    const float2 v = quad16(q.rv, q.nr, ph.deltaReg, ph.kr);
+    store_pair(A, g.nKs, pr.x, pr.y, pack_h2(v.x * ph.scale, v.y * ph.scale));
+}
+
+// Near pairs by the learned correction: value = K + M t e^{ikr}, t from the
+// 9-64-64-2 tanh MLP in BF16 (weights and activations rounded to BF16, FP32
+// accumulation) on the BF16 matrix cores (v_mfma_f32_32x32x16_bf16). A wave
+// evaluates 32 pairs: features -> LDS -> A operand of layer 1; each hidden
+// layer's accumulators go through LDS (tanh, BF16) to become the next A
+// operand; the 64 -> 2 output layer is a per-lane dot product. The layers'
+// B operands are built once per workgroup in fragment order in LDS.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kHP = 72;  // bf16 per activation row in LDS (64 + 8: 16-byte aligned rows)
+
+__global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
+                                                  int count, uint4 *__restrict__ A) {
+    __shared__ bf16x8 sW1[2][64];
+    __shared__ bf16x8 sW2[4][2][64];
+    __shared__ float sB1[64], sB2[64], sW3[2][64], sB3[2];
+    __shared__ __attribute__((aligned(16))) __bf16 sH[4][32][kHP];
+    const float *P = ph.mlp;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int h = lane >> 5, j = lane & 31;
+    // layer B fragments: lane l holds B[k 8 (l/32) .. +7][col l % 32] of its tile
+    for (int idx = threadIdx.x; idx < 2 * 64; idx += 256) {
+        const int c = idx >> 6, l = idx & 63, col = 32 * c + (l & 31), k0 = 8 * (l >> 5);
+        bf16x8 v;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = (__bf16)(k0 + q < kMlpIn ? P[col * kMlpIn + k0 + q] : 0.0f);
+        sW1[c][l] = v;
+    }
+    for (int idx = threadIdx.x; idx < 4 * 2 * 64; idx += 256) {
+        const int st = idx >> 7, c = (idx >> 6) & 1, l = idx & 63, col = 32 * c + (l & 31), k0 = 16 * st + 8 * (l >> 5);
+        bf16x8 v;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = (__bf16)P[kOffW2 + col * 64 + k0 + q];
+        sW2[st][c][l] = v;
+    }
+    for (int idx = threadIdx.x; idx < 64; idx += 256) {
+        sB1[idx] = (float)(__bf16)P[kOffB1 + idx];
+        sB2[idx] = (float)(__bf16)P[kOffB2 + idx];
+        sW3[0][idx] = (float)(__bf16)P[kOffW3 + idx];
+        sW3[1][idx] = (float)(__bf16)P[kOffW3 + 64 + idx];
+    }
+    if (threadIdx.x < 2) sB3[threadIdx.x] = (float)(__bf16)P[kOffB3 + threadIdx.x];
+    const float lam = 2.0f * 3.14159265358979f / ph.kr;
+    const float d = ph.deltaReg;
+    __bf16 (*H)[kHP] = sH[wave];
+    // workgroup-uniform trip count: tiles of 128 pairs (32 per wave)
+    for (int base = blockIdx.x * 128; base < count; base += gridDim.x * 128) {
+        const int i = base + wave * 32 + j;
+        const bool valid = i < count;
+        int2 pr = make_int2(0, 0);
+        float2 K = make_float2(0.f, 0.f);
+        float M = 0.f, cs_kr = 1.f, sn_kr = 0.f;
+        __syncthreads();  // the previous tile's reads of H are done
+        if (h == 0) {
+            bf16x8 f0, f1;
+#pragma unroll
+            for (int q = 0; q < 8; q++) { f0[q] = (__bf16)0.0f; f1[q] = (__bf16)0.0f; }
+            if (valid) {
+                pr = list[i];
+                const NearGeom q = near_geom(g, ph, pr.x, pr.y);
+                const float r = q.r;
+                const float cs = q.rv.z / r, ci = -(q.rv.x * q.nr.x + q.rv.y * q.nr.y + q.rv.z * q.nr.z) / r;
+                K = g_point(r, cs, ci, ph.kr, 0.0f);  // unregularised point value
+                M = 0.9f * (ph.kr * kInv4PiT / r + 1.0f / (r * r));
+                sincosf(ph.kr * r, &sn_kr, &cs_kr);
+                const float f[kMlpIn] = {r / lam - 1.0f, (d / lam - 0.19f) * 14.0f, q.rv.x / r, q.rv.y / r,
+                                         2.0f * cs - 1.0f, q.nr.x, q.nr.y, q.nr.z, 0.5f * logf(r / d)};
+#pragma unroll
+                for (int qq = 0; qq < 8; qq++) f0[qq] = (__bf16)f[qq];
+                f1[0] = (__bf16)f[8];
+            }
+            *reinterpret_cast<bf16x8 *>(&H[j][0]) = f0;
+            *reinterpret_cast<bf16x8 *>(&H[j][8]) = f1;
+        }
+        __syncthreads();
+        // layer 1: [32 x 16] x [16 x 64]
+        f32x16 acc[2];
+        {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&H[j][8 * h]);
+#pragma unroll
+            for (int c = 0; c < 2; c++) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, sW1[c][lane], f32x16{}, 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = (e >> 2) * 8 + h * 4 + (e & 3), col = 32 * c + j;
+                H[row][col] = (__bf16)tanhf(acc[c][e] + sB1[col]);
+            }
+        __syncthreads();
+        // layer 2: [32 x 64] x [64 x 64]
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[c] = f32x16{};
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&H[j][16 * st + 8 * h]);
+#pragma unroll
+            for (int c = 0; c < 2; c++) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, sW2[st][c][lane], acc[c], 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = (e >> 2) * 8 + h * 4 + (e & 3), col = 32 * c + j;
+                H[row][col] = (__bf16)tanhf(acc[c][e] + sB2[col]);
+            }
+        __syncthreads();
+        // layer 3: lane (j, h) -> output h of pair j
+        float o = sB3[h];
+        for (int k = 0; k < 64; k++) o = fmaf(sW3[h][k], (float)H[j][k], o);
+        const float o1 = __shfl(o, j + 32);
+        if (h == 0 && valid) {
+            // K + M t e^{ikr}
+            const float re = K.x + M * (o * cs_kr - o1 * sn_kr), im = K.y + M * (o * sn_kr + o1 * cs_kr);
+            store_pair(A, g.nKs, pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
+        }
+    }
+}
+
+// Appends (row, j) for the lanes with pred set; called by the whole wave.
+__device__ inline void near_append(bool pred, int2 *list, int *count, int cap, int row, int j) {
+    const uint64_t m = __ballot(pred);
+    if (!m) return;
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0);
+    if (pred) {
+        const int idx = base + __popcll(m & ((1ull << __lane_id()) - 1ull));
+        if (idx < cap) list[idx] = make_int2(row, j);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// S build: one lane = one row, 4 emitters of one K step.
+// S build: one lane = one row, 4 emitters of one K step. Near pairs (mode 1,
+// 2) are left zero and listed for the near-field kernels.
 template <bool ATT, int NEAR>
 __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
                                                  uint4 *__restrict__ A, int ksPerWave) {
-    __shared__ float sw[NEAR == 2 ? kMlpParams : 1];
-    if (NEAR == 2) {
-        for (int q = threadIdx.x; q < kMlpParams; q += blockDim.x) sw[q] = bf16_round(ph.mlp[q]);
-        __syncthreads();
-    }
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int nKsChunks = (g.nKs + ksPerWave - 1) / ksPerWave;
@@ -159,6 +291,7 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
         for (int q = 0; q < 4; q++) {
             const int j = ks * 8 + h * 4 + q;
             float re = 0.f, im = 0.f;
+            bool isNear = false, isQuad = false;
             if (rvalid && j < g.n && ((mrow[j] >> bit) & 1ull)) {
                 const float4 ep = g.p[j], en = g.nrm[j];
                 float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
@@ -169,17 +302,9 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 float dr = vx * rn.x + vy * rn.y + vz * rn.z;
                 float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
                 if (NEAR != 0 && r < ph.nearRadius) {
-                    // emitter frame of the emitter's facet
-                    const int fe = __float_as_int(en.w);
-                    const float4 xa = ph.frames[2 * fe], ya = ph.frames[2 * fe + 1];
-                    const float3 rv = make_float3(vx * xa.x + vy * xa.y + vz * xa.z, vx * ya.x + vy * ya.y + vz * ya.z,
-                                                  vx * en.x + vy * en.y + vz * en.z);
-                    const float3 nr = make_float3(rn.x * xa.x + rn.y * xa.y + rn.z * xa.z,
-                                                  rn.x * ya.x + rn.y * ya.y + rn.z * ya.z,
-                                                  rn.x * en.x + rn.y * en.y + rn.z * en.z);
-                    const float2 v = near_value<NEAR>(rv, nr, ph.deltaReg, ph.kr, sw);
-                    re = v.x * ph.scale;
-                    im = v.y * ph.scale;
+                    // evaluated afterwards from the near lists (k_near_quad / k_near_mlp)
+                    isNear = true;
+                    isQuad = NEAR == 1 || r < ph.deltaReg;  // the learned range starts at r = delta
                 } else if (r < ph.deltaReg) {
                     re = wgt * (-ph.alpha * kInv4PiT);
                     im = wgt * (ph.kr * kInv4PiT);
@@ -203,6 +328,10 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 }
             }
             w[q] = pack_h2(re, im);
+            if (NEAR != 0) {  // wave-uniform appends
+                near_append(isNear && isQuad, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
+                near_append(isNear && !isQuad, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
+            }
         }
         A[((size_t)rt * g.nKs + ks) * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);
     }
@@ -412,6 +541,21 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
     else if (ph.alpha != 0.0f) k_build_s<true, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
     else k_build_s<false, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
     SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st) {
+    if (nQuad > ph.nearCap || nMlp > ph.nearCap) { set_error("near lists overflow"); return -2; }
+    if (nQuad > 0) {
+        k_near_quad<<<(nQuad + 255) / 256, 256, 0, st>>>(g, ph, ph.nearQ, nQuad, A);
+        SR_HIP(hipGetLastError());
+    }
+    if (nMlp > 0) {
+        if (!ph.mlp) { set_error("learned near field without a network"); return -2; }
+        const int grid = std::min((nMlp + 127) / 128, 2048);
+        k_near_mlp<<<grid, 256, 0, st>>>(g, ph, ph.nearM, nMlp, A);
+        SR_HIP(hipGetLastError());
+    }
     return 0;
 }
 

@@ -290,7 +290,7 @@ def kernel_stats(family):
 
 _sig("sr_set_transfer_mode", [_i])
 _sig("sr_transfer_mode", [])
-KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9})
+KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9, "near": 10})
 
 
 _sig("sr_set_gather_mode", [_i])
