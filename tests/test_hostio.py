@@ -67,3 +67,22 @@ def test_colour_map_levels_and_wheel(api):
     assert np.argmax(rgba[1, :3]) == 1
     assert rgba[2, 1] > 0.99 and rgba[2, 2] > 0.99 and abs(rgba[2, 0] - 0.2) < 1e-5
     assert np.argmax(rgba[3, :3]) == 2
+
+
+FIXTURE_TRIHEDRAL = os.path.join(os.path.dirname(__file__), "golden", "trihedral_reflector.stl")
+
+
+def test_reference_trihedral_fixture(api):
+    """The reference's trihedral mesh (run_scripts/trihedral_reflector.stl, copied as a data fixture):
+    36 triangles of a closed, plate-thick trihedral; every edge is shared (the plates' common edge 4 times)."""
+    t = api.read_stl(FIXTURE_TRIHEDRAL).astype(np.float64)
+    assert t.shape == (36, 3, 3)
+    area = 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1).sum()
+    assert abs(area - 6.12) < 1e-5
+    assert np.allclose(np.abs(t).max(), np.sqrt(0.5), atol=1e-6)
+    from collections import Counter
+    edges = Counter()
+    for tri in t:
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            edges[tuple(sorted([tuple(np.round(tri[a], 5)), tuple(np.round(tri[b], 5))]))] += 1
+    assert sorted(Counter(edges.values()).items()) == [(2, 52), (4, 1)]
