@@ -15,8 +15,9 @@ Three forms are timed back to back:
   mixed_fp16   the pair kernel stored once as a complex FP16 matrix, every
                bounce an FP16 MFMA GEMM with FP32 accumulation;
   learned      the same scene fragmented at lambda/4 (3.5x fewer fragments),
-               pairs closer than 2 lambda corrected by the learned near-field
-               network (trained once on the GPU before timing), mixed form.
+               pairs with 3 delta <= r < 2 lambda corrected by the learned
+               near-field network (trained once on the GPU before timing),
+               closer pairs by the 16x16 quadrature, mixed form.
 plain and mixed are checked against the FP64 oracle (point rule, lambda/7.5)
 on the incidences the CPU baseline runs; learned against the FP64 oracle with
 the 16x16 sub-fragment quadrature judge at lambda/4, and its field points
@@ -78,7 +79,9 @@ def time_form(scene, mode, steps, warmup, near=None):
 
     runner.load_scene(scene, transfer_mode=mode)
     if near:
-        api.set_near_mode(*near)
+        api.set_near_mode(*near[:2])
+        if len(near) > 2:
+            api.set_near_quadrature_radius(near[2])
     api.device_synchronize()
 
     def step():
@@ -165,10 +168,11 @@ def main():
                       targets=[(tris, lam / 4.0) for tris, _ in scene["targets"]])
         api.nf_init(1)
         t0 = time.perf_counter()
-        tr, va = api.nf_train(nsamples=262144, steps=4000, batch=4096, lr=3e-3, seed=7)
+        tr, va = api.nf_train(nsamples=262144, steps=60000, batch=4096, lr=3e-3, seed=7)
         api.device_synchronize()
         learned = dict(scene=coarse, lam=lam, train_s=time.perf_counter() - t0, train_err=tr, heldout_err=va)
-        forms.append(("learned_mixed_fp16_lambda4", 1, (2, 2.0), coarse, "learned"))
+        # learned correction for 3 delta <= r < 2 lambda, the quadrature closer in
+        forms.append(("learned_mixed_fp16_lambda4", 1, (2, 2.0, 3.0), coarse, "learned"))
     chosen = [f for f in forms if f[4] in want]
 
     results = {}

@@ -166,6 +166,9 @@ int sr_gather_mode(void);
  * 1 16x16 sub-fragment quadrature of the same kernel over the emitter's
  * cell, 2 the learned correction (a 9-64-64-2 MLP, BF16 inference). */
 int sr_set_near_mode(int mode, float radiusWavelengths);
+/* Learned mode: pairs closer than `deltas` fragment sizes (1 to 16, default 1)
+ * use the quadrature instead of the network. Reset to 1 by TearDownCuda. */
+int sr_set_near_quadrature_radius(float deltas);
 
 /* The learned correction's network: trained on the GPU with the library's own
  * forward/backward/Adam kernels on pairs labelled by the quadrature.

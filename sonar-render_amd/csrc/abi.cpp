@@ -226,6 +226,9 @@ int sr_set_gather_mode(int mode) {
     return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(gModel, mode); });
 }
 int sr_gather_mode(void) { return gModel.gatherMode; }
+int sr_set_near_quadrature_radius(float deltas) {
+    return guarded("set_near_quadrature_radius", [&] { return sr::model_set_near_quadrature_radius(gModel, deltas); });
+}
 int sr_set_near_mode(int mode, float radiusWavelengths) {
     return guarded("set_near_mode", [&] { return sr::model_set_near_mode(gModel, mode, radiusWavelengths); });
 }

@@ -14,8 +14,10 @@ constexpr int kOffB1 = 64 * kMlpIn, kOffW2 = kOffB1 + 64, kOffB2 = kOffW2 + 4096
 // Training pairs with quadrature labels; lengths in wavelengths.
 int nf_samples(uint32_t seed, int n, float dMin, float dMax, float rMax, float *X, float *Y, hipStream_t st);
 int mlp_forward(const float *P, const float *X, int n, float *H1, float *H2, float *Out, hipStream_t st);
+// partial: kMlpGradPartials doubles of scratch for the two-stage batch sums.
+constexpr int kMlpGradPartials = 32 * kMlpParams;
 int mlp_backward(const float *P, const float *X, const float *Y, int n, const float *H1, const float *H2,
-                 const float *Out, float *G3, float *D2, float *D1, float *grad, hipStream_t st);
+                 const float *Out, float *G3, float *D2, float *D1, double *partial, float *grad, hipStream_t st);
 int mlp_adam(float *P, const float *g, float *m, float *v, float lr, int t, hipStream_t st);
 int mlp_sq_err(const float *Out, const float *Y, int n, float *acc2, hipStream_t st);
 

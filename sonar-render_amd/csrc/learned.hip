@@ -104,12 +104,15 @@ __global__ void k_nf_samples(uint32_t seed, int n, float dMin, float dMax, float
     const float r = d + (rMax - d) * u0 * u0;
     float3 uv, nr;
     float cs, ci;
+    // half the receivers at right angles to the emitter: right-angle junctions
+    // (corner reflectors, boxes, fins) are where near pairs across facets occur
+    const bool perp = urand(seed, i, 1000) < 0.5f;
     for (int tr = 0;; tr++) {
         const float zc = urand(seed, i, 2 + 4 * tr);
         const float ph = 2.0f * kPi * urand(seed, i, 3 + 4 * tr);
         const float st = sqrtf(fmaxf(0.f, 1.f - zc * zc));
         uv = make_float3(st * cosf(ph), st * sinf(ph), zc);
-        const float zn = 2.f * urand(seed, i, 4 + 4 * tr) - 1.f;
+        const float zn = perp ? 0.f : 2.f * urand(seed, i, 4 + 4 * tr) - 1.f;
         const float pn = 2.0f * kPi * urand(seed, i, 5 + 4 * tr);
         const float sn = sqrtf(fmaxf(0.f, 1.f - zn * zn));
         nr = make_float3(sn * cosf(pn), sn * sinf(pn), zn);
@@ -195,32 +198,47 @@ __global__ void k_mlp_backward(const float *__restrict__ P, const float *__restr
     }
 }
 
-// One thread per parameter: the batch sum of its gradient terms.
+// Batch sums of the parameter gradients in two fixed-order stages: thread
+// (parameter p, chunk c) sums the chunk's samples in FP64, then one thread per
+// parameter adds the chunks in order.
+constexpr int kGradChunks = 32;
+
 __global__ void k_mlp_wgrad(const float *__restrict__ X, const float *__restrict__ H1, const float *__restrict__ H2,
                             const float *__restrict__ G3, const float *__restrict__ D2, const float *__restrict__ D1,
-                            int n, float *__restrict__ grad) {
+                            int n, double *__restrict__ partial) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
     if (p >= kMlpParams) return;
+    const int per = (n + kGradChunks - 1) / kGradChunks;
+    const int i0 = c * per, i1 = min(n, i0 + per);
     double acc = 0.0;
     if (p < kOffB1) {  // W1[o][q]
         const int o = p / kMlpIn, q = p % kMlpIn;
-        for (int i = 0; i < n; i++) acc += (double)D1[(size_t)i * 64 + o] * X[(size_t)i * kMlpIn + q];
+        for (int i = i0; i < i1; i++) acc += (double)D1[(size_t)i * 64 + o] * X[(size_t)i * kMlpIn + q];
     } else if (p < kOffW2) {  // b1
         const int o = p - kOffB1;
-        for (int i = 0; i < n; i++) acc += D1[(size_t)i * 64 + o];
+        for (int i = i0; i < i1; i++) acc += D1[(size_t)i * 64 + o];
     } else if (p < kOffB2) {  // W2[o][q]
         const int o = (p - kOffW2) / 64, q = (p - kOffW2) % 64;
-        for (int i = 0; i < n; i++) acc += (double)D2[(size_t)i * 64 + o] * H1[(size_t)i * 64 + q];
+        for (int i = i0; i < i1; i++) acc += (double)D2[(size_t)i * 64 + o] * H1[(size_t)i * 64 + q];
     } else if (p < kOffW3) {
         const int o = p - kOffB2;
-        for (int i = 0; i < n; i++) acc += D2[(size_t)i * 64 + o];
+        for (int i = i0; i < i1; i++) acc += D2[(size_t)i * 64 + o];
     } else if (p < kOffB3) {  // W3[o][q]
         const int o = (p - kOffW3) / 64, q = (p - kOffW3) % 64;
-        for (int i = 0; i < n; i++) acc += (double)G3[(size_t)i * 2 + o] * H2[(size_t)i * 64 + q];
+        for (int i = i0; i < i1; i++) acc += (double)G3[(size_t)i * 2 + o] * H2[(size_t)i * 64 + q];
     } else {
         const int o = p - kOffB3;
-        for (int i = 0; i < n; i++) acc += G3[(size_t)i * 2 + o];
+        for (int i = i0; i < i1; i++) acc += G3[(size_t)i * 2 + o];
     }
+    partial[(size_t)c * kMlpParams + p] = acc;
+}
+
+__global__ void k_mlp_wgrad_sum(const double *__restrict__ partial, float *__restrict__ grad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    double acc = 0.0;
+    for (int c = 0; c < kGradChunks; c++) acc += partial[(size_t)c * kMlpParams + p];
     grad[p] = (float)acc;
 }
 
@@ -257,10 +275,12 @@ int mlp_forward(const float *P, const float *X, int n, float *H1, float *H2, flo
 }
 
 int mlp_backward(const float *P, const float *X, const float *Y, int n, const float *H1, const float *H2,
-                 const float *Out, float *G3, float *D2, float *D1, float *grad, hipStream_t st) {
+                 const float *Out, float *G3, float *D2, float *D1, double *partial, float *grad, hipStream_t st) {
     k_mlp_backward<<<(n + 127) / 128, 128, 0, st>>>(P, Y, n, H1, H2, Out, G3, D2, D1);
     SR_HIP(hipGetLastError());
-    k_mlp_wgrad<<<(kMlpParams + 127) / 128, 128, 0, st>>>(X, H1, H2, G3, D2, D1, n, grad);
+    k_mlp_wgrad<<<dim3((kMlpParams + 127) / 128, kGradChunks), 128, 0, st>>>(X, H1, H2, G3, D2, D1, n, partial);
+    SR_HIP(hipGetLastError());
+    k_mlp_wgrad_sum<<<(kMlpParams + 255) / 256, 256, 0, st>>>(partial, grad);
     SR_HIP(hipGetLastError());
     return 0;
 }

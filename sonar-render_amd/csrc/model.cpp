@@ -390,12 +390,23 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
         ProfScope ps(m, PK_VIS, (double)tk.em.count * tk.rc.count, 0.0);
         if (vis_mask_launch(tk, bvh_of(m), m.vc, m.stream)) return -1;
     }
+    mb.valid = true;  // the traversal-overflow flag is checked when the projection finishes
+    *out = mk;
+    return 0;
+}
+
+// End of a projection: waits for the stream and reports a visibility traversal
+// overflow raised by any mask built on the way (the words are then dropped).
+int finish_projection(Model &m) {
     int herr = 0;
     SR_HIP(hipMemcpyAsync(&herr, m.errFlag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
     SR_HIP(hipStreamSynchronize(m.stream));
-    if (herr) { set_error("visibility traversal stack overflow"); return -1; }
-    mb.valid = true;
-    *out = mk;
+    if (herr) {
+        for (auto &kv : m.masks) kv.second.valid = false;
+        SR_HIP(hipMemset(m.errFlag, 0, sizeof(int)));
+        set_error("visibility traversal stack overflow");
+        return -1;
+    }
     return 0;
 }
 
@@ -422,8 +433,7 @@ int model_project_src(Model &m, bool fieldObjects) {
     for (int i = 0; i < (int)m.targets.size(); i++) if (run(m.targets[i], OBJ_TARGET, i)) return -1;
     if (fieldObjects)
         for (int i = 0; i < (int)m.fields.size(); i++) if (run(m.fields[i], OBJ_FIELD, i)) return -1;
-    SR_HIP(hipStreamSynchronize(m.stream));
-    return 0;
+    return finish_projection(m);
 }
 
 // One gather + reduce over all planes in groups of <= 16.
@@ -640,7 +650,7 @@ static int ensure_transfer(Model &m, int i) {
     TransferPhys ph{m.kf, m.atten, (float)((double)m.atten * 1.4426950408889634), o.delta, 0.9f, t.sS,
                     (double)m.kf / (2.0 * M_PI), m.nearMode,
                     (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr,
-                    nullptr, nullptr, nullptr, 0};
+                    nullptr, nullptr, nullptr, 0, m.nearQuadDeltas * o.delta};
     if (m.nearMode == 2) {
         NearField &nf = nearfield();
         if (!nf.trained) { set_error("learned near-field mode needs a trained network (sr_nf_train)"); return -3; }
@@ -799,8 +809,7 @@ int model_project_t2t(Model &m, int nref) {
             const double pairs = (double)o.nact * o.nact - o.sameFacetPairs;
             if (gather_planes(m, t, ph, KIND_F2F, r, m.nplanes, o.nact, o.nact, o.nact, o.ncell, pairs)) return -1;
         }
-    SR_HIP(hipStreamSynchronize(m.stream));
-    return 0;
+    return finish_projection(m);
 }
 
 // FACET_NO_RESP: every target into every field object (ModelCuda.cpp:280-301).
@@ -823,8 +832,7 @@ int model_project_t2field_objects(Model &m) {
             if (gather_planes(m, t, ph, KIND_F2F_FIELD, r, m.nplanes, A.ncell, 0, 0, B.ncell, (double)A.nact * B.nact))
                 return -1;
         }
-    SR_HIP(hipStreamSynchronize(m.stream));
-    return 0;
+    return finish_projection(m);
 }
 
 // FIELD_POINTS (ModelCuda.cpp:302-321): compensated FP32 gathers, FP64 totals.
@@ -845,8 +853,7 @@ int model_project_t2field_points(Model &m) {
         ReduceTask r{};
         if (gather_planes(m, t, ph, KIND_F2P, r, m.nplanes, A.ncell, 0, 0, 0, (double)A.nact * nq)) return -1;
     }
-    SR_HIP(hipStreamSynchronize(m.stream));
-    return 0;
+    return finish_projection(m);
 }
 
 // FACET_RESP between two target objects (OptiX.cu:427-430; never dispatched by
@@ -876,8 +883,7 @@ int model_project_resp(Model &m, int ia, int ib) {
     s.gateMode = GATE_ALL; s.gval = A->Pin; s.out = A->Pout; s.outAdd = 1;
     if (gather_planes(m, u, ph, KIND_F2F, s, m.nplanes, B->nact, A->nact, A->nact, 0, (double)A->nact * B->nact))
         return -1;
-    SR_HIP(hipStreamSynchronize(m.stream));
-    return 0;
+    return finish_projection(m);
 }
 
 // P_in <- P (active cells), P_out <- 0: the state a RESP call reads.
@@ -927,7 +933,7 @@ int model_get_object_pressure(Model &m, int type, int idx, int which, int plane,
 int model_get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t *out, long nwords) {
     if (int rc = check_rendered(m)) return rc;
     uint64_t *mk;
-    if (get_mask(m, kind, ta, ia, tb, ib, &mk)) return -1;
+    if (get_mask(m, kind, ta, ia, tb, ib, &mk) || finish_projection(m)) return -1;
     SR_HIP(hipMemcpy(out, mk, nwords * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return 0;
 }
@@ -959,6 +965,13 @@ int model_set_near_mode(Model &m, int mode, float radiusWl) {
     if (!(radiusWl > 0.f)) { set_error("near radius must be > 0 wavelengths"); return -2; }
     m.nearMode = mode;
     m.nearRadiusWl = radiusWl;
+    for (auto &o : m.targets) o.tc.valid = false;
+    return 0;
+}
+
+int model_set_near_quadrature_radius(Model &m, float deltas) {
+    if (!(deltas >= 1.0f) || deltas > 16.0f) { set_error("quadrature radius: 1 to 16 fragment sizes"); return -2; }
+    m.nearQuadDeltas = deltas;
     for (auto &o : m.targets) o.tc.valid = false;
     return 0;
 }
@@ -1013,6 +1026,7 @@ void model_teardown(Model &m) {
     m.masks.clear();
     m.transferMode = 0;
     m.nearMode = 0;
+    m.nearQuadDeltas = 1.0f;
     m.gatherMode = 0;
     dev_free(m.gatedList); dev_free(m.gatedCount);
     m.gatedCap = 0;

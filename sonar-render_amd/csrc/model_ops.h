@@ -30,6 +30,7 @@ int model_reset_pressures(Model &m);
 int model_clear_vis_cache(Model &m);
 int model_set_transfer_mode(Model &m, int mode);
 int model_set_gather_mode(Model &m, int mode);
+int model_set_near_quadrature_radius(Model &m, float deltas);
 int model_set_near_mode(Model &m, int mode, float radiusWl);
 void model_profile(Model &m, int on);
 void model_profile_reset(Model &m);

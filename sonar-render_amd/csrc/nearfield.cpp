@@ -114,16 +114,17 @@ int nf_eval(const float *X, const float *Y, int n, float *out, float *grad, cons
         return -1;
     SR_HIP(hipMemcpy(dX, X, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice));
     int rc = mlp_forward(dP, dX, n, H1, H2, O, nullptr);
+    double *part = nullptr;
     if (!rc && grad) {
-        if (dalloc(&dY, (size_t)n * 2)) return -1;
+        if (dalloc(&dY, (size_t)n * 2) || dalloc(&part, (size_t)kMlpGradPartials)) return -1;
         SR_HIP(hipMemcpy(dY, Y, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice));
-        rc = mlp_backward(dP, dX, dY, n, H1, H2, O, G3, D2, D1, dg, nullptr);
+        rc = mlp_backward(dP, dX, dY, n, H1, H2, O, G3, D2, D1, part, dg, nullptr);
     }
     SR_HIP(hipDeviceSynchronize());
     if (!rc && out) SR_HIP(hipMemcpy(out, O, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost));
     if (!rc && grad) SR_HIP(hipMemcpy(grad, dg, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
     for (void *p : {(void *)dX, (void *)dY, (void *)H1, (void *)H2, (void *)O, (void *)G3, (void *)D2, (void *)D1,
-                    (void *)dg})
+                    (void *)dg, (void *)part})
         dfree(p);
     if (own) dfree(dP);
     return rc;
@@ -142,6 +143,8 @@ int nf_train(int nsamples, int steps, int batch, float lr, uint32_t seed, float 
         dalloc(&G3, (size_t)batch * 2) || dalloc(&D2, (size_t)batch * 64) || dalloc(&D1, (size_t)batch * 64) ||
         dalloc(&g, kMlpParams) || dalloc(&m, kMlpParams) || dalloc(&v, kMlpParams) || dalloc(&acc, 4))
         return -1;
+    double *part = nullptr;
+    if (dalloc(&part, (size_t)kMlpGradPartials)) return -1;
     int rc = nf_samples(seed, nsamples, kNfDeltaMin, kNfDeltaMax, kNfRadius, X, Y, nullptr);
     if (!rc) rc = nf_samples(seed ^ 0xA5A5A5A5u, nval, kNfDeltaMin, kNfDeltaMax, kNfRadius, Xv, Yv, nullptr);
     SR_HIP(hipMemset(m, 0, kMlpParams * sizeof(float)));
@@ -152,7 +155,7 @@ int nf_train(int nsamples, int steps, int batch, float lr, uint32_t seed, float 
         const float *xb = X + (size_t)b * batch * kMlpIn, *yb = Y + (size_t)b * batch * 2;
         const float l = lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(3.14159265f * (float)(t - 1) / steps)));
         rc = mlp_forward(nf.P, xb, batch, H1, H2, O, nullptr);
-        if (!rc) rc = mlp_backward(nf.P, xb, yb, batch, H1, H2, O, G3, D2, D1, g, nullptr);
+        if (!rc) rc = mlp_backward(nf.P, xb, yb, batch, H1, H2, O, G3, D2, D1, part, g, nullptr);
         if (!rc) rc = mlp_adam(nf.P, g, m, v, l, t, nullptr);
     }
     auto relerr = [&](const float *xs, const float *ys, int n, float &out) -> int {
@@ -168,7 +171,7 @@ int nf_train(int nsamples, int steps, int batch, float lr, uint32_t seed, float 
     if (!rc) rc = relerr(Xv, Yv, nval, va);
     SR_HIP(hipDeviceSynchronize());
     for (void *p : {(void *)X, (void *)Y, (void *)Xv, (void *)Yv, (void *)H1, (void *)H2, (void *)O, (void *)G3,
-                    (void *)D2, (void *)D1, (void *)g, (void *)m, (void *)v, (void *)acc})
+                    (void *)D2, (void *)D1, (void *)g, (void *)m, (void *)v, (void *)acc, (void *)part})
         dfree(p);
     if (rc) return rc;
     std::vector<float> chk(kMlpParams);

@@ -26,6 +26,7 @@ struct TransferPhys {
     int2 *nearQ, *nearM;   // (row, emitter)
     int *nearCount;        // [2] device counters, zeroed by the caller
     int nearCap;           // capacity of each list
+    float quadRadius;      // learned mode: closer pairs use the quadrature (metres, >= delta)
 };
 
 struct TransferGemm {

@@ -74,9 +74,13 @@ __device__ inline float2 g_point(float r, float cs, float ci, float k, float reg
     return make_float2(w * re, w * im);
 }
 
+// 16 x 16 midpoint quadrature of the pair kernel over the emitter's cell.
+// The phase of each sub-point uses the hardware sine/cosine in revolutions
+// (r < 2 lambda here, so r k / 2 pi < 2 keeps ~1e-7 of a revolution).
 __device__ float2 quad16(float3 rv, float3 nr, float d, float k) {
     const int n = 16;
     const float h = d / n;
+    const float krev = k * 0.159154943091895336f;
     float2 acc = make_float2(0.f, 0.f);
     for (int sy = 0; sy < n; sy++)
         for (int sx = 0; sx < n; sx++) {
@@ -85,9 +89,21 @@ __device__ float2 quad16(float3 rv, float3 nr, float d, float k) {
             const float r = sqrtf(vx * vx + vy * vy + vz * vz);
             const float cs = vz / r, ci = -(vx * nr.x + vy * nr.y + vz * nr.z) / r;
             if (!(cs >= 1e-6f) || !(ci >= 1e-6f)) continue;
-            const float2 v = g_point(r, cs, ci, k, h);
-            acc.x += v.x;
-            acc.y += v.y;
+            float re, im;
+            if (r < h) {
+                re = 0.f;
+                im = k * kInv4PiT;
+            } else {
+                float x = r * krev;
+                x -= rintf(x);
+                const float s = __builtin_amdgcn_sinf(x), c = __builtin_amdgcn_cosf(x);
+                const float a = k * kInv4PiT / r;
+                re = a * s + 1.0f / (r * r);
+                im = -a * c;
+            }
+            const float w = cs * ci * 0.9f;
+            acc.x += w * re;
+            acc.y += w * im;
         }
     return make_float2(acc.x * (1.0f / 256.0f), acc.y * (1.0f / 256.0f));
 }
@@ -252,17 +268,28 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
     }
 }
 
-// Appends (row, j) for the lanes with pred set; called by the whole wave.
-__device__ inline void near_append(bool pred, int2 *list, int *count, int cap, int row, int j) {
+// Near pairs are staged per wave in LDS and flushed to the global lists with
+// one atomic per batch (called by the whole wave; counts are wave-uniform).
+constexpr int kNearStage = 256;
+
+// entries: this wave's LDS staging area; n: its fill (wave-uniform register)
+__device__ inline void near_flush(int2 *entries, int &n, int2 *list, int *count, int cap) {
+    if (!n) return;
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(count, n);
+    base = __shfl(base, 0);
+    for (int k = __lane_id(); k < n; k += 64)
+        if (base + k < cap) list[base + k] = entries[k];
+    n = 0;
+}
+
+__device__ inline void near_stage(int2 *entries, int &n, bool pred, int2 *list, int *count, int cap, int row, int j) {
     const uint64_t m = __ballot(pred);
     if (!m) return;
-    int base = 0;
-    if (__lane_id() == 0) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, 0);
-    if (pred) {
-        const int idx = base + __popcll(m & ((1ull << __lane_id()) - 1ull));
-        if (idx < cap) list[idx] = make_int2(row, j);
-    }
+    const int cnt = __popcll(m);
+    if (n + cnt > kNearStage) near_flush(entries, n, list, count, cap);
+    if (pred) entries[n + __popcll(m & ((1ull << __lane_id()) - 1ull))] = make_int2(row, j);
+    n += cnt;
 }
 
 // ---------------------------------------------------------------------------
@@ -271,6 +298,9 @@ __device__ inline void near_append(bool pred, int2 *list, int *count, int cap, i
 template <bool ATT, int NEAR>
 __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
                                                  uint4 *__restrict__ A, int ksPerWave) {
+    __shared__ int2 stage[NEAR != 0 ? 4 : 1][2][NEAR != 0 ? kNearStage : 1];
+    int2 *stQ = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][0], *stM = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][1];
+    int nQ = 0, nM = 0;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int nKsChunks = (g.nKs + ksPerWave - 1) / ksPerWave;
@@ -304,7 +334,7 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 if (NEAR != 0 && r < ph.nearRadius) {
                     // evaluated afterwards from the near lists (k_near_quad / k_near_mlp)
                     isNear = true;
-                    isQuad = NEAR == 1 || r < ph.deltaReg;  // the learned range starts at r = delta
+                    isQuad = NEAR == 1 || r < ph.quadRadius;  // the learned range starts at r >= delta
                 } else if (r < ph.deltaReg) {
                     re = wgt * (-ph.alpha * kInv4PiT);
                     im = wgt * (ph.kr * kInv4PiT);
@@ -328,12 +358,16 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 }
             }
             w[q] = pack_h2(re, im);
-            if (NEAR != 0) {  // wave-uniform appends
-                near_append(isNear && isQuad, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
-                near_append(isNear && !isQuad, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
+            if (NEAR != 0) {  // wave-uniform
+                near_stage(stQ, nQ, isNear && isQuad, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
+                near_stage(stM, nM, isNear && !isQuad, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
             }
         }
         A[((size_t)rt * g.nKs + ks) * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    if (NEAR != 0) {
+        near_flush(stQ, nQ, ph.nearQ, ph.nearCount, ph.nearCap);
+        near_flush(stM, nM, ph.nearM, ph.nearCount + 1, ph.nearCap);
     }
 }
 
@@ -485,6 +519,7 @@ __global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes,
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool z = false;
     if (i < n)
+#pragma unroll 8
         for (int b = 0; b < nplanes; b++) {
             const float2 v = Pin[(size_t)b * n + i];
             z |= (v.x == 0.0f && v.y == 0.0f);

@@ -364,6 +364,14 @@ def set_near_mode(mode, radius_wavelengths=2.0):
     _check(cpp_lib.sr_set_near_mode(int(mode), float(radius_wavelengths)), "set_near_mode")
 
 
+_sig("sr_set_near_quadrature_radius", [_f])
+
+
+def set_near_quadrature_radius(deltas):
+    """Learned mode: pairs closer than `deltas` fragment sizes use the 16x16 quadrature."""
+    _check(cpp_lib.sr_set_near_quadrature_radius(float(deltas)), "set_near_quadrature_radius")
+
+
 def nf_init(seed=1):
     _check(cpp_lib.sr_nf_init(seed), "nf_init")
 
