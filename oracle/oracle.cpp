@@ -57,6 +57,7 @@ struct Object {
     std::vector<size_t> cellOff;  // per facet offset into the flat cell arrays
     size_t ncell = 0;
     std::vector<cd> P, Pin, Pout;  // field objects alias all three to P
+    int triBase = -1;              // targets: index of the first facet among the occluder triangles
 };
 
 // ----------------------------------------------------------------------------
@@ -256,10 +257,13 @@ static cd expik(const cd &k, float r) {
     return cd(m * std::cos(z.imag()), m * std::sin(z.imag()));
 }
 
-static inline bool occluded(const Scene &s, V3 o, V3 d, float tmin, float tmax) {
-    if (s.bruteForce) return anyhit_brute(s.tris.data(), (int)s.tris.size(), s.vp, o, d, tmin, tmax);
-    return anyhit_bvh(s.bvh, s.tris.data(), s.vp, o, d, tmin, tmax);
+// skipA, skipB: the occluder triangles the ray's end points lie on (-1: a
+// source, field point or field object), which the ray never tests.
+static inline bool occluded(const Scene &s, V3 o, V3 d, float tmin, float tmax, int skipA, int skipB) {
+    if (s.bruteForce) return anyhit_brute(s.tris.data(), (int)s.tris.size(), s.vp, o, d, tmin, tmax, skipA, skipB);
+    return anyhit_bvh(s.bvh, s.tris.data(), s.vp, o, d, tmin, tmax, skipA, skipB);
 }
+static inline int tri_of(const Object &o, int facet) { return o.triBase >= 0 ? o.triBase + facet : -1; }
 
 // Ray set-up exactly as OptiX.cu: r = sqrtf(dot(v,v)); uv = v * (1.0f / r).
 static inline void ray_dir(V3 v, float &r, V3 &uv) {
@@ -286,7 +290,7 @@ static void project_src_to_object(Scene &s, Object &B) {
                     float cinc = dotp(V3{-uv.x, -uv.y, -uv.z}, F.normal);
                     if (B.type == OBJ_FIELD) cinc = 1;
                     if (cinc < 1e-6f) continue;
-                    if (occluded(s, pi, uv, s.vp.tminPoint, r - eps)) continue;
+                    if (occluded(s, pi, uv, s.vp.tminPoint, r - eps, -1, tri_of(B, b))) continue;
                     cd Pj = (double)cinc * (s.srcP[i] * expik(s.k, r)) / (double)r;
                     B.P[cj] += Pj;
                 }
@@ -321,7 +325,7 @@ static void project_object_to_points(Scene &s, Object &A) {
                         float r; V3 uv; ray_dir(v, r, uv);
                         float cs = dotp(V3{-uv.x, -uv.y, -uv.z}, F.normal);
                         if (cs < 1e-6f) continue;
-                        if (occluded(s, pj, uv, s.vp.tminPoint, r - eps)) continue;
+                        if (occluded(s, pj, uv, s.vp.tminPoint, r - eps, -1, tri_of(A, a))) continue;
                         float invr2 = 1 / (r * r);  // FP32 in the reference
                         cd G = -expik(s.k, r) / fourpi * (ik / (double)r) + (double)invr2;
                         float Acs = Ai * cs;
@@ -383,7 +387,7 @@ static void project_facets(Scene &s, Object &A, Object &B, int mode) {
                                 float cinc = dotp(V3{-uv.x, -uv.y, -uv.z}, FB.normal);
                                 if (B.type == OBJ_FIELD) cinc = 1;
                                 if (cinc < 1e-6f) continue;
-                                if (occluded(s, pi, uv, s.vp.tminFrag, r - eps)) continue;
+                                if (occluded(s, pi, uv, s.vp.tminFrag, r - eps, tri_of(A, a), tri_of(B, b))) continue;
                                 if (s.nearSub > 0 && r < s.nearRadius) {
                                     // near-field judge: quadrature over each emitter's own cell
                                     const bool fld = (B.type == OBJ_FIELD);
@@ -499,8 +503,10 @@ int orc_render(void *h) {
     s.tris.clear();
     float scale = 1.0f;
     auto upd = [&](V3 p) { scale = std::max(scale, std::max(std::fabs(p.x), std::max(std::fabs(p.y), std::fabs(p.z)))); };
-    for (auto &o : s.targets)
+    for (auto &o : s.targets) {
+        o.triBase = (int)s.tris.size();
         for (auto &f : o.facets) { s.tris.push_back(make_tri(f.v1, f.v2, f.v3)); upd(f.v1); upd(f.v2); upd(f.v3); }
+    }
     for (auto &o : s.fields)
         for (auto &f : o.facets) { upd(f.v1); upd(f.v2); upd(f.v3); }
     for (auto &p : s.srcPos) upd(p);
@@ -637,14 +643,16 @@ void orc_set_object_pressure(void *h, int type, int idx, int which, const double
 }
 
 // Direct any-hit queries on the scene's occluders (for visibility parity tests).
+// skipA, skipB: per-ray end-point triangles (may be null: none).
 void orc_anyhit(void *h, const float *o, const float *d, const float *tmin, const float *tmax, long n,
-                unsigned char *hit, int brute) {
+                unsigned char *hit, int brute, const int *skipA, const int *skipB) {
     Scene &s = *(Scene *)h;
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) {
         V3 O{o[3 * i], o[3 * i + 1], o[3 * i + 2]}, D{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
-        hit[i] = brute ? anyhit_brute(s.tris.data(), (int)s.tris.size(), s.vp, O, D, tmin[i], tmax[i])
-                       : anyhit_bvh(s.bvh, s.tris.data(), s.vp, O, D, tmin[i], tmax[i]);
+        const int a = skipA ? skipA[i] : -1, b = skipB ? skipB[i] : -1;
+        hit[i] = brute ? anyhit_brute(s.tris.data(), (int)s.tris.size(), s.vp, O, D, tmin[i], tmax[i], a, b)
+                       : anyhit_bvh(s.bvh, s.tris.data(), s.vp, O, D, tmin[i], tmax[i], a, b);
     }
 }
 void orc_vis_params(void *h, float *out) {

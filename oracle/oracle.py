@@ -47,7 +47,7 @@ for name, args, res in [
     ("orc_object_frames", [_v, _i, _i, _fp], None),
     ("orc_get_object_pressure", [_v, _i, _i, _i, _dp], None),
     ("orc_set_object_pressure", [_v, _i, _i, _i, _dp], None),
-    ("orc_anyhit", [_v, _fp, _fp, _fp, _fp, ctypes.c_long, _u8p, _i], None),
+    ("orc_anyhit", [_v, _fp, _fp, _fp, _fp, ctypes.c_long, _u8p, _i, _ip, _ip], None),
     ("orc_vis_params", [_v, _fp], None), ("orc_kf", [_v], _f),
     ("orc_set_near_quadrature", [_v, _i, _f], None),
 ]:
@@ -169,14 +169,18 @@ class Scene:
         v = np.ascontiguousarray(np.stack([np.real(values), np.imag(values)], 1).astype(np.float64))
         _lib.orc_set_object_pressure(self.h, t, i, which, v.ctypes.data_as(_dp))
 
-    def anyhit(self, o, d, tmin, tmax, brute=False):
+    def anyhit(self, o, d, tmin, tmax, brute=False, skip_a=None, skip_b=None):
+        """skip_a, skip_b: per-ray occluder triangles the end points lie on (never tested)."""
         o, d = _f32(o).reshape(-1, 3), _f32(d).reshape(-1, 3)
         n = len(o)
         tmin = _f32(np.broadcast_to(tmin, (n,)))
         tmax = _f32(np.broadcast_to(tmax, (n,)))
         out = np.zeros(n, np.uint8)
+        sa = None if skip_a is None else np.ascontiguousarray(np.broadcast_to(skip_a, (n,)), np.int32)
+        sb = None if skip_b is None else np.ascontiguousarray(np.broadcast_to(skip_b, (n,)), np.int32)
         _lib.orc_anyhit(self.h, o.ctypes.data_as(_fp), d.ctypes.data_as(_fp), tmin.ctypes.data_as(_fp),
-                        tmax.ctypes.data_as(_fp), n, out.ctypes.data_as(_u8p), 1 if brute else 0)
+                        tmax.ctypes.data_as(_fp), n, out.ctypes.data_as(_u8p), 1 if brute else 0,
+                        None if sa is None else sa.ctypes.data_as(_ip), None if sb is None else sb.ctypes.data_as(_ip))
         return out.astype(bool)
 
     def vis_params(self):

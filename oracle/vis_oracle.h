@@ -76,9 +76,12 @@ static inline bool mt_hit(const Tri &T, float padTri, V3 o, V3 d, float tmin, fl
            h.z >= T.bmin.z - padTri && h.z <= T.bmax.z + padTri;
 }
 
-static inline bool anyhit_brute(const Tri *tris, int n, const VisParams &vp, V3 o, V3 d, float tmin, float tmax) {
+// skipA, skipB: the triangles the ray's two end points lie on (or -1); a ray
+// never tests them (it leaves one and ends on the other).
+static inline bool anyhit_brute(const Tri *tris, int n, const VisParams &vp, V3 o, V3 d, float tmin, float tmax,
+                                int skipA = -1, int skipB = -1) {
     for (int i = 0; i < n; i++)
-        if (mt_hit(tris[i], vp.padTri, o, d, tmin, tmax)) return true;
+        if (i != skipA && i != skipB && mt_hit(tris[i], vp.padTri, o, d, tmin, tmax)) return true;
     return false;
 }
 
@@ -155,7 +158,8 @@ static inline bool box_hit(const BvhNode &n, float pad, V3 o, V3 inv, float tmin
     return tn <= tf && tf >= tmin - pad && tn <= tmax + pad;
 }
 
-static inline bool anyhit_bvh(const Bvh &b, const Tri *tris, const VisParams &vp, V3 o, V3 d, float tmin, float tmax) {
+static inline bool anyhit_bvh(const Bvh &b, const Tri *tris, const VisParams &vp, V3 o, V3 d, float tmin, float tmax,
+                              int skipA = -1, int skipB = -1) {
     if (b.nodes.empty()) return false;
     V3 inv{safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     int stack[128];
@@ -165,8 +169,10 @@ static inline bool anyhit_bvh(const Bvh &b, const Tri *tris, const VisParams &vp
         const BvhNode &n = b.nodes[stack[--sp]];
         if (!box_hit(n, vp.padNode, o, inv, tmin, tmax)) continue;
         if (n.left < 0) {
-            for (int i = n.first; i < n.first + n.count; i++)
-                if (mt_hit(tris[b.idx[i]], vp.padTri, o, d, tmin, tmax)) return true;
+            for (int i = n.first; i < n.first + n.count; i++) {
+                const int ti = b.idx[i];
+                if (ti != skipA && ti != skipB && mt_hit(tris[ti], vp.padTri, o, d, tmin, tmax)) return true;
+            }
         } else {
             stack[sp++] = n.left;
             stack[sp++] = n.right;

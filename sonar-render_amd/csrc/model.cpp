@@ -246,7 +246,8 @@ int model_render(Model &m) {
     auto upd = [&](float x, float y, float z) {
         scale = std::max(scale, std::max(std::fabs(x), std::max(std::fabs(y), std::fabs(z))));
     };
-    for (auto &o : m.targets)
+    for (auto &o : m.targets) {
+        o.triBase = (int)(tris.size() / 4);
         for (auto &F : o.frames) {
             Vec3 a = F.v1, b = F.v2, c = F.v3;
             Vec3 e1{b.x - a.x, b.y - a.y, b.z - a.z}, e2{c.x - a.x, c.y - a.y, c.z - a.z};
@@ -255,11 +256,16 @@ int model_render(Model &m) {
             tris.push_back(make_float4(a.x, a.y, a.z, e1.x));
             tris.push_back(make_float4(e1.y, e1.z, e2.x, e2.y));
             tris.push_back(make_float4(e2.z, mn.x, mn.y, mn.z));
-            tris.push_back(make_float4(mx.x, mx.y, mx.z, 0.f));
+            // w: the triangle's index in load order (the facet a ray end point may lie on)
+            const int id = (int)(tris.size() / 4);
+            float idBits;
+            std::memcpy(&idBits, &id, sizeof(float));
+            tris.push_back(make_float4(mx.x, mx.y, mx.z, idBits));
             lo = make_float3(std::min(lo.x, mn.x), std::min(lo.y, mn.y), std::min(lo.z, mn.z));
             hi = make_float3(std::max(hi.x, mx.x), std::max(hi.y, mx.y), std::max(hi.z, mx.z));
             upd(a.x, a.y, a.z); upd(b.x, b.y, b.z); upd(c.x, c.y, c.z);
         }
+    }
     for (auto &o : m.fields)
         for (auto &F : o.frames) { upd(F.v1.x, F.v1.y, F.v1.z); upd(F.v2.x, F.v2.y, F.v2.z); upd(F.v3.x, F.v3.y, F.v3.z); }
     for (size_t i = 0; i < m.srcHost.size(); i += 3) upd(m.srcHost[i], m.srcHost[i + 1], m.srcHost[i + 2]);
@@ -318,6 +324,7 @@ int model_set_source_amplitude(Model &m, int i, float re, float im) {
 // Describes the visibility words of one (emitter set, receiver set); no launch.
 int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, size_t &words) {
     tk = MaskTask{};
+    tk.emTriBase = tk.rcTriBase = -1;
     Object *A = (kind == MK_P2F) ? nullptr : object_at(m, ta, ia);
     Object *B = (kind == MK_P2F || kind >= MK_NORESP) ? object_at(m, tb, ib) : nullptr;
     const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
@@ -325,18 +332,21 @@ int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, 
     case MK_P2F:
         tk.em = PointSet{m.srcP, nullptr, ns};
         tk.rc = PointSet{B->cellP, B->cellN, (int)B->ncell};
+        tk.rcTriBase = B->type == OBJ_TARGET ? B->triBase : -1;
         tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 0; tk.useNormalRc = (B->type == OBJ_TARGET);
         tk.tmin = m.vc.tminPoint;
         break;
     case MK_F2P:
         tk.em = PointSet{A->actP, A->actN, A->nact};
         tk.rc = PointSet{m.fldP, nullptr, nq};
+        tk.emTriBase = A->triBase;
         tk.origin = ORIGIN_RECEIVER; tk.useNormalEm = 1; tk.useNormalRc = 0;
         tk.tmin = m.vc.tminPoint;
         break;
     case MK_SELF:
         tk.em = PointSet{A->actP, A->actN, A->nact};
         tk.rc = tk.em;
+        tk.emTriBase = tk.rcTriBase = A->triBase;
         tk.origin = ORIGIN_LOWER_FACET; tk.useNormalEm = 1; tk.useNormalRc = 1; tk.sameObject = 1;
         tk.upperOnly = 1;  // the set is symmetric: half the rays, the rest by bit transposition
         tk.tmin = m.vc.tminFrag;
@@ -344,18 +354,24 @@ int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, 
     case MK_NORESP:
         tk.em = PointSet{A->actP, A->actN, A->nact};
         tk.rc = PointSet{B->actP, B->actN, B->nact};
+        tk.emTriBase = A->triBase;
+        tk.rcTriBase = B->type == OBJ_TARGET ? B->triBase : -1;
         tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 1; tk.useNormalRc = (B->type == OBJ_TARGET);
         tk.tmin = m.vc.tminFrag;
         break;
     case MK_RESP_FWD:
         tk.em = PointSet{A->actP, A->actN, A->nact};
         tk.rc = PointSet{B->actP, B->actN, B->nact};
+        tk.emTriBase = A->triBase;
+        tk.rcTriBase = B->triBase;
         tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 1; tk.useNormalRc = 1;
         tk.tmin = m.vc.tminFrag;
         break;
     case MK_RESP_REC:
         tk.em = PointSet{B->actP, B->actN, B->nact};
         tk.rc = PointSet{A->actP, A->actN, A->nact};
+        tk.emTriBase = B->triBase;
+        tk.rcTriBase = A->triBase;
         tk.origin = ORIGIN_RECEIVER; tk.useNormalEm = 1; tk.useNormalRc = 1;
         tk.tmin = m.vc.tminFrag;
         break;

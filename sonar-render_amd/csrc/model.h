@@ -35,6 +35,7 @@ struct Object {
     int *actCell = nullptr;    // active -> cell
     float4 *frameTab = nullptr;  // per facet: xAxis, yAxis
     double sameFacetPairs = 0; // sum over facets of (active cells)^2
+    int triBase = -1;          // targets: index of the first facet among the occluder triangles
     std::vector<int> actFacetHost;
     // mixed-precision transfer matrix (built once per geometry and frequency)
     struct Transfer {

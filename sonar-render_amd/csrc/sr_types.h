@@ -27,7 +27,7 @@ struct VisConst {
 
 // Triangle record for the shadow test: 4 x float4 = 64 B.
 //   q0 = (v0.x, v0.y, v0.z, e1.x)   q1 = (e1.y, e1.z, e2.x, e2.y)
-//   q2 = (e2.z, bmin.x, bmin.y, bmin.z)  q3 = (bmax.x, bmax.y, bmax.z, 0)
+//   q2 = (e2.z, bmin.x, bmin.y, bmin.z)  q3 = (bmax.x, bmax.y, bmax.z, triangle index bits)
 // BVH node (internal): 4 x float4 = 64 B.
 //   n0 = (Lmin.xyz, L child bits)  n1 = (Lmax.xyz, R child bits)
 //   n2 = (Rmin.xyz, 0)             n3 = (Rmax.xyz, 0)
@@ -57,6 +57,10 @@ struct MaskTask {
     float tmin;
     uint64_t *mask;       // [ceil(rc.count/64)][em.count]
     int *errFlag;
+    // occluder triangle of each point = base + its facet id (normal w bits);
+    // -1: that side is not an occluder (sources, field points, field objects).
+    // A ray never tests the triangles its end points lie on.
+    int emTriBase, rcTriBase;
 };
 
 }  // namespace sr

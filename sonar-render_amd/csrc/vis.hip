@@ -175,8 +175,9 @@ __device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
 __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Returns this lane's "occluded" flag. active: lane has a ray to trace.
+// skipA, skipB: the triangles this lane's end points lie on (-1: none); never tested.
 __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk, bool active, F3 o, F3 d,
-                                   float tmin, float tmax, int *errFlag) {
+                                   float tmin, float tmax, int *errFlag, int skipA, int skipB) {
     if (bvh.ntri <= 0) return false;
     bool hit = false;
     unsigned long long live = ballot64(active);
@@ -184,8 +185,10 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
     if (bvh.ntri <= kBruteTris) {
         // a handful of occluders: test every triangle, no tree
         for (int ti = 0; ti < bvh.ntri; ti++) {
-            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], bvh.tris[4 * ti + 3]);
-            if (active && !hit) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
+            const float4 q3 = bvh.tris[4 * ti + 3];
+            const int id = __float_as_int(q3.w);
+            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], q3);
+            if (active && !hit && id != skipA && id != skipB) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
             if (ballot64(active && !hit) == 0) break;
         }
         return hit;
@@ -206,14 +209,18 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
         // leaves are resolved immediately
         if (cl < 0 && bl) {
             int ti = ~cl;
-            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], bvh.tris[4 * ti + 3]);
-            if (hl && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
+            const float4 q3 = bvh.tris[4 * ti + 3];
+            const int id = __float_as_int(q3.w);
+            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], q3);
+            if (hl && id != skipA && id != skipB && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
             bl = 0;
         }
         if (cr < 0 && br) {
             int ti = ~cr;
-            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], bvh.tris[4 * ti + 3]);
-            if (hr && !hit && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
+            const float4 q3 = bvh.tris[4 * ti + 3];
+            const int id = __float_as_int(q3.w);
+            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], q3);
+            if (hr && !hit && id != skipA && id != skipB && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
             br = 0;
         }
         unsigned long long remaining = ballot64(active && !hit);
@@ -285,7 +292,9 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
             float c = f3dot(F3{-uv.x, -uv.y, -uv.z}, nt);
             if (!(c >= 1e-6f)) pair = false;
         }
-        bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag);
+        const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
+        const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
+        bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc);
         unsigned long long word = ballot64(pair && !occ);
         if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
     }

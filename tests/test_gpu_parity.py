@@ -84,17 +84,17 @@ def test_masks_sphere_exact(api):
     cf, nf, actf = _oracle_tables(s, 2, 0)
     ns, nq = len(sc["sources"]), len(sc["points"])
     # point -> fragment, every cell of the target
-    ref = maskref.p2f(s, sc["sources"], c["pr"], n, True)
+    ref = maskref.p2f(s, sc["sources"], c["pr"], n, True, cell_tri=c["facet"])
     got = api.mask_bits(0, ns, len(c["area"]), tb=0, ib=0)
     print("P2F visible", ref.sum(), "of", ref.size)
     assert np.array_equal(got, ref)
     # fragment -> point
-    ref = maskref.f2p(s, c["pe"][act], n[act], sc["points"])
+    ref = maskref.f2p(s, c["pe"][act], n[act], sc["points"], em_tri=c["facet"][act])
     got = api.mask_bits(1, int(act.sum()), nq, ta=0, ia=0)
     print("F2P visible", ref.sum(), "of", ref.size)
     assert ref.sum() > 0 and np.array_equal(got, ref)
     # target -> field object
-    ref = maskref.f2f(s, c["pe"][act], n[act], cf["pe"][actf], None, rc_field=True)
+    ref = maskref.f2f(s, c["pe"][act], n[act], cf["pe"][actf], None, rc_field=True, em_tri=c["facet"][act])
     got = api.mask_bits(3, int(act.sum()), int(actf.sum()), ta=0, ia=0, tb=2, ib=0)
     print("NO_RESP visible", ref.sum(), "of", ref.size)
     assert ref.sum() > 0 and np.array_equal(got, ref)
