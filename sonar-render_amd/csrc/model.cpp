@@ -662,7 +662,7 @@ static int ensure_transfer(Model &m, int i) {
     const double d = o.delta;
     const double smax = 0.9 * std::max(kabs / (4 * M_PI), kabs * grow / (4 * M_PI * d) + 1.0 / (d * d));
     t.sS = (float)(16384.0 / smax);
-    TransferGeom g{o.actP, o.actN, n, nKs, nRt};
+    TransferGeom g{o.actP, o.actN, n, nKs, nRt, t.wgSkip};
     TransferPhys ph{m.kf, m.atten, (float)((double)m.atten * 1.4426950408889634), o.delta, 0.9f, t.sS,
                     (double)m.kf / (2.0 * M_PI), m.nearMode,
                     (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr,

@@ -11,6 +11,7 @@ struct TransferGeom {
     int n;              // active cells
     int nKs;            // K steps of 8 emitters (ceil(n/8), padded to a multiple of 8)
     int nRt;            // row tiles of 32 (padded to a multiple of 4)
+    const int2 *wgSkip; // per 4-row-tile group: K steps [x, y) that hold only zeros (not written)
 };
 
 struct TransferPhys {

@@ -315,7 +315,9 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
     const uint64_t *mrow = mask + (size_t)(rvalid ? (row >> 6) : 0) * g.n;
     const int bit = row & 63;
     const int ks0 = kc * ksPerWave, ks1 = min(g.nKs, ks0 + ksPerWave);
+    const int2 skip = g.wgSkip ? g.wgSkip[rt >> 2] : make_int2(0, 0);  // never read by the GEMM
     for (int ks = ks0; ks < ks1; ks++) {
+        if (ks >= skip.x && ks < skip.y) continue;  // wave-uniform
         uint32_t w[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
