@@ -498,7 +498,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         {
             // flops: the complex multiply-accumulates, 8 per pair and plane, on the matrix cores
             ProfScope ps(m, PK_F2F, pairs * nb, pairs * 8.0 * nb);
-            if (f2f_mfma_launch(g, ph, m.stream)) return -1;
+            if (f2f_mfma_launch(g, ph, KIND_F2F, m.stream)) return -1;
         }
         ReduceTask r = rt;
         r.partL = m.scratchL;
@@ -554,7 +554,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         c.nrhs = nb;
         c.eval = t.eval + b0 * evalStride;
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (f2f_mfma_launch(c, ph, m.stream)) return -1;
+        if (f2f_mfma_launch(c, ph, KIND_F2F, m.stream)) return -1;
         if (gated_replace_launch(m.scratchL, c.nsplit, ngated, m.gatedList, t.gval + b0 * gvalStride, gvalStride, nb,
                                  rt.out + b0 * outStride, rt.pcell + b0 * pcellStride, pcellStride, rt.rowCell,
                                  nullptr, m.stream))
@@ -563,8 +563,46 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
     return 0;
 }
 
+// Fragment->field point on the FP32 matrix cores, groups of up to 64 planes:
+// the emitters' total pressures are first gathered into a compact [plane][emitter]
+// block, partial sums are FP32 within a split of 32 emitters and FP64 across splits.
+static int gather_points_mfma(Model &m, GatherTask t, const Phys &ph, int nplanes, long evalStride, double pairs) {
+    const int nsplit = std::max(1, std::min(2048, (t.ne + 31) / 32));
+    t.emPerSplit = ((t.ne + nsplit - 1) / nsplit + 31) / 32 * 32;
+    t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
+    const int nb0 = std::min(64, nplanes);
+    // scratchL holds the partials, scratchU the compact emitter values
+    if (ensure_scratch(m, std::max((size_t)t.nsplit * 64 * t.nr, (size_t)nb0 * t.ne))) return -1;
+    for (int b0 = 0; b0 < nplanes; b0 += 64) {
+        const int nb = std::min(64, nplanes - b0);
+        {
+            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);  // data movement, counted with the reductions
+            if (prime_launch(t.eval + b0 * evalStride, evalStride, t.ecell, t.ne, nb, m.scratchU, t.ne, m.stream))
+                return -1;
+        }
+        GatherTask g = t;
+        g.nrhs = nb;
+        g.ecell = nullptr;
+        g.eval = m.scratchU;
+        g.evalStride = t.ne;
+        g.partL = m.scratchL;
+        g.partU = nullptr;
+        {
+            ProfScope ps(m, PK_F2P, pairs * nb, pairs * 8.0 * nb);
+            if (f2f_mfma_launch(g, ph, KIND_F2P, m.stream)) return -1;
+        }
+        ProfScope pr(m, PK_REDUCE, 0.0, 0.0);
+        if (reduce_points_launch(m.scratchL, nullptr, g.nsplit, 64, nb, t.nr, m.fldVal + (size_t)b0 * t.nr, t.nr,
+                                 m.stream))
+            return -1;
+    }
+    return 0;
+}
+
 static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, ReduceTask rt, int nplanes,
                          long evalStride, long gvalStride, long outStride, long pcellStride, double pairs) {
+    if (kind == KIND_F2P && m.gatherMode == 0 && nplanes >= 16)
+        return gather_points_mfma(m, t, ph, nplanes, evalStride, pairs);
     if (kind == KIND_F2F && m.gatherMode == 0 && nplanes >= 16 && !t.ecell &&
         (t.gateMode == GATE_NONE || (t.gateMode == GATE_UPPER && t.gval == t.eval && gvalStride == evalStride)))
         return gather_planes_mfma(m, t, ph, rt, nplanes, evalStride, gvalStride, outStride, pcellStride, pairs);

@@ -86,16 +86,18 @@ struct P2FTask {
 
 int gather_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st);
 int reduce_launch(const ReduceTask &t, hipStream_t st);
+// lo may be null (partials without a compensation term)
 int reduce_points_launch(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs, int nq,
                          double2 *fld, long fldStride, hipStream_t st);
 int p2f_launch(const P2FTask &t, const Phys &ph, hipStream_t st);
 int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
                  long outStride, hipStream_t st);
 int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st);
-// Fragment->fragment for 1..64 planes with the complex products on the FP32
-// matrix cores. Partial sums [split][64][nr] (plane stride 64); the gate is
-// not applied (gateMode is ignored).
-int f2f_mfma_launch(const GatherTask &t, const Phys &ph, hipStream_t st);
+// Fragment->fragment (kind KIND_F2F) or fragment->field point (KIND_F2P) for
+// 1..64 planes with the complex products on the FP32 matrix cores. Partial
+// sums [split][64][nr] (plane stride 64), FP32 within one emitter split; the
+// gate is not applied (gateMode is ignored).
+int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st);
 // Gated rows (list of active indices) as a compact receiver set: positions,
 // normals, first lower-facet bound and (when mask is given) visibility words
 // [ceil(count/64)][ne] restricted to the lower-facet emitters.

@@ -257,7 +257,9 @@ __device__ __attribute__((always_inline)) inline void mfma_tile_dma(const Gather
     }
 }
 
-template <bool ATT>
+// KIND_F2F: receivers are fragments (both cosines); KIND_F2P: receivers are
+// field points (emitter cosine only, the phase split for long ranges).
+template <int KIND, bool ATT>
 __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
     __shared__ MfmaTile TA, TB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
     const int nrb = (t.nr + 63) >> 6;
     const int rb0 = blockIdx.x * 2;
     const float4 rp = rvalid ? t.rp[row] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 rn = rvalid ? t.rn[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 rn = (rvalid && KIND == KIND_F2F) ? t.rn[row] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int mslot = wave >> 1;
     const int mshift = (wave & 1) * 32 + j;
     const int e0 = split * t.emPerSplit;
@@ -299,7 +301,7 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
             const bool bit = ke < ne && ((T.m[mslot][ke] >> mshift) & 1ull);
             const float4 ep = T.p[ke];
             float Kre, Kim;
-            pair_kernel<KIND_F2F, ATT>(ph, rp, rn, ep, T.n[ke], bit, Kre, Kim);
+            pair_kernel<KIND, ATT>(ph, rp, rn, ep, T.n[ke], bit, Kre, Kim);
             Kre *= ep.w;  // emitter area
             Kim *= ep.w;
             const auto sw =
@@ -546,8 +548,8 @@ __global__ __launch_bounds__(256) void k_reduce_points(const float2 *hi, const f
     double sr = 0.0, si = 0.0;
     for (int s = lane; s < nsplit; s += 64) {
         const size_t o = ((size_t)s * nbStride + b) * nq + q;
-        sr += (double)hi[o].x + (double)lo[o].x;
-        si += (double)hi[o].y + (double)lo[o].y;
+        sr += (double)hi[o].x + (lo ? (double)lo[o].x : 0.0);
+        si += (double)hi[o].y + (lo ? (double)lo[o].y : 0.0);
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -678,15 +680,22 @@ int gated_replace_launch(const float2 *part, int nsplit, int count, const int *l
     return 0;
 }
 
-int f2f_mfma_launch(const GatherTask &t, const Phys &ph, hipStream_t st) {
+int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st) {
     if (t.nr <= 0 || t.ne <= 0) return 0;
     if (t.ecell || t.nrhs < 1 || t.nrhs > 64 || t.emPerSplit <= 0 || (long)t.nsplit * t.emPerSplit < t.ne) {
         set_error("f2f_mfma: needs compact emitters, 1..64 planes and a covering emitter split");
         return -2;
     }
+    if (kind != KIND_F2F && kind != KIND_F2P) { set_error("f2f_mfma: fragment or field-point receivers only"); return -2; }
+    if (kind == KIND_F2F && !t.rn) { set_error("f2f_mfma: fragment receivers need normals"); return -2; }
     dim3 grid((t.nr + 127) / 128, t.nsplit);
-    if (ph.alpha != 0.0f) k_f2f_mfma<true><<<grid, 256, 0, st>>>(t, ph);
-    else k_f2f_mfma<false><<<grid, 256, 0, st>>>(t, ph);
+    if (kind == KIND_F2F) {
+        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2F, true><<<grid, 256, 0, st>>>(t, ph);
+        else k_f2f_mfma<KIND_F2F, false><<<grid, 256, 0, st>>>(t, ph);
+    } else {
+        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2P, true><<<grid, 256, 0, st>>>(t, ph);
+        else k_f2f_mfma<KIND_F2P, false><<<grid, 256, 0, st>>>(t, ph);
+    }
     SR_HIP(hipGetLastError());
     return 0;
 }

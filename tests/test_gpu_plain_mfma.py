@@ -93,3 +93,20 @@ def test_plain_mfma_64_planes_and_remainder(api):
     assert e < 1e-5
     ref = oracle.run_scene(sc, plane_source=69)
     assert rel_l2(_flat(mf, 69), np.concatenate([ref["points"], ref["targets"][0]])) < 2e-4
+
+
+def test_field_points_on_matrix_cores(api):
+    """Fragment->field point for 64 planes on the matrix cores (FP32 within 32-emitter
+    splits, FP64 across them) against the compensated vector-unit gather and the oracle."""
+    sc = scenes.config_trihedral(f=10e3, n_side=8, bounces=0)
+    _banner("F2P on matrix cores, 64 planes, single scattering")
+    mf, _ = _run(api, sc, 0)
+    va, _ = _run(api, sc, 1)
+    e = rel_l2(mf["points"].ravel(), va["points"].ravel())
+    print("field points, matrix cores vs vector units rel-L2", e)
+    assert e < 1e-5
+    for p in (0, 31, 63):
+        ref = oracle.run_scene(sc, plane_source=p)
+        ep = rel_l2(mf["points"][p], ref["points"])
+        print("plane", p, "field points vs oracle", ep)
+        assert ep < 1e-4
