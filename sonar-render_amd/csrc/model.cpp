@@ -176,6 +176,19 @@ struct ProfScope {
     }
 };
 
+// A device count copied into pinned host word `slot`; the host waits for that
+// copy alone (word_wait), not for the work queued after it.
+int word_readback(Model &m, const int *dev, int slot) {
+    SR_HIP(hipMemcpyAsync(m.hostWords + slot, dev, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    SR_HIP(hipEventRecord(m.wordEv[slot], m.stream));
+    return 0;
+}
+int word_wait(Model &m, int slot, int &out) {
+    SR_HIP(hipEventSynchronize(m.wordEv[slot]));
+    out = m.hostWords[slot];
+    return 0;
+}
+
 }  // namespace
 
 void set_error(const std::string &msg) {
@@ -234,6 +247,10 @@ int model_set_batched_sources(Model &m, int on) {
 int model_render(Model &m) {
     if (m.rendered) { set_error("render_cuda called twice without TearDownCuda"); return -3; }
     if (!m.stream) SR_HIP(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+    if (!m.hostWords) {  // kept, like the stream, for the life of the process
+        SR_HIP(hipHostMalloc((void **)&m.hostWords, Model::kWords * sizeof(int), hipHostMallocDefault));
+        for (auto &e : m.wordEv) SR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
     m.nplanes = m.batchedSources ? std::max(1, ns) : 1;
     for (auto &o : m.targets) if (upload_object(m, o)) return -1;
@@ -476,15 +493,15 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         if (!m.gatedCount && dev_alloc(&m.gatedCount, 1)) return -1;
         if (counters_zero_launch(m.gatedCount, 1, m.stream)) return -1;
         if (transfer_gated_rows(t.gval, t.nr, nplanes, m.gatedList, m.gatedCount, m.stream)) return -1;
-        SR_HIP(hipMemcpyAsync(&ngated, m.gatedCount, sizeof(int), hipMemcpyDeviceToHost, m.stream));
-        SR_HIP(hipStreamSynchronize(m.stream));
+        if (word_readback(m, m.gatedCount, 0)) return -1;  // read after the main passes are queued
     }
     // ~2048 workgroups of 128 receivers; emitter ranges in whole LDS tiles of 32
     const int groups = (t.nr + 127) / 128;
     int nsplit = std::max(1, std::min((2048 + groups - 1) / groups, t.ne / 64));
     t.emPerSplit = ((t.ne + nsplit - 1) / nsplit + 31) / 32 * 32;
     t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
-    if (ensure_scratch(m, (size_t)t.nsplit * 64 * t.nr)) return -1;
+    // also covers the compact set's partials: splits <= ceil(512 / groups), so split count x rows <= 512 x 128 + nr + 128
+    if (ensure_scratch(m, std::max((size_t)t.nsplit * t.nr, (size_t)512 * 128 + t.nr + 128) * 64)) return -1;
     for (int b0 = 0; b0 < nplanes; b0 += 64) {
         const int nb = std::min(64, nplanes - b0);
         GatherTask g = t;
@@ -517,6 +534,8 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         ProfScope pr(m, PK_REDUCE, 0.0, 0.0);
         if (reduce_launch(r, m.stream)) return -1;
     }
+    if (!gated) return 0;
+    if (word_wait(m, 0, ngated)) return -1;
     if (!ngated) return 0;
     if (outStride != gvalStride) { set_error("gated MFMA gather: P_in and P_out strides differ"); return -2; }
     // the gated rows as a compact receiver set whose words hold only lower-facet emitters
@@ -547,7 +566,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
     int cs = std::max(1, std::min((512 + cgroups - 1) / cgroups, t.ne / 64));
     c.emPerSplit = ((t.ne + cs - 1) / cs + 31) / 32 * 32;
     c.nsplit = (t.ne + c.emPerSplit - 1) / c.emPerSplit;
-    if (ensure_scratch(m, (size_t)c.nsplit * 64 * ngated)) return -1;
+    if ((size_t)c.nsplit * 64 * ngated > m.scratchElems) { set_error("gated MFMA gather: scratch bound"); return -2; }
     c.partL = m.scratchL;
     for (int b0 = 0; b0 < nplanes; b0 += 64) {
         const int nb = std::min(64, nplanes - b0);
@@ -711,6 +730,8 @@ static int ensure_transfer(Model &m, int i) {
         ph.mlp = nf.P;
     }
     int counts[2] = {0, 0};
+    const double key[4] = {(double)m.nearMode, (double)m.nearRadiusWl, (double)m.nearQuadDeltas, (double)m.kf};
+    const bool known = std::equal(key, key + 4, t.nearKey);
     for (int pass = 0; pass < 2; pass++) {
         if (m.nearMode != 0) {
             if (!t.nearCount && dev_alloc(&t.nearCount, 2)) return -1;
@@ -727,9 +748,19 @@ static int ensure_transfer(Model &m, int i) {
             if (transfer_build_s(g, ph, mk, t.A, m.stream)) return -1;
         }
         if (m.nearMode == 0) break;
+        if (known) {  // same lists as the last build, which fitted
+            counts[0] = t.nearKnown[0];
+            counts[1] = t.nearKnown[1];
+            break;
+        }
         SR_HIP(hipMemcpyAsync(counts, t.nearCount, 2 * sizeof(int), hipMemcpyDeviceToHost, m.stream));
         SR_HIP(hipStreamSynchronize(m.stream));
-        if (counts[0] <= t.nearCap && counts[1] <= t.nearCap) break;
+        if (counts[0] <= t.nearCap && counts[1] <= t.nearCap) {
+            std::copy(key, key + 4, t.nearKey);
+            t.nearKnown[0] = counts[0];
+            t.nearKnown[1] = counts[1];
+            break;
+        }
         dev_free(t.nearQ); dev_free(t.nearM);
         t.nearCap = std::max(counts[0], counts[1]) + std::max(counts[0], counts[1]) / 4;
         if (dev_alloc(&t.nearQ, t.nearCap) || dev_alloc(&t.nearM, t.nearCap)) return -1;
@@ -755,15 +786,17 @@ static int bounce_mixed(Model &m, int i) {
     const int ksPer = ((t.nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
     nsplit = (t.nKs + ksPer - 1) / ksPer;
     // rows with a zero input in some plane (the gate), counted before the product
+    // and read back while the first plane group's product runs
     if (counters_zero_launch(t.gatedCount, 1, m.stream)) return -1;
     if (transfer_gated_rows(o.Pin, n, m.nplanes, t.gatedList, t.gatedCount, m.stream)) return -1;
-    int gated = 0;
-    SR_HIP(hipMemcpyAsync(&gated, t.gatedCount, sizeof(int), hipMemcpyDeviceToHost, m.stream));
-    SR_HIP(hipStreamSynchronize(m.stream));
+    if (word_readback(m, t.gatedCount, 0)) return -1;
+    // partials of the main product, and of the compact set (split count x rows <= 1024 x 128 + n + 128)
+    if (ensure_scratch(m, std::max((size_t)nsplit * n, (size_t)1024 * 128 + n + 128) * 64)) return -1;
+    int gated = -1;  // not yet known
+    int nRtC = 0, nsplitC = 1, ksPerC = t.nKs;
     // the gated rows as compact row tiles of S (lower-facet emitters only)
-    const int nRtC = ((gated + 127) / 128) * 4;
-    int nsplitC = 1, ksPerC = t.nKs;
-    if (gated) {
+    auto compact_set = [&]() -> int {
+        nRtC = ((gated + 127) / 128) * 4;
         if (m.gCap < gated) {
             dev_free(m.gPc); dev_free(m.gNc); dev_free(m.gFs);
             m.gCap = 0;
@@ -786,8 +819,9 @@ static int bounce_mixed(Model &m, int i) {
         nsplitC = std::max(1, std::min(1024 / (nRtC / 4), t.nKs / 64));
         ksPerC = ((t.nKs + nsplitC - 1) / nsplitC + 7) / 8 * 8;
         nsplitC = (t.nKs + ksPerC - 1) / ksPerC;
-    }
-    if (ensure_scratch(m, std::max((size_t)nsplit * 64 * n, (size_t)nsplitC * 64 * gated))) return -1;
+        if ((size_t)nsplitC * 64 * gated > m.scratchElems) { set_error("transfer bounce: scratch bound"); return -2; }
+        return 0;
+    };
     for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
         const int np = std::min(64, m.nplanes - g0);
         const float2 *pin = o.Pin + (size_t)g0 * n;
@@ -823,6 +857,10 @@ static int bounce_mixed(Model &m, int i) {
         {
             ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
             if (reduce_launch(r, m.stream)) return -1;
+        }
+        if (gated < 0) {  // the count has long landed by now; the product above keeps the GPU busy
+            if (word_wait(m, 0, gated)) return -1;
+            if (gated && compact_set()) return -1;
         }
         if (!gated) continue;
         TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, m.scratchL};

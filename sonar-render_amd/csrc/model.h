@@ -50,6 +50,11 @@ struct Object {
         int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
         int *nearCount = nullptr;
         int nearCap = 0;
+        // near-pair counts of the last build and the settings they hold for: the
+        // counts depend only on geometry, visibility and these, so a rebuild with
+        // the same settings needs no read-back
+        int nearKnown[2] = {0, 0};
+        double nearKey[4] = {-1, 0, 0, 0};
         float *colScale = nullptr, *colInv = nullptr;
         float sS = 1.f;
         double aBytesRead = 0;  // bytes of S one GEMM pass reads (same-facet steps are skipped)
@@ -74,6 +79,10 @@ struct Model {
 
     // device
     hipStream_t stream = nullptr;
+    // counts read back while the stream runs on: pinned words, one event each
+    static constexpr int kWords = 4;
+    int *hostWords = nullptr;
+    hipEvent_t wordEv[kWords] = {};
     float4 *srcP = nullptr;
     float2 *srcAmp = nullptr;
     float4 *fldP = nullptr;
