@@ -164,17 +164,19 @@ int sr_gather_mode(void);
 /* Near-field terms of the transfer matrix (mode 1 transfer only; pairs of
  * fragments closer than radiusWavelengths): 0 the reference's point rule,
  * 1 16x16 sub-fragment quadrature of the same kernel over the emitter's
- * cell, 2 the learned correction (a 9-64-64-2 MLP, BF16 inference). */
+ * cell, 2 the learned correction (a 10-64-64-2 MLP, BF16 inference). */
 int sr_set_near_mode(int mode, float radiusWavelengths);
-/* Learned mode: pairs closer than `deltas` fragment sizes (1 to 16, default 1)
- * use the quadrature instead of the network. Reset to 1 by TearDownCuda. */
+/* Learned mode: pairs closer than `deltas` fragment sizes (1 to 16, default 3)
+ * use the quadrature instead of the network. The network is trained on pairs
+ * at 3 or more fragment sizes; a smaller radius applies it outside that
+ * range. Reset to 3 by TearDownCuda. */
 int sr_set_near_quadrature_radius(float deltas);
 
 /* The learned correction's network: trained on the GPU with the library's own
  * forward/backward/Adam kernels on pairs labelled by the quadrature.
  * report2: relative RMS error on training and held-out pairs. Parameters are
- * 4930 floats (W1[64][9] b1 W2[64][64] b2 W3[2][64] b3). sr_nf_samples
- * returns labelled pairs (9 features, 2 targets each); sr_nf_forward the FP32
+ * 4994 floats (W1[64][10] b1 W2[64][64] b2 W3[2][64] b3). sr_nf_samples
+ * returns labelled pairs (10 features, 2 targets each); sr_nf_forward the FP32
  * network output; sr_nf_gradient the loss gradient of a given parameter set
  * on given pairs (L = sum |out - y|^2 / n). */
 int sr_nf_init(unsigned seed);

@@ -7,15 +7,22 @@
 // square centred on the emitter point (cosine culls and the r < delta/16
 // regularisation applied at every sub-point).
 //
-// A small MLP (9 -> 64 -> 64 -> 2, tanh) learns that correction from pairs
+// A small MLP (10 -> 64 -> 64 -> 2, tanh) learns that correction from pairs
 // labelled by the quadrature, in units of the wavelength, so one network
-// serves every frequency. In the emitter's frame (x = xAxis, y = yAxis,
-// z = normal), with u the unit vector emitter -> receiver:
+// serves every frequency. It corrects a cheap base rule rather than the point
+// value: the 2 x 2 Gauss-Legendre average of the same kernel over the cell
+// (near_rules.h), whose remainder is a smooth fourth-order term. In the
+// emitter's frame (x = xAxis, y = yAxis, z = normal), with u the unit vector
+// emitter -> receiver:
 //   features: r/lambda - 1, 14 (delta/lambda - 0.19), u.x, u.y, 2 u.z - 1,
-//             n_r.x, n_r.y, n_r.z, ln(r/delta)/2
-//   target:   t = (Q - K_true) e^{-ikr} / M,  M = 0.9 (k/(4 pi r) + 1/r^2),
-// with K_true = cos_s cos_i 0.9 G(r) (per unit area), for delta <= r < 2 lambda.
-// Pairs closer than delta keep the quadrature as fallback.
+//             n_r.x, n_r.y, n_r.z, ln(r/delta)/2, and c = -(r u . n_r)/delta
+//             clamped to [-3, 3]: where the receiver's plane crosses the
+//             emitter cell (the receiver cosine culls part of the cell there,
+//             which is most of what the base rule misses)
+//   target:   t = g (Q - G2) e^{-ikr} / M,  M = 0.9 (k/(4 pi r) + 1/r^2),
+// g = kNfTargetGain,
+// G2 the base rule (per unit area), for 3 delta <= r < 2 lambda.
+// Pairs closer than that keep the quadrature as fallback.
 // Training (forward, backward, weight gradients, Adam) and inference are the
 // kernels below; inference rounds weights and activations to BF16.
 
@@ -25,6 +32,7 @@
 
 #include "launch.h"
 #include "learned.h"
+#include "near_rules.h"
 
 namespace sr {
 
@@ -79,29 +87,17 @@ __device__ float2 near_quad16(float3 rv, float3 nr, bool fieldDst, float d, floa
     return make_float2(acc.x / (n * n), acc.y / (n * n));
 }
 
-__device__ inline void features(float r, float lam, float d, float3 u, float3 nr, float *f) {
-    f[0] = r / lam - 1.0f;
-    f[1] = (d / lam - 0.19f) * 14.0f;
-    f[2] = u.x;
-    f[3] = u.y;
-    f[4] = 2.0f * u.z - 1.0f;
-    f[5] = nr.x;
-    f[6] = nr.y;
-    f[7] = nr.z;
-    f[8] = 0.5f * logf(r / d);
-}
-
 // Training pairs in the emitter's frame, lengths in wavelengths (k = 2 pi).
 __global__ void k_nf_samples(uint32_t seed, int n, float dMin, float dMax, float rMax, float *X, float *Y) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float k = 2.0f * kPi, lam = 1.0f;
     const float d = dMin + (dMax - dMin) * urand(seed, i, 0);
-    // distance in [delta, rMax], denser close in where the correction is large;
-    // closer pairs are left to the quadrature (the point rule's r < delta
-    // regularisation makes them ill-conditioned for a correction)
+    // distance in [3 delta, rMax], denser close in where the correction is
+    // large; closer pairs are left to the quadrature
     const float u0 = urand(seed, i, 1);
-    const float r = d + (rMax - d) * u0 * u0;
+    const float r0 = kNfRMinDeltas * d;
+    const float r = r0 + (rMax - r0) * u0 * u0;
     float3 uv, nr;
     float cs, ci;
     // half the receivers at right angles to the emitter: right-angle junctions
@@ -122,15 +118,15 @@ __global__ void k_nf_samples(uint32_t seed, int n, float dMin, float dMax, float
     }
     const float3 rv = make_float3(uv.x * r, uv.y * r, uv.z * r);
     float f[kMlpIn];
-    features(r, lam, d, uv, nr, f);
+    near_features(r, lam, d, uv, nr, f);
     for (int q = 0; q < kMlpIn; q++) X[(size_t)i * kMlpIn + q] = f[q];
     const float2 Q = near_quad16(rv, nr, false, d, k);
-    const float2 K = point_kernel(r, cs, ci, k, 0.0f);
-    const float M = 0.9f * (k * kInv4PiL / r + 1.0f / (r * r));
+    const float2 K = near_gauss2(rv, nr, d, k);
+    const float M = 0.9f * (k * kInv4PiL / r + 1.0f / (r * r)) * (1.0f / kNfTargetGain);
     float s, c;
     sincosf(k * r, &s, &c);
     const float dre = Q.x - K.x, dim = Q.y - K.y;
-    // (dre + i dim) e^{-ikr} / M
+    // g (dre + i dim) e^{-ikr} / M
     Y[(size_t)i * 2] = (dre * c + dim * s) / M;
     Y[(size_t)i * 2 + 1] = (dim * c - dre * s) / M;
 }

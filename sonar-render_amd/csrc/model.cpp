@@ -1118,7 +1118,7 @@ void model_teardown(Model &m) {
     m.masks.clear();
     m.transferMode = 0;
     m.nearMode = 0;
-    m.nearQuadDeltas = 1.0f;
+    m.nearQuadDeltas = kNfRMinDeltas;
     m.gatherMode = 0;
     dev_free(m.gatedList); dev_free(m.gatedCount);
     m.gatedCap = 0;

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "fragmenter.h"
+#include "learned.h"
 #include "sr_types.h"
 #include "sums.h"
 
@@ -105,7 +106,7 @@ struct Model {
     size_t gMcCap = 0;
     int nearMode = 0;      // transfer matrix near field: 0 point rule, 1 16x16 quadrature, 2 learned
     float nearRadiusWl = 2.0f;
-    float nearQuadDeltas = 1.0f;  // learned mode: pairs closer than this many fragment sizes use the quadrature
+    float nearQuadDeltas = kNfRMinDeltas;  // learned mode: pairs closer than this many fragment sizes use the quadrature
     float2 *scratchL = nullptr, *scratchU = nullptr;
     size_t scratchElems = 0;
     // per-launch profiling (HIP events on the library's stream)

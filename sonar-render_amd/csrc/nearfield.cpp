@@ -68,7 +68,7 @@ int nf_init(uint32_t seed) {
 }
 
 int nf_set_params(const float *in, int n) {
-    if (n != kMlpParams) { set_error("near-field MLP has 4930 parameters"); return -2; }
+    if (n != kMlpParams) { set_error("near-field MLP has 4994 parameters"); return -2; }
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
     SR_HIP(hipMemcpy(nf.P, in, n * sizeof(float), hipMemcpyHostToDevice));
@@ -77,7 +77,7 @@ int nf_set_params(const float *in, int n) {
 }
 
 int nf_get_params(float *out, int n) {
-    if (n < kMlpParams) { set_error("buffer shorter than 4930 parameters"); return -2; }
+    if (n < kMlpParams) { set_error("buffer shorter than 4994 parameters"); return -2; }
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
     SR_HIP(hipMemcpy(out, nf.P, kMlpParams * sizeof(float), hipMemcpyDeviceToHost));

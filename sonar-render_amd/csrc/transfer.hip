@@ -31,6 +31,7 @@
 #include "pair_kernel.h"
 #include "transfer.h"
 #include "learned.h"
+#include "near_rules.h"
 
 namespace sr {
 
@@ -51,28 +52,6 @@ __device__ inline uint32_t pack_h2(float a, float b) {
 // Near-field values (per unit emitter area) for a pair closer than nearRadius.
 // rv, nr: receiver point (relative to the emitter point) and normal, in the
 // emitter's frame (x = xAxis, y = yAxis, z = normal).
-
-__device__ inline float bf16_round(float x) {
-    uint32_t u = __float_as_uint(x);
-    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
-    return __uint_as_float(u);
-}
-
-__device__ inline float2 g_point(float r, float cs, float ci, float k, float reg) {
-    float re, im;
-    if (r < reg) {
-        re = 0.f;
-        im = k * kInv4PiT;
-    } else {
-        float s, c;
-        sincosf(k * r, &s, &c);
-        const float a = k * kInv4PiT / r;
-        re = a * s + 1.0f / (r * r);
-        im = -a * c;
-    }
-    const float w = cs * ci * 0.9f;
-    return make_float2(w * re, w * im);
-}
 
 // 16 x 16 midpoint quadrature of the pair kernel over the emitter's cell.
 // The phase of each sub-point uses the hardware sine/cosine in revolutions
@@ -148,8 +127,9 @@ __global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys 
     store_pair(A, g.nKs, pr.x, pr.y, pack_h2(v.x * ph.scale, v.y * ph.scale));
 }
 
-// Near pairs by the learned correction: value = K + M t e^{ikr}, t from the
-// 9-64-64-2 tanh MLP in BF16 (weights and activations rounded to BF16, FP32
+// Near pairs by the learned correction: value = G2 + (M / g) t e^{ikr} with G2 the
+// 2 x 2 Gauss base rule (near_rules.h), t from the
+// 10-64-64-2 tanh MLP in BF16 (weights and activations rounded to BF16, FP32
 // accumulation) on the BF16 matrix cores (v_mfma_f32_32x32x16_bf16). A wave
 // evaluates 32 pairs: features -> LDS -> A operand of layer 1; each hidden
 // layer's accumulators go through LDS (tanh, BF16) to become the next A
@@ -208,15 +188,15 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
                 pr = list[i];
                 const NearGeom q = near_geom(g, ph, pr.x, pr.y);
                 const float r = q.r;
-                const float cs = q.rv.z / r, ci = -(q.rv.x * q.nr.x + q.rv.y * q.nr.y + q.rv.z * q.nr.z) / r;
-                K = g_point(r, cs, ci, ph.kr, 0.0f);  // unregularised point value
-                M = 0.9f * (ph.kr * kInv4PiT / r + 1.0f / (r * r));
+                K = near_gauss2(q.rv, q.nr, d, ph.kr);  // the base rule the network corrects
+                M = 0.9f * (ph.kr * kInv4PiT / r + 1.0f / (r * r)) * (1.0f / kNfTargetGain);
                 sincosf(ph.kr * r, &sn_kr, &cs_kr);
-                const float f[kMlpIn] = {r / lam - 1.0f, (d / lam - 0.19f) * 14.0f, q.rv.x / r, q.rv.y / r,
-                                         2.0f * cs - 1.0f, q.nr.x, q.nr.y, q.nr.z, 0.5f * logf(r / d)};
+                float f[kMlpIn];
+                near_features(r, lam, d, make_float3(q.rv.x / r, q.rv.y / r, q.rv.z / r), q.nr, f);
 #pragma unroll
                 for (int qq = 0; qq < 8; qq++) f0[qq] = (__bf16)f[qq];
-                f1[0] = (__bf16)f[8];
+#pragma unroll
+                for (int qq = 8; qq < kMlpIn; qq++) f1[qq - 8] = (__bf16)f[qq];
             }
             *reinterpret_cast<bf16x8 *>(&H[j][0]) = f0;
             *reinterpret_cast<bf16x8 *>(&H[j][8]) = f1;
@@ -261,7 +241,7 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
         for (int k = 0; k < 64; k++) o = fmaf(sW3[h][k], (float)H[j][k], o);
         const float o1 = __shfl(o, j + 32);
         if (h == 0 && valid) {
-            // K + M t e^{ikr}
+            // G2 + (M / g) t e^{ikr}
             const float re = K.x + M * (o * cs_kr - o1 * sn_kr), im = K.y + M * (o * sn_kr + o1 * cs_kr);
             store_pair(A, g.nKs, pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
         }

@@ -355,8 +355,8 @@ _sig("sr_nf_samples", [ctypes.c_uint, _i, _fp, _fp])
 _sig("sr_nf_forward", [_fp, _i, _fp])
 _sig("sr_nf_gradient", [_fp, _fp, _fp, _i, _fp])
 
-NF_PARAMS = 4930
-NF_IN = 9
+NF_PARAMS = 4994
+NF_IN = 10
 NEAR_POINT, NEAR_QUADRATURE, NEAR_LEARNED = 0, 1, 2
 
 
@@ -368,7 +368,8 @@ _sig("sr_set_near_quadrature_radius", [_f])
 
 
 def set_near_quadrature_radius(deltas):
-    """Learned mode: pairs closer than `deltas` fragment sizes use the 16x16 quadrature."""
+    """Learned mode: pairs closer than `deltas` fragment sizes (default 3, the network's
+    training range starts there) use the 16x16 quadrature."""
     _check(cpp_lib.sr_set_near_quadrature_radius(float(deltas)), "set_near_quadrature_radius")
 
 

@@ -1,19 +1,28 @@
 """FP64 numpy restatement of the near-field learning problem (test infrastructure).
 
 * label(x): the 16 x 16 sub-fragment quadrature of the pair kernel over the
-  emitter's cell, minus the unregularised point value, times e^{-ikr} / M,
-  rebuilt from the features alone (lengths in wavelengths, emitter frame).
-* forward / gradient: the 9-64-64-2 tanh network and the gradient of
+  emitter's cell, minus the base rule (the unregularised kernel averaged over
+  the cell's 2 x 2 Gauss-Legendre points, numpy's leggauss nodes), times
+  e^{-ikr} / M and the target gain, rebuilt from the features alone (lengths in wavelengths,
+  emitter frame).
+* forward / gradient: the 10-64-64-2 tanh network and the gradient of
   L = sum_i |out_i - y_i|^2 / n, by explicit back-propagation.
 """
 import numpy as np
 
-OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, NPAR = 576, 640, 4736, 4800, 4928, 4930
+NIN = 10  # network inputs
+OFF_B1 = 64 * NIN
+OFF_W2 = OFF_B1 + 64
+OFF_B2 = OFF_W2 + 64 * 64
+OFF_W3 = OFF_B2 + 64
+OFF_B3 = OFF_W3 + 2 * 64
+NPAR = OFF_B3 + 2
+TARGET_GAIN = 4096.0  # the library's kNfTargetGain
 
 
 def unpack(p):
     p = np.asarray(p, np.float64)
-    return (p[:OFF_B1].reshape(64, 9), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(64, 64), p[OFF_B2:OFF_W3],
+    return (p[:OFF_B1].reshape(64, NIN), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(64, 64), p[OFF_B2:OFF_W3],
             p[OFF_W3:OFF_B3].reshape(2, 64), p[OFF_B3:NPAR])
 
 
@@ -42,6 +51,22 @@ def _g(r, cs, ci, k, reg):
     return cs * ci * 0.9 * G
 
 
+def gauss2(rv, nr, d, k):
+    """The base rule: the kernel (no regularisation, cosine culls) at the 2 x 2
+    Gauss-Legendre points of the d x d emitter cell, equally weighted."""
+    x, _ = np.polynomial.legendre.leggauss(2)
+    o = x * d / 2
+    acc = 0.0
+    for oy in o:
+        for ox in o:
+            v = np.array([rv[0] - ox, rv[1] - oy, rv[2]])
+            rs = np.linalg.norm(v)
+            css, cis = v[2] / rs, -(v @ nr) / rs
+            if css >= 1e-6 and cis >= 1e-6:
+                acc = acc + _g(rs, css, cis, k, 0.0)
+    return acc / 4
+
+
 def label(x):
     """Target of one feature vector (k = 2 pi, lambda = 1)."""
     k = 2 * np.pi
@@ -50,7 +75,6 @@ def label(x):
     u = np.array([x[2], x[3], (x[4] + 1.0) / 2.0], np.float64)
     nr = np.array(x[5:8], np.float64)
     rv = u * r
-    cs, ci = u[2], -u @ nr
     h = d / 16
     o = (np.arange(16) + 0.5) * h - 0.5 * d
     ox, oy = np.meshgrid(o, o, indexing="xy")
@@ -59,7 +83,7 @@ def label(x):
     css, cis = v[..., 2] / rs, -(v @ nr) / rs
     ok = (css >= 1e-6) & (cis >= 1e-6)
     Q = np.where(ok, _g(rs, css, cis, k, h), 0).sum() / 256
-    K = _g(r, cs, ci, k, 0.0)
+    K = gauss2(rv, nr, d, k)
     M = 0.9 * (k / (4 * np.pi * r) + 1 / r ** 2)
-    t = (Q - K) * np.exp(-1j * k * r) / M
+    t = TARGET_GAIN * (Q - K) * np.exp(-1j * k * r) / M
     return np.array([t.real, t.imag])

@@ -58,12 +58,12 @@ def test_training_reduces_heldout_error(api):
     api.nf_init(1)
     X, Y = api.nf_samples(99, 4096)
     e0 = np.linalg.norm(api.nf_forward(X) - Y) / np.linalg.norm(Y)
-    tr, va = api.nf_train(nsamples=131072, steps=1500, batch=4096, lr=3e-3, seed=7)
+    tr, va = api.nf_train(nsamples=131072, steps=8000, batch=4096, lr=3e-3, seed=7)
     e1 = np.linalg.norm(api.nf_forward(X) - Y) / np.linalg.norm(Y)
     print("untrained", e0, "trained: train", tr, "held-out", va, "fresh", e1)
-    # 1500 steps bring the relative label error from ~19 (random init) to ~0.8;
-    # the scene-level gain of a longer run is the next test's subject
-    assert va < 0.9 and e1 < 0.9 and e1 < 0.05 * e0
+    # the targets are the base rule's remainder scaled to order one, so an
+    # untrained network sits near 1; 8000 steps explain most of the variance
+    assert va < 0.8 and e1 < 0.8 and e1 < 0.7 * e0
     assert abs(va - e1) < 0.1  # held-out and fresh samples agree: no over-fitting
 
 
@@ -98,18 +98,27 @@ def test_quadrature_mode_matches_oracle_judge(api):
 
 
 def test_learned_mode_gain_over_point_rule(api):
+    """Against the judge: the learned mode (2 x 2 Gauss base rule + trained
+    network) beats the point rule, and the trained network beats its own base
+    rule alone (the same mode with every weight zero, so the network adds 0)."""
     sc, lam = _trihedral_near_case()
-    _banner("learned near field vs point rule, both against the judge")
+    _banner("learned near field vs point rule and vs its base rule, against the judge")
     api.nf_init(1)
-    print("train / held-out error", api.nf_train(nsamples=262144, steps=4000, batch=4096, lr=3e-3, seed=7))
-    assert np.all(np.isfinite(api.nf_params()))
+    print("train / held-out error", api.nf_train(nsamples=262144, steps=30000, batch=4096, lr=3e-3, seed=7))
+    trained = api.nf_params()
+    assert np.all(np.isfinite(trained))
     pt = _mixed_run(api, sc, api.NEAR_POINT)
     ml = _mixed_run(api, sc, api.NEAR_LEARNED)
-    gains = []
+    api.nf_set_params(np.zeros_like(trained))
+    base = _mixed_run(api, sc, api.NEAR_LEARNED)
+    api.nf_set_params(trained)
+    gains, net_gains = [], []
     for p in range(len(sc["sources"])):
         ref = oracle.run_scene(sc, plane_source=p, near=(16, 2 * lam))
         b = np.concatenate([ref["points"], ref["targets"][0]])
-        e_pt, e_ml = rel_l2(_flat(pt, p), b), rel_l2(_flat(ml, p), b)
-        print("plane", p, "point rule", e_pt, "learned", e_ml)
+        e_pt, e_ml, e_b = rel_l2(_flat(pt, p), b), rel_l2(_flat(ml, p), b), rel_l2(_flat(base, p), b)
+        print("plane", p, "point rule", e_pt, "base rule", e_b, "learned", e_ml)
         gains.append(e_pt / e_ml)
+        net_gains.append(e_b / e_ml)
     assert min(gains) > 1.5
+    assert min(net_gains) > 1.2
