@@ -244,9 +244,12 @@ def main():
         "value": P["gpairs_per_s"], "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": P["ms_per_scene"], "s_per_scene": round(P["ms_per_scene"] / 1e3, 6),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f16" if primary.startswith("mixed") else "f32", "data": "synthetic",
+        # the transfer forms multiply in FP16 (and the learned near field in BF16)
+        "dtype": "f32" if primary == "plain_fp32" else "f16", "data": "synthetic",
         "config": {"workload": scene["name"], "form": primary, "incidences": n_planes,
-                   "bounces": scene.get("bounces", 0), "pairs_per_scene": pairs, "parallelism": "single GPU"},
+                   "bounces": scene.get("bounces", 0), "pairs_per_scene": P["pairs_per_scene"],
+                   "fragment": "lambda/4" if primary.startswith("learned") else "lambda/7.5",
+                   "parallelism": "single GPU"},
         "rel_l2_vs_fp64": P["rel_l2_vs_fp64"], "speedup_mixed_vs_plain": speedup,
         "speedup_learned_vs_plain": speedup_l,
         "forms": per_form, "roofline": P["roofline"], "cpu_baseline": cpu,

@@ -816,7 +816,9 @@ static int bounce_mixed(Model &m, int i) {
                                  m.stream) ||
             transfer_gather_rows(t.A, t.nKs, t.gatedList, m.gFs, gated, nRtC, t.Ac, m.stream))
             return -1;
-        nsplitC = std::max(1, std::min(1024 / (nRtC / 4), t.nKs / 64));
+        // few rows: split K finer (down to single chunks, at most 48 splits: the
+        // replacement sums every split's partial) so the grid still fills the chip
+        nsplitC = std::max(1, std::min({1024 / (nRtC / 4), t.nKs / 8, 48}));
         ksPerC = ((t.nKs + nsplitC - 1) / nsplitC + 7) / 8 * 8;
         nsplitC = (t.nKs + ksPerC - 1) / ksPerC;
         if ((size_t)nsplitC * 64 * gated > m.scratchElems) { set_error("transfer bounce: scratch bound"); return -2; }

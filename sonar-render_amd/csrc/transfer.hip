@@ -53,15 +53,17 @@ __device__ inline uint32_t pack_h2(float a, float b) {
 // rv, nr: receiver point (relative to the emitter point) and normal, in the
 // emitter's frame (x = xAxis, y = yAxis, z = normal).
 
-// 16 x 16 midpoint quadrature of the pair kernel over the emitter's cell.
-// The phase of each sub-point uses the hardware sine/cosine in revolutions
-// (r < 2 lambda here, so r k / 2 pi < 2 keeps ~1e-7 of a revolution).
-__device__ float2 quad16(float3 rv, float3 nr, float d, float k) {
+// One row (sy) of the 16 x 16 midpoint quadrature of the pair kernel over the
+// emitter's cell, unnormalised. The phase of each sub-point uses the hardware
+// sine/cosine in revolutions (r < 2 lambda here, so r k / 2 pi < 2 keeps
+// ~1e-7 of a revolution).
+__device__ float2 quad16_row(float3 rv, float3 nr, float d, float k, int sy) {
     const int n = 16;
     const float h = d / n;
     const float krev = k * 0.159154943091895336f;
     float2 acc = make_float2(0.f, 0.f);
-    for (int sy = 0; sy < n; sy++)
+    {
+#pragma unroll 4
         for (int sx = 0; sx < n; sx++) {
             const float vx = rv.x - ((sx + 0.5f) * h - 0.5f * d), vy = rv.y - ((sy + 0.5f) * h - 0.5f * d);
             const float vz = rv.z;
@@ -84,7 +86,8 @@ __device__ float2 quad16(float3 rv, float3 nr, float d, float k) {
             acc.x += w * re;
             acc.y += w * im;
         }
-    return make_float2(acc.x * (1.0f / 256.0f), acc.y * (1.0f / 256.0f));
+    }
+    return acc;
 }
 
 // Geometry of one near pair in the emitter's frame (x = xAxis, y = yAxis,
@@ -116,15 +119,27 @@ __device__ inline void store_pair(uint4 *A, int nKs, int row, int j, uint32_t w)
     q[j & 3] = w;
 }
 
-// Near pairs by 16x16 sub-fragment quadrature: one lane per pair.
+// Near pairs by 16x16 sub-fragment quadrature: 16 lanes per pair, one row of
+// sub-points each, summed across the lanes.
 __global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
                                                    int count, uint4 *__restrict__ A) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int2 pr = list[i];
-    const NearGeom q = near_geom(g, ph, pr.x, pr.y);
-    const float2 v = quad16(q.rv, q.nr, ph.deltaReg, ph.kr);
-    store_pair(A, g.nKs, pr.x, pr.y, pack_h2(v.x * ph.scale, v.y * ph.scale));
+    const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sy = threadIdx.x & 15;
+    const bool valid = i < count;  // uniform over the pair's 16 lanes
+    int2 pr = make_int2(0, 0);
+    float2 v = make_float2(0.f, 0.f);
+    if (valid) {
+        pr = list[i];
+        const NearGeom q = near_geom(g, ph, pr.x, pr.y);
+        v = quad16_row(q.rv, q.nr, ph.deltaReg, ph.kr, sy);
+    }
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {
+        v.x += __shfl_xor(v.x, m);
+        v.y += __shfl_xor(v.y, m);
+    }
+    const float sc = ph.scale * (1.0f / 256.0f);
+    if (valid && sy == 0) store_pair(A, g.nKs, pr.x, pr.y, pack_h2(v.x * sc, v.y * sc));
 }
 
 // Near pairs by the learned correction: value = G2 + (M / g) t e^{ikr} with G2 the
@@ -564,7 +579,7 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st) {
     if (nQuad > ph.nearCap || nMlp > ph.nearCap) { set_error("near lists overflow"); return -2; }
     if (nQuad > 0) {
-        k_near_quad<<<(nQuad + 255) / 256, 256, 0, st>>>(g, ph, ph.nearQ, nQuad, A);
+        k_near_quad<<<(unsigned)(((long)nQuad * 16 + 255) / 256), 256, 0, st>>>(g, ph, ph.nearQ, nQuad, A);
         SR_HIP(hipGetLastError());
     }
     if (nMlp > 0) {
