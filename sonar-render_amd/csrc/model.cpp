@@ -135,7 +135,7 @@ int upload_object(Model &m, Object &o) {
 void free_transfer(Object &o) {
     auto &t = o.tc;
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
-    dev_free(t.gatedList); dev_free(t.gatedCount); dev_free(t.Ac); dev_free(t.skipZero);
+    dev_free(t.gatedList); dev_free(t.state); dev_free(t.rowFs); dev_free(t.Ac); dev_free(t.skipZero);
     dev_free(t.nearQ); dev_free(t.nearM); dev_free(t.nearCount);
     t = Object::Transfer{};
 }
@@ -688,10 +688,18 @@ static int ensure_transfer(Model &m, int i) {
         const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
             dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
-            dev_alloc(&t.gatedList, n) || dev_alloc(&t.gatedCount, 1))
+            dev_alloc(&t.gatedList, n) || dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes)) ||
+            dev_alloc(&t.rowFs, n))
             return -1;
         t.nKs = nKs;
         t.nRt = nRt;
+        {
+            const auto &fa = o.actFacetHost;
+            std::vector<int> fs(n);
+            for (int r = 0; r < n; r++)
+                fs[r] = (r > 0 && fa[r] == fa[r - 1]) ? fs[r - 1] : r;  // rows are sorted by facet
+            SR_HIP(hipMemcpy(t.rowFs, fs.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        }
         // A group of 128 rows on one facet f never reads the K chunks (64 emitters)
         // that lie wholly inside f's emitter range: same-facet pairs are zero.
         const auto &fa = o.actFacetHost;
@@ -776,7 +784,7 @@ static int ensure_transfer(Model &m, int i) {
 // One bounce of target i in the transfer form: P_out = S x for every row, then
 // the rows whose own P_in is zero in some plane get the product of their
 // lower-facet part of S (the zero-input gate) for those planes, P += P_out.
-static int bounce_mixed(Model &m, int i) {
+static int bounce_mixed(Model &m, int i, bool first) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     const int n = o.nact;
@@ -785,11 +793,23 @@ static int bounce_mixed(Model &m, int i) {
     int nsplit = std::max(1, std::min(1024 / std::max(1, wgRows), t.nKs / 64));
     const int ksPer = ((t.nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
     nsplit = (t.nKs + ksPer - 1) / ksPer;
+    // bounce state: this bounce's set (zero), and the other one, cleared here for the next
+    const int stWords = 1 + m.nplanes;
+    if (first) {
+        if (counters_zero_launch(t.state, 2 * stWords, m.stream)) return -1;
+        t.stateSet = 0;
+    }
+    int *cur = t.state + (size_t)t.stateSet * stWords, *nxt = t.state + (size_t)(t.stateSet ^ 1) * stWords;
+    t.stateSet ^= 1;
+    unsigned *colMax = reinterpret_cast<unsigned *>(cur + 1);
     // rows with a zero input in some plane (the gate), counted before the product
     // and read back while the first plane group's product runs
-    if (counters_zero_launch(t.gatedCount, 1, m.stream)) return -1;
-    if (transfer_gated_rows(o.Pin, n, m.nplanes, t.gatedList, t.gatedCount, m.stream)) return -1;
-    if (word_readback(m, t.gatedCount, 0)) return -1;
+    if (transfer_gated_rows(o.Pin, n, m.nplanes, t.gatedList, cur, m.stream)) return -1;
+    if (word_readback(m, cur, 0)) return -1;
+    {
+        ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
+        if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
+    }
     // partials of the main product, and of the compact set (split count x rows <= 1024 x 128 + n + 128)
     if (ensure_scratch(m, std::max((size_t)nsplit * n, (size_t)1024 * 128 + n + 128) * 64)) return -1;
     int gated = -1;  // not yet known
@@ -797,12 +817,6 @@ static int bounce_mixed(Model &m, int i) {
     // the gated rows as compact row tiles of S (lower-facet emitters only)
     auto compact_set = [&]() -> int {
         nRtC = ((gated + 127) / 128) * 4;
-        if (m.gCap < gated) {
-            dev_free(m.gPc); dev_free(m.gNc); dev_free(m.gFs);
-            m.gCap = 0;
-            if (dev_alloc(&m.gPc, gated) || dev_alloc(&m.gNc, gated) || dev_alloc(&m.gFs, gated)) return -1;
-            m.gCap = gated;
-        }
         const size_t need = (size_t)nRtC * t.nKs * 64;
         if (t.acCap < need || t.skipZeroCap < nRtC / 4) {
             dev_free(t.Ac); dev_free(t.skipZero);
@@ -812,10 +826,7 @@ static int bounce_mixed(Model &m, int i) {
             t.acCap = need; t.skipZeroCap = nRtC / 4;
         }
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (gated_compact_launch(t.gatedList, gated, o.actP, o.actN, nullptr, n, m.gPc, m.gNc, m.gFs, nullptr,
-                                 m.stream) ||
-            transfer_gather_rows(t.A, t.nKs, t.gatedList, m.gFs, gated, nRtC, t.Ac, m.stream))
-            return -1;
+        if (transfer_gather_rows(t.A, t.nKs, t.gatedList, t.rowFs, gated, nRtC, t.Ac, m.stream)) return -1;
         // few rows: split K finer (down to single chunks, at most 48 splits: the
         // replacement sums every split's partial) so the grid still fills the chip
         nsplitC = std::max(1, std::min({1024 / (nRtC / 4), t.nKs / 8, 48}));
@@ -829,8 +840,9 @@ static int bounce_mixed(Model &m, int i) {
         const float2 *pin = o.Pin + (size_t)g0 * n;
         {
             ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
-            if (transfer_col_scale(o.actP, pin, n, n, np, t.colScale, t.colInv, t.sS, m.stream)) return -1;
-            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, t.colScale, t.B, m.stream)) return -1;
+            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + g0, t.sS, t.colScale, t.colInv, t.B,
+                                 m.stream))
+                return -1;
         }
         TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL};
         {
@@ -887,7 +899,7 @@ int model_project_t2t(Model &m, int nref) {
             if (!o.nact) continue;
             std::swap(o.Pin, o.Pout);  // SwapOutputToInputPressure; P_out is fully rewritten below
             if (m.transferMode == 1) {
-                if (ensure_transfer(m, i) || bounce_mixed(m, i)) return -1;
+                if (ensure_transfer(m, i) || bounce_mixed(m, i, it == 0)) return -1;
                 continue;
             }
             uint64_t *mk;

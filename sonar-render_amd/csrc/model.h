@@ -43,7 +43,12 @@ struct Object {
         uint4 *A = nullptr, *B = nullptr;
         int nKs = 0, nRt = 0;
         int2 *wgSkip = nullptr;    // per 128-row group: zero K-step range (its own single facet)
-        int *gatedList = nullptr, *gatedCount = nullptr;  // rows with a zero input in some plane
+        int *gatedList = nullptr;  // rows with a zero input in some plane
+        // bounce state, two sets used in turn: [0] gated-row count, [1 + b] max |A P_in| of plane b
+        // (float bits); each bounce clears the other set for the next
+        int *state = nullptr;
+        int stateSet = 0;
+        int *rowFs = nullptr;      // per active row: first active index on its facet
         uint4 *Ac = nullptr;       // ... as compact row tiles of A (lower-facet emitters only)
         int2 *skipZero = nullptr;  // empty skip ranges for those tiles
         size_t acCap = 0;

@@ -405,6 +405,7 @@ __global__ void k_gated_replace(const float2 *__restrict__ part, int nsplit, int
     const float2 g = Pin[(size_t)b * stride + r];
     if (!(g.x == 0.0f && g.y == 0.0f)) return;
     float2 L = make_float2(0.f, 0.f);
+#pragma unroll 8
     for (int s = 0; s < nsplit; s++) {
         const float2 v = part[((size_t)s * 64 + b) * count + i];
         L.x += v.x;

@@ -42,14 +42,18 @@ struct TransferGemm {
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
 // Writes the listed near pairs' values into S (counts read back from nearCount).
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st);
-int transfer_col_scale(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, float *scale,
-                       float *invScale, float sS, hipStream_t st);
-int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes, const float *scale,
-                     uint4 *B, hipStream_t st);
+// Per-plane max |A_j P_in[b][j]| into colMax (float bits, zeroed beforehand);
+// also zeroes nextWords ints at nextState (the next bounce's state set).
+int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, unsigned *colMax,
+                     int *nextState, int nextWords, hipStream_t st);
+// B of one plane group, scaled from colMax; stores each plane's scale and the
+// inverse 1 / (scale sS) the reduction applies.
+int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
+                     const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st);
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
 // Gated rows as compact row tiles of the transfer matrix (nRtC tiles of 32,
-// a multiple of 4): emitters at or above each row's facet bound fsc[i] zeroed.
-int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *fsc, int count, int nRtC, uint4 *Ac,
+// a multiple of 4): emitters at or above row r's facet bound rowFs[r] zeroed.
+int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, int count, int nRtC, uint4 *Ac,
                          hipStream_t st);
 // Lists the rows with an exactly zero input pressure in some plane (count: device int, zeroed by the caller).
 int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st);

@@ -369,36 +369,54 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
 }
 
 // Per-incidence scale: 2^13 / max(|Re X|, |Im X|) with X = A_j P_in[j,b].
-__global__ __launch_bounds__(256) void k_col_scale(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
-                                                   long stride, int n, float *__restrict__ scale,
-                                                   float *__restrict__ invScale, float sS) {
-    __shared__ float red[256];
+// Per-plane max |A_j P_in[b][j]| over the active cells (the FP16 scaling of
+// B), atomically into colMax[b] as float bits (non-negative floats order like
+// their bit patterns). Grid (planes, row chunks). Block (0, 0) also clears the
+// other bounce-state set, which the next bounce uses.
+__global__ __launch_bounds__(256) void k_col_max(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
+                                                 long stride, int n, unsigned *__restrict__ colMax,
+                                                 int *__restrict__ nextState, int nextWords) {
+    __shared__ float red[4];
     const int b = blockIdx.x;
+    if (b == 0 && blockIdx.y == 0)
+        for (int k = threadIdx.x; k < nextWords; k += 256) nextState[k] = 0;
+    const int per = (n + gridDim.y - 1) / gridDim.y;
+    const int j0 = blockIdx.y * per, j1 = min(n, j0 + per);
     float m = 0.f;
-    for (int j = threadIdx.x; j < n; j += 256) {
-        float2 v = Pin[b * stride + j];
-        float a = p[j].w;
+#pragma unroll 4
+    for (int j = j0 + threadIdx.x; j < j1; j += 256) {
+        const float2 v = Pin[b * stride + j];
+        const float a = p[j].w;
         m = fmaxf(m, fmaxf(fabsf(a * v.x), fabsf(a * v.y)));
     }
-    red[threadIdx.x] = m;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        float mx = red[0];
-        float sc = mx > 0.f ? 8192.0f / mx : 1.0f;
-        scale[b] = sc;
-        invScale[b] = 1.0f / (sc * sS);
-    }
+    if (threadIdx.x == 0)
+        atomicMax(&colMax[b], __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+// The FP16 scale of plane b's B column: its largest entry becomes 8192.
+__device__ inline float col_scale_of(unsigned bits) {
+    const float mx = __uint_as_float(bits);
+    return mx > 0.f ? 8192.0f / mx : 1.0f;
 }
 
 // B build for one group of up to 64 incidences.
+// Scales come from colMax (k_col_max); threads 0..nplanes-1 also store each
+// plane's scale and the inverse the reduction applies (1 / (scale sS)).
 __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
                                                  long stride, int n, int nKs, int nplanes,
-                                                 const float *__restrict__ scale, uint4 *__restrict__ B) {
+                                                 const unsigned *__restrict__ colMax, float sS,
+                                                 float *__restrict__ scaleOut, float *__restrict__ invOut,
+                                                 uint4 *__restrict__ B) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nplanes) {
+        const float sc = col_scale_of(colMax[t]);
+        scaleOut[t] = sc;
+        invOut[t] = 1.0f / (sc * sS);
+    }
     if (t >= (long)nKs * 4 * 64) return;
     const int lane = (int)(t & 63);
     const int ct = (int)((t >> 6) & 3);
@@ -413,7 +431,7 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
         float xr = 0.f, xi = 0.f;
         if (j < n && b < nplanes) {
             float2 v = Pin[b * stride + j];
-            float a = p[j].w * scale[b];
+            float a = p[j].w * col_scale_of(colMax[b]);
             xr = a * v.x;
             xi = a * v.y;
         }
@@ -536,7 +554,7 @@ __global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes,
 // goes through k_transfer_gemm with the bounce's B: the gated rows see exactly
 // the operator the other rows see, near-field terms included.
 __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int *__restrict__ list,
-                                const int *__restrict__ fsc, int count, int nRtC, uint4 *__restrict__ Ac) {
+                                const int *__restrict__ rowFs, int count, int nRtC, uint4 *__restrict__ Ac) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nRtC * nKs * 64) return;
     const int lane = (int)(t & 63);
@@ -545,7 +563,7 @@ __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int 
     const int i = ct * 32 + (lane & 31), h = lane >> 5;
     uint4 q = make_uint4(0, 0, 0, 0);
     if (i < count) {
-        const int r = list[i], fs = fsc[i];
+        const int r = list[i], fs = rowFs[r];
         const int j0 = ks * 8 + h * 4;  // first emitter of this fragment
         if (j0 < fs) {
             q = A[((size_t)(r >> 5) * nKs + ks) * 64 + (r & 31) + 32 * h];
@@ -591,19 +609,21 @@ int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int 
     return 0;
 }
 
-int transfer_col_scale(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, float *scale,
-                       float *invScale, float sS, hipStream_t st) {
-    if (nplanes <= 0) return 0;
-    k_col_scale<<<nplanes, 256, 0, st>>>(p, Pin, stride, n, scale, invScale, sS);
+int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, unsigned *colMax,
+                     int *nextState, int nextWords, hipStream_t st) {
+    if (nplanes <= 0 || n <= 0) return 0;
+    const dim3 grid(nplanes, std::max(1, std::min(16, n / 1024)));
+    k_col_max<<<grid, 256, 0, st>>>(p, Pin, stride, n, colMax, nextState, nextWords);
     SR_HIP(hipGetLastError());
     return 0;
 }
 
-int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes, const float *scale,
-                     uint4 *B, hipStream_t st) {
-    const long tot = (long)nKs * 256;
+int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
+                     const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st) {
+    const long tot = std::max((long)nKs * 256, (long)nplanes);
     if (tot <= 0) return 0;
-    k_build_b<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, scale, B);
+    k_build_b<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, colMax, sS, scale,
+                                                             invScale, B);
     SR_HIP(hipGetLastError());
     return 0;
 }
@@ -620,11 +640,11 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     return 0;
 }
 
-int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *fsc, int count, int nRtC, uint4 *Ac,
+int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, int count, int nRtC, uint4 *Ac,
                          hipStream_t st) {
     const long tot = (long)nRtC * nKs * 64;
     if (tot <= 0) return 0;
-    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, fsc, count, nRtC, Ac);
+    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, rowFs, count, nRtC, Ac);
     SR_HIP(hipGetLastError());
     return 0;
 }
