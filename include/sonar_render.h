@@ -200,6 +200,11 @@ int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, f
  * algorithmic HBM bytes (transfer GEMM: S, B and the partial sums). */
 int sr_profile(int on);
 int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
+/* Measurement hook with no counterpart in the reference: replays the last
+ * transfer-form bounce product (k_transfer_gemm over the same S and B) `reps`
+ * times back to back and returns the mean time per launch, free of the
+ * per-launch event overhead of sr_kernel_stats. Overwrites only scratch. */
+int sr_time_transfer_gemm(int reps, double *msPerLaunch);
 
 /* STL meshes (binary or ASCII in, binary out): 9 floats per triangle. */
 long sr_read_stl(const char *path, float *out9, long cap);

@@ -259,6 +259,10 @@ int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double 
     });
 }
 
+int sr_time_transfer_gemm(int reps, double *msPerLaunch) {
+    return guarded("time_transfer_gemm", [&] { return sr::model_time_transfer_gemm(gModel, reps, msPerLaunch); });
+}
+
 long sr_read_stl(const char *path, float *out, long cap) {
     long rc = -1;
     guarded("read_stl", [&] {

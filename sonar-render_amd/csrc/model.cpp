@@ -573,7 +573,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         c.nrhs = nb;
         c.eval = t.eval + b0 * evalStride;
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (f2f_mfma_launch(c, ph, KIND_F2F, m.stream)) return -1;
+        if (f2f_mfma_launch(c, ph, KIND_F2F, m.stream, true)) return -1;
         if (gated_replace_launch(m.scratchL, c.nsplit, ngated, m.gatedList, t.gval + b0 * gvalStride, gvalStride, nb,
                                  rt.out + b0 * outStride, rt.pcell + b0 * pcellStride, pcellStride, rt.rowCell,
                                  nullptr, m.stream))
@@ -845,6 +845,8 @@ static int bounce_mixed(Model &m, int i, bool first) {
                 return -1;
         }
         TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL};
+        m.lastGemm = gm;
+        m.haveLastGemm = true;
         {
             const double pairs = ((double)n * n - o.sameFacetPairs) * np;
             // flops of the GEMM as executed: 2 * rows * 128 columns * 2 * K (padded)
@@ -877,7 +879,7 @@ static int bounce_mixed(Model &m, int i, bool first) {
             if (gated && compact_set()) return -1;
         }
         if (!gated) continue;
-        TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, m.scratchL};
+        TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, m.scratchL, 1};
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
         if (transfer_gemm(gc, m.stream) ||
             gated_replace_launch(m.scratchL, nsplitC, gated, t.gatedList, pin, n, np, o.Pout + (size_t)g0 * n,
@@ -1120,6 +1122,28 @@ int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *p
     return 0;
 }
 
+int model_time_transfer_gemm(Model &m, int reps, double *msPerLaunch) {
+    if (!m.haveLastGemm || !m.rendered) { set_error("no transfer-form bounce has run since render_cuda"); return -3; }
+    if (reps < 1 || reps > 10000) { set_error("reps: 1 to 10000"); return -2; }
+    hipEvent_t a, b;
+    SR_HIP(hipEventCreate(&a));
+    SR_HIP(hipEventCreate(&b));
+    int rc = transfer_gemm(m.lastGemm, m.stream);  // warm
+    if (!rc) SR_HIP(hipEventRecord(a, m.stream));
+    for (int r = 0; !rc && r < reps; r++) rc = transfer_gemm(m.lastGemm, m.stream);
+    float ms = 0.f;
+    if (!rc) {
+        SR_HIP(hipEventRecord(b, m.stream));
+        SR_HIP(hipEventSynchronize(b));
+        SR_HIP(hipEventElapsedTime(&ms, a, b));
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    *msPerLaunch = (double)ms / reps;
+    return 0;
+}
+
 void model_teardown(Model &m) {
     model_profile_reset(m);
     if (m.stream) (void)hipStreamSynchronize(m.stream);
@@ -1131,6 +1155,7 @@ void model_teardown(Model &m) {
     for (auto &kv : m.masks) (void)hipFree(kv.second.p);
     m.masks.clear();
     m.transferMode = 0;
+    m.haveLastGemm = false;
     m.nearMode = 0;
     m.nearQuadDeltas = kNfRMinDeltas;
     m.gatherMode = 0;

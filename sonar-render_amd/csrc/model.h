@@ -14,6 +14,7 @@
 
 #include "fragmenter.h"
 #include "learned.h"
+#include "transfer.h"
 #include "sr_types.h"
 #include "sums.h"
 
@@ -114,6 +115,9 @@ struct Model {
     float nearQuadDeltas = kNfRMinDeltas;  // learned mode: pairs closer than this many fragment sizes use the quadrature
     float2 *scratchL = nullptr, *scratchU = nullptr;
     size_t scratchElems = 0;
+    // the last transfer-form bounce product, for sr_time_transfer_gemm
+    TransferGemm lastGemm{};
+    bool haveLastGemm = false;
     // per-launch profiling (HIP events on the library's stream)
     bool profiling = false;
     struct Rec { int kind; hipEvent_t a, b; double pairs, flops, bytes; };

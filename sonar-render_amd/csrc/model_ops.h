@@ -35,6 +35,7 @@ int model_set_near_mode(Model &m, int mode, float radiusWl);
 void model_profile(Model &m, int on);
 void model_profile_reset(Model &m);
 int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
+int model_time_transfer_gemm(Model &m, int reps, double *msPerLaunch);
 void model_teardown(Model &m);
 
 // Visibility-word task of one pair set (kinds as sr_get_mask), without launching it.

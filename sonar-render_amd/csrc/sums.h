@@ -97,7 +97,7 @@ int any_zero_launch(const float2 *v, long n, int *flag, hipStream_t st);
 // 1..64 planes with the complex products on the FP32 matrix cores. Partial
 // sums [split][64][nr] (plane stride 64), FP32 within one emitter split; the
 // gate is not applied (gateMode is ignored).
-int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st);
+int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st, bool compactSet = false);
 // Gated rows (list of active indices) as a compact receiver set: positions,
 // normals, first lower-facet bound and (when mask is given) visibility words
 // [ceil(count/64)][ne] restricted to the lower-facet emitters.

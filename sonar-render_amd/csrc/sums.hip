@@ -259,7 +259,8 @@ __device__ __attribute__((always_inline)) inline void mfma_tile_dma(const Gather
 
 // KIND_F2F: receivers are fragments (both cosines); KIND_F2P: receivers are
 // field points (emitter cosine only, the phase split for long ranges).
-template <int KIND, bool ATT>
+// ROLE 1: the compact set of gated rows (identical code, its own name in profiles).
+template <int KIND, bool ATT, int ROLE>
 __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
     __shared__ MfmaTile TA, TB;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -703,7 +704,7 @@ int gated_replace_launch(const float2 *part, int nsplit, int count, const int *l
     return 0;
 }
 
-int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st) {
+int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st, bool compactSet) {
     if (t.nr <= 0 || t.ne <= 0) return 0;
     if (t.ecell || t.nrhs < 1 || t.nrhs > 64 || t.emPerSplit <= 0 || (long)t.nsplit * t.emPerSplit < t.ne) {
         set_error("f2f_mfma: needs compact emitters, 1..64 planes and a covering emitter split");
@@ -712,12 +713,15 @@ int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t s
     if (kind != KIND_F2F && kind != KIND_F2P) { set_error("f2f_mfma: fragment or field-point receivers only"); return -2; }
     if (kind == KIND_F2F && !t.rn) { set_error("f2f_mfma: fragment receivers need normals"); return -2; }
     dim3 grid((t.nr + 127) / 128, t.nsplit);
-    if (kind == KIND_F2F) {
-        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2F, true><<<grid, 256, 0, st>>>(t, ph);
-        else k_f2f_mfma<KIND_F2F, false><<<grid, 256, 0, st>>>(t, ph);
+    if (kind == KIND_F2F && compactSet) {
+        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2F, true, 1><<<grid, 256, 0, st>>>(t, ph);
+        else k_f2f_mfma<KIND_F2F, false, 1><<<grid, 256, 0, st>>>(t, ph);
+    } else if (kind == KIND_F2F) {
+        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2F, true, 0><<<grid, 256, 0, st>>>(t, ph);
+        else k_f2f_mfma<KIND_F2F, false, 0><<<grid, 256, 0, st>>>(t, ph);
     } else {
-        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2P, true><<<grid, 256, 0, st>>>(t, ph);
-        else k_f2f_mfma<KIND_F2P, false><<<grid, 256, 0, st>>>(t, ph);
+        if (ph.alpha != 0.0f) k_f2f_mfma<KIND_F2P, true, 0><<<grid, 256, 0, st>>>(t, ph);
+        else k_f2f_mfma<KIND_F2P, false, 0><<<grid, 256, 0, st>>>(t, ph);
     }
     SR_HIP(hipGetLastError());
     return 0;

@@ -37,6 +37,7 @@ struct TransferGemm {
     int nsplit, ksPerSplit;
     const int2 *wgSkip;                   // per 4-row-tile group: K steps [x, y) that hold only zeros
     float2 *partL;                        // [split][64][n]
+    int compactSet = 0;                   // 1: the gated rows' compact set (same kernel, its own name in profiles)
 };
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);

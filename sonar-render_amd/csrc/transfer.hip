@@ -453,6 +453,9 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
 constexpr int kKC = 8;
 static_assert(kKC == 8, "K steps and split sizes are padded to multiples of 8");
 
+// ROLE 0: a bounce's product; 1: the compact set of gated rows (identical
+// code, instantiated twice so that profiles tell the two launches apart).
+template <int ROLE>
 __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     __shared__ uint4 sB[2][kKC * 4 * 64];
     const int wave = threadIdx.x >> 6;
@@ -635,7 +638,8 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
         return -2;
     }
     dim3 grid((t.nRt + 3) / 4, t.nsplit);
-    k_transfer_gemm<<<grid, 256, 0, st>>>(t);
+    if (t.compactSet) k_transfer_gemm<1><<<grid, 256, 0, st>>>(t);
+    else k_transfer_gemm<0><<<grid, 256, 0, st>>>(t);
     SR_HIP(hipGetLastError());
     return 0;
 }

@@ -288,6 +288,16 @@ def kernel_stats(family):
     return dict(ms=ms.value, launches=n.value, pairs=pairs.value, flops=flops.value, bytes=by.value)
 
 
+_sig("sr_time_transfer_gemm", [_i, ctypes.POINTER(ctypes.c_double)])
+
+
+def time_transfer_gemm(reps=20):
+    """Mean ms per launch of the last transfer-form bounce product, replayed back to back."""
+    ms = ctypes.c_double()
+    _check(cpp_lib.sr_time_transfer_gemm(int(reps), ctypes.byref(ms)), "time_transfer_gemm")
+    return ms.value
+
+
 _sig("sr_set_transfer_mode", [_i])
 _sig("sr_transfer_mode", [])
 KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9, "near": 10})
