@@ -58,6 +58,7 @@ def _configs():
         "trihedral": lambda: scenes.config_trihedral(),
         "sphere": lambda: scenes.config_sphere(),
         "plate": lambda: scenes.config_plate(),
+        "hull": lambda: scenes.config_hull(),
     }
 
 
@@ -175,7 +176,8 @@ def main():
     want = args.forms.split(",")
     forms = [("plain_fp32", 0, None, scene, "plain"), ("mixed_fp16_mfma", 1, None, scene, "mixed")]
     learned = None
-    if "learned" in want and scene.get("bounces", 0):
+    # the near-field correction is defined for attenuation-free media only
+    if "learned" in want and scene.get("bounces", 0) and not scene.get("atten"):
         import scenes
         import sonar_api as api
         lam = scene["c"] / scene["f"]

@@ -146,6 +146,89 @@ def config_trihedral(f=20e3, c=C_WATER, frac=7.5, n_side=8, bounces=3, radius=15
         batched=True)
 
 
+def _orient_outward(tris, inside):
+    """Reorders each triangle's vertices so that unit((v1-v2) x (v1-v3)) points away from inside(centroid)."""
+    t = np.asarray(tris, dtype=np.float64).copy()
+    n = np.cross(t[:, 0] - t[:, 1], t[:, 0] - t[:, 2])
+    c = t.mean(axis=1)
+    flip = np.einsum("ij,ij->i", n, c - inside(c)) < 0
+    t[flip] = t[flip][:, [0, 2, 1]]
+    return t
+
+
+def capped_cylinder(length=6.0, radius=0.5, n_around=48, n_along=24, n_cap=8):
+    """Cylinder along x (body from -length/2 to +length/2) with hemispherical end caps; outward normals."""
+    half = length / 2.0
+    phi = np.linspace(0.0, 2.0 * np.pi, n_around + 1)
+    xs = np.linspace(-half, half, n_along + 1)
+    tris = []
+    for i in range(n_along):
+        for j in range(n_around):
+            a = (xs[i], radius * np.cos(phi[j]), radius * np.sin(phi[j]))
+            b = (xs[i + 1], radius * np.cos(phi[j]), radius * np.sin(phi[j]))
+            c = (xs[i + 1], radius * np.cos(phi[j + 1]), radius * np.sin(phi[j + 1]))
+            d = (xs[i], radius * np.cos(phi[j + 1]), radius * np.sin(phi[j + 1]))
+            tris += [(a, b, c), (a, c, d)]
+    th = np.linspace(0.0, np.pi / 2, n_cap + 1)  # 0 at the rim, pi/2 at the pole
+    for sgn in (-1.0, 1.0):
+        def p(k, j):
+            return (sgn * (half + radius * np.sin(th[k])), radius * np.cos(th[k]) * np.cos(phi[j]),
+                    radius * np.cos(th[k]) * np.sin(phi[j]))
+        for k in range(n_cap):
+            for j in range(n_around):
+                if k == n_cap - 1:
+                    tris.append((p(k, j), p(k, j + 1), p(k + 1, 0)))
+                else:
+                    tris += [(p(k, j), p(k, j + 1), p(k + 1, j + 1)), (p(k, j), p(k + 1, j + 1), p(k + 1, j))]
+    t = np.array(tris, dtype=np.float64)
+
+    def inside(c):  # nearest point of the axis segment
+        return np.stack([np.clip(c[:, 0], -half, half), np.zeros(len(c)), np.zeros(len(c))], 1)
+    return _orient_outward(t, inside).astype(np.float32)
+
+
+def box(lo, hi):
+    """Axis-aligned closed box, 12 triangles, outward normals."""
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    v = np.array([[lo[0] if i & 1 == 0 else hi[0], lo[1] if i & 2 == 0 else hi[1], lo[2] if i & 4 == 0 else hi[2]]
+                  for i in range(8)])
+    quads = [(0, 1, 3, 2), (4, 5, 7, 6), (0, 1, 5, 4), (2, 3, 7, 6), (0, 2, 6, 4), (1, 3, 7, 5)]
+    t = np.array([(v[a], v[b], v[c]) for a, b, c, d in quads] + [(v[a], v[c], v[d]) for a, b, c, d in quads])
+    centre = (lo + hi) / 2
+    return _orient_outward(t, lambda c: np.broadcast_to(centre, c.shape)).astype(np.float32)
+
+
+def hull_with_fin(length=6.0, radius=0.5, n_around=48, n_along=24, n_cap=8):
+    """Capped-cylinder hull with a 1 x 0.3 x 0.05 m box fin on top (its foot 5 cm inside the hull)."""
+    fin = box((-0.5, -0.025, radius - 0.05), (0.5, 0.025, radius + 0.25))
+    return np.concatenate([capped_cylinder(length, radius, n_around, n_along, n_cap), fin])
+
+
+def config_hull(f=8e3, c=C_WATER, frac=7.5, n_angles=720, radius=50.0, atten_db_per_m=0.5, bounces=2,
+                n_around=48, n_along=24, n_cap=8, field_cells=0):
+    """Capped-cylinder hull with a fin, monostatic angles around it in the horizontal plane.
+
+    Attenuation is given in dB/m and converted to the Np/m the pair kernel takes
+    (the reference feeds the dB/m figure straight into Im k, mh.py:72; here the
+    physical conversion is used). field_cells > 0 adds a field-surface plane of
+    that many cells a side (x-y plane, 2 m below the axis).
+    """
+    lam = c / f
+    a = np.arange(n_angles) * (2.0 * np.pi / n_angles)
+    src = (radius * np.stack([np.cos(a), np.sin(a), np.zeros_like(a)], 1)).astype(np.float32)
+    fields = []
+    if field_cells:
+        d = lam / frac
+        L = field_cells * d
+        plate = make_rectangle(L, L)
+        plate[:, :, 2] -= 2.0
+        fields.append((plate, d))
+    return dict(name=f"hull_6m_{f / 1e3:g}kHz", f=f, c=c, atten=atten_db_per_m / (20.0 / math.log(10.0)),
+                targets=[(hull_with_fin(6.0, 0.5, n_around, n_along, n_cap), lam / frac)], fields=fields,
+                sources=src, points=src.copy(), bounces=bounces, project_field_objects=bool(fields),
+                batched=True)
+
+
 def random_convex_polyhedron(rng, ntri_min=8, ntri_max=40, extent=(0.3, 1.5)):
     """Convex hull of random points: 8-40 outward-facing triangles, seeded."""
     from scipy.spatial import ConvexHull
