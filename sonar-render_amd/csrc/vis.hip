@@ -176,18 +176,22 @@ __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane
 
 // Returns this lane's "occluded" flag. active: lane has a ray to trace.
 // skipA, skipB: the triangles this lane's end points lie on (-1: none); never tested.
+// ldsTris: the triangle records staged in LDS when there are at most kBruteTris
+// of them (null: read them from global memory).
 __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk, bool active, F3 o, F3 d,
-                                   float tmin, float tmax, int *errFlag, int skipA, int skipB) {
+                                   float tmin, float tmax, int *errFlag, int skipA, int skipB,
+                                   const float4 *ldsTris = nullptr) {
     if (bvh.ntri <= 0) return false;
     bool hit = false;
     unsigned long long live = ballot64(active);
     if (live == 0) return false;
     if (bvh.ntri <= kBruteTris) {
         // a handful of occluders: test every triangle, no tree
+        const float4 *tr = ldsTris ? ldsTris : bvh.tris;
         for (int ti = 0; ti < bvh.ntri; ti++) {
-            const float4 q3 = bvh.tris[4 * ti + 3];
+            const float4 q3 = tr[4 * ti + 3];
             const int id = __float_as_int(q3.w);
-            TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], q3);
+            TriRec t = tri_unpack(tr[4 * ti], tr[4 * ti + 1], tr[4 * ti + 2], q3);
             if (active && !hit && id != skipA && id != skipB) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
             if (ballot64(active && !hit) == 0) break;
         }
@@ -247,9 +251,18 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
 }
 
 // One wave's share: receiver block rb, emitters [ec*emPerWave, (ec+1)*emPerWave).
+// The emitters' records are read 64 at a time, one per lane, and handed to
+// the whole wave by lane reads, so no iteration waits on memory for them.
+__device__ inline float4 lane_float4(float4 v, int k) {
+    return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), k)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), k)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), k)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), k)));
+}
+
 __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &tk, const Bvh &bvh,
                                                                const VisConst &vc, int rb, int ec, int emPerWave,
-                                                               int *stk) {
+                                                               int *stk, const float4 *ldsTris = nullptr) {
     const int lane = __lane_id();
     const int r = rb * 64 + lane;
     const bool rvalid = r < tk.rc.count;
@@ -259,10 +272,22 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
     int e0 = ec * emPerWave;
     const int e1 = min(tk.em.count, e0 + emPerWave);
     if (tk.upperOnly) e0 = max(e0, rb * 64);
+    float4 epRun = make_float4(0.f, 0.f, 0.f, 0.f), enRun = epRun;
     for (int e = e0; e < e1; e++) {
-        float4 ep = tk.em.p[e];
-        float4 en = tk.em.n ? tk.em.n[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int k = (e - e0) & 63;
+        if (k == 0) {  // the next 64 emitters, one per lane
+            const int el = min(e + lane, e1 - 1);
+            epRun = tk.em.p[el];
+            enRun = tk.em.n ? tk.em.n[el] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const float4 ep = lane_float4(epRun, k);
+        const float4 en = lane_float4(enRun, k);
         bool pair = rvalid;
+        // an emitter on every lane's own facet pairs with none of them: no ray set-up
+        if (tk.origin == ORIGIN_LOWER_FACET && ballot64(rvalid && __float_as_int(en.w) != rfacet) == 0ull) {
+            if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = 0ull;
+            continue;
+        }
         // Orientation: which end the ray leaves from and which normal culls where.
         bool fromEm = true;
         if (tk.origin == ORIGIN_RECEIVER) fromEm = false;
@@ -294,7 +319,7 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
         }
         const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
         const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
-        bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc);
+        bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc, ldsTris);
         unsigned long long word = ballot64(pair && !occ);
         if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
     }
@@ -304,12 +329,18 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, VisConst vc, int emPerWave) {
     __shared__ int stkAll[WAVES][kStack];
+    __shared__ float4 sTri[4 * kBruteTris];  // few occluders: their records, staged once per block
+    const bool brute = bvh.ntri > 0 && bvh.ntri <= kBruteTris;
+    if (brute)
+        for (int q = threadIdx.x; q < 4 * bvh.ntri; q += 64 * WAVES) sTri[q] = bvh.tris[q];
+    __syncthreads();
     const int wave = threadIdx.x >> 6;
     const int nrb = (tk.rc.count + 63) >> 6;
     const int nEmChunks = (tk.em.count + emPerWave - 1) / emPerWave;
     const long gw = (long)blockIdx.x * WAVES + wave;
     if (gw >= (long)nrb * nEmChunks) return;
-    vis_wave(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave]);
+    vis_wave(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave],
+             brute ? sTri : nullptr);
 }
 
 // Persistent form over many tasks (scenes): each wave pulls (task, receiver
