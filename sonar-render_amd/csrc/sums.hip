@@ -196,12 +196,14 @@ __global__ __launch_bounds__(256) void k_gather_multi(const GatherTask *__restri
 // Fragment->fragment, 16..64 incidences: FP32 pair kernel on the VALU, complex
 // multiply-accumulate on the FP32 matrix cores (v_mfma_f32_32x32x2_f32).
 //
-// As a real product, P_out[r][b] = sum_e [Kr Ki](r,e) x [[Xr Xi], [-Xi Xr]](e,b)
-// with X = A_e P_in[b][e]: K = 2 per emitter (re, im), 128 columns (Re of 64
-// planes, then Im of 64 planes). A wave owns 32 receivers and all 128 columns
-// (4 accumulator tiles of 32 x 32); its 64 lanes evaluate 2 emitters x 32
-// receivers at once, and one v_permlane32_swap turns the lanes' (Kr, Ki)
-// into the A operands of both emitters:
+// With X = A_e P_in[b][e], the complex product takes three real products
+// (Gauss): P1 = K_r X_r, P2 = K_i X_i, P3 = (K_r + K_i)(X_r + X_i), summed over
+// the emitters, then Re = P1 - P2 and Im = P3 - P1 - P2 - 6 matrix operations
+// per emitter pair instead of 8. A wave owns 32 receivers and 64 planes (6
+// accumulator tiles of 32 x 32: three products x two halves of the planes);
+// its 64 lanes evaluate 2 emitters x 32 receivers at once, and the K dimension
+// of each v_mfma_f32_32x32x2_f32 is those two emitters, so every lane's own
+// kernel value is directly its A operand:
 //   A operand: lane l holds A[row l % 32][k l / 32]; B: lane l holds B[k l / 32][col l % 32];
 //   C: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, col l % 32).
 // Emitter tiles (pressures of all planes, positions, normals, visibility
@@ -279,9 +281,11 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
     const int e0 = split * t.emPerSplit;
     const int e1 = min(t.ne, e0 + t.emPerSplit);
 
-    f32x16 acc[4];
+    // three real products per complex term (Gauss): P1 = sum Kr Xr, P2 = sum Ki Xi,
+    // P3 = sum (Kr + Ki)(Xr + Xi); Re = P1 - P2, Im = P3 - P1 - P2. Tiles: [product][planes 0-31, 32-63]
+    f32x16 acc[6];
 #pragma unroll
-    for (int c = 0; c < 4; c++) acc[c] = f32x16{};
+    for (int c = 0; c < 6; c++) acc[c] = f32x16{};
 
     // The two tile buffers are separate LDS objects and every phase names them
     // statically, so the compiler sees that reads of one never wait on copies
@@ -305,23 +309,17 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
             pair_kernel<KIND, ATT>(ph, rp, rn, ep, T.n[ke], bit, Kre, Kim);
             Kre *= ep.w;  // emitter area
             Kim *= ep.w;
-            const auto sw =
-                __builtin_amdgcn_permlane32_swap(__float_as_uint(Kre), __float_as_uint(Kim), false, false);
-            const float a0 = __uint_as_float(sw[0]), a1 = __uint_as_float(sw[1]);
-            // B of this lane: h = 0 (multiplies Kr): Xr(j), Xr(32+j), Xi(j), Xi(32+j);
-            //                 h = 1 (multiplies Ki): -Xi(j), -Xi(32+j), Xr(j), Xr(32+j)
-            const float2 x00 = *reinterpret_cast<const float2 *>(&T.x[j][2 * k]);
-            const float2 x01 = *reinterpret_cast<const float2 *>(&T.x[32 + j][2 * k]);
-            const float2 x10 = *reinterpret_cast<const float2 *>(&T.x[j][2 * k + 2]);
-            const float2 x11 = *reinterpret_cast<const float2 *>(&T.x[32 + j][2 * k + 2]);
-            acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? -x00.y : x00.x, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? -x01.y : x01.x, acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? x00.x : x00.y, acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, h ? x01.x : x01.y, acc[3], 0, 0, 0);
-            acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? -x10.y : x10.x, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? -x11.y : x11.x, acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? x10.x : x10.y, acc[2], 0, 0, 0);
-            acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, h ? x11.x : x11.y, acc[3], 0, 0, 0);
+            // K = the two emitters: lane (j, h) holds A[j][h] (its own emitter k + h) and
+            // B[h][col j] = X of emitter k + h, plane j (lo) and 32 + j (hi)
+            const float2 xl = *reinterpret_cast<const float2 *>(&T.x[j][2 * ke]);
+            const float2 xh = *reinterpret_cast<const float2 *>(&T.x[32 + j][2 * ke]);
+            const float ks = Kre + Kim;
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(Kre, xl.x, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(Kre, xh.x, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_32x32x2f32(Kim, xl.y, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_32x32x2f32(Kim, xh.y, acc[3], 0, 0, 0);
+            acc[4] = __builtin_amdgcn_mfma_f32_32x32x2f32(ks, xl.x + xl.y, acc[4], 0, 0, 0);
+            acc[5] = __builtin_amdgcn_mfma_f32_32x32x2f32(ks, xh.x + xh.y, acc[5], 0, 0, 0);
         }
     };
     if (e0 < e1) mfma_tile_dma(t, e0, rb0, nrb, wave, lane, TA);
@@ -340,7 +338,7 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
         base += kMT;
     }
     if (!waveActive) return;
-    // column j of tiles 0/2 is plane j, of tiles 1/3 plane 32 + j
+    // column j of tiles 0/2/4 is plane j, of tiles 1/3/5 plane 32 + j
 #pragma unroll
     for (int c = 0; c < 2; c++) {
         const int b = c * 32 + j;
@@ -349,7 +347,8 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
         for (int i = 0; i < 16; i++) {
             const int r = rt * 32 + (i >> 2) * 8 + h * 4 + (i & 3);
             if (r >= t.nr) continue;
-            t.partL[((size_t)split * 64 + b) * t.nr + r] = make_float2(acc[c][i], acc[c + 2][i]);
+            const float p1 = acc[c][i], p2 = acc[c + 2][i], p3 = acc[c + 4][i];
+            t.partL[((size_t)split * 64 + b) * t.nr + r] = make_float2(p1 - p2, p3 - p1 - p2);
         }
     }
 }
