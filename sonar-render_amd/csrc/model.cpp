@@ -158,23 +158,51 @@ int ensure_scratch(Model &m, size_t elems) {
 
 Bvh bvh_of(const Model &m) { return Bvh{m.trisSorted, m.nodes, m.ntri}; }
 
-// Brackets one launch with HIP events on the model's stream when profiling.
+// Brackets one launch with HIP events on the model's stream (or st) when profiling.
 struct ProfScope {
     Model &m;
     int kind;
     double pairs, flops, bytes;
+    hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(Model &mm, int k, double p, double f, double by = 0.0) : m(mm), kind(k), pairs(p), flops(f), bytes(by) {
+    ProfScope(Model &mm, int k, double p, double f, double by = 0.0, hipStream_t st = nullptr)
+        : m(mm), kind(k), pairs(p), flops(f), bytes(by), s(st ? st : mm.stream) {
         if (!m.profiling) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
-        (void)hipEventRecord(a, m.stream);
+        (void)hipEventRecord(a, s);
     }
     ~ProfScope() {
         if (!a) return;
-        (void)hipEventRecord(b, m.stream);
+        (void)hipEventRecord(b, s);
         m.recs.push_back(Model::Rec{kind, a, b, pairs, flops, bytes});
     }
 };
+
+int ensure_scratch_c(Model &m, size_t elems) {
+    if (elems <= m.scratchCElems) return 0;
+    dev_free(m.scratchC);
+    m.scratchCElems = 0;
+    if (dev_alloc(&m.scratchC, elems)) return -1;
+    m.scratchCElems = elems;
+    return 0;
+}
+
+// Marks the main stream's current point; side work queued after side_follow
+// starts there, concurrently with what the main stream runs next.
+int side_mark(Model &m) {
+    SR_HIP(hipEventRecord(m.sideIn, m.stream));
+    return 0;
+}
+int side_follow(Model &m) {
+    SR_HIP(hipStreamWaitEvent(m.side, m.sideIn, 0));
+    return 0;
+}
+// The main stream continues after everything queued on the side stream so far.
+int side_join(Model &m) {
+    SR_HIP(hipEventRecord(m.sideOut, m.side));
+    SR_HIP(hipStreamWaitEvent(m.stream, m.sideOut, 0));
+    return 0;
+}
 
 // A device count copied into pinned host word `slot`; the host waits for that
 // copy alone (word_wait), not for the work queued after it.
@@ -247,6 +275,11 @@ int model_set_batched_sources(Model &m, int on) {
 int model_render(Model &m) {
     if (m.rendered) { set_error("render_cuda called twice without TearDownCuda"); return -3; }
     if (!m.stream) SR_HIP(hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking));
+    if (!m.side) {
+        SR_HIP(hipStreamCreateWithFlags(&m.side, hipStreamNonBlocking));
+        SR_HIP(hipEventCreateWithFlags(&m.sideIn, hipEventDisableTiming));
+        SR_HIP(hipEventCreateWithFlags(&m.sideOut, hipEventDisableTiming));
+    }
     if (!m.hostWords) {  // kept, like the stream, for the life of the process
         SR_HIP(hipHostMalloc((void **)&m.hostWords, Model::kWords * sizeof(int), hipHostMallocDefault));
         for (auto &e : m.wordEv) SR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -495,6 +528,10 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         if (transfer_gated_rows(t.gval, t.nr, nplanes, m.gatedList, m.gatedCount, m.stream)) return -1;
         if (word_readback(m, m.gatedCount, 0)) return -1;  // read after the main passes are queued
     }
+    // one plane group: the gated rows' compact set runs on the side stream, concurrently
+    // with the main passes (it reads only the inputs, positions and visibility words)
+    const bool overlap = gated && nplanes <= 64;
+    if (overlap && side_mark(m)) return -1;
     // ~2048 workgroups of 128 receivers; emitter ranges in whole LDS tiles of 32
     const int groups = (t.nr + 127) / 128;
     int nsplit = std::max(1, std::min((2048 + groups - 1) / groups, t.ne / 64));
@@ -552,9 +589,11 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         if (dev_alloc(&m.gMc, words)) return -1;
         m.gMcCap = words;
     }
+    const hipStream_t cst = overlap ? m.side : m.stream;
+    if (overlap && (ensure_scratch_c(m, ((size_t)512 * 128 + t.nr + 128) * 64) || side_follow(m))) return -1;
     {
-        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (gated_compact_launch(m.gatedList, ngated, t.rp, t.rn, t.mask, t.ne, m.gPc, m.gNc, m.gFs, m.gMc, m.stream))
+        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cst);
+        if (gated_compact_launch(m.gatedList, ngated, t.rp, t.rn, t.mask, t.ne, m.gPc, m.gNc, m.gFs, m.gMc, cst))
             return -1;
     }
     GatherTask c = t;
@@ -566,15 +605,22 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
     int cs = std::max(1, std::min((512 + cgroups - 1) / cgroups, t.ne / 64));
     c.emPerSplit = ((t.ne + cs - 1) / cs + 31) / 32 * 32;
     c.nsplit = (t.ne + c.emPerSplit - 1) / c.emPerSplit;
-    if ((size_t)c.nsplit * 64 * ngated > m.scratchElems) { set_error("gated MFMA gather: scratch bound"); return -2; }
-    c.partL = m.scratchL;
+    if ((size_t)c.nsplit * 64 * ngated > (overlap ? m.scratchCElems : m.scratchElems)) {
+        set_error("gated MFMA gather: scratch bound");
+        return -2;
+    }
+    c.partL = overlap ? m.scratchC : m.scratchL;
     for (int b0 = 0; b0 < nplanes; b0 += 64) {
         const int nb = std::min(64, nplanes - b0);
         c.nrhs = nb;
         c.eval = t.eval + b0 * evalStride;
+        {
+            ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cst);
+            if (f2f_mfma_launch(c, ph, KIND_F2F, cst, true)) return -1;
+        }
+        if (overlap && side_join(m)) return -1;
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (f2f_mfma_launch(c, ph, KIND_F2F, m.stream, true)) return -1;
-        if (gated_replace_launch(m.scratchL, c.nsplit, ngated, m.gatedList, t.gval + b0 * gvalStride, gvalStride, nb,
+        if (gated_replace_launch(c.partL, c.nsplit, ngated, m.gatedList, t.gval + b0 * gvalStride, gvalStride, nb,
                                  rt.out + b0 * outStride, rt.pcell + b0 * pcellStride, pcellStride, rt.rowCell,
                                  nullptr, m.stream))
             return -1;
@@ -815,7 +861,7 @@ static int bounce_mixed(Model &m, int i, bool first) {
     int gated = -1;  // not yet known
     int nRtC = 0, nsplitC = 1, ksPerC = t.nKs;
     // the gated rows as compact row tiles of S (lower-facet emitters only)
-    auto compact_set = [&]() -> int {
+    auto compact_set = [&](hipStream_t cs) -> int {
         nRtC = ((gated + 127) / 128) * 4;
         const size_t need = (size_t)nRtC * t.nKs * 64;
         if (t.acCap < need || t.skipZeroCap < nRtC / 4) {
@@ -825,16 +871,24 @@ static int bounce_mixed(Model &m, int i, bool first) {
             SR_HIP(hipMemset(t.skipZero, 0, (nRtC / 4) * sizeof(int2)));
             t.acCap = need; t.skipZeroCap = nRtC / 4;
         }
-        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (transfer_gather_rows(t.A, t.nKs, t.gatedList, t.rowFs, gated, nRtC, t.Ac, m.stream)) return -1;
+        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cs);
+        if (transfer_gather_rows(t.A, t.nKs, t.gatedList, t.rowFs, gated, nRtC, t.Ac, cs)) return -1;
         // few rows: split K finer (down to single chunks, at most 48 splits: the
         // replacement sums every split's partial) so the grid still fills the chip
         nsplitC = std::max(1, std::min({1024 / (nRtC / 4), t.nKs / 8, 48}));
         ksPerC = ((t.nKs + nsplitC - 1) / nsplitC + 7) / 8 * 8;
         nsplitC = (t.nKs + ksPerC - 1) / ksPerC;
-        if ((size_t)nsplitC * 64 * gated > m.scratchElems) { set_error("transfer bounce: scratch bound"); return -2; }
+        if ((size_t)nsplitC * 64 * gated > (cs == m.side ? m.scratchCElems : m.scratchElems)) {
+            set_error("transfer bounce: scratch bound");
+            return -2;
+        }
         return 0;
     };
+    // One plane group: the compact set runs on the side stream, concurrently with the
+    // main product (it needs only B and the gated list); more groups share B, so
+    // their compact sets run in line on the main stream.
+    const bool overlap = m.nplanes <= 64;
+    if (overlap && ensure_scratch_c(m, ((size_t)1024 * 128 + n + 128) * 64)) return -1;
     for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
         const int np = std::min(64, m.nplanes - g0);
         const float2 *pin = o.Pin + (size_t)g0 * n;
@@ -844,6 +898,7 @@ static int bounce_mixed(Model &m, int i, bool first) {
                                  m.stream))
                 return -1;
         }
+        if (overlap && side_mark(m)) return -1;
         TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL};
         m.lastGemm = gm;
         m.haveLastGemm = true;
@@ -876,13 +931,22 @@ static int bounce_mixed(Model &m, int i, bool first) {
         }
         if (gated < 0) {  // the count has long landed by now; the product above keeps the GPU busy
             if (word_wait(m, 0, gated)) return -1;
-            if (gated && compact_set()) return -1;
+            if (gated) {
+                if (overlap && side_follow(m)) return -1;
+                if (compact_set(overlap ? m.side : m.stream)) return -1;
+            }
         }
         if (!gated) continue;
-        TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, m.scratchL, 1};
+        hipStream_t cs = overlap ? m.side : m.stream;
+        float2 *cpart = overlap ? m.scratchC : m.scratchL;
+        TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, cpart, 1};
+        {
+            ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cs);
+            if (transfer_gemm(gc, cs)) return -1;
+        }
+        if (overlap && side_join(m)) return -1;
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (transfer_gemm(gc, m.stream) ||
-            gated_replace_launch(m.scratchL, nsplitC, gated, t.gatedList, pin, n, np, o.Pout + (size_t)g0 * n,
+        if (gated_replace_launch(cpart, nsplitC, gated, t.gatedList, pin, n, np, o.Pout + (size_t)g0 * n,
                                  o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv, m.stream))
             return -1;
     }
@@ -1103,12 +1167,14 @@ void model_profile(Model &m, int on) {
 
 void model_profile_reset(Model &m) {
     if (m.stream) (void)hipStreamSynchronize(m.stream);
+    if (m.side) (void)hipStreamSynchronize(m.side);
     for (auto &r : m.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     m.recs.clear();
 }
 
 int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes) {
     if (m.stream) SR_HIP(hipStreamSynchronize(m.stream));
+    if (m.side) SR_HIP(hipStreamSynchronize(m.side));
     double t = 0, p = 0, f = 0, b = 0;
     long n = 0;
     for (auto &r : m.recs) {
@@ -1147,6 +1213,7 @@ int model_time_transfer_gemm(Model &m, int reps, double *msPerLaunch) {
 void model_teardown(Model &m) {
     model_profile_reset(m);
     if (m.stream) (void)hipStreamSynchronize(m.stream);
+    if (m.side) (void)hipStreamSynchronize(m.side);
     for (auto &o : m.targets) free_object(o);
     for (auto &o : m.fields) free_object(o);
     m.targets.clear();
@@ -1166,8 +1233,9 @@ void model_teardown(Model &m) {
     m.gMcCap = 0;
     dev_free(m.srcP); dev_free(m.srcAmp); dev_free(m.fldP); dev_free(m.fldVal);
     dev_free(m.trisSorted); dev_free(m.nodes); dev_free(m.errFlag);
-    dev_free(m.scratchL); dev_free(m.scratchU);
+    dev_free(m.scratchL); dev_free(m.scratchU); dev_free(m.scratchC);
     m.scratchElems = 0;
+    m.scratchCElems = 0;
     m.srcHost.clear();
     m.fldHost.clear();
     m.ntri = 0;

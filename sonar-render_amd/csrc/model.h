@@ -86,6 +86,12 @@ struct Model {
 
     // device
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for the gated rows' compact set, which
+    // overlaps the bounce's main product; two events order it against the main stream
+    hipStream_t side = nullptr;
+    hipEvent_t sideIn = nullptr, sideOut = nullptr;
+    float2 *scratchC = nullptr;  // the compact set's partial sums
+    size_t scratchCElems = 0;
     // counts read back while the stream runs on: pinned words, one event each
     static constexpr int kWords = 4;
     int *hostWords = nullptr;

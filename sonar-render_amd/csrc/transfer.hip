@@ -581,7 +581,9 @@ __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int 
 // ---------------------------------------------------------------------------
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st) {
-    const int ksPerWave = 16;
+    // runs of 16 K steps per wave, shorter when that leaves fewer than ~16k waves
+    int ksPerWave = 16;
+    while (ksPerWave > 4 && (long)g.nRt * ((g.nKs + ksPerWave - 1) / ksPerWave) < 16384) ksPerWave /= 2;
     const long waves = (long)g.nRt * ((g.nKs + ksPerWave - 1) / ksPerWave);
     const long blocks = (waves + 3) / 4;
     if (blocks <= 0) return 0;
