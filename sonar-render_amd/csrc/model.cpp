@@ -136,6 +136,7 @@ void free_transfer(Object &o) {
     auto &t = o.tc;
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
     dev_free(t.gatedList); dev_free(t.state); dev_free(t.rowFs); dev_free(t.Ac); dev_free(t.skipZero);
+    dev_free(t.rowAct); dev_free(t.actRow); dev_free(t.kB); dev_free(t.cls);
     dev_free(t.nearQ); dev_free(t.nearM); dev_free(t.nearCount);
     t = Object::Transfer{};
 }
@@ -728,8 +729,24 @@ static int ensure_transfer(Model &m, int i) {
     if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
     const int n = o.nact;
     const int nKs = ((n + 7) / 8 + 7) / 8 * 8;  // K steps of 8 emitters, whole chunks of 8 steps
-    const int nRt = ((n + 127) / 128) * 4;
-    if (!t.A || t.nKs != nKs || t.nRt != nRt) {
+    // Facet ranges of the active rows (sorted by facet). When every facet holds at
+    // least 128 rows and padding each to whole 128-row groups adds at most a
+    // quarter, the rows of S are padded so: every group is then single-facet and
+    // the zero-input gate is applied inside the product.
+    const auto &fa = o.actFacetHost;
+    std::vector<int2> fr;  // [fs, fe) per facet present
+    for (int r = 0; r < n; r++)
+        if (r == 0 || fa[r] != fa[r - 1]) fr.push_back(make_int2(r, r + 1)); else fr.back().y = r + 1;
+    long padded = 0;
+    bool big = !fr.empty();
+    for (auto &f : fr) {
+        big = big && (f.y - f.x) >= 128;
+        padded += ((f.y - f.x) + 127) / 128 * 128;
+    }
+    const bool gateIn = big && padded <= n + n / 4;
+    const int nPad = gateIn ? (int)padded : n;
+    const int nRt = ((nPad + 127) / 128) * 4;
+    if (!t.A || t.nKs != nKs || t.nRt != nRt || t.gateInGemm != gateIn) {
         free_transfer(o);
         const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
@@ -739,31 +756,74 @@ static int ensure_transfer(Model &m, int i) {
             return -1;
         t.nKs = nKs;
         t.nRt = nRt;
+        t.nPad = nPad;
+        t.gateInGemm = gateIn;
         {
-            const auto &fa = o.actFacetHost;
             std::vector<int> fs(n);
             for (int r = 0; r < n; r++)
-                fs[r] = (r > 0 && fa[r] == fa[r - 1]) ? fs[r - 1] : r;  // rows are sorted by facet
+                fs[r] = (r > 0 && fa[r] == fa[r - 1]) ? fs[r - 1] : r;
             SR_HIP(hipMemcpy(t.rowFs, fs.data(), n * sizeof(int), hipMemcpyHostToDevice));
         }
-        // A group of 128 rows on one facet f never reads the K chunks (64 emitters)
-        // that lie wholly inside f's emitter range: same-facet pairs are zero.
-        const auto &fa = o.actFacetHost;
+        // Per 128-row group: its facet range (single-facet groups only), the K chunks
+        // wholly inside it (same-facet pairs are zero: never read) and, with the gate
+        // inside the product, the first K step of the higher facets.
+        std::vector<int> rowAct(gateIn ? (size_t)nRt * 32 : 0, -1), actRow(gateIn ? n : 0);
+        std::vector<int2> grp(nWg, make_int2(-1, -1));
+        if (gateIn) {
+            int base = 0;
+            for (auto &f : fr) {
+                for (int r = f.x; r < f.y; r++) {
+                    rowAct[base + (r - f.x)] = r;
+                    actRow[r] = base + (r - f.x);
+                }
+                for (int w = base / 128; w < (base + (f.y - f.x) + 127) / 128; w++) grp[w] = f;
+                base += ((f.y - f.x) + 127) / 128 * 128;
+            }
+        } else {
+            for (int w = 0; w < nWg; w++) {
+                const int r0 = std::min(n - 1, w * 128), r1 = std::min(n - 1, w * 128 + 127);
+                if (fa[r0] == fa[r1])
+                    for (auto &f : fr)
+                        if (r0 >= f.x && r0 < f.y) grp[w] = f;
+            }
+        }
         std::vector<int2> skip(nWg, make_int2(0, 0));
+        std::vector<int> kB(nWg, nKs);
         double steps = 0;
         for (int w = 0; w < nWg; w++) {
-            const int r0 = std::min(n - 1, w * 128), r1 = std::min(n - 1, w * 128 + 127);
-            if (fa[r0] == fa[r1]) {
-                const int f = fa[r0];
-                const int fs = (int)(std::lower_bound(fa.begin(), fa.end(), f) - fa.begin());
-                const int fe = (int)(std::upper_bound(fa.begin(), fa.end(), f) - fa.begin());
+            if (grp[w].x >= 0) {
+                const int fs = grp[w].x, fe = grp[w].y;
                 const int cLo = ((fs + 7) / 8 + 7) / 8, cHi = (fe / 8) / 8;  // chunks wholly inside [fs, fe)
                 if (cHi > cLo) skip[w] = make_int2(cLo * 8, cHi * 8);
+                kB[w] = cHi * 8;  // this chunk holds facet f's tail (zero) and higher facets
             }
             steps += 4.0 * (nKs - (skip[w].y - skip[w].x));
         }
         SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
         t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
+        // split K (whole chunks of 8 steps) so that the grid holds ~1024 workgroups
+        int nsplit = std::max(1, std::min(1024 / std::max(1, nWg), nKs / 64));
+        t.ksPer = ((nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
+        t.nsplit = (nKs + t.ksPer - 1) / t.ksPer;
+        if (gateIn) {
+            // which partials each (group, split) writes, by the product's own walk over K
+            std::vector<unsigned char> cls((size_t)nWg * t.nsplit, 0);
+            for (int w = 0; w < nWg; w++)
+                for (int s = 0; s < t.nsplit; s++) {
+                    const int ks0 = s * t.ksPer, ks1 = std::min(nKs, ks0 + t.ksPer);
+                    auto next = [&](int kb) { return (kb >= skip[w].x && kb < skip[w].y) ? skip[w].y : kb; };
+                    unsigned char c = 0;
+                    for (int kb = next(ks0); kb < ks1; kb = next(kb + 8)) c |= kb < kB[w] ? 1 : 2;
+                    cls[(size_t)w * t.nsplit + s] = c;
+                }
+            if (dev_alloc(&t.rowAct, rowAct.size()) || dev_alloc(&t.actRow, n) || dev_alloc(&t.kB, nWg) ||
+                dev_alloc(&t.cls, cls.size()))
+                return -1;
+            SR_HIP(hipMemcpy(t.rowAct, rowAct.data(), rowAct.size() * sizeof(int), hipMemcpyHostToDevice));
+            SR_HIP(hipMemcpy(t.actRow, actRow.data(), n * sizeof(int), hipMemcpyHostToDevice));
+            SR_HIP(hipMemcpy(t.kB, kB.data(), nWg * sizeof(int), hipMemcpyHostToDevice));
+            SR_HIP(hipMemcpy(t.cls, cls.data(), cls.size(), hipMemcpyHostToDevice));
+        }
     }
     // One FP16 scale for S: |cos_s cos_i| <= 1, |G| <= |k|/(4 pi delta) e^{max(0,-alpha) D} + 1/delta^2
     // for r >= delta and |k|/(4 pi) below it.
@@ -773,7 +833,7 @@ static int ensure_transfer(Model &m, int i) {
     const double d = o.delta;
     const double smax = 0.9 * std::max(kabs / (4 * M_PI), kabs * grow / (4 * M_PI * d) + 1.0 / (d * d));
     t.sS = (float)(16384.0 / smax);
-    TransferGeom g{o.actP, o.actN, n, nKs, nRt, t.wgSkip};
+    TransferGeom g{o.actP, o.actN, n, nKs, nRt, t.wgSkip, t.rowAct, t.actRow};
     TransferPhys ph{m.kf, m.atten, (float)((double)m.atten * 1.4426950408889634), o.delta, 0.9f, t.sS,
                     (double)m.kf / (2.0 * M_PI), m.nearMode,
                     (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr,
@@ -834,11 +894,7 @@ static int bounce_mixed(Model &m, int i, bool first) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     const int n = o.nact;
-    const int wgRows = t.nRt / 4;
-    // split K (in whole chunks of 8 steps) so that the grid holds ~1024 workgroups
-    int nsplit = std::max(1, std::min(1024 / std::max(1, wgRows), t.nKs / 64));
-    const int ksPer = ((t.nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
-    nsplit = (t.nKs + ksPer - 1) / ksPer;
+    const int nsplit = t.nsplit, ksPer = t.ksPer;  // split K, chosen with the transfer matrix
     // bounce state: this bounce's set (zero), and the other one, cleared here for the next
     const int stWords = 1 + m.nplanes;
     if (first) {
@@ -848,6 +904,41 @@ static int bounce_mixed(Model &m, int i, bool first) {
     int *cur = t.state + (size_t)t.stateSet * stWords, *nxt = t.state + (size_t)(t.stateSet ^ 1) * stWords;
     t.stateSet ^= 1;
     unsigned *colMax = reinterpret_cast<unsigned *>(cur + 1);
+    if (t.gateInGemm) {
+        // Every 128-row group is single-facet: each split's partial sums are split at
+        // the group's higher-facet boundary and the reduction applies the gate.
+        {
+            ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
+            if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
+        }
+        if (ensure_scratch(m, (size_t)nsplit * 64 * t.nPad)) return -1;
+        for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
+            const int np = std::min(64, m.nplanes - g0);
+            const float2 *pin = o.Pin + (size_t)g0 * n;
+            {
+                ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
+                if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + g0, t.sS, t.colScale, t.colInv, t.B,
+                                     m.stream))
+                    return -1;
+            }
+            TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL, 0, t.kB,
+                            m.scratchU};
+            m.lastGemm = gm;
+            m.haveLastGemm = true;
+            {
+                const double pairs = ((double)n * n - o.sameFacetPairs) * np;
+                const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
+                const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + (double)nsplit * np * n * 8;
+                ProfScope ps(m, PK_GEMM, pairs, fl, by);
+                if (transfer_gemm(gm, m.stream)) return -1;
+            }
+            ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.actRow, pin,
+                             o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv};
+            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
+            if (transfer_reduce_gate(r, m.stream)) return -1;
+        }
+        return 0;
+    }
     // rows with a zero input in some plane (the gate), counted before the product
     // and read back while the first plane group's product runs
     if (transfer_gated_rows(o.Pin, n, m.nplanes, t.gatedList, cur, m.stream)) return -1;
@@ -1191,12 +1282,17 @@ int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *p
 int model_time_transfer_gemm(Model &m, int reps, double *msPerLaunch) {
     if (!m.haveLastGemm || !m.rendered) { set_error("no transfer-form bounce has run since render_cuda"); return -3; }
     if (reps < 1 || reps > 10000) { set_error("reps: 1 to 10000"); return -2; }
+    // the partial sums go to the current scratch (later projections may have regrown it)
+    TransferGemm g = m.lastGemm;
+    if ((size_t)g.nsplit * 64 * g.n > m.scratchElems) { set_error("scratch smaller than the product's partials"); return -3; }
+    g.partL = m.scratchL;
+    if (g.partU) g.partU = m.scratchU;
     hipEvent_t a, b;
     SR_HIP(hipEventCreate(&a));
     SR_HIP(hipEventCreate(&b));
-    int rc = transfer_gemm(m.lastGemm, m.stream);  // warm
+    int rc = transfer_gemm(g, m.stream);  // warm
     if (!rc) SR_HIP(hipEventRecord(a, m.stream));
-    for (int r = 0; !rc && r < reps; r++) rc = transfer_gemm(m.lastGemm, m.stream);
+    for (int r = 0; !rc && r < reps; r++) rc = transfer_gemm(g, m.stream);
     float ms = 0.f;
     if (!rc) {
         SR_HIP(hipEventRecord(b, m.stream));

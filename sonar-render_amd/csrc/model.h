@@ -50,6 +50,15 @@ struct Object {
         int *state = nullptr;
         int stateSet = 0;
         int *rowFs = nullptr;      // per active row: first active index on its facet
+        // Zero-input gate inside the product (meshes whose facets each hold >= 128
+        // active cells): rows of S padded so that every facet starts a 128-row
+        // group; per group the first K step of the higher facets (kB); per
+        // (group, split) which partial sums the split writes (cls: bit 0 lower, 1 higher)
+        bool gateInGemm = false;
+        int nPad = 0;              // rows of S (padded)
+        int *rowAct = nullptr, *actRow = nullptr, *kB = nullptr;
+        unsigned char *cls = nullptr;
+        int nsplit = 1, ksPer = 8; // split-K of the bounce product
         uint4 *Ac = nullptr;       // ... as compact row tiles of A (lower-facet emitters only)
         int2 *skipZero = nullptr;  // empty skip ranges for those tiles
         size_t acCap = 0;

@@ -12,6 +12,11 @@ struct TransferGeom {
     int nKs;            // K steps of 8 emitters (ceil(n/8), padded to a multiple of 8)
     int nRt;            // row tiles of 32 (padded to a multiple of 4)
     const int2 *wgSkip; // per 4-row-tile group: K steps [x, y) that hold only zeros (not written)
+    // Rows of S may be padded so that every facet starts a 128-row group: rowAct
+    // maps a row of S to its active cell (-1: padding), actRow the other way.
+    // Both null: row = active cell.
+    const int *rowAct = nullptr;
+    const int *actRow = nullptr;
 };
 
 struct TransferPhys {
@@ -38,6 +43,28 @@ struct TransferGemm {
     const int2 *wgSkip;                   // per 4-row-tile group: K steps [x, y) that hold only zeros
     float2 *partL;                        // [split][64][n]
     int compactSet = 0;                   // 1: the gated rows' compact set (same kernel, its own name in profiles)
+    // Zero-input gate inside the product: per 128-row group, the first K step of
+    // the higher-facet emitters (whole chunks). A workgroup writes what it sums
+    // below that step to partL and from it on to partU (null: partL only).
+    const int *kB = nullptr;
+    float2 *partU = nullptr;
+};
+
+// Reduction of a gated product: per (plane, active row), the lower-facet
+// partials always and the higher-facet ones where the row's own input is not
+// zero (src/OptiX.cu:241-245); cls[group][split] bit 0/1 says which of the
+// split's partials exist. Applies the column scale, writes P_out and adds to P.
+struct ReduceGateTask {
+    const float2 *partL, *partU;
+    const unsigned char *cls;
+    int nsplit, nPadRows, nrhs, n;
+    const int *actRow;          // active row -> row of S (null: identity)
+    const float2 *gval;         // P_in [plane][n]
+    float2 *out;                // P_out [plane][n]
+    float2 *pcell;              // P [plane][pcellStride] through rowCell
+    long pcellStride;
+    const int *rowCell;
+    const float *colInvScale;
 };
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
@@ -52,10 +79,11 @@ int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
                      const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st);
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
+int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st);
 // Gated rows as compact row tiles of the transfer matrix (nRtC tiles of 32,
 // a multiple of 4): emitters at or above row r's facet bound rowFs[r] zeroed.
 int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, int count, int nRtC, uint4 *Ac,
-                         hipStream_t st);
+                         hipStream_t st, const int *actRow = nullptr);
 // Lists the rows with an exactly zero input pressure in some plane (count: device int, zeroed by the caller).
 int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st);
 

@@ -139,7 +139,8 @@ __global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys 
         v.y += __shfl_xor(v.y, m);
     }
     const float sc = ph.scale * (1.0f / 256.0f);
-    if (valid && sy == 0) store_pair(A, g.nKs, pr.x, pr.y, pack_h2(v.x * sc, v.y * sc));
+    if (valid && sy == 0)
+        store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(v.x * sc, v.y * sc));
 }
 
 // Near pairs by the learned correction: value = G2 + (M / g) t e^{ikr} with G2 the
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
         if (h == 0 && valid) {
             // G2 + (M / g) t e^{ikr}
             const float re = K.x + M * (o * cs_kr - o1 * sn_kr), im = K.y + M * (o * sn_kr + o1 * cs_kr);
-            store_pair(A, g.nKs, pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
+            store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
         }
     }
 }
@@ -302,9 +303,10 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
     const int rt = wave / nKsChunks;
     const int kc = wave % nKsChunks;
     if (rt >= g.nRt) return;
-    const int row = rt * 32 + (lane & 31);
+    const int srow = rt * 32 + (lane & 31);  // row of S
     const int h = lane >> 5;
-    const bool rvalid = row < g.n;
+    const int row = g.rowAct ? g.rowAct[srow] : srow;  // its active cell (-1: padding)
+    const bool rvalid = row >= 0 && row < g.n;
     const float4 rp = rvalid ? g.p[row] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 rn = rvalid ? g.nrm[row] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint64_t *mrow = mask + (size_t)(rvalid ? (row >> 6) : 0) * g.n;
@@ -468,10 +470,28 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     const int2 skip = t.wgSkip[blockIdx.x];  // chunk-aligned K steps [x, y) that hold only zeros
     const uint4 *Arow = t.A + (size_t)(rtValid ? rt : 0) * t.nKs * 64 + lane;
     const uint4 *Bt = t.B + threadIdx.x;     // thread x stages element x of every K step
+    // in-product gate: K steps from kBnd on are the group's higher-facet emitters
+    const int kBnd = t.kB ? t.kB[blockIdx.x] : 0x7fffffff;
+    int kLast = -1;  // the last chunk summed
 
     f32x16 acc[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[c] = f32x16{};
+    // C layout: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, column l%32).
+    auto store_part = [&](float2 *part) {
+        if (!rtValid) return;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int b = c * 32 + (lane & 31);
+            if (b >= t.nplanes) continue;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int row = rt * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+                if (row >= t.n) continue;
+                part[((size_t)split * 64 + b) * t.n + row] = make_float2(acc[c][i], acc[c + 2][i]);
+            }
+        }
+    };
 
     auto next_chunk = [&](int kb) { return (kb >= skip.x && kb < skip.y) ? skip.y : kb; };
     // B goes global -> LDS directly (global_load_lds_dwordx4: lane l of wave w
@@ -507,6 +527,11 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
                 acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[c], 0, 0, 0);           \
             }                                                                                        \
         }                                                                                            \
+        kLast = kb;                                                                                  \
+        if (kb < kBnd && kn >= kBnd && kn < ks1) { /* lower-facet part done, higher follows */       \
+            store_part(t.partL);                                                                     \
+            _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                         \
+        }                                                                                            \
         kb = kn;                                                                                     \
     }
     while (kb < ks1) {
@@ -515,19 +540,32 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
         SR_GEMM_PHASE(a1, a0, 1)
     }
 #undef SR_GEMM_PHASE
-    if (!rtValid) return;
-    // C layout: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, column l%32).
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-        const int b = c * 32 + (lane & 31);
-        if (b >= t.nplanes) continue;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int row = rt * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-            if (row >= t.n) continue;
-            t.partL[((size_t)split * 64 + b) * t.n + row] = make_float2(acc[c][i], acc[c + 2][i]);
-        }
+    if (!t.kB) store_part(t.partL);  // every split's partial, zero when it summed nothing
+    else if (kLast >= 0) store_part(kLast < kBnd ? t.partL : t.partU);
+}
+
+// Gated reduction: thread = (plane, active row).
+__global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)t.nrhs * t.n) return;
+    const int b = (int)(i / t.n), r = (int)(i % t.n);
+    const int sr = t.actRow ? t.actRow[r] : r;
+    const unsigned char *cls = t.cls + (size_t)(sr >> 7) * t.nsplit;
+    float2 L = make_float2(0.f, 0.f), U = make_float2(0.f, 0.f);
+    for (int s = 0; s < t.nsplit; s++) {
+        const size_t o = ((size_t)s * 64 + b) * t.nPadRows + sr;
+        const unsigned c = cls[s];
+        if (c & 1u) { const float2 a = t.partL[o]; L.x += a.x; L.y += a.y; }
+        if (c & 2u) { const float2 a = t.partU[o]; U.x += a.x; U.y += a.y; }
     }
+    const float2 gv = t.gval[(size_t)b * t.n + r];
+    const float g = (gv.x == 0.0f && gv.y == 0.0f) ? 0.0f : 1.0f;
+    const float sc = t.colInvScale[b];
+    const float2 v = make_float2((L.x + g * U.x) * sc, (L.y + g * U.y) * sc);
+    t.out[(size_t)b * t.n + r] = v;
+    float2 &P = t.pcell[(size_t)b * t.pcellStride + t.rowCell[r]];
+    P.x += v.x;
+    P.y += v.y;
 }
 
 // Rows whose own input pressure is exactly zero in some plane (the zero-input
@@ -557,7 +595,8 @@ __global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes,
 // goes through k_transfer_gemm with the bounce's B: the gated rows see exactly
 // the operator the other rows see, near-field terms included.
 __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int *__restrict__ list,
-                                const int *__restrict__ rowFs, int count, int nRtC, uint4 *__restrict__ Ac) {
+                                const int *__restrict__ rowFs, int count, int nRtC, uint4 *__restrict__ Ac,
+                                const int *__restrict__ actRow) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nRtC * nKs * 64) return;
     const int lane = (int)(t & 63);
@@ -569,7 +608,8 @@ __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int 
         const int r = list[i], fs = rowFs[r];
         const int j0 = ks * 8 + h * 4;  // first emitter of this fragment
         if (j0 < fs) {
-            q = A[((size_t)(r >> 5) * nKs + ks) * 64 + (r & 31) + 32 * h];
+            const int sr = actRow ? actRow[r] : r;  // the row of S
+            q = A[((size_t)(sr >> 5) * nKs + ks) * 64 + (sr & 31) + 32 * h];
             if (j0 + 1 >= fs) q.y = 0u;
             if (j0 + 2 >= fs) q.z = 0u;
             if (j0 + 3 >= fs) q.w = 0u;
@@ -646,11 +686,19 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     return 0;
 }
 
+int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st) {
+    const long n = (long)t.nrhs * t.n;
+    if (n <= 0) return 0;
+    k_reduce_gate<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(t);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
 int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, int count, int nRtC, uint4 *Ac,
-                         hipStream_t st) {
+                         hipStream_t st, const int *actRow) {
     const long tot = (long)nRtC * nKs * 64;
     if (tot <= 0) return 0;
-    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, rowFs, count, nRtC, Ac);
+    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, rowFs, count, nRtC, Ac, actRow);
     SR_HIP(hipGetLastError());
     return 0;
 }
