@@ -316,6 +316,7 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
     for (int ks = ks0; ks < ks1; ks++) {
         if (ks >= skip.x && ks < skip.y) continue;  // wave-uniform
         uint32_t w[4];
+        unsigned nearBits = 0, quadBits = 0;  // per q: listed near pair, by quadrature
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int j = ks * 8 + h * 4 + q;
@@ -357,9 +358,17 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 }
             }
             w[q] = pack_h2(re, im);
-            if (NEAR != 0) {  // wave-uniform
-                near_stage(stQ, nQ, isNear && isQuad, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
-                near_stage(stM, nM, isNear && !isQuad, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
+            nearBits |= (unsigned)isNear << q;
+            quadBits |= (unsigned)isQuad << q;
+        }
+        // near pairs are rare: one ballot decides whether this step lists any
+        if (NEAR != 0 && __ballot(nearBits != 0u)) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = ks * 8 + h * 4 + q;
+                const bool nr = (nearBits >> q) & 1u, qd = (quadBits >> q) & 1u;
+                near_stage(stQ, nQ, nr && qd, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
+                near_stage(stM, nM, nr && !qd, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
             }
         }
         A[((size_t)rt * g.nKs + ks) * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);

@@ -446,8 +446,9 @@ int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipS
     constexpr int W = 4;
     const long nrb = (tk.rc.count + 63) / 64;
     // Enough waves to fill 256 CUs several times over, but runs of emitters long
-    // enough that the receiver block's data is reused.
+    // enough that the receiver block's data is reused (shorter runs for small sets).
     int emPerWave = 16;
+    while (emPerWave > 1 && nrb * ((tk.em.count + emPerWave - 1) / emPerWave) < 8192) emPerWave /= 2;
     long waves = nrb * ((tk.em.count + emPerWave - 1) / emPerWave);
     while (waves > (1L << 22) && emPerWave < 1024) {
         emPerWave *= 2;
