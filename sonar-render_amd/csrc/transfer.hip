@@ -317,13 +317,23 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
         if (ks >= skip.x && ks < skip.y) continue;  // wave-uniform
         uint32_t w[4];
         unsigned nearBits = 0, quadBits = 0;  // per q: listed near pair, by quadrature
+        // the step's emitter records and visibility words, all requested before any is used
+        float4 epq[4], enq[4];
+        uint64_t mq[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int jj = min(ks * 8 + h * 4 + q, g.n - 1);
+            mq[q] = mrow[jj];
+            epq[q] = g.p[jj];
+            enq[q] = g.nrm[jj];
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int j = ks * 8 + h * 4 + q;
             float re = 0.f, im = 0.f;
             bool isNear = false, isQuad = false;
-            if (rvalid && j < g.n && ((mrow[j] >> bit) & 1ull)) {
-                const float4 ep = g.p[j], en = g.nrm[j];
+            if (rvalid && j < g.n && ((mq[q] >> bit) & 1ull)) {
+                const float4 ep = epq[q], en = enq[q];
                 float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
                 float r2 = vx * vx + vy * vy + vz * vz;
                 float rinv = __builtin_amdgcn_rsqf(r2);
@@ -561,11 +571,17 @@ __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
     const int sr = t.actRow ? t.actRow[r] : r;
     const unsigned char *cls = t.cls + (size_t)(sr >> 7) * t.nsplit;
     float2 L = make_float2(0.f, 0.f), U = make_float2(0.f, 0.f);
+    // both partials are read (in bounds either way) and the unwritten one is
+    // selected away, so the loads of all splits are in flight together
+#pragma unroll 4
     for (int s = 0; s < t.nsplit; s++) {
         const size_t o = ((size_t)s * 64 + b) * t.nPadRows + sr;
         const unsigned c = cls[s];
-        if (c & 1u) { const float2 a = t.partL[o]; L.x += a.x; L.y += a.y; }
-        if (c & 2u) { const float2 a = t.partU[o]; U.x += a.x; U.y += a.y; }
+        const float2 a = t.partL[o], u = t.partU[o];
+        L.x += (c & 1u) ? a.x : 0.f;
+        L.y += (c & 1u) ? a.y : 0.f;
+        U.x += (c & 2u) ? u.x : 0.f;
+        U.y += (c & 2u) ? u.y : 0.f;
     }
     const float2 gv = t.gval[(size_t)b * t.n + r];
     const float g = (gv.x == 0.0f && gv.y == 0.0f) ? 0.0f : 1.0f;
