@@ -220,6 +220,8 @@ int word_wait(Model &m, int slot, int &out) {
 
 }  // namespace
 
+int settle(Model &m);
+
 void set_error(const std::string &msg) {
     std::lock_guard<std::mutex> g(gErrMu);
     gErr = msg;
@@ -367,6 +369,7 @@ int model_set_source_amplitude(Model &m, int i, float re, float im) {
     const int ns = (int)(m.srcHost.size() / 3);
     if (!m.rendered) { set_error("source amplitudes live on the GPU: call render_cuda first"); return -3; }
     if (i < 0 || i >= ns) { set_error("source index out of range"); return -2; }
+    if (int rc = settle(m)) return rc;  // queued projections still read the amplitudes
     float2 a = make_float2(re, im);
     SR_HIP(hipMemcpy(m.srcAmp + i, &a, sizeof(float2), hipMemcpyHostToDevice));
     return 0;
@@ -462,16 +465,29 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
     return 0;
 }
 
-// End of a projection: waits for the stream and reports a visibility traversal
-// overflow raised by any mask built on the way (the words are then dropped).
+// End of a projection. Projections return once their work is queued on the
+// library's stream (the host queues the next call's work while this one runs);
+// the visibility traversal's overflow flag follows the work into pinned memory
+// and the next call that reads results (settle) reports it.
 int finish_projection(Model &m) {
-    int herr = 0;
-    SR_HIP(hipMemcpyAsync(&herr, m.errFlag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
-    SR_HIP(hipStreamSynchronize(m.stream));
-    if (herr) {
+    SR_HIP(hipMemcpyAsync(m.hostWords + 1, m.errFlag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    m.errPending = true;
+    return 0;
+}
+
+// Waits for all queued work; reports a traversal overflow raised by any
+// projection since the last settle (its visibility words are then dropped and
+// the results since are not valid).
+int settle(Model &m) {
+    if (m.stream) SR_HIP(hipStreamSynchronize(m.stream));
+    if (m.side) SR_HIP(hipStreamSynchronize(m.side));
+    if (!m.errPending) return 0;
+    m.errPending = false;
+    if (m.hostWords[1]) {
+        m.hostWords[1] = 0;
         for (auto &kv : m.masks) kv.second.valid = false;
         SR_HIP(hipMemset(m.errFlag, 0, sizeof(int)));
-        set_error("visibility traversal stack overflow");
+        set_error("visibility traversal stack overflow in a projection since the last result read");
         return -1;
     }
     return 0;
@@ -1166,6 +1182,7 @@ int model_get_field_points(Model &m, double *out, long n, int plane) {
     if (!m.rendered) { set_error("no field points on the GPU: call render_cuda first"); return -3; }
     if (!out && n > 0) { set_error("null output buffer"); return -2; }
     if (plane < 0 || plane >= m.nplanes) { set_error("plane out of range"); return -2; }
+    if (int rc = settle(m)) return rc;
     const long cnt = std::min<long>(n, nq);
     if (cnt > 0)
         SR_HIP(hipMemcpy(out, m.fldVal + (size_t)plane * nq, cnt * sizeof(double2), hipMemcpyDeviceToHost));
@@ -1178,6 +1195,7 @@ int model_get_object_pressure(Model &m, int type, int idx, int which, int plane,
     if (!o) { set_error("object index out of range"); return -2; }
     if (plane < 0 || plane >= m.nplanes) { set_error("plane out of range"); return -2; }
     if (n < o->ncell) { set_error("output buffer shorter than the object's cell count"); return -2; }
+    if (int rc = settle(m)) return rc;
     std::vector<float2> h(o->ncell, make_float2(0.f, 0.f));
     if (which == 0 || o->type == OBJ_FIELD) {
         if (o->ncell)
@@ -1196,13 +1214,14 @@ int model_get_object_pressure(Model &m, int type, int idx, int which, int plane,
 int model_get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t *out, long nwords) {
     if (int rc = check_rendered(m)) return rc;
     uint64_t *mk;
-    if (get_mask(m, kind, ta, ia, tb, ib, &mk) || finish_projection(m)) return -1;
+    if (get_mask(m, kind, ta, ia, tb, ib, &mk) || finish_projection(m) || settle(m)) return -1;
     SR_HIP(hipMemcpy(out, mk, nwords * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return 0;
 }
 
 int model_reset_pressures(Model &m) {
     if (int rc = check_rendered(m)) return rc;
+    if (int rc = settle(m)) return rc;
     const size_t np = (size_t)m.nplanes;
     for (auto *v : {&m.targets, &m.fields})
         for (auto &o : *v) {
@@ -1212,12 +1231,11 @@ int model_reset_pressures(Model &m) {
         }
     const size_t nq = m.fldHost.size() / 3;
     if (nq) SR_HIP(hipMemsetAsync(m.fldVal, 0, np * nq * sizeof(double2), m.stream));
-    SR_HIP(hipStreamSynchronize(m.stream));
     return 0;
 }
 
 int model_clear_vis_cache(Model &m) {
-    if (m.stream) SR_HIP(hipStreamSynchronize(m.stream));
+    // the words are rewritten by later work on the same stream: no wait needed
     for (auto &kv : m.masks) kv.second.valid = false;  // buffers are kept and rewritten
     for (auto &o : m.targets) o.tc.valid = false;
     return 0;

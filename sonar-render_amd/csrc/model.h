@@ -103,8 +103,9 @@ struct Model {
     size_t scratchCElems = 0;
     // counts read back while the stream runs on: pinned words, one event each
     static constexpr int kWords = 4;
-    int *hostWords = nullptr;
+    int *hostWords = nullptr;  // [0] gated-row count, [1] visibility overflow flag
     hipEvent_t wordEv[kWords] = {};
+    bool errPending = false;   // a projection's overflow flag is on its way to hostWords[1]
     float4 *srcP = nullptr;
     float2 *srcAmp = nullptr;
     float4 *fldP = nullptr;

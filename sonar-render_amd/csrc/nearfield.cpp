@@ -61,6 +61,7 @@ int nf_ensure(NearField &nf) {
 int nf_init(uint32_t seed) {
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
+    SR_HIP(hipDeviceSynchronize());  // queued projections may still read the parameters
     std::vector<float> P = init_params(seed);
     SR_HIP(hipMemcpy(nf.P, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
     nf.trained = false;
@@ -71,6 +72,7 @@ int nf_set_params(const float *in, int n) {
     if (n != kMlpParams) { set_error("near-field MLP has 4994 parameters"); return -2; }
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
+    SR_HIP(hipDeviceSynchronize());  // queued projections may still read the parameters
     SR_HIP(hipMemcpy(nf.P, in, n * sizeof(float), hipMemcpyHostToDevice));
     nf.trained = true;
     return 0;
@@ -134,6 +136,7 @@ int nf_eval(const float *X, const float *Y, int n, float *out, float *grad, cons
 int nf_train(int nsamples, int steps, int batch, float lr, uint32_t seed, float *report) {
     NearField &nf = nearfield();
     if (nf_ensure(nf)) return -1;
+    SR_HIP(hipDeviceSynchronize());  // queued projections may still read the parameters
     if (nsamples < batch || batch <= 0 || steps <= 0) { set_error("nf_train: bad sizes"); return -2; }
     const int nval = 16384;
     float *X, *Y, *Xv, *Yv, *H1, *H2, *O, *G3, *D2, *D1, *g, *m, *v, *acc;
