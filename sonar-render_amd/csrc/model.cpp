@@ -1219,9 +1219,10 @@ int model_get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t 
     return 0;
 }
 
+// Queued behind any work in flight (stream order): scenes can follow each other
+// without the host waiting in between.
 int model_reset_pressures(Model &m) {
     if (int rc = check_rendered(m)) return rc;
-    if (int rc = settle(m)) return rc;
     const size_t np = (size_t)m.nplanes;
     for (auto *v : {&m.targets, &m.fields})
         for (auto &o : *v) {
