@@ -286,7 +286,11 @@ int model_render(Model &m) {
     if (!m.hostWords) {  // kept, like the stream, for the life of the process
         SR_HIP(hipHostMalloc((void **)&m.hostWords, Model::kWords * sizeof(int), hipHostMallocDefault));
         for (auto &e : m.wordEv) SR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        SR_HIP(hipHostMalloc((void **)&m.errHost, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+        SR_HIP(hipHostGetDevicePointer((void **)&m.errFlag, m.errHost, 0));
     }
+    *(volatile int *)m.errHost = 0;
+    m.errPending = false;
     const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
     m.nplanes = m.batchedSources ? std::max(1, ns) : 1;
     for (auto &o : m.targets) if (upload_object(m, o)) return -1;
@@ -341,8 +345,6 @@ int model_render(Model &m) {
         dev_free(unsorted);
         if (rc) return rc;
     }
-    if (dev_alloc(&m.errFlag, 2)) return -1;
-    SR_HIP(hipMemset(m.errFlag, 0, 2 * sizeof(int)));
 
     // Points (ModelCuda.cpp:39-94); sources carry amplitude 1+0i (Model.cpp:18).
     if (ns) {
@@ -467,10 +469,9 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
 
 // End of a projection. Projections return once their work is queued on the
 // library's stream (the host queues the next call's work while this one runs);
-// the visibility traversal's overflow flag follows the work into pinned memory
-// and the next call that reads results (settle) reports it.
+// the visibility kernels store a traversal overflow straight into a pinned word,
+// which the next call that reads results (settle) checks.
 int finish_projection(Model &m) {
-    SR_HIP(hipMemcpyAsync(m.hostWords + 1, m.errFlag, sizeof(int), hipMemcpyDeviceToHost, m.stream));
     m.errPending = true;
     return 0;
 }
@@ -483,10 +484,10 @@ int settle(Model &m) {
     if (m.side) SR_HIP(hipStreamSynchronize(m.side));
     if (!m.errPending) return 0;
     m.errPending = false;
-    if (m.hostWords[1]) {
-        m.hostWords[1] = 0;
+    volatile int *err = m.errHost;  // no work is queued: the host may clear it
+    if (*err) {
+        *err = 0;
         for (auto &kv : m.masks) kv.second.valid = false;
-        SR_HIP(hipMemset(m.errFlag, 0, sizeof(int)));
         set_error("visibility traversal stack overflow in a projection since the last result read");
         return -1;
     }
@@ -871,7 +872,8 @@ static int ensure_transfer(Model &m, int i) {
                 if (dev_alloc(&t.nearQ, t.nearCap) || dev_alloc(&t.nearM, t.nearCap)) return -1;
             }
             ph.nearQ = t.nearQ; ph.nearM = t.nearM; ph.nearCount = t.nearCount; ph.nearCap = t.nearCap;
-            if (counters_zero_launch(t.nearCount, 2, m.stream)) return -1;
+            if (!t.nearClear && counters_zero_launch(t.nearCount, 2, m.stream)) return -1;
+            t.nearClear = false;  // the build counts into it
         }
         {
             ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, 0.0);
@@ -898,6 +900,7 @@ static int ensure_transfer(Model &m, int i) {
     if (m.nearMode != 0) {
         ProfScope ps(m, PK_NEAR, (double)counts[0] + counts[1], 0.0);
         if (transfer_near(g, ph, counts[0], counts[1], t.A, m.stream)) return -1;
+        t.nearClear = true;  // cleared by the near-pair kernels, or never counted into
     }
     t.valid = true;
     return 0;
@@ -906,19 +909,20 @@ static int ensure_transfer(Model &m, int i) {
 // One bounce of target i in the transfer form: P_out = S x for every row, then
 // the rows whose own P_in is zero in some plane get the product of their
 // lower-facet part of S (the zero-input gate) for those planes, P += P_out.
-static int bounce_mixed(Model &m, int i, bool first) {
+static int bounce_mixed(Model &m, int i) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     const int n = o.nact;
     const int nsplit = t.nsplit, ksPer = t.ksPer;  // split K, chosen with the transfer matrix
     // bounce state: this bounce's set (zero), and the other one, cleared here for the next
     const int stWords = 1 + m.nplanes;
-    if (first) {
+    if (!t.stateClear) {  // first use, or a bounce that stopped before clearing the next set
         if (counters_zero_launch(t.state, 2 * stWords, m.stream)) return -1;
         t.stateSet = 0;
     }
     int *cur = t.state + (size_t)t.stateSet * stWords, *nxt = t.state + (size_t)(t.stateSet ^ 1) * stWords;
     t.stateSet ^= 1;
+    t.stateClear = false;  // until the column maximum below has cleared the next set
     unsigned *colMax = reinterpret_cast<unsigned *>(cur + 1);
     if (t.gateInGemm) {
         // Every 128-row group is single-facet: each split's partial sums are split at
@@ -926,6 +930,7 @@ static int bounce_mixed(Model &m, int i, bool first) {
         {
             ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
             if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
+            t.stateClear = true;
         }
         if (ensure_scratch(m, (size_t)nsplit * 64 * t.nPad)) return -1;
         for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
@@ -962,6 +967,7 @@ static int bounce_mixed(Model &m, int i, bool first) {
     {
         ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
         if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
+        t.stateClear = true;
     }
     // partials of the main product, and of the compact set (split count x rows <= 1024 x 128 + n + 128)
     if (ensure_scratch(m, std::max((size_t)nsplit * n, (size_t)1024 * 128 + n + 128) * 64)) return -1;
@@ -1072,7 +1078,7 @@ int model_project_t2t(Model &m, int nref) {
             if (!o.nact) continue;
             std::swap(o.Pin, o.Pout);  // SwapOutputToInputPressure; P_out is fully rewritten below
             if (m.transferMode == 1) {
-                if (ensure_transfer(m, i) || bounce_mixed(m, i, it == 0)) return -1;
+                if (ensure_transfer(m, i) || bounce_mixed(m, i)) return -1;
                 continue;
             }
             uint64_t *mk;
@@ -1223,16 +1229,20 @@ int model_get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t 
 // without the host waiting in between.
 int model_reset_pressures(Model &m) {
     if (int rc = check_rendered(m)) return rc;
-    const size_t np = (size_t)m.nplanes;
+    const long np = m.nplanes;
+    std::vector<float2 *> p;
+    std::vector<long> n;
+    auto add = [&](float2 *q, long cnt) {
+        if (q && cnt > 0) { p.push_back(q); n.push_back(cnt); }
+    };
     for (auto *v : {&m.targets, &m.fields})
         for (auto &o : *v) {
-            if (o.ncell) SR_HIP(hipMemsetAsync(o.P, 0, np * o.ncell * sizeof(float2), m.stream));
-            if (o.Pin) SR_HIP(hipMemsetAsync(o.Pin, 0, np * o.nact * sizeof(float2), m.stream));
-            if (o.Pout) SR_HIP(hipMemsetAsync(o.Pout, 0, np * o.nact * sizeof(float2), m.stream));
+            add(o.P, np * o.ncell);
+            add(o.Pin, np * o.nact);
+            add(o.Pout, np * o.nact);
         }
-    const size_t nq = m.fldHost.size() / 3;
-    if (nq) SR_HIP(hipMemsetAsync(m.fldVal, 0, np * nq * sizeof(double2), m.stream));
-    return 0;
+    add(reinterpret_cast<float2 *>(m.fldVal), 2 * np * (long)(m.fldHost.size() / 3));  // double2 = 2 float2
+    return zero_ranges_launch(p.data(), n.data(), (int)p.size(), m.stream);
 }
 
 int model_clear_vis_cache(Model &m) {
@@ -1347,7 +1357,7 @@ void model_teardown(Model &m) {
     m.gCap = 0;
     m.gMcCap = 0;
     dev_free(m.srcP); dev_free(m.srcAmp); dev_free(m.fldP); dev_free(m.fldVal);
-    dev_free(m.trisSorted); dev_free(m.nodes); dev_free(m.errFlag);
+    dev_free(m.trisSorted); dev_free(m.nodes);
     dev_free(m.scratchL); dev_free(m.scratchU); dev_free(m.scratchC);
     m.scratchElems = 0;
     m.scratchCElems = 0;

@@ -49,6 +49,7 @@ struct Object {
         // (float bits); each bounce clears the other set for the next
         int *state = nullptr;
         int stateSet = 0;
+        bool stateClear = false;   // the set stateSet names is known to be zero
         int *rowFs = nullptr;      // per active row: first active index on its facet
         // Zero-input gate inside the product (meshes whose facets each hold >= 128
         // active cells): rows of S padded so that every facet starts a 128-row
@@ -65,6 +66,7 @@ struct Object {
         int skipZeroCap = 0;
         int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
         int *nearCount = nullptr;
+        bool nearClear = false;    // nearCount is known to be zero (the near-pair kernels clear it)
         int nearCap = 0;
         // near-pair counts of the last build and the settings they hold for: the
         // counts depend only on geometry, visibility and these, so a rebuild with
@@ -103,9 +105,12 @@ struct Model {
     size_t scratchCElems = 0;
     // counts read back while the stream runs on: pinned words, one event each
     static constexpr int kWords = 4;
-    int *hostWords = nullptr;  // [0] gated-row count, [1] visibility overflow flag
+    int *hostWords = nullptr;  // [0] gated-row count
     hipEvent_t wordEv[kWords] = {};
-    bool errPending = false;   // a projection's overflow flag is on its way to hostWords[1]
+    // the visibility traversal's overflow flag: a mapped, coherent pinned word that
+    // the kernels store into, read by the host after a stream wait with no copy
+    int *errHost = nullptr, *errFlag = nullptr;  // host view, device view
+    bool errPending = false;  // projections since the last result read may have raised it
     float4 *srcP = nullptr;
     float2 *srcAmp = nullptr;
     float4 *fldP = nullptr;
@@ -114,7 +119,6 @@ struct Model {
     float4 *nodes = nullptr;
     int ntri = 0;
     VisConst vc{};
-    int *errFlag = nullptr;
     struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; };
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix

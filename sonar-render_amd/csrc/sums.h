@@ -139,6 +139,9 @@ int reduce_points_multi_launch(const PointsReduceTask *tasks, const int2 *items,
 int prime_multi_launch(const PrimeTask *tasks, const int2 *items, int nitems, hipStream_t st);
 int zero_multi_launch(const ZeroTask *tasks, const int2 *items, int nitems, hipStream_t st);
 int counters_zero_launch(int *counters, int n, hipStream_t st);
+// Zeroes count float2 ranges (start p[i], n[i] elements) with one launch per 8 ranges
+// (a kernel rather than memset nodes: callers may capture it in a graph).
+int zero_ranges_launch(float2 *const *p, const long *n, int count, hipStream_t st);
 
 // Split-K choice: enough waves to occupy every SIMD a few times.
 int choose_splits(int nrb, int ne, int minPerSplit);

@@ -627,6 +627,16 @@ __global__ void k_counters_zero(int *c, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) c[i] = 0;
 }
+struct ZeroRanges {
+    float2 *p[8];
+    long n[8];
+};
+// blockIdx.y: the range; a grid-stride loop over its elements
+__global__ __launch_bounds__(256) void k_zero_ranges(ZeroRanges z) {
+    float2 *p = z.p[blockIdx.y];
+    const long n = z.n[blockIdx.y];
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = make_float2(0.f, 0.f);
+}
 
 // ---------------------------------------------------------------------------
 
@@ -824,6 +834,24 @@ int zero_multi_launch(const ZeroTask *tasks, const int2 *items, int nitems, hipS
 int counters_zero_launch(int *counters, int n, hipStream_t st) {
     k_counters_zero<<<(n + 255) / 256, 256, 0, st>>>(counters, n);
     SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int zero_ranges_launch(float2 *const *p, const long *n, int count, hipStream_t st) {
+    for (int c0 = 0; c0 < count; c0 += 8) {
+        ZeroRanges z{};
+        const int k = std::min(8, count - c0);
+        long mx = 0;
+        for (int i = 0; i < k; i++) {
+            z.p[i] = p[c0 + i];
+            z.n[i] = p[c0 + i] ? n[c0 + i] : 0;
+            mx = std::max(mx, z.n[i]);
+        }
+        if (mx <= 0) continue;
+        const unsigned gx = (unsigned)std::min<long>(512, (mx + 255) / 256);
+        k_zero_ranges<<<dim3(gx, k), 256, 0, st>>>(z);
+        SR_HIP(hipGetLastError());
+    }
     return 0;
 }
 

@@ -30,7 +30,7 @@ struct TransferPhys {
     const float *mlp;      // kMlpParams FP32 weights (rounded to BF16 on load)
     // near pairs found by the S build: by quadrature (nearQ) and learned (nearM)
     int2 *nearQ, *nearM;   // (row, emitter)
-    int *nearCount;        // [2] device counters, zeroed by the caller
+    int *nearCount;        // [2] device counters: zero before a build; the near-pair kernels clear them
     int nearCap;           // capacity of each list
     float quadRadius;      // learned mode: closer pairs use the quadrature (metres, >= delta)
 };
@@ -68,7 +68,8 @@ struct ReduceGateTask {
 };
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
-// Writes the listed near pairs' values into S (counts read back from nearCount).
+// Writes the listed near pairs' values into S (counts read back from nearCount)
+// and clears nearCount for the next build.
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st);
 // Per-plane max |A_j P_in[b][j]| into colMax (float bits, zeroed beforehand);
 // also zeroes nextWords ints at nextState (the next bounce's state set).

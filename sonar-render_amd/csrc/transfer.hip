@@ -123,6 +123,7 @@ __device__ inline void store_pair(uint4 *A, int nKs, int row, int j, uint32_t w)
 // sub-points each, summed across the lanes.
 __global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
                                                    int count, uint4 *__restrict__ A) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;  // the next build counts from zero
     const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
     const int sy = threadIdx.x & 15;
     const bool valid = i < count;  // uniform over the pair's 16 lanes
@@ -156,6 +157,7 @@ constexpr int kHP = 72;  // bf16 per activation row in LDS (64 + 8: 16-byte alig
 
 __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
                                                   int count, uint4 *__restrict__ A) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;  // the next build counts from zero
     __shared__ bf16x8 sW1[2][64];
     __shared__ bf16x8 sW2[4][2][64];
     __shared__ float sB1[64], sB2[64], sW3[2][64], sB3[2];

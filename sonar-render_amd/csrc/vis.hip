@@ -231,7 +231,10 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
         if (remaining == 0) break;
         int next = -1;
         if (bl && br) {
-            if (sp >= kStack) { if (lane == 0) atomicExch(errFlag, 2); break; }
+            if (sp >= kStack) {  // a plain system-scope store: the flag may be a mapped host word
+                if (lane == 0) __hip_atomic_store(errFlag, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
             if (lane == 0) stk[sp] = cr;
             sp++;
             next = cl;
