@@ -646,28 +646,20 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
     return 0;
 }
 
-// Fragment->field point on the FP32 matrix cores, groups of up to 64 planes:
-// the emitters' total pressures are first gathered into a compact [plane][emitter]
-// block, partial sums are FP32 within a split of 32 emitters and FP64 across splits.
+// Fragment->field point on the FP32 matrix cores, groups of up to 64 planes: the
+// emitters' total pressures are read through the emitter->cell map as the tiles
+// are loaded; partial sums are FP32 within a split of 32 emitters and FP64 across splits.
 static int gather_points_mfma(Model &m, GatherTask t, const Phys &ph, int nplanes, long evalStride, double pairs) {
     const int nsplit = std::max(1, std::min(2048, (t.ne + 31) / 32));
     t.emPerSplit = ((t.ne + nsplit - 1) / nsplit + 31) / 32 * 32;
     t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
-    const int nb0 = std::min(64, nplanes);
-    // scratchL holds the partials, scratchU the compact emitter values
-    if (ensure_scratch(m, std::max((size_t)t.nsplit * 64 * t.nr, (size_t)nb0 * t.ne))) return -1;
+    if (ensure_scratch(m, (size_t)t.nsplit * 64 * t.nr)) return -1;  // the partials
     for (int b0 = 0; b0 < nplanes; b0 += 64) {
         const int nb = std::min(64, nplanes - b0);
-        {
-            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);  // data movement, counted with the reductions
-            if (prime_launch(t.eval + b0 * evalStride, evalStride, t.ecell, t.ne, nb, m.scratchU, t.ne, m.stream))
-                return -1;
-        }
         GatherTask g = t;
         g.nrhs = nb;
-        g.ecell = nullptr;
-        g.eval = m.scratchU;
-        g.evalStride = t.ne;
+        g.eval = t.eval + b0 * evalStride;
+        g.evalStride = evalStride;
         g.partL = m.scratchL;
         g.partU = nullptr;
         {
@@ -909,37 +901,44 @@ static int ensure_transfer(Model &m, int i) {
 // One bounce of target i in the transfer form: P_out = S x for every row, then
 // the rows whose own P_in is zero in some plane get the product of their
 // lower-facet part of S (the zero-input gate) for those planes, P += P_out.
-static int bounce_mixed(Model &m, int i) {
+// first: the first bounce of a projection; more: another bounce of this target
+// follows in the same projection (its P_in is this bounce's P_out).
+static int bounce_mixed(Model &m, int i, bool first, bool more) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     const int n = o.nact;
     const int nsplit = t.nsplit, ksPer = t.ksPer;  // split K, chosen with the transfer matrix
-    // bounce state: this bounce's set (zero), and the other one, cleared here for the next
+    // Bounce state, two sets used in turn: this bounce's set, zero or holding the
+    // column maxima of P_in that the previous bounce's reduction found; the other
+    // set is cleared here for the next bounce.
     const int stWords = 1 + m.nplanes;
-    if (!t.stateClear) {  // first use, or a bounce that stopped before clearing the next set
+    const bool haveMax = !first && t.gateInGemm && t.maxReady;
+    t.maxReady = false;
+    if (!haveMax && !t.stateClear) {  // first use, or a bounce that stopped before clearing the next set
         if (counters_zero_launch(t.state, 2 * stWords, m.stream)) return -1;
         t.stateSet = 0;
     }
     int *cur = t.state + (size_t)t.stateSet * stWords, *nxt = t.state + (size_t)(t.stateSet ^ 1) * stWords;
     t.stateSet ^= 1;
-    t.stateClear = false;  // until the column maximum below has cleared the next set
+    t.stateClear = false;  // until this bounce has cleared the next set
     unsigned *colMax = reinterpret_cast<unsigned *>(cur + 1);
     if (t.gateInGemm) {
         // Every 128-row group is single-facet: each split's partial sums are split at
         // the group's higher-facet boundary and the reduction applies the gate.
-        {
+        if (!haveMax) {
             ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
             if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
-            t.stateClear = true;
         }
         if (ensure_scratch(m, (size_t)nsplit * 64 * t.nPad)) return -1;
         for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
             const int np = std::min(64, m.nplanes - g0);
             const float2 *pin = o.Pin + (size_t)g0 * n;
             {
+                // with the maxima from the previous reduction, the first group's B build clears the next set
+                const bool clr = haveMax && g0 == 0;
                 ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
                 if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + g0, t.sS, t.colScale, t.colInv, t.B,
-                                     m.stream))
+                                     m.stream, clr ? nxt : nullptr, clr ? stWords : 0))
                     return -1;
             }
             TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL, 0, t.kB,
@@ -953,11 +952,15 @@ static int bounce_mixed(Model &m, int i) {
                 ProfScope ps(m, PK_GEMM, pairs, fl, by);
                 if (transfer_gemm(gm, m.stream)) return -1;
             }
+            // another bounce follows: the reduction also finds its column maxima (into the next set)
             ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.actRow, pin,
-                             o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv};
+                             o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
+                             o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + g0 : nullptr};
             ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
             if (transfer_reduce_gate(r, m.stream)) return -1;
         }
+        if (more) t.maxReady = true;
+        else t.stateClear = true;
         return 0;
     }
     // rows with a zero input in some plane (the gate), counted before the product
@@ -967,7 +970,6 @@ static int bounce_mixed(Model &m, int i) {
     {
         ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
         if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
-        t.stateClear = true;
     }
     // partials of the main product, and of the compact set (split count x rows <= 1024 x 128 + n + 128)
     if (ensure_scratch(m, std::max((size_t)nsplit * n, (size_t)1024 * 128 + n + 128) * 64)) return -1;
@@ -1063,6 +1065,7 @@ static int bounce_mixed(Model &m, int i) {
                                  o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv, m.stream))
             return -1;
     }
+    t.stateClear = true;  // the column-max pass cleared the next set
     return 0;
 }
 
@@ -1078,7 +1081,7 @@ int model_project_t2t(Model &m, int nref) {
             if (!o.nact) continue;
             std::swap(o.Pin, o.Pout);  // SwapOutputToInputPressure; P_out is fully rewritten below
             if (m.transferMode == 1) {
-                if (ensure_transfer(m, i) || bounce_mixed(m, i)) return -1;
+                if (ensure_transfer(m, i) || bounce_mixed(m, i, it == 0, it + 1 < nref)) return -1;
                 continue;
             }
             uint64_t *mk;

@@ -50,6 +50,7 @@ struct Object {
         int *state = nullptr;
         int stateSet = 0;
         bool stateClear = false;   // the set stateSet names is known to be zero
+        bool maxReady = false;     // ... or holds the column maxima of P_in (from the last reduction)
         int *rowFs = nullptr;      // per active row: first active index on its facet
         // Zero-input gate inside the product (meshes whose facets each hold >= 128
         // active cells): rows of S padded so that every facet starts a 128-row

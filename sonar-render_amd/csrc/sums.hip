@@ -228,12 +228,15 @@ struct MfmaTile {
 
 // Requests tile [base, base + kMT) into T. Emitter indices are clamped into
 // range (the values past the tile's end are never used: their bits read as 0).
+// GATHER: the pressures are read through t.ecell (emitter -> column of eval).
+template <bool GATHER>
 __device__ __attribute__((always_inline)) inline void mfma_tile_dma(const GatherTask &t, int base, int rb0, int nrb,
                                                                     int wave, int lane, MfmaTile &T) {
     const int last = t.ne - 1;
     // x: plane q = 16 w + i, one dword per lane: emitter lane / 2, component lane & 1
     {
-        const int e = min(base + (lane >> 1), last);
+        int e = min(base + (lane >> 1), last);
+        if (GATHER && t.ecell) e = t.ecell[e];
         for (int i = 0; i < 16; i++) {
             const int q = wave * 16 + i;
             if (q >= t.nrhs) break;  // wave-uniform; the columns of absent planes are never written out
@@ -322,18 +325,18 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
             acc[5] = __builtin_amdgcn_mfma_f32_32x32x2f32(ks, xh.x + xh.y, acc[5], 0, 0, 0);
         }
     };
-    if (e0 < e1) mfma_tile_dma(t, e0, rb0, nrb, wave, lane, TA);
+    if (e0 < e1) mfma_tile_dma<KIND == KIND_F2P>(t, e0, rb0, nrb, wave, lane, TA);
     int base = e0;
     while (base < e1) {
         __builtin_amdgcn_s_waitcnt(0);  // this wave's share of TA has landed
         __syncthreads();                // TA complete; every wave is done with TB
-        if (base + kMT < e1) mfma_tile_dma(t, base + kMT, rb0, nrb, wave, lane, TB);
+        if (base + kMT < e1) mfma_tile_dma<KIND == KIND_F2P>(t, base + kMT, rb0, nrb, wave, lane, TB);
         compute(TA, min(kMT, e1 - base));
         base += kMT;
         if (base >= e1) break;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (base + kMT < e1) mfma_tile_dma(t, base + kMT, rb0, nrb, wave, lane, TA);
+        if (base + kMT < e1) mfma_tile_dma<KIND == KIND_F2P>(t, base + kMT, rb0, nrb, wave, lane, TA);
         compute(TB, min(kMT, e1 - base));
         base += kMT;
     }
@@ -715,8 +718,10 @@ int gated_replace_launch(const float2 *part, int nsplit, int count, const int *l
 
 int f2f_mfma_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st, bool compactSet) {
     if (t.nr <= 0 || t.ne <= 0) return 0;
-    if (t.ecell || t.nrhs < 1 || t.nrhs > 64 || t.emPerSplit <= 0 || (long)t.nsplit * t.emPerSplit < t.ne) {
-        set_error("f2f_mfma: needs compact emitters, 1..64 planes and a covering emitter split");
+    if ((t.ecell && kind != KIND_F2P) || t.nrhs < 1 || t.nrhs > 64 || t.emPerSplit <= 0 ||
+        (long)t.nsplit * t.emPerSplit < t.ne) {
+        set_error("f2f_mfma: needs compact emitters (field points: or an emitter->cell map), 1..64 planes and a "
+                  "covering emitter split");
         return -2;
     }
     if (kind != KIND_F2F && kind != KIND_F2P) { set_error("f2f_mfma: fragment or field-point receivers only"); return -2; }

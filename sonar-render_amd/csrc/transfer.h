@@ -65,6 +65,10 @@ struct ReduceGateTask {
     long pcellStride;
     const int *rowCell;
     const float *colInvScale;
+    // when set: per plane, max |a_r P_out[b][r]| (a_r = actP[r].w) as float bits by
+    // atomicMax, zero beforehand: the next bounce's B scales without a pass over P_in
+    const float4 *actP;
+    unsigned *colMaxNext;
 };
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
@@ -76,9 +80,11 @@ int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int 
 int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, unsigned *colMax,
                      int *nextState, int nextWords, hipStream_t st);
 // B of one plane group, scaled from colMax; stores each plane's scale and the
-// inverse 1 / (scale sS) the reduction applies.
+// inverse 1 / (scale sS) the reduction applies. clearState (nullable): clearWords
+// ints zeroed (the next bounce's state set, when no column-max pass runs).
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
-                     const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st);
+                     const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st,
+                     int *clearState = nullptr, int clearWords = 0);
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
 int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st);
 // Gated rows as compact row tiles of the transfer matrix (nRtC tiles of 32,

@@ -433,8 +433,10 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
                                                  long stride, int n, int nKs, int nplanes,
                                                  const unsigned *__restrict__ colMax, float sS,
                                                  float *__restrict__ scaleOut, float *__restrict__ invOut,
-                                                 uint4 *__restrict__ B) {
+                                                 uint4 *__restrict__ B, int *__restrict__ clearState,
+                                                 int clearWords) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (clearState && t < clearWords) clearState[t] = 0;
     if (t < nplanes) {
         const float sc = col_scale_of(colMax[t]);
         scaleOut[t] = sc;
@@ -565,11 +567,11 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     else if (kLast >= 0) store_part(kLast < kBnd ? t.partL : t.partU);
 }
 
-// Gated reduction: thread = (plane, active row).
-__global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)t.nrhs * t.n) return;
-    const int b = (int)(i / t.n), r = (int)(i % t.n);
+// One (plane, active row) of the gated reduction: sums the splits' lower- and
+// higher-facet partials, applies the zero-input gate and the column scale, stores
+// P_out and adds it into P. Returns the stored value (zero past the last row).
+__device__ inline float2 reduce_gate_row(const ReduceGateTask &t, int b, int r) {
+    if (r >= t.n) return make_float2(0.f, 0.f);
     const int sr = t.actRow ? t.actRow[r] : r;
     const unsigned char *cls = t.cls + (size_t)(sr >> 7) * t.nsplit;
     float2 L = make_float2(0.f, 0.f), U = make_float2(0.f, 0.f);
@@ -593,6 +595,28 @@ __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
     float2 &P = t.pcell[(size_t)b * t.pcellStride + t.rowCell[r]];
     P.x += v.x;
     P.y += v.y;
+    return v;
+}
+
+// Gated reduction: a workgroup = 256 active rows of one plane (blockIdx.y).
+__global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
+    const int b = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
+    const float2 v = reduce_gate_row(t, b, r);
+    if (!t.colMaxNext) return;  // uniform
+    __shared__ float red[4];
+    float m = 0.f;
+    if (r < t.n) {
+        const float a = t.actP[r].w;
+        m = fmaxf(fabsf(a * v.x), fabsf(a * v.y));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        if (m > 0.f) atomicMax(&t.colMaxNext[b], __float_as_uint(m));
+    }
 }
 
 // Rows whose own input pressure is exactly zero in some plane (the zero-input
@@ -691,11 +715,12 @@ int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int
 }
 
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
-                     const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st) {
-    const long tot = std::max((long)nKs * 256, (long)nplanes);
+                     const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st,
+                     int *clearState, int clearWords) {
+    const long tot = std::max({(long)nKs * 256, (long)nplanes, clearState ? (long)clearWords : 0L});
     if (tot <= 0) return 0;
     k_build_b<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, colMax, sS, scale,
-                                                             invScale, B);
+                                                             invScale, B, clearState, clearWords);
     SR_HIP(hipGetLastError());
     return 0;
 }
@@ -714,9 +739,9 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
 }
 
 int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st) {
-    const long n = (long)t.nrhs * t.n;
-    if (n <= 0) return 0;
-    k_reduce_gate<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(t);
+    if (t.n <= 0 || t.nrhs <= 0) return 0;
+    if (t.colMaxNext && !t.actP) { set_error("gated reduction: column maxima need the fragment weights"); return -2; }
+    k_reduce_gate<<<dim3((unsigned)((t.n + 255) / 256), (unsigned)t.nrhs), 256, 0, st>>>(t);
     SR_HIP(hipGetLastError());
     return 0;
 }

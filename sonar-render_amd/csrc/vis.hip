@@ -368,24 +368,32 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *_
 
 // Fills the words with emitter e < 64*rb of a symmetric set from the computed
 // ones: word (rb, 64*eb + i) bit j = word (eb, 64*rb + j) bit i, for eb < rb.
-__global__ __launch_bounds__(256) void k_mask_transpose(uint64_t *mask, int n) {
-    const int nrb = (n + 63) >> 6;
-    const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = __lane_id();
-    // pair -> (rb, eb) with 0 <= eb < rb < nrb, row-major over the strictly lower triangle
-    const long npairs = (long)nrb * (nrb - 1) / 2;
-    if (pair >= npairs) return;
+// One wave per block pair, pair -> (rb, eb) with 0 <= eb < rb row-major over the
+// strictly lower triangle; lane i gathers its transposed word from 64 ballots
+// and the wave stores the 64 words together.
+__device__ inline void transpose_block_pair(uint64_t *mask, int n, long pair, int lane) {
     int rb = (int)((1.0 + sqrt(1.0 + 8.0 * (double)pair)) / 2.0);
     while ((long)rb * (rb - 1) / 2 > pair) rb--;
     while ((long)(rb + 1) * rb / 2 <= pair) rb++;
     const int eb = (int)(pair - (long)rb * (rb - 1) / 2);
     const int r = rb * 64 + lane;
     const uint64_t w = (r < n) ? mask[(size_t)eb * n + r] : 0ull;
+    uint64_t mine = 0ull;
+#pragma unroll 8
     for (int i = 0; i < 64; i++) {
         const uint64_t col = __ballot((w >> i) & 1ull);
-        const int e = eb * 64 + i;
-        if (lane == i && e < n) mask[(size_t)rb * n + e] = col;
+        if (lane == i) mine = col;
     }
+    const int e = eb * 64 + lane;
+    if (e < n) mask[(size_t)rb * n + e] = mine;
+}
+
+__global__ __launch_bounds__(256) void k_mask_transpose(uint64_t *mask, int n) {
+    const int nrb = (n + 63) >> 6;
+    const long pair = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long npairs = (long)nrb * (nrb - 1) / 2;
+    if (pair >= npairs) return;  // uniform over the wave
+    transpose_block_pair(mask, n, pair, __lane_id());
 }
 
 // ---------------------------------------------------------------------------
@@ -473,23 +481,9 @@ int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipS
 __global__ __launch_bounds__(256) void k_mask_transpose_multi(const MaskTask *__restrict__ tasks,
                                                               const int2 *__restrict__ items, int nitems) {
     const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (w >= nitems) return;
+    if (w >= nitems) return;  // uniform over the wave
     const int2 it = items[w];
-    const MaskTask &tk = tasks[it.x];
-    const int n = tk.rc.count;
-    const long pair = it.y;
-    const int lane = __lane_id();
-    int rb = (int)((1.0 + sqrt(1.0 + 8.0 * (double)pair)) / 2.0);
-    while ((long)rb * (rb - 1) / 2 > pair) rb--;
-    while ((long)(rb + 1) * rb / 2 <= pair) rb++;
-    const int eb = (int)(pair - (long)rb * (rb - 1) / 2);
-    const int r = rb * 64 + lane;
-    const uint64_t wd = (r < n) ? tk.mask[(size_t)eb * n + r] : 0ull;
-    for (int i = 0; i < 64; i++) {
-        const uint64_t col = __ballot((wd >> i) & 1ull);
-        const int e = eb * 64 + i;
-        if (lane == i && e < n) tk.mask[(size_t)rb * n + e] = col;
-    }
+    transpose_block_pair(tasks[it.x].mask, tasks[it.x].rc.count, it.y, __lane_id());
 }
 
 int mask_transpose_multi_launch(const MaskTask *tasks, const int2 *items, int nitems, hipStream_t st) {

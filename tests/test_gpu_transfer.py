@@ -94,3 +94,44 @@ def test_mixed_gated_rows_exact(api, f):
         print("plane", p, "mixed", rel_l2(a, b), "plain", rel_l2(c, b))
         assert rel_l2(a, b) < TOL_MIXED
         assert rel_l2(c, b) < 2e-4
+
+
+def test_mixed_two_plane_groups_repeated_scenes(api):
+    """80 incidences: every bounce runs two plane groups (64 + 16), and four of the
+    incidences leave plates unlit. Each bounce's reduction also finds the column
+    scales of the next bounce's input, per group. The same model then renders
+    scenes of 3, 1 and 2 bounces back to back (queued with no host wait between
+    them), each equal to that scene on a fresh model."""
+    sc = scenes.config_trihedral(f=5e3, n_side=9, bounces=3)
+    unlit = np.array([[-6.0, 4.0, 13.0], [5.0, -3.0, 12.0], [4.0, 5.0, 13.0], [3.0, 4.0, -0.5]], np.float32)
+    sc["sources"] = np.concatenate([sc["sources"][:76], unlit])
+    sc["points"] = sc["sources"].copy()
+    _banner("mixed form: trihedral 5 kHz, 80 incidences (two plane groups), 3 bounces")
+    mixed = runner.run_scene(sc, transfer_mode=1)
+    plain = runner.run_scene(sc, transfer_mode=0)
+    e = rel_l2(np.concatenate([mixed["points"].ravel(), mixed["targets"][0].ravel()]),
+               np.concatenate([plain["points"].ravel(), plain["targets"][0].ravel()]))
+    print("mixed vs plain rel-L2", e)
+    assert e < TOL_MIXED
+    for p in (0, 63, 64, 79):
+        ref = oracle.run_scene(sc, plane_source=p)
+        a = np.concatenate([mixed["points"][p], mixed["targets"][0][p]])
+        b = np.concatenate([ref["points"], ref["targets"][0]])
+        print("plane", p, "mixed vs oracle", rel_l2(a, b))
+        assert rel_l2(a, b) < TOL_MIXED
+    fresh = {3: mixed}
+    for nb in (1, 2):
+        fresh[nb] = runner.run_scene(dict(sc, bounces=nb), transfer_mode=1)
+    _banner("mixed form: scenes of 3, 1, 2 bounces back to back on one model")
+    runner.load_scene(sc, transfer_mode=1)
+    for nb in (3, 1, 2):
+        api.reset_pressures()
+        runner.project_scene(dict(sc, bounces=nb))
+        out = runner.read_results(sc)
+        for key in ("points", "targets"):
+            x = out[key] if key == "points" else out[key][0]
+            y = fresh[nb][key] if key == "points" else fresh[nb][key][0]
+            d = rel_l2(x.ravel(), y.ravel())
+            print(nb, "bounces", key, "vs fresh model rel-L2", d)
+            assert d < 1e-6
+    api.TearDownCuda()
