@@ -810,10 +810,12 @@ static int ensure_transfer(Model &m, int i) {
         }
         SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
         t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
-        // split K (whole chunks of 8 steps) so that the grid holds ~1024 workgroups
-        int nsplit = std::max(1, std::min(1024 / std::max(1, nWg), nKs / 64));
+        // split K (whole chunks of 8 steps, at least 32 steps a split) so that the
+        // grid holds ~1024 workgroups
+        int nsplit = std::max(1, std::min(1024 / std::max(1, nWg), nKs / 32));
         t.ksPer = ((nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
         t.nsplit = (nKs + t.ksPer - 1) / t.ksPer;
+        t.partRows = (double)t.nsplit * n;  // every split writes every row
         if (gateIn) {
             // which partials each (group, split) writes, by the product's own walk over K
             std::vector<unsigned char> cls((size_t)nWg * t.nsplit, 0);
@@ -825,6 +827,13 @@ static int ensure_transfer(Model &m, int i) {
                     for (int kb = next(ks0); kb < ks1; kb = next(kb + 8)) c |= kb < kB[w] ? 1 : 2;
                     cls[(size_t)w * t.nsplit + s] = c;
                 }
+            // partial-sum rows one pass writes per plane: each partial a split writes, over its group's active rows
+            t.partRows = 0;
+            for (int w = 0; w < nWg; w++) {
+                int act = 0;
+                for (int k = 0; k < 128; k++) act += rowAct[(size_t)w * 128 + k] >= 0;
+                for (int s = 0; s < t.nsplit; s++) t.partRows += (double)act * __builtin_popcount(cls[(size_t)w * t.nsplit + s]);
+            }
             if (dev_alloc(&t.rowAct, rowAct.size()) || dev_alloc(&t.actRow, n) || dev_alloc(&t.kB, nWg) ||
                 dev_alloc(&t.cls, cls.size()))
                 return -1;
@@ -948,12 +957,12 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             {
                 const double pairs = ((double)n * n - o.sameFacetPairs) * np;
                 const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
-                const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + (double)nsplit * np * n * 8;
+                const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + t.partRows * np * 8;
                 ProfScope ps(m, PK_GEMM, pairs, fl, by);
                 if (transfer_gemm(gm, m.stream)) return -1;
             }
             // another bounce follows: the reduction also finds its column maxima (into the next set)
-            ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.actRow, pin,
+            ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.rowAct, pin,
                              o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
                              o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + g0 : nullptr};
             ProfScope ps(m, PK_REDUCE, 0.0, 0.0);

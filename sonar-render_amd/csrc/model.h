@@ -77,6 +77,7 @@ struct Object {
         float *colScale = nullptr, *colInv = nullptr;
         float sS = 1.f;
         double aBytesRead = 0;  // bytes of S one GEMM pass reads (same-facet steps are skipped)
+        double partRows = 0;    // partial-sum rows one GEMM pass writes, per plane
         bool valid = false;
     } tc;
     // pressure planes (nplanes each)

@@ -58,7 +58,7 @@ struct ReduceGateTask {
     const float2 *partL, *partU;
     const unsigned char *cls;
     int nsplit, nPadRows, nrhs, n;
-    const int *actRow;          // active row -> row of S (null: identity)
+    const int *rowAct;          // row of S -> active row (-1: padding; null: identity, nPadRows == n)
     const float2 *gval;         // P_in [plane][n]
     float2 *out;                // P_out [plane][n]
     float2 *pcell;              // P [plane][pcellStride] through rowCell

@@ -291,84 +291,104 @@ __device__ inline void near_stage(int2 *entries, int &n, bool pred, int2 *list, 
 }
 
 // ---------------------------------------------------------------------------
-// S build: one lane = one row, 4 emitters of one K step. Near pairs (mode 1,
-// 2) are left zero and listed for the near-field kernels.
+// S build: a wave = 64 consecutive rows of S (two 32-row tiles), one lane per
+// row, all 8 emitters of each K step. The step's emitter records and
+// visibility words are then the same for every lane: they are read through the
+// scalar cache (constant address space: the kernel never writes them) and the
+// vector units take them as scalar operands. Each lane stores its row's two
+// fragments (emitters 0-3, 4-7) into the MFMA A layout of its tile. Near pairs
+// (modes 1, 2) are left zero and listed for the near-field kernels.
+typedef const __attribute__((address_space(4))) float const_f32;
+typedef const __attribute__((address_space(4))) uint64_t const_u64;
+
 template <bool ATT, int NEAR>
 __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
                                                  uint4 *__restrict__ A, int ksPerWave) {
     __shared__ int2 stage[NEAR != 0 ? 4 : 1][2][NEAR != 0 ? kNearStage : 1];
     int2 *stQ = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][0], *stM = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][1];
     int nQ = 0, nM = 0;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // wave-uniform by construction; readfirstlane lets the compiler see it
+    const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     const int nKsChunks = (g.nKs + ksPerWave - 1) / ksPerWave;
-    const int rt = wave / nKsChunks;
+    const int rp64 = wave / nKsChunks;  // 64-row block of S
     const int kc = wave % nKsChunks;
-    if (rt >= g.nRt) return;
-    const int srow = rt * 32 + (lane & 31);  // row of S
-    const int h = lane >> 5;
+    if (rp64 >= g.nRt / 2) return;  // uniform (nRt is a multiple of 4)
+    const int srow = rp64 * 64 + lane;  // row of S
     const int row = g.rowAct ? g.rowAct[srow] : srow;  // its active cell (-1: padding)
     const bool rvalid = row >= 0 && row < g.n;
     const float4 rp = rvalid ? g.p[row] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 rn = rvalid ? g.nrm[row] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint64_t *mrow = mask + (size_t)(rvalid ? (row >> 6) : 0) * g.n;
-    const int bit = row & 63;
+    // The wave's rows are consecutive active cells (one facet's, or the identity
+    // map), so their visibility words come from two 64-row blocks at most: b0
+    // (lane 0's, or the first valid lane's) and b0 + 1.
     const int ks0 = kc * ksPerWave, ks1 = min(g.nKs, ks0 + ksPerWave);
-    const int2 skip = g.wgSkip ? g.wgSkip[rt >> 2] : make_int2(0, 0);  // never read by the GEMM
-    for (int ks = ks0; ks < ks1; ks++) {
-        if (ks >= skip.x && ks < skip.y) continue;  // wave-uniform
-        uint32_t w[4];
-        unsigned nearBits = 0, quadBits = 0;  // per q: listed near pair, by quadrature
-        // the step's emitter records and visibility words, all requested before any is used
-        float4 epq[4], enq[4];
-        uint64_t mq[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int jj = min(ks * 8 + h * 4 + q, g.n - 1);
-            mq[q] = mrow[jj];
-            epq[q] = g.p[jj];
-            enq[q] = g.nrm[jj];
+    const int2 skip = g.wgSkip ? g.wgSkip[rp64 >> 1] : make_int2(0, 0);  // never read by the GEMM
+    const int tile = srow >> 5, tl = srow & 31;
+    const uint64_t valid = __ballot(rvalid);
+    if (!valid) {  // padding rows only: zero fragments
+        for (int ks = ks0; ks < ks1; ks++) {
+            if (ks >= skip.x && ks < skip.y) continue;
+            uint4 *dst = A + ((size_t)tile * g.nKs + ks) * 64 + tl;
+            dst[0] = make_uint4(0u, 0u, 0u, 0u);
+            dst[32] = make_uint4(0u, 0u, 0u, 0u);
         }
+        return;
+    }
+    const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)valid) - 1);
+    const int b0 = __builtin_amdgcn_readfirstlane(__shfl(row, first) >> 6);
+    const int nrbM = (g.n + 63) >> 6;
+    const int b1 = min(b0 + 1, nrbM - 1);
+    const bool upper = rvalid && (row >> 6) != b0;  // this lane's words are block b0 + 1's
+    const int bit = row & 63;
+    const const_f32 *P = (const const_f32 *)g.p, *N = (const const_f32 *)g.nrm;  // float4 records
+    const const_u64 *M0 = (const const_u64 *)(mask + (size_t)b0 * g.n), *M1 = (const const_u64 *)(mask + (size_t)b1 * g.n);
+    for (int ks = ks0; ks < ks1; ks++) {
+        if (ks >= skip.x && ks < skip.y) continue;  // uniform
+        uint32_t w[8];
+        unsigned nearBits = 0, quadBits = 0;  // per emitter: listed near pair, by quadrature
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int j = ks * 8 + h * 4 + q;
-            float re = 0.f, im = 0.f;
-            bool isNear = false, isQuad = false;
-            if (rvalid && j < g.n && ((mq[q] >> bit) & 1ull)) {
-                const float4 ep = epq[q], en = enq[q];
-                float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
-                float r2 = vx * vx + vy * vy + vz * vz;
-                float rinv = __builtin_amdgcn_rsqf(r2);
-                float r = r2 * rinv;
-                float de = vx * en.x + vy * en.y + vz * en.z;
-                float dr = vx * rn.x + vy * rn.y + vz * rn.z;
-                float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
-                if (NEAR != 0 && r < ph.nearRadius) {
-                    // evaluated afterwards from the near lists (k_near_quad / k_near_mlp)
-                    isNear = true;
-                    isQuad = NEAR == 1 || r < ph.quadRadius;  // the learned range starts at r >= delta
-                } else if (r < ph.deltaReg) {
-                    re = wgt * (-ph.alpha * kInv4PiT);
-                    im = wgt * (ph.kr * kInv4PiT);
-                } else {
-                    // FP64 reduction of k r to one revolution, then FP32 sin/cos.
-                    double rev = (double)r * ph.krevD;
-                    float x = (float)(rev - rint(rev));
-                    float c = __builtin_amdgcn_cosf(x), s = __builtin_amdgcn_sinf(x);
-                    float a = rinv * kInv4PiT;
-                    if (ATT) a *= __builtin_amdgcn_exp2f(-ph.alphaLog2e * r);
-                    float Gre, Gim;
-                    if (ATT) {
-                        Gre = a * fmaf(ph.alpha, c, ph.kr * s) + rinv * rinv;
-                        Gim = -a * fmaf(ph.kr, c, -ph.alpha * s);
-                    } else {
-                        Gre = (a * ph.kr) * s + rinv * rinv;
-                        Gim = -(a * ph.kr) * c;
-                    }
-                    re = wgt * Gre;
-                    im = wgt * Gim;
-                }
+        for (int q = 0; q < 8; q++) {
+            const int j = ks * 8 + q;
+            const int jj = min(j, g.n - 1);
+            const float4 ep = make_float4(P[4 * jj], P[4 * jj + 1], P[4 * jj + 2], 0.f);
+            const float4 en = make_float4(N[4 * jj], N[4 * jj + 1], N[4 * jj + 2], 0.f);
+            const uint64_t w0 = M0[jj], w1 = M1[jj];
+            const uint64_t word = upper ? w1 : w0;
+            // every lane evaluates the far-field form and selects (the excluded
+            // cases' values, even non-finite ones at r = 0, are never selected)
+            const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
+            const float r2 = vx * vx + vy * vy + vz * vz;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            const float de = vx * en.x + vy * en.y + vz * en.z;
+            const float dr = vx * rn.x + vy * rn.y + vz * rn.z;
+            const float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
+            // FP64 reduction of k r to one revolution, then FP32 sin/cos.
+            const double rev = (double)r * ph.krevD;
+            const float x = (float)(rev - rint(rev));
+            const float c = __builtin_amdgcn_cosf(x), sn = __builtin_amdgcn_sinf(x);
+            float a = rinv * kInv4PiT;
+            if (ATT) a *= __builtin_amdgcn_exp2f(-ph.alphaLog2e * r);
+            float Gre, Gim;
+            if (ATT) {
+                Gre = a * fmaf(ph.alpha, c, ph.kr * sn) + rinv * rinv;
+                Gim = -a * fmaf(ph.kr, c, -ph.alpha * sn);
+            } else {
+                Gre = (a * ph.kr) * sn + rinv * rinv;
+                Gim = -(a * ph.kr) * c;
             }
+            const bool vis = rvalid && j < g.n && ((word >> bit) & 1ull);
+            // near pairs: evaluated afterwards from the near lists (k_near_quad / k_near_mlp)
+            const bool isNear = NEAR != 0 && vis && r < ph.nearRadius;
+            // the learned range starts at r >= delta
+            const bool isQuad = isNear && (NEAR == 1 || r < ph.quadRadius);
+            const bool reg = r < ph.deltaReg;
+            float re = reg ? wgt * (-ph.alpha * kInv4PiT) : wgt * Gre;
+            float im = reg ? wgt * (ph.kr * kInv4PiT) : wgt * Gim;
+            const bool keep = vis && !isNear;
+            re = keep ? re : 0.f;
+            im = keep ? im : 0.f;
             w[q] = pack_h2(re, im);
             nearBits |= (unsigned)isNear << q;
             quadBits |= (unsigned)isQuad << q;
@@ -376,14 +396,17 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
         // near pairs are rare: one ballot decides whether this step lists any
         if (NEAR != 0 && __ballot(nearBits != 0u)) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int j = ks * 8 + h * 4 + q;
+            for (int q = 0; q < 8; q++) {
+                const int j = ks * 8 + q;
                 const bool nr = (nearBits >> q) & 1u, qd = (quadBits >> q) & 1u;
                 near_stage(stQ, nQ, nr && qd, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
                 near_stage(stM, nM, nr && !qd, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
             }
         }
-        A[((size_t)rt * g.nKs + ks) * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);
+        // fragment h of row tl of the tile: emitters 4h .. 4h + 3
+        uint4 *dst = A + ((size_t)tile * g.nKs + ks) * 64 + tl;
+        dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        dst[32] = make_uint4(w[4], w[5], w[6], w[7]);
     }
     if (NEAR != 0) {
         near_flush(stQ, nQ, ph.nearQ, ph.nearCount, ph.nearCap);
@@ -567,26 +590,41 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     else if (kLast >= 0) store_part(kLast < kBnd ? t.partL : t.partU);
 }
 
-// One (plane, active row) of the gated reduction: sums the splits' lower- and
+// One (plane, row of S) of the gated reduction: sums the splits' lower- and
 // higher-facet partials, applies the zero-input gate and the column scale, stores
-// P_out and adds it into P. Returns the stored value (zero past the last row).
-__device__ inline float2 reduce_gate_row(const ReduceGateTask &t, int b, int r) {
-    if (r >= t.n) return make_float2(0.f, 0.f);
-    const int sr = t.actRow ? t.actRow[r] : r;
-    const unsigned char *cls = t.cls + (size_t)(sr >> 7) * t.nsplit;
+// P_out and adds it into P. Threads run over the rows of S (padding included) so
+// that the partials' addresses and the group's split classes (cls, uniform over
+// a wave: waves never straddle a 128-row group) need no lookup first; the active
+// row comes from rowAct alongside. Returns the stored value, zero for padding.
+__device__ inline float2 reduce_gate_row(const ReduceGateTask &t, int b, int sr, int &r) {
+    r = -1;
+    if (sr >= t.nPadRows) return make_float2(0.f, 0.f);
+    r = t.rowAct ? t.rowAct[sr] : sr;
+    const int grp = __builtin_amdgcn_readfirstlane(sr >> 7);
+    const unsigned char *cls = t.cls + (size_t)grp * t.nsplit;
     float2 L = make_float2(0.f, 0.f), U = make_float2(0.f, 0.f);
-    // both partials are read (in bounds either way) and the unwritten one is
-    // selected away, so the loads of all splits are in flight together
-#pragma unroll 4
+    // One load per split, unconditional so that the splits' loads are in flight
+    // together: its lower-facet partial if it wrote one, else its higher-facet one
+    // (an unwritten partial is in bounds and selected away). The split that spans
+    // the group's facet boundary wrote both: its higher-facet partial follows.
+#pragma unroll 8
     for (int s = 0; s < t.nsplit; s++) {
         const size_t o = ((size_t)s * 64 + b) * t.nPadRows + sr;
         const unsigned c = cls[s];
-        const float2 a = t.partL[o], u = t.partU[o];
-        L.x += (c & 1u) ? a.x : 0.f;
-        L.y += (c & 1u) ? a.y : 0.f;
-        U.x += (c & 2u) ? u.x : 0.f;
-        U.y += (c & 2u) ? u.y : 0.f;
+        const float2 v = ((c & 1u) ? t.partL : t.partU)[o];
+        const bool toL = (c & 1u) != 0u, toU = c == 2u;
+        L.x += toL ? v.x : 0.f;
+        L.y += toL ? v.y : 0.f;
+        U.x += toU ? v.x : 0.f;
+        U.y += toU ? v.y : 0.f;
     }
+    for (int s = 0; s < t.nsplit; s++)
+        if (cls[s] == 3u) {
+            const float2 u = t.partU[((size_t)s * 64 + b) * t.nPadRows + sr];
+            U.x += u.x;
+            U.y += u.y;
+        }
+    if (r < 0) return make_float2(0.f, 0.f);
     const float2 gv = t.gval[(size_t)b * t.n + r];
     const float g = (gv.x == 0.0f && gv.y == 0.0f) ? 0.0f : 1.0f;
     const float sc = t.colInvScale[b];
@@ -598,14 +636,15 @@ __device__ inline float2 reduce_gate_row(const ReduceGateTask &t, int b, int r) 
     return v;
 }
 
-// Gated reduction: a workgroup = 256 active rows of one plane (blockIdx.y).
+// Gated reduction: a workgroup = 256 rows of S for one plane (blockIdx.y).
 __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
-    const int b = blockIdx.y, r = blockIdx.x * 256 + threadIdx.x;
-    const float2 v = reduce_gate_row(t, b, r);
+    const int b = blockIdx.y;
+    int r;
+    const float2 v = reduce_gate_row(t, b, blockIdx.x * 256 + threadIdx.x, r);
     if (!t.colMaxNext) return;  // uniform
     __shared__ float red[4];
     float m = 0.f;
-    if (r < t.n) {
+    if (r >= 0) {
         const float a = t.actP[r].w;
         m = fmaxf(fabsf(a * v.x), fabsf(a * v.y));
     }
@@ -672,10 +711,11 @@ __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int 
 // ---------------------------------------------------------------------------
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st) {
-    // runs of 16 K steps per wave, shorter when that leaves fewer than ~16k waves
+    // runs of 16 K steps per wave (64 rows each), shorter when that leaves fewer than ~8k waves
+    if (g.nRt % 4) { set_error("S build: row tiles must be padded to 4"); return -2; }
     int ksPerWave = 16;
-    while (ksPerWave > 4 && (long)g.nRt * ((g.nKs + ksPerWave - 1) / ksPerWave) < 16384) ksPerWave /= 2;
-    const long waves = (long)g.nRt * ((g.nKs + ksPerWave - 1) / ksPerWave);
+    while (ksPerWave > 4 && (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave) < 8192) ksPerWave /= 2;
+    const long waves = (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave);
     const long blocks = (waves + 3) / 4;
     if (blocks <= 0) return 0;
     if (ph.nearMode != 0 && ph.alpha != 0.0f) {
@@ -741,7 +781,8 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
 int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st) {
     if (t.n <= 0 || t.nrhs <= 0) return 0;
     if (t.colMaxNext && !t.actP) { set_error("gated reduction: column maxima need the fragment weights"); return -2; }
-    k_reduce_gate<<<dim3((unsigned)((t.n + 255) / 256), (unsigned)t.nrhs), 256, 0, st>>>(t);
+    if (!t.rowAct && t.nPadRows != t.n) { set_error("gated reduction: padded rows need the row map"); return -2; }
+    k_reduce_gate<<<dim3((unsigned)((t.nPadRows + 255) / 256), (unsigned)t.nrhs), 256, 0, st>>>(t);
     SR_HIP(hipGetLastError());
     return 0;
 }
