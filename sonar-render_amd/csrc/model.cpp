@@ -760,7 +760,7 @@ static int ensure_transfer(Model &m, int i) {
         const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
             dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
-            dev_alloc(&t.gatedList, n) || dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes)) ||
+            dev_alloc(&t.gatedList, n) || dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes * kColStride)) ||
             dev_alloc(&t.rowFs, n))
             return -1;
         t.nKs = nKs;
@@ -920,7 +920,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
     // Bounce state, two sets used in turn: this bounce's set, zero or holding the
     // column maxima of P_in that the previous bounce's reduction found; the other
     // set is cleared here for the next bounce.
-    const int stWords = 1 + m.nplanes;
+    const int stWords = 1 + m.nplanes * kColStride;
     const bool haveMax = !first && t.gateInGemm && t.maxReady;
     t.maxReady = false;
     if (!haveMax && !t.stateClear) {  // first use, or a bounce that stopped before clearing the next set
@@ -946,7 +946,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
                 // with the maxima from the previous reduction, the first group's B build clears the next set
                 const bool clr = haveMax && g0 == 0;
                 ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
-                if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + g0, t.sS, t.colScale, t.colInv, t.B,
+                if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + (size_t)g0 * kColStride, t.sS, t.colScale, t.colInv, t.B,
                                      m.stream, clr ? nxt : nullptr, clr ? stWords : 0))
                     return -1;
             }
@@ -964,7 +964,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             // another bounce follows: the reduction also finds its column maxima (into the next set)
             ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.rowAct, pin,
                              o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
-                             o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + g0 : nullptr};
+                             o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + (size_t)g0 * kColStride : nullptr};
             ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
             if (transfer_reduce_gate(r, m.stream)) return -1;
         }
@@ -1018,7 +1018,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
         const float2 *pin = o.Pin + (size_t)g0 * n;
         {
             ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
-            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + g0, t.sS, t.colScale, t.colInv, t.B,
+            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + (size_t)g0 * kColStride, t.sS, t.colScale, t.colInv, t.B,
                                  m.stream))
                 return -1;
         }

@@ -45,7 +45,7 @@ struct Object {
         int nKs = 0, nRt = 0;
         int2 *wgSkip = nullptr;    // per 128-row group: zero K-step range (its own single facet)
         int *gatedList = nullptr;  // rows with a zero input in some plane
-        // bounce state, two sets used in turn: [0] gated-row count, [1 + b] max |A P_in| of plane b
+        // bounce state, two sets used in turn: [0] gated-row count, [1 + b kColStride] max |A P_in| of plane b
         // (float bits); each bounce clears the other set for the next
         int *state = nullptr;
         int stateSet = 0;

@@ -54,6 +54,10 @@ struct TransferGemm {
 // partials always and the higher-facet ones where the row's own input is not
 // zero (src/OptiX.cu:241-245); cls[group][split] bit 0/1 says which of the
 // split's partials exist. Applies the column scale, writes P_out and adds to P.
+// Words between consecutive planes' column maxima in a bounce state set: each
+// plane's atomics hit a cache line of their own.
+constexpr int kColStride = 32;
+
 struct ReduceGateTask {
     const float2 *partL, *partU;
     const unsigned char *cls;
@@ -75,7 +79,7 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
 // Writes the listed near pairs' values into S (counts read back from nearCount)
 // and clears nearCount for the next build.
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st);
-// Per-plane max |A_j P_in[b][j]| into colMax (float bits, zeroed beforehand);
+// Per-plane max |A_j P_in[b][j]| into colMax[b kColStride] (float bits, zeroed beforehand);
 // also zeroes nextWords ints at nextState (the next bounce's state set).
 int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int nplanes, unsigned *colMax,
                      int *nextState, int nextWords, hipStream_t st);

@@ -416,7 +416,7 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
 
 // Per-incidence scale: 2^13 / max(|Re X|, |Im X|) with X = A_j P_in[j,b].
 // Per-plane max |A_j P_in[b][j]| over the active cells (the FP16 scaling of
-// B), atomically into colMax[b] as float bits (non-negative floats order like
+// B), atomically into colMax[b * kColStride] as float bits (non-negative floats order like
 // their bit patterns). Grid (planes, row chunks). Block (0, 0) also clears the
 // other bounce-state set, which the next bounce uses.
 __global__ __launch_bounds__(256) void k_col_max(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(256) void k_col_max(const float4 *__restrict__ p, c
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        atomicMax(&colMax[b], __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+        atomicMax(&colMax[b * kColStride], __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
 }
 
 // The FP16 scale of plane b's B column: its largest entry becomes 8192.
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (clearState && t < clearWords) clearState[t] = 0;
     if (t < nplanes) {
-        const float sc = col_scale_of(colMax[t]);
+        const float sc = col_scale_of(colMax[t * kColStride]);
         scaleOut[t] = sc;
         invOut[t] = 1.0f / (sc * sS);
     }
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
         float xr = 0.f, xi = 0.f;
         if (j < n && b < nplanes) {
             float2 v = Pin[b * stride + j];
-            float a = p[j].w * col_scale_of(colMax[b]);
+            float a = p[j].w * col_scale_of(colMax[b * kColStride]);
             xr = a * v.x;
             xi = a * v.y;
         }
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
     __syncthreads();
     if (threadIdx.x == 0) {
         m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        if (m > 0.f) atomicMax(&t.colMaxNext[b], __float_as_uint(m));
+        if (m > 0.f) atomicMax(&t.colMaxNext[b * kColStride], __float_as_uint(m));
     }
 }
 
