@@ -42,6 +42,16 @@ __device__ inline float2 near_gauss2(float3 rv, float3 nr, float d, float k) {
     return make_float2(0.25f * acc.x, 0.25f * acc.y);
 }
 
+// Points 2 half and 2 half + 1 of the same rule, summed, unscaled: the two
+// halves' sum times 1/4 is near_gauss2 (up to the order of the additions).
+__device__ inline float2 near_gauss2_half(float3 rv, float3 nr, float d, float k, int half) {
+    const float o = 0.288675134594812882f * d;
+    const float oy = half ? o : -o;
+    const float2 a = near_kernel_at(rv.x + o, rv.y - oy, rv.z, nr, k);
+    const float2 b = near_kernel_at(rv.x - o, rv.y - oy, rv.z, nr, k);
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+
 // The network's inputs (lengths in any unit, lam the wavelength; u the unit
 // vector emitter -> receiver and nr the receiver normal, emitter frame).
 __device__ inline float plane_offset(float r, float d, float3 u, float3 nr) {

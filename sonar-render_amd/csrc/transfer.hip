@@ -155,6 +155,13 @@ __global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int kHP = 72;  // bf16 per activation row in LDS (64 + 8: 16-byte aligned rows)
 
+// tanh x = 1 - 2 / (e^{2x} + 1) on the hardware exp2 and reciprocal (a few ulp
+// in FP32, far below the BF16 rounding of the activations it feeds); +-1 at the
+// extremes (e^{2x} -> inf or 0), no cancellation issue at the BF16 scale.
+__device__ inline float tanh_hw(float x) {
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(x * 2.8853900817779268f) + 1.0f);
+}
+
 __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
                                                   int count, uint4 *__restrict__ A) {
     if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;  // the next build counts from zero
@@ -198,15 +205,18 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
         float2 K = make_float2(0.f, 0.f);
         float M = 0.f, cs_kr = 1.f, sn_kr = 0.f;
         __syncthreads();  // the previous tile's reads of H are done
-        if (h == 0) {
-            bf16x8 f0, f1;
+        // both halves of the wave take pair j: each evaluates two points of the
+        // base rule the network corrects; half 0 also the features and e^{ikr}
+        bf16x8 f0, f1;
 #pragma unroll
-            for (int q = 0; q < 8; q++) { f0[q] = (__bf16)0.0f; f1[q] = (__bf16)0.0f; }
-            if (valid) {
-                pr = list[i];
-                const NearGeom q = near_geom(g, ph, pr.x, pr.y);
+        for (int q = 0; q < 8; q++) { f0[q] = (__bf16)0.0f; f1[q] = (__bf16)0.0f; }
+        float2 Kh = make_float2(0.f, 0.f);
+        if (valid) {
+            pr = list[i];
+            const NearGeom q = near_geom(g, ph, pr.x, pr.y);
+            Kh = near_gauss2_half(q.rv, q.nr, d, ph.kr, h);
+            if (h == 0) {
                 const float r = q.r;
-                K = near_gauss2(q.rv, q.nr, d, ph.kr);  // the base rule the network corrects
                 M = 0.9f * (ph.kr * kInv4PiT / r + 1.0f / (r * r)) * (1.0f / kNfTargetGain);
                 sincosf(ph.kr * r, &sn_kr, &cs_kr);
                 float f[kMlpIn];
@@ -216,6 +226,9 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
 #pragma unroll
                 for (int qq = 8; qq < kMlpIn; qq++) f1[qq - 8] = (__bf16)f[qq];
             }
+        }
+        K = make_float2(0.25f * (Kh.x + __shfl_xor(Kh.x, 32)), 0.25f * (Kh.y + __shfl_xor(Kh.y, 32)));
+        if (h == 0) {
             *reinterpret_cast<bf16x8 *>(&H[j][0]) = f0;
             *reinterpret_cast<bf16x8 *>(&H[j][8]) = f1;
         }
@@ -233,7 +246,7 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int row = (e >> 2) * 8 + h * 4 + (e & 3), col = 32 * c + j;
-                H[row][col] = (__bf16)tanhf(acc[c][e] + sB1[col]);
+                H[row][col] = (__bf16)tanh_hw(acc[c][e] + sB1[col]);
             }
         __syncthreads();
         // layer 2: [32 x 64] x [64 x 64]
@@ -251,12 +264,17 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int row = (e >> 2) * 8 + h * 4 + (e & 3), col = 32 * c + j;
-                H[row][col] = (__bf16)tanhf(acc[c][e] + sB2[col]);
+                H[row][col] = (__bf16)tanh_hw(acc[c][e] + sB2[col]);
             }
         __syncthreads();
         // layer 3: lane (j, h) -> output h of pair j
         float o = sB3[h];
-        for (int k = 0; k < 64; k++) o = fmaf(sW3[h][k], (float)H[j][k], o);
+#pragma unroll
+        for (int k8 = 0; k8 < 8; k8++) {
+            const bf16x8 hv = *reinterpret_cast<const bf16x8 *>(&H[j][8 * k8]);
+#pragma unroll
+            for (int q = 0; q < 8; q++) o = fmaf(sW3[h][8 * k8 + q], (float)hv[q], o);
+        }
         const float o1 = __shfl(o, j + 32);
         if (h == 0 && valid) {
             // G2 + (M / g) t e^{ikr}
