@@ -773,39 +773,60 @@ static int ensure_transfer(Model &m, int i) {
                 fs[r] = (r > 0 && fa[r] == fa[r - 1]) ? fs[r - 1] : r;
             SR_HIP(hipMemcpy(t.rowFs, fs.data(), n * sizeof(int), hipMemcpyHostToDevice));
         }
-        // Per 128-row group: its facet range (single-facet groups only), the K chunks
-        // wholly inside it (same-facet pairs are zero: never read) and, with the gate
-        // inside the product, the first K step of the higher facets.
+        // Runs of consecutive facets whose normals are bitwise identical: a pair
+        // between two of them fails one of the two cosine culls (the cosines are
+        // exact negatives of each other), so S is zero there as within one facet.
+        std::vector<int2> run(fr.size());
+        for (size_t q = 0; q < fr.size();) {
+            const Vec3 &nq = o.frames[fa[fr[q].x]].normal;
+            size_t e = q + 1;
+            for (; e < fr.size(); e++) {
+                const Vec3 &ne = o.frames[fa[fr[e].x]].normal;
+                if (!(ne.x == nq.x && ne.y == nq.y && ne.z == nq.z)) break;
+            }
+            for (size_t k = q; k < e; k++) run[k] = make_int2(fr[q].x, fr[e - 1].y);
+            q = e;
+        }
+        // Per 128-row group: its facet range (single-facet groups only) and that
+        // facet's run, the K chunks wholly inside the run (zero: never read) and,
+        // with the gate inside the product, the first K step of the higher facets.
         std::vector<int> rowAct(gateIn ? (size_t)nRt * 32 : 0, -1), actRow(gateIn ? n : 0);
-        std::vector<int2> grp(nWg, make_int2(-1, -1));
+        std::vector<int2> grp(nWg, make_int2(-1, -1)), grpRun(nWg, make_int2(-1, -1));
         if (gateIn) {
             int base = 0;
-            for (auto &f : fr) {
+            for (size_t q = 0; q < fr.size(); q++) {
+                const int2 f = fr[q];
                 for (int r = f.x; r < f.y; r++) {
                     rowAct[base + (r - f.x)] = r;
                     actRow[r] = base + (r - f.x);
                 }
-                for (int w = base / 128; w < (base + (f.y - f.x) + 127) / 128; w++) grp[w] = f;
+                for (int w = base / 128; w < (base + (f.y - f.x) + 127) / 128; w++) {
+                    grp[w] = f;
+                    grpRun[w] = run[q];
+                }
                 base += ((f.y - f.x) + 127) / 128 * 128;
             }
         } else {
             for (int w = 0; w < nWg; w++) {
                 const int r0 = std::min(n - 1, w * 128), r1 = std::min(n - 1, w * 128 + 127);
-                if (fa[r0] == fa[r1])
-                    for (auto &f : fr)
-                        if (r0 >= f.x && r0 < f.y) grp[w] = f;
+                for (size_t q = 0; q < fr.size(); q++) {
+                    if (r0 >= fr[q].x && r0 < fr[q].y && fa[r0] == fa[r1]) grp[w] = fr[q];
+                    // rows of several facets of one run share its zero chunks too
+                    if (r0 >= run[q].x && r0 < run[q].y && r1 < run[q].y) grpRun[w] = run[q];
+                }
             }
         }
         std::vector<int2> skip(nWg, make_int2(0, 0));
         std::vector<int> kB(nWg, nKs);
         double steps = 0;
         for (int w = 0; w < nWg; w++) {
-            if (grp[w].x >= 0) {
-                const int fs = grp[w].x, fe = grp[w].y;
-                const int cLo = ((fs + 7) / 8 + 7) / 8, cHi = (fe / 8) / 8;  // chunks wholly inside [fs, fe)
+            if (grpRun[w].x >= 0) {
+                const int rs = grpRun[w].x, re = grpRun[w].y;
+                const int cLo = ((rs + 7) / 8 + 7) / 8, cHi = (re / 8) / 8;  // chunks wholly inside [rs, re)
                 if (cHi > cLo) skip[w] = make_int2(cLo * 8, cHi * 8);
-                kB[w] = cHi * 8;  // this chunk holds facet f's tail (zero) and higher facets
             }
+            // the chunk holding the group's facet's tail (zero) and the higher facets' start
+            if (grp[w].x >= 0) kB[w] = ((grp[w].y / 8) / 8) * 8;
             steps += 4.0 * (nKs - (skip[w].y - skip[w].x));
         }
         SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
@@ -996,7 +1017,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             t.acCap = need; t.skipZeroCap = nRtC / 4;
         }
         ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cs);
-        if (transfer_gather_rows(t.A, t.nKs, t.gatedList, t.rowFs, gated, nRtC, t.Ac, cs)) return -1;
+        if (transfer_gather_rows(t.A, t.nKs, t.gatedList, t.rowFs, t.wgSkip, gated, nRtC, t.Ac, cs)) return -1;
         // few rows: split K finer (down to single chunks, at most 48 splits: the
         // replacement sums every split's partial) so the grid still fills the chip
         nsplitC = std::max(1, std::min({1024 / (nRtC / 4), t.nKs / 8, 48}));

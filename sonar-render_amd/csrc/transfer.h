@@ -92,9 +92,10 @@ int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
 int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st);
 // Gated rows as compact row tiles of the transfer matrix (nRtC tiles of 32,
-// a multiple of 4): emitters at or above row r's facet bound rowFs[r] zeroed.
-int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, int count, int nRtC, uint4 *Ac,
-                         hipStream_t st, const int *actRow = nullptr);
+// a multiple of 4): emitters at or above row r's facet bound rowFs[r] zeroed, and
+// the K chunks S never holds (wgSkip of the row's 128-row group) written as zeros.
+int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, const int2 *wgSkip, int count,
+                         int nRtC, uint4 *Ac, hipStream_t st, const int *actRow = nullptr);
 // Lists the rows with an exactly zero input pressure in some plane (count: device int, zeroed by the caller).
 int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st);
 

@@ -703,8 +703,8 @@ __global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes,
 // goes through k_transfer_gemm with the bounce's B: the gated rows see exactly
 // the operator the other rows see, near-field terms included.
 __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int *__restrict__ list,
-                                const int *__restrict__ rowFs, int count, int nRtC, uint4 *__restrict__ Ac,
-                                const int *__restrict__ actRow) {
+                                const int *__restrict__ rowFs, const int2 *__restrict__ wgSkip, int count, int nRtC,
+                                uint4 *__restrict__ Ac, const int *__restrict__ actRow) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nRtC * nKs * 64) return;
     const int lane = (int)(t & 63);
@@ -715,8 +715,9 @@ __global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int 
     if (i < count) {
         const int r = list[i], fs = rowFs[r];
         const int j0 = ks * 8 + h * 4;  // first emitter of this fragment
-        if (j0 < fs) {
-            const int sr = actRow ? actRow[r] : r;  // the row of S
+        const int sr = actRow ? actRow[r] : r;  // the row of S
+        const int2 sk = wgSkip ? wgSkip[sr >> 7] : make_int2(0, 0);
+        if (j0 < fs && !(ks >= sk.x && ks < sk.y)) {  // chunks S never holds are zero
             q = A[((size_t)(sr >> 5) * nKs + ks) * 64 + (sr & 31) + 32 * h];
             if (j0 + 1 >= fs) q.y = 0u;
             if (j0 + 2 >= fs) q.z = 0u;
@@ -805,11 +806,12 @@ int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st) {
     return 0;
 }
 
-int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, int count, int nRtC, uint4 *Ac,
-                         hipStream_t st, const int *actRow) {
+int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, const int2 *wgSkip, int count,
+                         int nRtC, uint4 *Ac, hipStream_t st, const int *actRow) {
     const long tot = (long)nRtC * nKs * 64;
     if (tot <= 0) return 0;
-    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, rowFs, count, nRtC, Ac, actRow);
+    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, rowFs, wgSkip, count, nRtC, Ac,
+                                                                   actRow);
     SR_HIP(hipGetLastError());
     return 0;
 }
