@@ -118,12 +118,13 @@ def roofline(stats, mode, workload):
     if mode == 0:
         st = stats["f2f"]
         ach = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
-        # achieved counts the conventional 8 flops per complex multiply-accumulate; the
-        # kernel issues 6 (three real products, Gauss) on the matrix cores
+        # achieved counts the conventional 8 flops per complex multiply-accumulate the kernel
+        # executed (it skips emitter pairs that no receiver of a wave sees; the count comes
+        # from the kernel itself); it issues 6 (three real products, Gauss) on the matrix cores
         return {"bound": "mfma", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
                 "mfma_issued_tflops": round(ach * 0.75, 3),
-                "kernel": "k_f2f_mfma (complex products on FP32 MFMA 32x32x2 as three real products; 8 flops per pair and plane counted, Green's function on the VALU)",
+                "kernel": "k_f2f_mfma (complex products on FP32 MFMA 32x32x2 as three real products; 8 flops per executed pair and plane counted, Green's function on the VALU)",
                 "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4), "launches": st["launches"]}
     st = stats["gemm"]
     n = max(1, st["launches"])

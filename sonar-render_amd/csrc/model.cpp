@@ -166,6 +166,16 @@ struct ProfScope {
     double pairs, flops, bytes;
     hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
+    int workSlot = -1;
+    double flopsPerStep = 0.0;
+    // A counter the launch adds its executed work to (profiling only, null otherwise);
+    // the record's flops then come from it at flopsPerStep each.
+    unsigned long long *work(double perStep) {
+        if (!a || !m.workCnt || m.workUsed >= m.workCap) return nullptr;
+        workSlot = m.workUsed++;
+        flopsPerStep = perStep;
+        return m.workCnt + workSlot;
+    }
     ProfScope(Model &mm, int k, double p, double f, double by = 0.0, hipStream_t st = nullptr)
         : m(mm), kind(k), pairs(p), flops(f), bytes(by), s(st ? st : mm.stream) {
         if (!m.profiling) return;
@@ -175,7 +185,7 @@ struct ProfScope {
     ~ProfScope() {
         if (!a) return;
         (void)hipEventRecord(b, s);
-        m.recs.push_back(Model::Rec{kind, a, b, pairs, flops, bytes});
+        m.recs.push_back(Model::Rec{kind, a, b, pairs, flops, bytes, workSlot, flopsPerStep});
     }
 };
 
@@ -568,8 +578,11 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         g.partL = m.scratchL;
         g.partU = nullptr;
         {
-            // flops: the complex multiply-accumulates, 8 per pair and plane, on the matrix cores
+            // flops: the complex multiply-accumulates executed on the matrix cores, 8 per
+            // (pair, plane), counted by the kernel (emitter pairs no receiver of a wave sees are
+            // skipped); without a counter, every pair
             ProfScope ps(m, PK_F2F, pairs * nb, pairs * 8.0 * nb);
+            g.workCount = ps.work(2.0 * 32.0 * 8.0 * nb);
             if (f2f_mfma_launch(g, ph, KIND_F2F, m.stream)) return -1;
         }
         ReduceTask r = rt;
@@ -1316,6 +1329,15 @@ int model_set_transfer_mode(Model &m, int mode) {
 void model_profile(Model &m, int on) {
     model_profile_reset(m);
     m.profiling = on != 0;
+    if (m.profiling && !m.workCnt) {  // kept for the life of the process once profiling was used
+        if (hipMalloc((void **)&m.workCnt, 4096 * sizeof(unsigned long long)) == hipSuccess) m.workCap = 4096;
+        else m.workCnt = nullptr;
+    }
+    if (m.workCnt) {  // the library's stream orders it before the counting launches
+        (void)hipMemsetAsync(m.workCnt, 0, m.workCap * sizeof(unsigned long long), m.stream);
+        if (m.stream) (void)hipStreamSynchronize(m.stream);
+    }
+    m.workUsed = 0;
 }
 
 void model_profile_reset(Model &m) {
@@ -1323,6 +1345,11 @@ void model_profile_reset(Model &m) {
     if (m.side) (void)hipStreamSynchronize(m.side);
     for (auto &r : m.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     m.recs.clear();
+    if (m.workCnt) {
+        (void)hipMemsetAsync(m.workCnt, 0, m.workCap * sizeof(unsigned long long), m.stream);
+        if (m.stream) (void)hipStreamSynchronize(m.stream);
+    }
+    m.workUsed = 0;
 }
 
 int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes) {
@@ -1330,11 +1357,15 @@ int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *p
     if (m.side) SR_HIP(hipStreamSynchronize(m.side));
     double t = 0, p = 0, f = 0, b = 0;
     long n = 0;
+    std::vector<unsigned long long> work(m.workUsed);
+    if (m.workUsed)
+        SR_HIP(hipMemcpy(work.data(), m.workCnt, m.workUsed * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (auto &r : m.recs) {
         if (r.kind != kind) continue;
         float e = 0.f;
         SR_HIP(hipEventElapsedTime(&e, r.a, r.b));
-        t += e; p += r.pairs; f += r.flops; b += r.bytes; n++;
+        const double fl = r.workSlot >= 0 ? (double)work[r.workSlot] * r.flopsPerStep : r.flops;
+        t += e; p += r.pairs; f += fl; b += r.bytes; n++;
     }
     *ms = t; *launches = n; *pairs = p; *flops = f;
     if (bytes) *bytes = b;

@@ -142,8 +142,11 @@ struct Model {
     bool haveLastGemm = false;
     // per-launch profiling (HIP events on the library's stream)
     bool profiling = false;
-    struct Rec { int kind; hipEvent_t a, b; double pairs, flops, bytes; };
+    // workSlot >= 0: flops = (executed steps counted in workCnt[workSlot]) x flopsPerStep
+    struct Rec { int kind; hipEvent_t a, b; double pairs, flops, bytes; int workSlot; double flopsPerStep; };
     std::vector<Rec> recs;
+    unsigned long long *workCnt = nullptr;  // device counters of executed matrix-core work (profiling)
+    int workCap = 0, workUsed = 0;
 };
 
 // Kernel families reported by the profiler.

@@ -53,6 +53,9 @@ struct GatherTask {
     // outputs: partial sums [split][NB][nr]
     float2 *partL;          // lower / ungated terms (F2P: high part)
     float2 *partU;          // gated terms (F2P: compensation)
+    // optional (profiling, matrix-core kernels): += the emitter-pair steps executed
+    // (each 2 emitters x 32 receivers x the launch's planes)
+    unsigned long long *workCount;
 };
 
 struct ReduceTask {

@@ -293,6 +293,7 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
     // The two tile buffers are separate LDS objects and every phase names them
     // statically, so the compiler sees that reads of one never wait on copies
     // into the other.
+    unsigned steps = 0;  // emitter-pair steps executed (wave-uniform)
     auto compute = [&](const MfmaTile &T, int ne) {
         // emitter pairs with any visible receiver of this wave: lane q < kMT/2 tests pair q
         bool act = false;
@@ -302,6 +303,7 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
             act = ((w >> ((wave & 1) * 32)) & 0xffffffffull) != 0ull;
         }
         uint64_t pairs = __ballot(act);
+        steps += (unsigned)__popcll(pairs);
         while (pairs) {  // a straight-line body keeps the accumulators in place
             const int k = 2 * __builtin_ctzll(pairs);
             pairs &= pairs - 1;
@@ -340,6 +342,7 @@ __global__ __launch_bounds__(256, 3) void k_f2f_mfma(GatherTask t, Phys ph) {
         compute(TB, min(kMT, e1 - base));
         base += kMT;
     }
+    if (t.workCount && lane == 0 && steps) atomicAdd(t.workCount, (unsigned long long)steps);
     if (!waveActive) return;
     // column j of tiles 0/2/4 is plane j, of tiles 1/3/5 plane 32 + j
 #pragma unroll
