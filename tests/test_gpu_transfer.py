@@ -135,3 +135,29 @@ def test_mixed_two_plane_groups_repeated_scenes(api):
             print(nb, "bounces", key, "vs fresh model rel-L2", d)
             assert d < 1e-6
     api.TearDownCuda()
+
+
+def test_mixed_dihedral_parallel_facet_runs(api):
+    """Two square plates at right angles, each two triangles with identical normals:
+    pairs within a plate always fail one cosine cull, so the product skips the K chunks
+    of the whole plate (not only of the row's own triangle), with the zero-input gate
+    inside the product. One incidence leaves a plate unlit. Exact against the oracle
+    to the transfer form's bound, and the plain form to its own."""
+    o = [0.0, 0.0, 0.0]
+    tris = np.array([[o, [1, 0, 0], [1, 1, 0]], [o, [1, 1, 0], [0, 1, 0]],
+                     [o, [0, 1, 0], [0, 1, 1]], [o, [0, 1, 1], [0, 0, 1]]], dtype=np.float32)
+    src = np.array([[6.0, 4.0, 8.0], [8.0, 3.0, 3.0], [-4.0, 5.0, 6.0]], np.float32)
+    lam = 1480.0 / 5e3
+    sc = dict(name="dihedral_4_facets", f=5e3, c=1480.0, atten=0.0, targets=[(tris, lam / 7.5)], fields=[],
+              sources=src, points=src.copy(), bounces=2, batched=True)
+    _banner("mixed form: dihedral of two-triangle plates, 5 kHz, 3 incidences, 2 bounces")
+    out = runner.run_scene(sc, transfer_mode=1)
+    plain = runner.run_scene(sc, transfer_mode=0)
+    for p in range(len(src)):
+        ref = oracle.run_scene(sc, plane_source=p)
+        a = np.concatenate([out["points"][p], out["targets"][0][p]])
+        b = np.concatenate([ref["points"], ref["targets"][0]])
+        c = np.concatenate([plain["points"][p], plain["targets"][0][p]])
+        print("plane", p, "mixed", rel_l2(a, b), "plain", rel_l2(c, b))
+        assert rel_l2(a, b) < TOL_MIXED
+        assert rel_l2(c, b) < 2e-4
