@@ -844,9 +844,20 @@ static int ensure_transfer(Model &m, int i) {
         }
         SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
         t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
-        // split K (whole chunks of 8 steps, at least 32 steps a split) so that the
-        // grid holds ~1024 workgroups
-        int nsplit = std::max(1, std::min(1024 / std::max(1, nWg), nKs / 32));
+        // Split K in whole chunks of 8 steps, at least 32 steps a split. The product
+        // holds two workgroups per CU (64 KiB of LDS each), so the grid runs in rounds
+        // of 2 x CUs workgroups: pick the split count with the least estimated time,
+        // rounds x (chunks per split + ~2 chunks of start-up and partial-sum store),
+        // which keeps the last round from running a few workgroups alone.
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
+        const long slots = 2L * cus, chunks = (nKs + 7) / 8;
+        int nsplit = 1;
+        double best = 1e300;
+        for (int sp = 1; sp <= std::max(1, nKs / 32); sp++) {
+            const double est = (double)((nWg * (long)sp + slots - 1) / slots) * ((chunks + sp - 1) / sp + 2);
+            if (est < best) { best = est; nsplit = sp; }
+        }
         t.ksPer = ((nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
         t.nsplit = (nKs + t.ksPer - 1) / t.ksPer;
         t.partRows = (double)t.nsplit * n;  // every split writes every row

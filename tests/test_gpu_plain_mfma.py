@@ -110,3 +110,30 @@ def test_field_points_on_matrix_cores(api):
         ep = rel_l2(mf["points"][p], ref["points"])
         print("plane", p, "field points vs oracle", ep)
         assert ep < 1e-4
+
+
+def test_plain_repeated_scenes_with_side_work(api):
+    """Scenes back to back on one model (queued with no host wait between them), the
+    visibility words rebuilt every scene, the gated rows' compact sets on the second
+    stream beside each bounce's main pass. Every scene equals the first bit for bit."""
+    sc = scenes.config_trihedral(f=5e3, n_side=4, bounces=2)
+    src = sc["sources"].copy()
+    src[::2, 0] *= -1.0  # half the incidences leave a plate unlit: gated rows every bounce
+    sc["sources"] = src.astype(np.float32)
+    sc["points"] = sc["sources"].copy()
+    _banner("plain form: 4 scenes back to back, visibility rebuilt each scene")
+    runner.load_scene(sc, transfer_mode=0)
+    outs = []
+    for _ in range(4):
+        api.reset_pressures()
+        api.clear_vis_cache()
+        runner.project_scene(sc)
+        outs.append(runner.read_results(sc))
+    api.TearDownCuda()
+    for k in range(1, 4):
+        assert np.array_equal(outs[k]["points"], outs[0]["points"])
+        assert np.array_equal(outs[k]["targets"][0], outs[0]["targets"][0])
+    ref = oracle.run_scene(sc, plane_source=0)
+    e = rel_l2(_flat(outs[3], 0), np.concatenate([ref["points"], ref["targets"][0]]))
+    print("last scene, plane 0 vs oracle", e)
+    assert e < 2e-4

@@ -101,7 +101,8 @@ def test_mixed_two_plane_groups_repeated_scenes(api):
     incidences leave plates unlit. Each bounce's reduction also finds the column
     scales of the next bounce's input, per group. The same model then renders
     scenes of 3, 1 and 2 bounces back to back (queued with no host wait between
-    them), each equal to that scene on a fresh model."""
+    them, the visibility words rebuilt each time), each equal to that scene on a fresh
+    model."""
     sc = scenes.config_trihedral(f=5e3, n_side=9, bounces=3)
     unlit = np.array([[-6.0, 4.0, 13.0], [5.0, -3.0, 12.0], [4.0, 5.0, 13.0], [3.0, 4.0, -0.5]], np.float32)
     sc["sources"] = np.concatenate([sc["sources"][:76], unlit])
@@ -126,6 +127,7 @@ def test_mixed_two_plane_groups_repeated_scenes(api):
     runner.load_scene(sc, transfer_mode=1)
     for nb in (3, 1, 2):
         api.reset_pressures()
+        api.clear_vis_cache()  # fresh visibility words and transfer matrix each scene
         runner.project_scene(dict(sc, bounces=nb))
         out = runner.read_results(sc)
         for key in ("points", "targets"):
