@@ -48,8 +48,8 @@ TOL = 1e-3
 # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (FETCH_SIZE x 2 + WRITE_SIZE,
 # the factor 2 from a 1 GiB copy calibration), per workload; the figures and commands are in
 # profiles/transfer_gemm_hbm_bytes.md.
-PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 858.2e6,
-               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 113.4e6}
+PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 823.8e6,
+               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 108.9e6}
 
 
 def _configs():
