@@ -135,7 +135,7 @@ int upload_object(Model &m, Object &o) {
 void free_transfer(Object &o) {
     auto &t = o.tc;
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
-    dev_free(t.gatedList); dev_free(t.state); dev_free(t.rowFs); dev_free(t.Ac); dev_free(t.skipZero);
+    dev_free(t.state); dev_free(t.rowFs);
     dev_free(t.rowAct); dev_free(t.actRow); dev_free(t.kB); dev_free(t.cls);
     dev_free(t.nearQ); dev_free(t.nearM); dev_free(t.nearCount);
     t = Object::Transfer{};
@@ -773,7 +773,7 @@ static int ensure_transfer(Model &m, int i) {
         const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
             dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
-            dev_alloc(&t.gatedList, n) || dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes * kColStride)) ||
+            dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes * kColStride)) ||
             dev_alloc(&t.rowFs, n))
             return -1;
         t.nKs = nKs;
@@ -860,7 +860,6 @@ static int ensure_transfer(Model &m, int i) {
         }
         t.ksPer = ((nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
         t.nsplit = (nKs + t.ksPer - 1) / t.ksPer;
-        t.partRows = (double)t.nsplit * n;  // every split writes every row
         if (gateIn) {
             // which partials each (group, split) writes, by the product's own walk over K
             std::vector<unsigned char> cls((size_t)nWg * t.nsplit, 0);
@@ -885,6 +884,12 @@ static int ensure_transfer(Model &m, int i) {
             SR_HIP(hipMemcpy(t.rowAct, rowAct.data(), rowAct.size() * sizeof(int), hipMemcpyHostToDevice));
             SR_HIP(hipMemcpy(t.actRow, actRow.data(), n * sizeof(int), hipMemcpyHostToDevice));
             SR_HIP(hipMemcpy(t.kB, kB.data(), nWg * sizeof(int), hipMemcpyHostToDevice));
+            SR_HIP(hipMemcpy(t.cls, cls.data(), cls.size(), hipMemcpyHostToDevice));
+        } else {
+            // two accumulators: every split writes both partials of every row
+            t.partRows = 2.0 * t.nsplit * n;
+            const std::vector<unsigned char> cls((size_t)nWg * t.nsplit, 3);
+            if (dev_alloc(&t.cls, cls.size())) return -1;
             SR_HIP(hipMemcpy(t.cls, cls.data(), cls.size(), hipMemcpyHostToDevice));
         }
     }
@@ -952,9 +957,12 @@ static int ensure_transfer(Model &m, int i) {
     return 0;
 }
 
-// One bounce of target i in the transfer form: P_out = S x for every row, then
-// the rows whose own P_in is zero in some plane get the product of their
-// lower-facet part of S (the zero-input gate) for those planes, P += P_out.
+// One bounce of target i in the transfer form: P_out = S x for every row, with
+// the zero-input gate (a row whose own P_in is zero in a plane keeps only its
+// lower-facet terms there) applied by the reduction to the product's two
+// partial sums, P += P_out. The product splits the terms at each 128-row
+// group's facet boundary (rows padded to single-facet groups) or, for any
+// other facet layout, with two accumulators per row.
 // first: the first bounce of a projection; more: another bounce of this target
 // follows in the same projection (its P_in is this bounce's P_out).
 static int bounce_mixed(Model &m, int i, bool first, bool more) {
@@ -966,7 +974,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
     // column maxima of P_in that the previous bounce's reduction found; the other
     // set is cleared here for the next bounce.
     const int stWords = 1 + m.nplanes * kColStride;
-    const bool haveMax = !first && t.gateInGemm && t.maxReady;
+    const bool haveMax = !first && t.maxReady;
     t.maxReady = false;
     if (!haveMax && !t.stateClear) {  // first use, or a bounce that stopped before clearing the next set
         if (counters_zero_launch(t.state, 2 * stWords, m.stream)) return -1;
@@ -976,99 +984,24 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
     t.stateSet ^= 1;
     t.stateClear = false;  // until this bounce has cleared the next set
     unsigned *colMax = reinterpret_cast<unsigned *>(cur + 1);
-    if (t.gateInGemm) {
-        // Every 128-row group is single-facet: each split's partial sums are split at
-        // the group's higher-facet boundary and the reduction applies the gate.
-        if (!haveMax) {
-            ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
-            if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
-        }
-        if (ensure_scratch(m, (size_t)nsplit * 64 * t.nPad)) return -1;
-        for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
-            const int np = std::min(64, m.nplanes - g0);
-            const float2 *pin = o.Pin + (size_t)g0 * n;
-            {
-                // with the maxima from the previous reduction, the first group's B build clears the next set
-                const bool clr = haveMax && g0 == 0;
-                ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
-                if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + (size_t)g0 * kColStride, t.sS, t.colScale, t.colInv, t.B,
-                                     m.stream, clr ? nxt : nullptr, clr ? stWords : 0))
-                    return -1;
-            }
-            TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL, 0, t.kB,
-                            m.scratchU};
-            m.lastGemm = gm;
-            m.haveLastGemm = true;
-            {
-                const double pairs = ((double)n * n - o.sameFacetPairs) * np;
-                const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
-                const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + t.partRows * np * 8;
-                ProfScope ps(m, PK_GEMM, pairs, fl, by);
-                if (transfer_gemm(gm, m.stream)) return -1;
-            }
-            // another bounce follows: the reduction also finds its column maxima (into the next set)
-            ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.rowAct, pin,
-                             o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
-                             o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + (size_t)g0 * kColStride : nullptr};
-            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
-            if (transfer_reduce_gate(r, m.stream)) return -1;
-        }
-        if (more) t.maxReady = true;
-        else t.stateClear = true;
-        return 0;
-    }
-    // rows with a zero input in some plane (the gate), counted before the product
-    // and read back while the first plane group's product runs
-    if (transfer_gated_rows(o.Pin, n, m.nplanes, t.gatedList, cur, m.stream)) return -1;
-    if (word_readback(m, cur, 0)) return -1;
-    {
+    if (!haveMax) {
         ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
         if (transfer_col_max(o.actP, o.Pin, n, n, m.nplanes, colMax, nxt, stWords, m.stream)) return -1;
     }
-    // partials of the main product, and of the compact set (split count x rows <= 1024 x 128 + n + 128)
-    if (ensure_scratch(m, std::max((size_t)nsplit * n, (size_t)1024 * 128 + n + 128) * 64)) return -1;
-    int gated = -1;  // not yet known
-    int nRtC = 0, nsplitC = 1, ksPerC = t.nKs;
-    // the gated rows as compact row tiles of S (lower-facet emitters only)
-    auto compact_set = [&](hipStream_t cs) -> int {
-        nRtC = ((gated + 127) / 128) * 4;
-        const size_t need = (size_t)nRtC * t.nKs * 64;
-        if (t.acCap < need || t.skipZeroCap < nRtC / 4) {
-            dev_free(t.Ac); dev_free(t.skipZero);
-            t.acCap = 0; t.skipZeroCap = 0;
-            if (dev_alloc(&t.Ac, need) || dev_alloc(&t.skipZero, nRtC / 4)) return -1;
-            SR_HIP(hipMemset(t.skipZero, 0, (nRtC / 4) * sizeof(int2)));
-            t.acCap = need; t.skipZeroCap = nRtC / 4;
-        }
-        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cs);
-        if (transfer_gather_rows(t.A, t.nKs, t.gatedList, t.rowFs, t.wgSkip, gated, nRtC, t.Ac, cs)) return -1;
-        // few rows: split K finer (down to single chunks, at most 48 splits: the
-        // replacement sums every split's partial) so the grid still fills the chip
-        nsplitC = std::max(1, std::min({1024 / (nRtC / 4), t.nKs / 8, 48}));
-        ksPerC = ((t.nKs + nsplitC - 1) / nsplitC + 7) / 8 * 8;
-        nsplitC = (t.nKs + ksPerC - 1) / ksPerC;
-        if ((size_t)nsplitC * 64 * gated > (cs == m.side ? m.scratchCElems : m.scratchElems)) {
-            set_error("transfer bounce: scratch bound");
-            return -2;
-        }
-        return 0;
-    };
-    // One plane group: the compact set runs on the side stream, concurrently with the
-    // main product (it needs only B and the gated list); more groups share B, so
-    // their compact sets run in line on the main stream.
-    const bool overlap = m.nplanes <= 64;
-    if (overlap && ensure_scratch_c(m, ((size_t)1024 * 128 + n + 128) * 64)) return -1;
+    if (ensure_scratch(m, (size_t)nsplit * 64 * t.nPad)) return -1;  // partL and partU
     for (int g0 = 0; g0 < m.nplanes; g0 += 64) {
         const int np = std::min(64, m.nplanes - g0);
         const float2 *pin = o.Pin + (size_t)g0 * n;
         {
+            // with the maxima from the previous reduction, the first group's B build clears the next set
+            const bool clr = haveMax && g0 == 0;
             ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
-            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + (size_t)g0 * kColStride, t.sS, t.colScale, t.colInv, t.B,
-                                 m.stream))
+            if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + (size_t)g0 * kColStride, t.sS, t.colScale,
+                                 t.colInv, t.B, m.stream, clr ? nxt : nullptr, clr ? stWords : 0))
                 return -1;
         }
-        if (overlap && side_mark(m)) return -1;
-        TransferGemm gm{t.A, t.B, n, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL};
+        TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL,
+                        t.gateInGemm ? t.kB : nullptr, m.scratchU, t.gateInGemm ? nullptr : t.rowFs};
         m.lastGemm = gm;
         m.haveLastGemm = true;
         {
@@ -1076,50 +1009,19 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             // flops of the GEMM as executed: 2 * rows * 128 columns * 2 * K (padded)
             const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
             // algorithmic HBM bytes: S read once, B read once, partial sums written
-            const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + (double)nsplit * np * n * 8;
+            const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + t.partRows * np * 8;
             ProfScope ps(m, PK_GEMM, pairs, fl, by);
             if (transfer_gemm(gm, m.stream)) return -1;
         }
-        ReduceTask r{};
-        r.partL = m.scratchL;
-        r.partU = nullptr;
-        r.nsplit = nsplit;
-        r.nbStride = 64;
-        r.nrhs = np;
-        r.nr = n;
-        r.gateMode = GATE_NONE;
-        r.out = o.Pout + (size_t)g0 * n;
-        r.outStride = n;
-        r.pcell = o.P + (size_t)g0 * o.ncell;
-        r.pcellStride = o.ncell;
-        r.rowCell = o.actCell;
-        r.colInvScale = t.colInv;
-        {
-            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
-            if (reduce_launch(r, m.stream)) return -1;
-        }
-        if (gated < 0) {  // the count has long landed by now; the product above keeps the GPU busy
-            if (word_wait(m, 0, gated)) return -1;
-            if (gated) {
-                if (overlap && side_follow(m)) return -1;
-                if (compact_set(overlap ? m.side : m.stream)) return -1;
-            }
-        }
-        if (!gated) continue;
-        hipStream_t cs = overlap ? m.side : m.stream;
-        float2 *cpart = overlap ? m.scratchC : m.scratchL;
-        TransferGemm gc{t.Ac, t.B, gated, t.nKs, nRtC, np, nsplitC, ksPerC, t.skipZero, cpart, 1};
-        {
-            ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0, 0.0, cs);
-            if (transfer_gemm(gc, cs)) return -1;
-        }
-        if (overlap && side_join(m)) return -1;
-        ProfScope ps(m, PK_GATED_FIX, 0.0, 0.0);
-        if (gated_replace_launch(cpart, nsplitC, gated, t.gatedList, pin, n, np, o.Pout + (size_t)g0 * n,
-                                 o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv, m.stream))
-            return -1;
+        // another bounce follows: the reduction also finds its column maxima (into the next set)
+        ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.rowAct, pin,
+                         o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
+                         o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + (size_t)g0 * kColStride : nullptr};
+        ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
+        if (transfer_reduce_gate(r, m.stream)) return -1;
     }
-    t.stateClear = true;  // the column-max pass cleared the next set
+    if (more) t.maxReady = true;
+    else t.stateClear = true;
     return 0;
 }
 

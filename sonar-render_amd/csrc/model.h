@@ -44,14 +44,13 @@ struct Object {
         uint4 *A = nullptr, *B = nullptr;
         int nKs = 0, nRt = 0;
         int2 *wgSkip = nullptr;    // per 128-row group: zero K-step range (its own single facet)
-        int *gatedList = nullptr;  // rows with a zero input in some plane
-        // bounce state, two sets used in turn: [0] gated-row count, [1 + b kColStride] max |A P_in| of plane b
+        // bounce state, two sets used in turn: [0] spare, [1 + b kColStride] max |A P_in| of plane b
         // (float bits); each bounce clears the other set for the next
         int *state = nullptr;
         int stateSet = 0;
         bool stateClear = false;   // the set stateSet names is known to be zero
         bool maxReady = false;     // ... or holds the column maxima of P_in (from the last reduction)
-        int *rowFs = nullptr;      // per active row: first active index on its facet
+        int *rowFs = nullptr;      // per active row: first active index on its facet (the two-accumulator gate)
         // Zero-input gate inside the product (meshes whose facets each hold >= 128
         // active cells): rows of S padded so that every facet starts a 128-row
         // group; per group the first K step of the higher facets (kB); per
@@ -61,10 +60,6 @@ struct Object {
         int *rowAct = nullptr, *actRow = nullptr, *kB = nullptr;
         unsigned char *cls = nullptr;
         int nsplit = 1, ksPer = 8; // split-K of the bounce product
-        uint4 *Ac = nullptr;       // ... as compact row tiles of A (lower-facet emitters only)
-        int2 *skipZero = nullptr;  // empty skip ranges for those tiles
-        size_t acCap = 0;
-        int skipZeroCap = 0;
         int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
         int *nearCount = nullptr;
         bool nearClear = false;    // nearCount is known to be zero (the near-pair kernels clear it)

@@ -42,12 +42,15 @@ struct TransferGemm {
     int nsplit, ksPerSplit;
     const int2 *wgSkip;                   // per 4-row-tile group: K steps [x, y) that hold only zeros
     float2 *partL;                        // [split][64][n]
-    int compactSet = 0;                   // 1: the gated rows' compact set (same kernel, its own name in profiles)
     // Zero-input gate inside the product: per 128-row group, the first K step of
     // the higher-facet emitters (whole chunks). A workgroup writes what it sums
     // below that step to partL and from it on to partU (null: partL only).
     const int *kB = nullptr;
     float2 *partU = nullptr;
+    // Zero-input gate by two accumulators (any facet layout): per row of S, the
+    // first emitter of its own facet; the product sends the terms below it to
+    // partL and the rest to partU, both for every split (null: one accumulator).
+    const int *rowFs = nullptr;
 };
 
 // Reduction of a gated product: per (plane, active row), the lower-facet
@@ -91,12 +94,8 @@ int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int
                      int *clearState = nullptr, int clearWords = 0);
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
 int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st);
-// Gated rows as compact row tiles of the transfer matrix (nRtC tiles of 32,
-// a multiple of 4): emitters at or above row r's facet bound rowFs[r] zeroed, and
-// the K chunks S never holds (wgSkip of the row's 128-row group) written as zeros.
-int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, const int2 *wgSkip, int count,
-                         int nRtC, uint4 *Ac, hipStream_t st, const int *actRow = nullptr);
-// Lists the rows with an exactly zero input pressure in some plane (count: device int, zeroed by the caller).
+// Lists the rows with an exactly zero input pressure in some plane (count: device int,
+// zeroed by the caller): the plain form's gated rows.
 int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st);
 
 }  // namespace sr

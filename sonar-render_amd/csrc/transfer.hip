@@ -513,16 +513,23 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
 // the 4 waves; A fragments stream from HBM straight into registers. Both are
 // requested one chunk ahead of the MFMAs that use them (two LDS buffers, two
 // A register sets). Chunks whose emitters all lie on the workgroup's single
-// facet hold only zeros (same-facet pairs never interact) and are skipped.
-// One FP32 accumulator: rows that the zero-input gate affects are recomputed
-// afterwards as compact row tiles (k_gather_a_rows).
+// facet (or its run of facets with one normal) hold only zeros and are skipped.
+// The zero-input gate needs each row's lower-facet terms (emitters below its
+// own facet) apart from the rest, in one of two ways:
+//   kB (rows of S padded so that each 128-row group is single-facet): the
+//     group's chunks below kB are lower-facet, so one accumulator is stored to
+//     partL when the walk passes kB and to partU at the end;
+//   DUAL (any facet layout): two accumulators, A split per row at its own
+//     facet start (rowFs), each stored to partL / partU; chunks wholly below
+//     or above every row of the wave take one product, the others two (the
+//     product is HBM-bound: the matrix cores have room for the second).
 constexpr int kKC = 8;
 static_assert(kKC == 8, "K steps and split sizes are padded to multiples of 8");
 
-// ROLE 0: a bounce's product; 1: the compact set of gated rows (identical
-// code, instantiated twice so that profiles tell the two launches apart).
+// ROLE 0: one accumulator (the kB gate, or none); 2: DUAL.
 template <int ROLE>
 __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
+    constexpr bool DUAL = ROLE == 2;
     __shared__ uint4 sB[2][kKC * 4 * 64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -535,14 +542,31 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     const uint4 *Arow = t.A + (size_t)(rtValid ? rt : 0) * t.nKs * 64 + lane;
     const uint4 *Bt = t.B + threadIdx.x;     // thread x stages element x of every K step
     // in-product gate: K steps from kBnd on are the group's higher-facet emitters
-    const int kBnd = t.kB ? t.kB[blockIdx.x] : 0x7fffffff;
+    const int kBnd = (!DUAL && t.kB) ? t.kB[blockIdx.x] : 0x7fffffff;
     int kLast = -1;  // the last chunk summed
+    // DUAL: this lane's row's facet start, and the wave's least and greatest
+    int fsRow = 0, fsMin = 0, fsMax = 0;
+    if (DUAL) {
+        const int row = rt * 32 + (lane & 31);
+        const bool rv = rtValid && row < t.n;
+        fsRow = rv ? t.rowFs[row] : 0;
+        int lo = rv ? fsRow : 0x7fffffff, hi = rv ? fsRow : 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+        }
+        fsMin = __builtin_amdgcn_readfirstlane(lo);
+        fsMax = __builtin_amdgcn_readfirstlane(hi);
+    }
 
-    f32x16 acc[4];
+    f32x16 acc[4], accU[DUAL ? 4 : 1];
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[c] = f32x16{};
+#pragma unroll
+    for (int c = 0; c < (DUAL ? 4 : 1); c++) accU[c] = f32x16{};
     // C layout: element i of lane l is (row (i/4)*8 + (l/32)*4 + i%4, column l%32).
-    auto store_part = [&](float2 *part) {
+    auto store_part = [&](float2 *part, const f32x16 *ac) {
         if (!rtValid) return;
 #pragma unroll
         for (int c = 0; c < 2; c++) {
@@ -552,7 +576,38 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
             for (int i = 0; i < 16; i++) {
                 const int row = rt * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
                 if (row >= t.n) continue;
-                part[((size_t)split * 64 + b) * t.n + row] = make_float2(acc[c][i], acc[c + 2][i]);
+                part[((size_t)split * 64 + b) * t.n + row] = make_float2(ac[c][i], ac[c + 2][i]);
+            }
+        }
+    };
+    // one K step's four products into an accumulator set
+    auto mfma_step = [&](f32x16 *ac, uint4 a, int bb, int s) {
+        const half8 av = __builtin_bit_cast(half8, a);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const half8 bv = __builtin_bit_cast(half8, sB[bb][(s * 4 + c) * 64 + lane]);
+            ac[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, ac[c], 0, 0, 0);
+        }
+    };
+    // DUAL: the chunk's steps, each split at the row's facet start (lane (j, h)
+    // holds emitters 8 k + 4 h .. + 3 of row j, one complex FP16 word each)
+    auto dual_chunk = [&](const uint4 *a, int kb, int bb) {
+        if ((kb + kKC) * 8 <= fsMin) {  // below every row's facet: lower-facet terms
+#pragma unroll
+            for (int s = 0; s < kKC; s++) mfma_step(acc, a[s], bb, s);
+        } else if (kb * 8 >= fsMax) {  // at or above every row's facet start
+#pragma unroll
+            for (int s = 0; s < kKC; s++) mfma_step(accU, a[s], bb, s);
+        } else {
+#pragma unroll
+            for (int s = 0; s < kKC; s++) {
+                const int j0 = (kb + s) * 8 + 4 * (lane >> 5);
+                const uint4 q = a[s];
+                const uint4 lo = make_uint4(j0 < fsRow ? q.x : 0u, j0 + 1 < fsRow ? q.y : 0u,
+                                            j0 + 2 < fsRow ? q.z : 0u, j0 + 3 < fsRow ? q.w : 0u);
+                const uint4 hi = make_uint4(q.x ^ lo.x, q.y ^ lo.y, q.z ^ lo.z, q.w ^ lo.w);
+                mfma_step(acc, lo, bb, s);
+                mfma_step(accU, hi, bb, s);
             }
         }
     };
@@ -584,17 +639,15 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
             stage_b(kn, BUF ^ 1);                                                                    \
             _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = Arow[(size_t)(kn + s) * 64];  \
         }                                                                                            \
-        _Pragma("unroll") for (int s = 0; s < kKC; s++) {                                            \
-            const half8 av = __builtin_bit_cast(half8, ACUR[s]);                                     \
-            _Pragma("unroll") for (int c = 0; c < 4; c++) {                                          \
-                const half8 bv = __builtin_bit_cast(half8, sB[BUF][(s * 4 + c) * 64 + lane]);        \
-                acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[c], 0, 0, 0);           \
+        if (DUAL) {                                                                                  \
+            dual_chunk(ACUR, kb, BUF);                                                               \
+        } else {                                                                                     \
+            _Pragma("unroll") for (int s = 0; s < kKC; s++) mfma_step(acc, ACUR[s], BUF, s);        \
+            kLast = kb;                                                                              \
+            if (kb < kBnd && kn >= kBnd && kn < ks1) { /* lower-facet part done, higher follows */   \
+                store_part(t.partL, acc);                                                            \
+                _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                     \
             }                                                                                        \
-        }                                                                                            \
-        kLast = kb;                                                                                  \
-        if (kb < kBnd && kn >= kBnd && kn < ks1) { /* lower-facet part done, higher follows */       \
-            store_part(t.partL);                                                                     \
-            _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                         \
         }                                                                                            \
         kb = kn;                                                                                     \
     }
@@ -604,8 +657,34 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
         SR_GEMM_PHASE(a1, a0, 1)
     }
 #undef SR_GEMM_PHASE
-    if (!t.kB) store_part(t.partL);  // every split's partial, zero when it summed nothing
-    else if (kLast >= 0) store_part(kLast < kBnd ? t.partL : t.partU);
+    if (DUAL) {  // both partials of every split, zero where it summed nothing
+        store_part(t.partL, acc);
+        store_part(t.partU, accU);
+    } else if (!t.kB) {
+        store_part(t.partL, acc);  // every split's partial, zero when it summed nothing
+    } else if (kLast >= 0) {
+        store_part(kLast < kBnd ? t.partL : t.partU, acc);
+    }
+}
+
+// Rows whose own input pressure is exactly zero in some plane (the zero-input
+// gate of OptiX.cu:241-245 drops their higher-facet terms there).
+__global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes, int *__restrict__ list,
+                             int *__restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool z = false;
+    if (i < n)
+#pragma unroll 8
+        for (int b = 0; b < nplanes; b++) {
+            const float2 v = Pin[(size_t)b * n + i];
+            z |= (v.x == 0.0f && v.y == 0.0f);
+        }
+    const unsigned long long m = __ballot(z);
+    if (!m) return;
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0);
+    if (z) list[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
 }
 
 // One (plane, row of S) of the gated reduction: sums the splits' lower- and
@@ -676,56 +755,6 @@ __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
     }
 }
 
-// Rows whose own input pressure is exactly zero in some plane (the zero-input
-// gate of OptiX.cu:241-245 drops their higher-facet terms there).
-__global__ void k_gated_rows(const float2 *__restrict__ Pin, int n, int nplanes, int *__restrict__ list,
-                             int *__restrict__ count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool z = false;
-    if (i < n)
-#pragma unroll 8
-        for (int b = 0; b < nplanes; b++) {
-            const float2 v = Pin[(size_t)b * n + i];
-            z |= (v.x == 0.0f && v.y == 0.0f);
-        }
-    const unsigned long long m = __ballot(z);
-    if (!m) return;
-    int base = 0;
-    if (__lane_id() == 0) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, 0);
-    if (z) list[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = i;
-}
-
-// Gated rows of a transfer-form bounce as a compact set of row tiles: row i
-// of the set is active row list[i], its FP16 pair values copied from the
-// transfer matrix with the emitters at or above its own facet zeroed (the
-// zero-input gate of OptiX.cu:241-245 keeps lower facets only). The set then
-// goes through k_transfer_gemm with the bounce's B: the gated rows see exactly
-// the operator the other rows see, near-field terms included.
-__global__ void k_gather_a_rows(const uint4 *__restrict__ A, int nKs, const int *__restrict__ list,
-                                const int *__restrict__ rowFs, const int2 *__restrict__ wgSkip, int count, int nRtC,
-                                uint4 *__restrict__ Ac, const int *__restrict__ actRow) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nRtC * nKs * 64) return;
-    const int lane = (int)(t & 63);
-    const long tk = t >> 6;
-    const int ks = (int)(tk % nKs), ct = (int)(tk / nKs);
-    const int i = ct * 32 + (lane & 31), h = lane >> 5;
-    uint4 q = make_uint4(0, 0, 0, 0);
-    if (i < count) {
-        const int r = list[i], fs = rowFs[r];
-        const int j0 = ks * 8 + h * 4;  // first emitter of this fragment
-        const int sr = actRow ? actRow[r] : r;  // the row of S
-        const int2 sk = wgSkip ? wgSkip[sr >> 7] : make_int2(0, 0);
-        if (j0 < fs && !(ks >= sk.x && ks < sk.y)) {  // chunks S never holds are zero
-            q = A[((size_t)(sr >> 5) * nKs + ks) * 64 + (sr & 31) + 32 * h];
-            if (j0 + 1 >= fs) q.y = 0u;
-            if (j0 + 2 >= fs) q.z = 0u;
-            if (j0 + 3 >= fs) q.w = 0u;
-        }
-    }
-    Ac[t] = q;
-}
 
 // ---------------------------------------------------------------------------
 
@@ -790,9 +819,20 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
         set_error("transfer GEMM: K steps, split size and row tiles must be padded (8, 8, 4)");
         return -2;
     }
+    if (t.rowFs && (t.kB || !t.partU)) {
+        set_error("transfer GEMM: two accumulators need partU and exclude kB");
+        return -2;
+    }
     dim3 grid((t.nRt + 3) / 4, t.nsplit);
-    if (t.compactSet) k_transfer_gemm<1><<<grid, 256, 0, st>>>(t);
+    if (t.rowFs) k_transfer_gemm<2><<<grid, 256, 0, st>>>(t);
     else k_transfer_gemm<0><<<grid, 256, 0, st>>>(t);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
+int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st) {
+    if (n <= 0) return 0;
+    k_gated_rows<<<(n + 255) / 256, 256, 0, st>>>(Pin, n, nplanes, list, count);
     SR_HIP(hipGetLastError());
     return 0;
 }
@@ -806,21 +846,6 @@ int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st) {
     return 0;
 }
 
-int transfer_gather_rows(const uint4 *A, int nKs, const int *list, const int *rowFs, const int2 *wgSkip, int count,
-                         int nRtC, uint4 *Ac, hipStream_t st, const int *actRow) {
-    const long tot = (long)nRtC * nKs * 64;
-    if (tot <= 0) return 0;
-    k_gather_a_rows<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(A, nKs, list, rowFs, wgSkip, count, nRtC, Ac,
-                                                                   actRow);
-    SR_HIP(hipGetLastError());
-    return 0;
-}
 
-int transfer_gated_rows(const float2 *Pin, int n, int nplanes, int *list, int *count, hipStream_t st) {
-    if (n <= 0) return 0;
-    k_gated_rows<<<(n + 255) / 256, 256, 0, st>>>(Pin, n, nplanes, list, count);
-    SR_HIP(hipGetLastError());
-    return 0;
-}
 
 }  // namespace sr
