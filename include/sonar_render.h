@@ -241,6 +241,24 @@ int sr_batch_set_tune(const int *in4);
 /* hipDeviceSynchronize on the library's runtime. */
 int sr_device_synchronize(void);
 
+/* Contexts: independent scenes (geometry, sources, field points, settings, device
+ * buffers, streams), one current per calling thread; every entry point above acts
+ * on the calling thread's current context. Handle 0 is the default context (the
+ * reference's single global Model, python_interface.cpp:11) and stays current
+ * until a thread selects another. Shared by all contexts: the learned near-field
+ * network (sr_nf_*) and the batched scenes (sr_batch_*). Replaces the
+ * reference's one-model process (python_interface.cpp:11), per the build
+ * contract of one context per handle.
+ * sr_context_create: a new empty context, returns its handle (> 0) or a negative status.
+ * sr_context_select: makes handle current for the calling thread (0: the default).
+ * sr_context_current: the calling thread's current handle.
+ * sr_context_destroy: tears a created context down and frees it; not 0, not the
+ *   calling thread's current one (another thread must not be using it). */
+int sr_context_create(void);
+int sr_context_select(int handle);
+int sr_context_current(void);
+int sr_context_destroy(int handle);
+
 /* Wall time of the last projection entry point, milliseconds. */
 float sr_last_call_ms(void);
 

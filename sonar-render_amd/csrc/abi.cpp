@@ -4,6 +4,8 @@
 #include <chrono>
 #include <cstdio>
 #include <exception>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 
@@ -16,7 +18,14 @@
 
 namespace {
 
-sr::Model gModel;  // python_interface.cpp:11
+sr::Model gModel;  // python_interface.cpp:11: the default context, handle 0
+// created contexts by handle, and each thread's current one (null: the default)
+std::mutex gCtxMu;
+std::map<int, std::unique_ptr<sr::Model>> gCtx;
+int gNextCtx = 1;
+thread_local sr::Model *tCur = nullptr;
+thread_local int tCurId = 0;
+sr::Model &cur() { return tCur ? *tCur : gModel; }
 // Render colour settings (ModelCuda.hpp:45-48 defaults), set by SoundVisualisationInit.
 float gMaxDb = -20.f, gMinDb = -90.f;
 bool gPhaseTarget = false, gPhaseField = false;
@@ -63,46 +72,46 @@ int sr_last_status(void) { return gStatus; }
 float sr_last_call_ms(void) { return gLastMs; }
 
 int sr_set_initial_conditions(float cp, float f, float a, float d) {
-    return guarded("set_initial_conditions", [&] { return sr::model_set_initial_conditions(gModel, cp, f, a, d); });
+    return guarded("set_initial_conditions", [&] { return sr::model_set_initial_conditions(cur(), cp, f, a, d); });
 }
 int sr_load_geometry(const float *v, int nf, int type, float res) {
-    return guarded("load_geometry", [&] { return sr::model_load_geometry(gModel, v, nf, type, res); });
+    return guarded("load_geometry", [&] { return sr::model_load_geometry(cur(), v, nf, type, res); });
 }
 int sr_load_field_points(const float *p, int n) {
-    return guarded("load_field_points", [&] { return sr::model_load_points(gModel, p, n, false); });
+    return guarded("load_field_points", [&] { return sr::model_load_points(cur(), p, n, false); });
 }
 int sr_load_source_points(const float *p, int n) {
-    return guarded("load_source_points", [&] { return sr::model_load_points(gModel, p, n, true); });
+    return guarded("load_source_points", [&] { return sr::model_load_points(cur(), p, n, true); });
 }
 int sr_set_batched_sources(int on) {
-    return guarded("set_batched_sources", [&] { return sr::model_set_batched_sources(gModel, on); });
+    return guarded("set_batched_sources", [&] { return sr::model_set_batched_sources(cur(), on); });
 }
-int sr_render(void) { return timed("render_cuda", [&] { return sr::model_render(gModel); }); }
+int sr_render(void) { return timed("render_cuda", [&] { return sr::model_render(cur()); }); }
 int sr_project_src_points_to_objects(int fo) {
-    return timed("ProjectSrcPointsToObjects", [&] { return sr::model_project_src(gModel, fo != 0); });
+    return timed("ProjectSrcPointsToObjects", [&] { return sr::model_project_src(cur(), fo != 0); });
 }
 int sr_project_target_to_field_objects(void) {
-    return timed("ProjectTargetToFieldObjects", [&] { return sr::model_project_t2field_objects(gModel); });
+    return timed("ProjectTargetToFieldObjects", [&] { return sr::model_project_t2field_objects(cur()); });
 }
 int sr_project_target_to_field_points(void) {
-    return timed("ProjectTargetToFieldPoints", [&] { return sr::model_project_t2field_points(gModel); });
+    return timed("ProjectTargetToFieldPoints", [&] { return sr::model_project_t2field_points(cur()); });
 }
 int sr_project_target_to_target_objects(int n) {
-    return timed("ProjectTargetToTargetObjects", [&] { return sr::model_project_t2t(gModel, n); });
+    return timed("ProjectTargetToTargetObjects", [&] { return sr::model_project_t2t(cur(), n); });
 }
-int sr_prepare_resp(int t) { return guarded("prepare_resp", [&] { return sr::model_prepare_resp(gModel, t); }); }
+int sr_prepare_resp(int t) { return guarded("prepare_resp", [&] { return sr::model_prepare_resp(cur(), t); }); }
 int sr_project_resp(int a, int b) {
-    return timed("project_resp", [&] { return sr::model_project_resp(gModel, a, b); });
+    return timed("project_resp", [&] { return sr::model_project_resp(cur(), a, b); });
 }
 int sr_get_field_point_pressures(sr_dcomplex *out, int n) {
     return guarded("GetFieldPointPressures", [&] {
-        int rc = sr::model_get_field_points(gModel, (double *)out, n, 0);
+        int rc = sr::model_get_field_points(cur(), (double *)out, n, 0);
         return rc < 0 ? rc : 0;
     });
 }
 int sr_get_field_point_pressures_plane(int plane, sr_dcomplex *out, int n) {
     return guarded("get_field_point_pressures_plane", [&] {
-        int rc = sr::model_get_field_points(gModel, (double *)out, n, plane);
+        int rc = sr::model_get_field_points(cur(), (double *)out, n, plane);
         return rc < 0 ? rc : 0;
     });
 }
@@ -110,16 +119,57 @@ int sr_get_field_point_pressures_c64(float *out, int n) {
     return guarded("get_field_point_pressures_c64", [&] {
         if (n <= 0) return 0;
         double *tmp = new double[2 * (size_t)n];
-        int rc = sr::model_get_field_points(gModel, tmp, n, 0);
+        int rc = sr::model_get_field_points(cur(), tmp, n, 0);
         for (int i = 0; rc > 0 && i < rc; i++) { out[2 * i] = (float)tmp[2 * i]; out[2 * i + 1] = (float)tmp[2 * i + 1]; }
         delete[] tmp;
         return rc < 0 ? rc : 0;
     });
 }
 int sr_set_source_amplitude(int s, float re, float im) {
-    return guarded("set_source_amplitude", [&] { return sr::model_set_source_amplitude(gModel, s, re, im); });
+    return guarded("set_source_amplitude", [&] { return sr::model_set_source_amplitude(cur(), s, re, im); });
 }
-int sr_num_planes(void) { return gModel.nplanes; }
+int sr_num_planes(void) { return cur().nplanes; }
+int sr_context_create(void) {
+    return guarded("context_create", [&] {
+        std::lock_guard<std::mutex> g(gCtxMu);
+        const int h = gNextCtx++;
+        gCtx[h] = std::make_unique<sr::Model>();
+        return h;
+    });
+}
+int sr_context_select(int handle) {
+    return guarded("context_select", [&] {
+        if (handle == 0) {
+            tCur = nullptr;
+            tCurId = 0;
+            return 0;
+        }
+        std::lock_guard<std::mutex> g(gCtxMu);
+        auto it = gCtx.find(handle);
+        if (it == gCtx.end()) { sr::set_error("no such context"); return -2; }
+        tCur = it->second.get();
+        tCurId = handle;
+        return 0;
+    });
+}
+int sr_context_current(void) { return tCurId; }
+int sr_context_destroy(int handle) {
+    return guarded("context_destroy", [&] {
+        if (handle == 0) { sr::set_error("the default context cannot be destroyed"); return -2; }
+        if (handle == tCurId) { sr::set_error("the calling thread's current context cannot be destroyed"); return -2; }
+        std::unique_ptr<sr::Model> m;
+        {
+            std::lock_guard<std::mutex> g(gCtxMu);
+            auto it = gCtx.find(handle);
+            if (it == gCtx.end()) { sr::set_error("no such context"); return -2; }
+            m = std::move(it->second);
+            gCtx.erase(it);
+        }
+        sr::model_release(*m);
+        return 0;
+    });
+}
+
 int sr_device_synchronize(void) {
     return guarded("device_synchronize", [&] {
         SR_HIP(hipDeviceSynchronize());
@@ -127,45 +177,45 @@ int sr_device_synchronize(void) {
     });
 }
 int sr_teardown(void) {
-    return guarded("TearDownCuda", [&] { sr::model_teardown(gModel); return 0; });
+    return guarded("TearDownCuda", [&] { sr::model_teardown(cur()); return 0; });
 }
 
 int sr_object_count(int type) {
-    if (type == sr::OBJ_TARGET) return (int)gModel.targets.size();
-    if (type == sr::OBJ_FIELD) return (int)gModel.fields.size();
-    if (type == sr::OBJ_SOURCE) return gModel.numSourceObjects;
+    if (type == sr::OBJ_TARGET) return (int)cur().targets.size();
+    if (type == sr::OBJ_FIELD) return (int)cur().fields.size();
+    if (type == sr::OBJ_SOURCE) return cur().numSourceObjects;
     return -2;
 }
 static sr::Object *obj(int type, int idx) {
-    auto &v = (type == sr::OBJ_FIELD) ? gModel.fields : gModel.targets;
+    auto &v = (type == sr::OBJ_FIELD) ? cur().fields : cur().targets;
     if (type != sr::OBJ_FIELD && type != sr::OBJ_TARGET) return nullptr;
     if (idx < 0 || idx >= (int)v.size()) return nullptr;
     return &v[idx];
 }
 long sr_object_cell_count(int type, int idx) {
     sr::Object *o = obj(type, idx);
-    if (!o || !gModel.rendered) return -2;
+    if (!o || !cur().rendered) return -2;
     return o->ncell;
 }
 int sr_object_active_count(int type, int idx) {
     sr::Object *o = obj(type, idx);
-    if (!o || !gModel.rendered) return -2;
+    if (!o || !cur().rendered) return -2;
     return o->nact;
 }
 int sr_object_active_cells(int type, int idx, int *out, int n) {
     sr::Object *o = obj(type, idx);
-    if (!o || !gModel.rendered || n < o->nact) return -2;
+    if (!o || !cur().rendered || n < o->nact) return -2;
     for (int k = 0; k < o->nact; k++) out[k] = o->actCellHost[k];
     return 0;
 }
 int sr_get_object_pressures(int type, int idx, int which, int plane, sr_dcomplex *out, long n) {
     return guarded("get_object_pressures",
-                   [&] { return sr::model_get_object_pressure(gModel, type, idx, which, plane, (double *)out, n); });
+                   [&] { return sr::model_get_object_pressure(cur(), type, idx, which, plane, (double *)out, n); });
 }
 
 long sr_mask_words(int kind, int ta, int ia, int tb, int ib) {
-    if (!gModel.rendered) return -3;
-    const long ns = (long)gModel.srcHost.size() / 3, nq = (long)gModel.fldHost.size() / 3;
+    if (!cur().rendered) return -3;
+    const long ns = (long)cur().srcHost.size() / 3, nq = (long)cur().fldHost.size() / 3;
     sr::Object *A = obj(ta, ia), *B = obj(tb, ib);
     long ne, nr;
     switch (kind) {
@@ -183,13 +233,13 @@ int sr_get_mask(int kind, int ta, int ia, int tb, int ib, unsigned long long *ou
         long need = sr_mask_words(kind, ta, ia, tb, ib);
         if (need < 0) { sr::set_error("bad mask request"); return (int)need; }
         if (nwords < need) { sr::set_error("mask buffer too short"); return -2; }
-        return sr::model_get_mask(gModel, kind, ta, ia, tb, ib, (uint64_t *)out, need);
+        return sr::model_get_mask(cur(), kind, ta, ia, tb, ib, (uint64_t *)out, need);
     });
 }
 int sr_vis_params(float *out) {
-    if (!gModel.rendered) return -3;
-    out[0] = gModel.vc.scale; out[1] = gModel.vc.padTri; out[2] = gModel.vc.padNode;
-    out[3] = gModel.vc.tminPoint; out[4] = gModel.vc.tminFrag;
+    if (!cur().rendered) return -3;
+    out[0] = cur().vc.scale; out[1] = cur().vc.padTri; out[2] = cur().vc.padNode;
+    out[3] = cur().vc.tminPoint; out[4] = cur().vc.tminFrag;
     return 0;
 }
 
@@ -213,24 +263,24 @@ long sr_fragment_facet(const float *v, float delta, int *grid3, float *area, flo
 }
 
 int sr_reset_pressures(void) {
-    return guarded("reset_pressures", [&] { return sr::model_reset_pressures(gModel); });
+    return guarded("reset_pressures", [&] { return sr::model_reset_pressures(cur()); });
 }
 int sr_clear_vis_cache(void) {
-    return guarded("clear_vis_cache", [&] { return sr::model_clear_vis_cache(gModel); });
+    return guarded("clear_vis_cache", [&] { return sr::model_clear_vis_cache(cur()); });
 }
 int sr_set_transfer_mode(int mode) {
-    return guarded("set_transfer_mode", [&] { return sr::model_set_transfer_mode(gModel, mode); });
+    return guarded("set_transfer_mode", [&] { return sr::model_set_transfer_mode(cur(), mode); });
 }
-int sr_transfer_mode(void) { return gModel.transferMode; }
+int sr_transfer_mode(void) { return cur().transferMode; }
 int sr_set_gather_mode(int mode) {
-    return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(gModel, mode); });
+    return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(cur(), mode); });
 }
-int sr_gather_mode(void) { return gModel.gatherMode; }
+int sr_gather_mode(void) { return cur().gatherMode; }
 int sr_set_near_quadrature_radius(float deltas) {
-    return guarded("set_near_quadrature_radius", [&] { return sr::model_set_near_quadrature_radius(gModel, deltas); });
+    return guarded("set_near_quadrature_radius", [&] { return sr::model_set_near_quadrature_radius(cur(), deltas); });
 }
 int sr_set_near_mode(int mode, float radiusWavelengths) {
-    return guarded("set_near_mode", [&] { return sr::model_set_near_mode(gModel, mode, radiusWavelengths); });
+    return guarded("set_near_mode", [&] { return sr::model_set_near_mode(cur(), mode, radiusWavelengths); });
 }
 int sr_nf_init(unsigned seed) { return guarded("nf_init", [&] { return sr::nf_init(seed); }); }
 int sr_nf_train(int nsamples, int steps, int batch, float lr, unsigned seed, float *report2) {
@@ -250,17 +300,17 @@ int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, f
     return guarded("nf_gradient", [&] { return sr::nf_eval(X, Y, n, nullptr, grad, params); });
 }
 int sr_profile(int on) {
-    return guarded("profile", [&] { sr::model_profile(gModel, on); return 0; });
+    return guarded("profile", [&] { sr::model_profile(cur(), on); return 0; });
 }
 int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes) {
     return guarded("kernel_stats", [&] {
         if (kind < 0 || kind >= sr::PK_COUNT) { sr::set_error("unknown kernel family"); return -2; }
-        return sr::model_kernel_stats(gModel, kind, ms, launches, pairs, flops, bytes);
+        return sr::model_kernel_stats(cur(), kind, ms, launches, pairs, flops, bytes);
     });
 }
 
 int sr_time_transfer_gemm(int reps, double *msPerLaunch) {
-    return guarded("time_transfer_gemm", [&] { return sr::model_time_transfer_gemm(gModel, reps, msPerLaunch); });
+    return guarded("time_transfer_gemm", [&] { return sr::model_time_transfer_gemm(cur(), reps, msPerLaunch); });
 }
 
 long sr_read_stl(const char *path, float *out, long cap) {
@@ -290,7 +340,7 @@ int sr_load_stl(const char *path, int type, float res) {
         std::string err;
         int n = sr::stl_read(path ? path : "", t, err);
         if (n < 0) { sr::set_error(err); return n; }
-        return sr::model_load_geometry(gModel, t.data(), n, type, res);
+        return sr::model_load_geometry(cur(), t.data(), n, type, res);
     });
 }
 int sr_colour_map(const double *p, long n, float minDb, float maxDb, int phase, float *rgba) {
@@ -301,7 +351,7 @@ int sr_colour_map(const double *p, long n, float minDb, float maxDb, int phase, 
 int sr_surface_rgba(int type, int idx, int plane, float *rgba, long n) {
     return guarded("surface_rgba", [&] {
         std::vector<double> p((size_t)2 * std::max<long>(n, 0));
-        int rc = sr::model_get_object_pressure(gModel, type, idx, 0, plane, p.data(), n);
+        int rc = sr::model_get_object_pressure(cur(), type, idx, 0, plane, p.data(), n);
         if (rc) return rc;
         const bool ph = (type == sr::OBJ_FIELD) ? gPhaseField : gPhaseTarget;
         sr::colour_map(p.data(), n, gMinDb, gMaxDb, ph, rgba);

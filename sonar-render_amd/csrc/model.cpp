@@ -1346,4 +1346,25 @@ void model_teardown(Model &m) {
     m.nplanes = 1;
 }
 
+void model_release(Model &m) {
+    model_teardown(m);
+    if (m.stream) (void)hipStreamSynchronize(m.stream);
+    if (m.side) (void)hipStreamSynchronize(m.side);
+    for (auto &e : m.wordEv)
+        if (e) (void)hipEventDestroy(e);
+    if (m.sideIn) (void)hipEventDestroy(m.sideIn);
+    if (m.sideOut) (void)hipEventDestroy(m.sideOut);
+    if (m.side) (void)hipStreamDestroy(m.side);
+    if (m.stream) (void)hipStreamDestroy(m.stream);
+    if (m.hostWords) (void)hipHostFree(m.hostWords);
+    if (m.errHost) (void)hipHostFree(m.errHost);
+    if (m.workCnt) (void)hipFree(m.workCnt);
+    m.stream = m.side = nullptr;
+    m.sideIn = m.sideOut = nullptr;
+    for (auto &e : m.wordEv) e = nullptr;
+    m.hostWords = m.errHost = m.errFlag = nullptr;
+    m.workCnt = nullptr;
+    m.workCap = m.workUsed = 0;
+}
+
 }  // namespace sr

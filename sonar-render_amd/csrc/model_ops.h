@@ -37,6 +37,8 @@ void model_profile_reset(Model &m);
 int model_kernel_stats(Model &m, int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);
 int model_time_transfer_gemm(Model &m, int reps, double *msPerLaunch);
 void model_teardown(Model &m);
+// Teardown plus the context's own streams, events and pinned words (a context that is discarded).
+void model_release(Model &m);
 
 // Visibility-word task of one pair set (kinds as sr_get_mask), without launching it.
 int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, size_t &words);

@@ -433,6 +433,34 @@ def device_synchronize():
     _check(cpp_lib.sr_device_synchronize(), "device_synchronize")
 
 
+_sig("sr_context_create", [])
+_sig("sr_context_select", [_i])
+_sig("sr_context_current", [])
+_sig("sr_context_destroy", [_i])
+
+
+def context_create():
+    """A new, empty scene context; returns its handle (> 0). Every other call acts on
+    the calling thread's current context (handle 0, the default, until one is selected)."""
+    h = cpp_lib.sr_context_create()
+    _check(h, "context_create")
+    return h
+
+
+def context_select(handle):
+    """Makes handle current for the calling thread (0: the default context)."""
+    _check(cpp_lib.sr_context_select(int(handle)), "context_select")
+
+
+def context_current():
+    return cpp_lib.sr_context_current()
+
+
+def context_destroy(handle):
+    """Tears a created context down and frees it (not 0, not the current one)."""
+    _check(cpp_lib.sr_context_destroy(int(handle)), "context_destroy")
+
+
 for _n, _a in [("sr_batch_clear", []), ("sr_batch_add_scene", [_fp, _i, _f, _f, _f, _f, _fp, _i, _fp, _i]),
                ("sr_batch_set_bounces", [_i]), ("sr_batch_prepare", []),
                ("sr_batch_run", [_i, _fp]), ("sr_batch_autotune", [_fp, _i]),

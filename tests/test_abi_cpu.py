@@ -67,3 +67,41 @@ def test_fragment_probe_shapes(api):
     assert area.shape == (ny, nx) and pe.shape == (nx * ny, 3) and pr.shape == (nx * ny, 3)
     assert 0 <= nneg <= nx
     assert abs(area.sum() - 0.5) < 1e-5
+
+
+def test_contexts_without_gpu():
+    """Context handles (no device work until render): each has its own scene state,
+    the calling thread's current one receives the calls, and misuse returns errors."""
+    import threading
+
+    import sonar_api as api
+    assert api.context_current() == 0
+    h1, h2 = api.context_create(), api.context_create()
+    assert h1 > 0 and h2 > 0 and h1 != h2
+    try:
+        api.context_select(h1)
+        assert api.context_current() == h1
+        api.set_batched_sources(True)
+        api.load_points_to_cuda(np.zeros((3, 3), np.float32), isSource=True)
+        api.context_select(h2)
+        api.load_points_to_cuda(np.zeros((5, 3), np.float32), isSource=True)
+        # another thread starts on the default context, whatever this one selected
+        seen = []
+        t = threading.Thread(target=lambda: seen.append(api.context_current()))
+        t.start()
+        t.join()
+        assert seen == [0]
+        with pytest.raises(api.SonarRenderError):
+            api.context_destroy(h2)  # current on this thread
+        with pytest.raises(api.SonarRenderError):
+            api.context_destroy(0)
+        with pytest.raises(api.SonarRenderError):
+            api.context_select(10 ** 6)
+        assert api.context_current() == h2  # a failed select keeps the current context
+    finally:
+        api.context_select(0)
+        api.context_destroy(h1)
+        api.context_destroy(h2)
+    with pytest.raises(api.SonarRenderError):
+        api.context_select(h1)  # destroyed
+    assert api.context_current() == 0
