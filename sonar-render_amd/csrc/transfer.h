@@ -30,7 +30,7 @@ struct TransferPhys {
     const float *mlp;      // kMlpParams FP32 weights (rounded to BF16 on load)
     // near pairs found by the S build: by quadrature (nearQ) and learned (nearM)
     int2 *nearQ, *nearM;   // (row, emitter)
-    int *nearCount;        // [2] device counters: zero before a build; the near-pair kernels clear them
+    int *nearCount;        // [2] device counters: zero before a build; the near-pair kernel clears them
     int nearCap;           // capacity of each list
     float quadRadius;      // learned mode: closer pairs use the quadrature (metres, >= delta)
 };

@@ -121,10 +121,9 @@ __device__ inline void store_pair(uint4 *A, int nKs, int row, int j, uint32_t w)
 
 // Near pairs by 16x16 sub-fragment quadrature: 16 lanes per pair, one row of
 // sub-points each, summed across the lanes.
-__global__ __launch_bounds__(256) void k_near_quad(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
-                                                   int count, uint4 *__restrict__ A) {
-    if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;  // the next build counts from zero
-    const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+__device__ inline void near_quad_block(const TransferGeom &g, const TransferPhys &ph, const int2 *__restrict__ list,
+                                       int count, uint4 *__restrict__ A, int bid) {
+    const long i = ((long)bid * blockDim.x + threadIdx.x) >> 4;
     const int sy = threadIdx.x & 15;
     const bool valid = i < count;  // uniform over the pair's 16 lanes
     int2 pr = make_int2(0, 0);
@@ -162,9 +161,8 @@ __device__ inline float tanh_hw(float x) {
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(x * 2.8853900817779268f) + 1.0f);
 }
 
-__global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys ph, const int2 *__restrict__ list,
-                                                  int count, uint4 *__restrict__ A) {
-    if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;  // the next build counts from zero
+__device__ inline void near_mlp_blocks(const TransferGeom &g, const TransferPhys &ph, const int2 *__restrict__ list,
+                                       int count, uint4 *__restrict__ A, int bid, int nblk) {
     __shared__ bf16x8 sW1[2][64];
     __shared__ bf16x8 sW2[4][2][64];
     __shared__ float sB1[64], sB2[64], sW3[2][64], sB3[2];
@@ -198,7 +196,7 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
     const float d = ph.deltaReg;
     __bf16 (*H)[kHP] = sH[wave];
     // workgroup-uniform trip count: tiles of 128 pairs (32 per wave)
-    for (int base = blockIdx.x * 128; base < count; base += gridDim.x * 128) {
+    for (int base = bid * 128; base < count; base += nblk * 128) {
         const int i = base + wave * 32 + j;
         const bool valid = i < count;
         int2 pr = make_int2(0, 0);
@@ -282,6 +280,16 @@ __global__ __launch_bounds__(256) void k_near_mlp(TransferGeom g, TransferPhys p
             store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
         }
     }
+}
+
+// Both near-pair lists in one launch: blocks [0, quadBlocks) take the quadrature
+// list (16 lanes per pair), the rest the learned list (128 pairs per block and
+// round). Block 0 clears the near counters for the next build (nothing here reads them).
+__global__ __launch_bounds__(256) void k_near(TransferGeom g, TransferPhys ph, int nQuad, int nMlp, int quadBlocks,
+                                              uint4 *__restrict__ A) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;
+    if ((int)blockIdx.x < quadBlocks) near_quad_block(g, ph, ph.nearQ, nQuad, A, blockIdx.x);
+    else near_mlp_blocks(g, ph, ph.nearM, nMlp, A, blockIdx.x - quadBlocks, gridDim.x - quadBlocks);
 }
 
 // Near pairs are staged per wave in LDS and flushed to the global lists with
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
                 Gim = -(a * ph.kr) * c;
             }
             const bool vis = rvalid && j < g.n && ((word >> bit) & 1ull);
-            // near pairs: evaluated afterwards from the near lists (k_near_quad / k_near_mlp)
+            // near pairs: evaluated afterwards from the near lists (k_near)
             const bool isNear = NEAR != 0 && vis && r < ph.nearRadius;
             // the learned range starts at r >= delta
             const bool isQuad = isNear && (NEAR == 1 || r < ph.quadRadius);
@@ -780,16 +788,12 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
 
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st) {
     if (nQuad > ph.nearCap || nMlp > ph.nearCap) { set_error("near lists overflow"); return -2; }
-    if (nQuad > 0) {
-        k_near_quad<<<(unsigned)(((long)nQuad * 16 + 255) / 256), 256, 0, st>>>(g, ph, ph.nearQ, nQuad, A);
-        SR_HIP(hipGetLastError());
-    }
-    if (nMlp > 0) {
-        if (!ph.mlp) { set_error("learned near field without a network"); return -2; }
-        const int grid = std::min((nMlp + 127) / 128, 2048);
-        k_near_mlp<<<grid, 256, 0, st>>>(g, ph, ph.nearM, nMlp, A);
-        SR_HIP(hipGetLastError());
-    }
+    if (nMlp > 0 && !ph.mlp) { set_error("learned near field without a network"); return -2; }
+    const long quadBlocks = ((long)nQuad * 16 + 255) / 256;
+    const int mlpBlocks = nMlp > 0 ? std::min((nMlp + 127) / 128, 2048) : 0;
+    if (quadBlocks + mlpBlocks == 0) return 0;
+    k_near<<<(unsigned)(quadBlocks + mlpBlocks), 256, 0, st>>>(g, ph, nQuad, nMlp, (int)quadBlocks, A);
+    SR_HIP(hipGetLastError());
     return 0;
 }
 
