@@ -13,8 +13,15 @@
  * negative status: -1 device/runtime failure, -2 bad argument, -3 wrong call
  * order.
  *
- * One process-global scene, as in the reference (python_interface.cpp:11).
- * Calls are synchronous and not re-entrant.
+ * Threading: the reference keeps one process-global scene
+ * (python_interface.cpp:11). Here that scene is context 0, and further
+ * independent scenes are contexts (sr_context_*): every entry point acts on
+ * the calling thread's current context, and one context must not be used by
+ * two threads at once. The status and message of a failed call
+ * (sr_last_status, sr_last_error) belong to the calling thread.
+ * Projections return once their kernels are queued on the context's stream;
+ * the calls that read results (GetFieldPointPressures, sr_get_*) wait for
+ * that work and report any error it raised.
  */
 #ifndef SONAR_RENDER_H
 #define SONAR_RENDER_H
@@ -37,8 +44,13 @@ typedef struct { double r, i; } sr_dcomplex;
 void set_initial_conditions(float cp, float frequency, float attenuation, float density);
 
 /* python_interface.cpp:26-41. v = num_facets x 9 floats (v1, v2, v3);
- * objectType 0 target, 1 source (stored, unused), 2 field; resolution is the
- * fragment edge delta. */
+ * objectType 0 target, 1 source, 2 field; resolution is the fragment edge
+ * delta. A source object (stored but never used by the reference,
+ * Model.cpp:33-35) is the piston-like emitter of README.md:2: every fragment
+ * at one coherent amplitude (1+0i, sr_set_source_object_amplitude),
+ * projected by ProjectSrcPointsToObjects onto the targets (and field objects)
+ * with the fragment->fragment sum without reciprocity. It never occludes, and
+ * needs unbatched source points. */
 void load_geometry(float *v, int num_facets, int objectType, float resolution);
 
 /* python_interface.cpp:43-50: n x 3 floats, appended. */
@@ -69,13 +81,15 @@ void ProjectTargetToTargetObjects(int number_reflections);
  * values (the reference ignores NumPoints, ModelCuda.cpp:206). */
 void GetFieldPointPressures(sr_dcomplex *field_points_pressure, int NumPoints);
 
-/* python_interface.cpp:92-96: the OpenGL/PNG render stays on the host and
- * is not part of this library; this records an error and returns.
- * sr_surface_rgba gives the per-cell colours that render would draw. */
+/* python_interface.cpp:92-96 (ModelGl.cpp:14-341): writes a PNG of the target
+ * objects then the field objects, each facet textured with its cells' colours,
+ * seen through a 45-degree perspective camera; viewSettings = eye (3), look-at
+ * point (3), up vector (3). Rendered on the host (no OpenGL): sr_render_png. */
 void RenderOpenGL(int width, int height, char *filename, float viewSettings[9]);
 
-/* python_interface.cpp:103-107: colour range and phase flags used by
- * sr_surface_rgba (defaults -20 / -90 dB as ModelCuda.hpp:45-48). */
+/* python_interface.cpp:103-107: colour range (dB) and phase flags of the
+ * render and of sr_surface_rgba, per context (defaults -20 / -90 dB and no
+ * phase, ModelCuda.hpp:45-48). */
 void SoundVisualisationInit(float maxDbValue, float minDbValue, bool renderPhaseTarget, bool renderPhaseField);
 
 /* python_interface.cpp:98-101: free the scene; the library can be loaded
@@ -111,6 +125,8 @@ int sr_set_batched_sources(int on);
 int sr_num_planes(void);
 int sr_get_field_point_pressures_plane(int plane, sr_dcomplex *out, int n);
 int sr_set_source_amplitude(int source, float re, float im);
+/* Coherent amplitude of source object `index` (default 1+0i). */
+int sr_set_source_object_amplitude(int index, float re, float im);
 
 /* FACET_RESP between two target objects (OptiX.cu:427-430; the reference
  * never dispatches it). sr_prepare_resp sets P_in <- P and P_out <- 0. */
@@ -127,8 +143,8 @@ int sr_get_object_pressures(int objectType, int index, int which, int plane, sr_
 
 /* Visibility words. kind: 0 sources->cells of (tb,ib), 1 active cells of
  * (ta,ia)->field points, 2 SELF of (ta,ia), 3 NO_RESP (ta,ia)->(tb,ib),
- * 4 RESP forward, 5 RESP reciprocal. Word [rblock][emitter], bit = receiver
- * rblock*64+bit contributes. */
+ * 4 RESP forward, 5 RESP reciprocal, 6 source object ia -> object (tb,ib).
+ * Word [rblock][emitter], bit = receiver rblock*64+bit contributes. */
 long sr_mask_words(int kind, int ta, int ia, int tb, int ib);
 int sr_get_mask(int kind, int ta, int ia, int tb, int ib, unsigned long long *out, long nwords);
 
@@ -216,6 +232,20 @@ int sr_load_stl(const char *path, int objectType, float resolution);
  * (phase) hue (arg p + pi) / 2 pi on an HSV wheel. rgba: 4 floats per value. */
 int sr_colour_map(const double *p2, long n, float minDb, float maxDb, int phase, float *rgba);
 int sr_surface_rgba(int objectType, int index, int plane, float *rgba, long n);
+
+/* Host render (RenderOpenGL's picture). sr_render_png writes incidence plane
+ * `plane` of the current scene to a PNG file; sr_render_rgb returns the same
+ * image as width x height x 3 bytes, rows top to bottom. sr_render_facets_rgb
+ * renders one object given as triangles, fragment edge delta and complex
+ * per-cell pressures (p2: 2 doubles per cell, cells of facet f after those of
+ * facets 0..f-1, cell (i,j) at j*nX+i) with the given colour settings, on the
+ * host only. sr_write_png writes an RGB image. */
+int sr_sound_visualisation_init(float maxDb, float minDb, int phaseTarget, int phaseField);
+int sr_render_png(const char *filename, int width, int height, const float *view9, int plane);
+int sr_render_rgb(int plane, int width, int height, const float *view9, unsigned char *rgb);
+int sr_render_facets_rgb(const float *v9, int nf, float delta, const double *p2, long ncell, int phase, float minDb,
+                         float maxDb, int width, int height, const float *view9, unsigned char *rgb);
+int sr_write_png(const char *path, int width, int height, const unsigned char *rgb);
 
 /* Batched independent scenes: each its own mesh (target), fragment edge,
  * sound speed, frequency, attenuation, sources (summed) and field points.

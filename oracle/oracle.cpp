@@ -206,6 +206,10 @@ struct Scene {
     float kf = 0.f;       // FP32 k (Model.cpp:50-52)
     cd k{0, 0};           // (double)k + i*attenuation
     std::vector<Object> targets, fields;
+    // source objects (type 1): fragmented like targets, every cell at the
+    // coherent amplitude srcObjAmp, projected by the fragment->fragment sum
+    std::vector<Object> sources;
+    std::vector<cd> srcObjAmp;
     std::vector<V3> srcPos, fldPos;
     std::vector<cd> srcP, fldP;
     // Occluders: all target triangles in load order (OptiX.cpp:273-292).
@@ -221,33 +225,38 @@ struct Scene {
 
 // Sub-fragment quadrature of one fragment->fragment term (test infrastructure's
 // definition of the near-field judge): the delta x delta square centred on the
-// emitter point is split into nsub x nsub cells; each cell centre is an
-// emitter with area A/nsub^2, cosine culls and the r < delta/nsub
-// regularisation of the point rule, in FP64. Returns the replacement for
-// A_e cos_s cos_i 0.9 G(r) of OptiX.cu:340-354.
-static cd near_quadrature(const Scene &s, const Facet &FA, float dA, float Ae, V3 pe, const Facet *FB, V3 pr,
-                          bool fieldDst) {
-    const int n = s.nearSub;
+// emitter point (axes xa, ya of the emitter's facet) is split into nsub x nsub
+// cells; each cell centre is an emitter with area A/nsub^2, the cosine culls
+// and the r < delta/nsub regularisation of the point rule (OptiX.cu:340-354),
+// in FP64. nb == nullptr: a field destination (receiver cosine 1). Returns the
+// replacement for A_e cos_s cos_i 0.9 G(r).
+static cd near_quadrature_pair(cd k, int n, float dA, float Ae, V3 pe, V3 xa, V3 ya, V3 na, const V3 *nb, V3 pr) {
     const double d = dA, h = d / n;
-    const cd ik = cd(0, 1) * s.k;
+    const cd ik = cd(0, 1) * k;
     cd acc(0, 0);
     for (int sy = 0; sy < n; sy++)
         for (int sx = 0; sx < n; sx++) {
             const double ox = ((sx + 0.5) * h - 0.5 * d), oy = ((sy + 0.5) * h - 0.5 * d);
-            const double px = pe.x + FA.xAxis.x * ox + FA.yAxis.x * oy;
-            const double py = pe.y + FA.xAxis.y * ox + FA.yAxis.y * oy;
-            const double pz = pe.z + FA.xAxis.z * ox + FA.yAxis.z * oy;
+            const double px = pe.x + xa.x * ox + ya.x * oy;
+            const double py = pe.y + xa.y * ox + ya.y * oy;
+            const double pz = pe.z + xa.z * ox + ya.z * oy;
             const double vx = pr.x - px, vy = pr.y - py, vz = pr.z - pz;
             const double r = std::sqrt(vx * vx + vy * vy + vz * vz);
-            const double cs = (vx * FA.normal.x + vy * FA.normal.y + vz * FA.normal.z) / r;
-            const double ci = fieldDst ? 1.0 : -(vx * FB->normal.x + vy * FB->normal.y + vz * FB->normal.z) / r;
+            const double cs = (vx * na.x + vy * na.y + vz * na.z) / r;
+            const double ci = nb ? -(vx * nb->x + vy * nb->y + vz * nb->z) / r : 1.0;
             if (!(cs >= 1e-6) || !(ci >= 1e-6)) continue;
             cd G;
             if (r < h) G = ik / (4.0 * M_PI);
-            else G = -std::exp(cd(0, 1) * s.k * r) / (4 * M_PI) * (ik / r) + 1.0 / (r * r);
+            else G = -std::exp(cd(0, 1) * k * r) / (4 * M_PI) * (ik / r) + 1.0 / (r * r);
             acc += cs * ci * 0.9 * G;
         }
     return acc * ((double)Ae / (double)(n * n));
+}
+
+static cd near_quadrature(const Scene &s, const Facet &FA, float dA, float Ae, V3 pe, const Facet *FB, V3 pr,
+                          bool fieldDst) {
+    return near_quadrature_pair(s.k, s.nearSub, dA, Ae, pe, FA.xAxis, FA.yAxis, FA.normal,
+                                fieldDst ? nullptr : &FB->normal, pr);
 }
 
 static cd expik(const cd &k, float r) {
@@ -440,6 +449,20 @@ void orc_destroy(void *h) { delete (Scene *)h; }
 void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 int orc_get_threads() { return omp_get_max_threads(); }
 void orc_set_bruteforce(void *h, int on) { ((Scene *)h)->bruteForce = on != 0; }
+// One pair of the near-field judge, outside any scene: emitter point pe with
+// its facet's axes xa, ya and normal na, cell edge delta and area Ae; receiver
+// point pr with normal nb (null: a field destination); k = kr + i atten.
+// out2 = the complex replacement of A_e cos_s cos_i 0.9 G(r).
+void orc_near_pair(double kr, double atten, int nsub, float delta, float Ae, const float *pe, const float *xa,
+                   const float *ya, const float *na, const float *nb, const float *pr, double *out2) {
+    auto v = [](const float *q) { return V3{q[0], q[1], q[2]}; };
+    V3 nbv = nb ? v(nb) : V3{0, 0, 0};
+    cd r = near_quadrature_pair(cd(kr, atten), nsub, delta, Ae, v(pe), v(xa), v(ya), v(na), nb ? &nbv : nullptr,
+                                v(pr));
+    out2[0] = r.real();
+    out2[1] = r.imag();
+}
+
 void orc_set_near_quadrature(void *h, int nsub, float radius) {
     ((Scene *)h)->nearSub = nsub;
     ((Scene *)h)->nearRadius = radius;
@@ -465,7 +488,17 @@ int orc_load_geometry(void *h, const float *v, int nf, int type, float res) {
     }
     if (type == OBJ_TARGET) s.targets.push_back(std::move(o));
     else if (type == OBJ_FIELD) s.fields.push_back(std::move(o));
-    return 0;  // source objects are stored but never used (Model.cpp:24-42)
+    else {
+        // Source objects: stored but never used by the reference (Model.cpp:24-42);
+        // here the piston-like emitter of README.md:2 (coherent pressure across the object).
+        s.sources.push_back(std::move(o));
+        s.srcObjAmp.push_back(cd(1, 0));
+    }
+    return 0;
+}
+void orc_set_source_object_amplitude(void *h, int i, double re, double im) {
+    Scene &s = *(Scene *)h;
+    if (i >= 0 && i < (int)s.srcObjAmp.size()) s.srcObjAmp[i] = cd(re, im);
 }
 
 void orc_load_source_points(void *h, const float *p, int n) {
@@ -500,6 +533,7 @@ int orc_render(void *h) {
     if (s.rendered) return -2;
     for (auto &o : s.targets) prep_object(o);
     for (auto &o : s.fields) prep_object(o);
+    for (auto &o : s.sources) prep_object(o);
     s.tris.clear();
     float scale = 1.0f;
     auto upd = [&](V3 p) { scale = std::max(scale, std::max(std::fabs(p.x), std::max(std::fabs(p.y), std::fabs(p.z)))); };
@@ -509,6 +543,8 @@ int orc_render(void *h) {
     }
     for (auto &o : s.fields)
         for (auto &f : o.facets) { upd(f.v1); upd(f.v2); upd(f.v3); }
+    for (auto &o : s.sources)  // never occluders (only targets are, OptiX.cpp:273-292)
+        for (auto &f : o.facets) { upd(f.v1); upd(f.v2); upd(f.v3); }
     for (auto &p : s.srcPos) upd(p);
     for (auto &p : s.fldPos) upd(p);
     s.vp = make_vis_params(scale);
@@ -517,12 +553,30 @@ int orc_render(void *h) {
     return 0;
 }
 
+// A source object's cells (all at its amplitude) into the total pressure P of
+// object B: the fragment->fragment sum without reciprocity (FACET_NO_RESP,
+// OptiX.cu:192-406), receivers at B's fragment points, B's cosine cull unless B
+// is a field object.
+static void project_source_object(Scene &s, Object &S, cd amp, Object &B) {
+    std::fill(S.P.begin(), S.P.end(), amp);
+    std::vector<cd> keep = B.Pout;
+    std::fill(B.Pout.begin(), B.Pout.end(), cd(0, 0));
+    project_facets(s, S, B, 2);
+    for (size_t c = 0; c < B.ncell; c++) B.P[c] += B.Pout[c];
+    B.Pout.swap(keep);
+}
+
 int orc_project_src_to_objects(void *h, int fieldObjects) {
     Scene &s = *(Scene *)h;
     if (!s.rendered) return -3;
     for (auto &o : s.targets) project_src_to_object(s, o);
     if (fieldObjects)
         for (auto &o : s.fields) project_src_to_object(s, o);
+    for (size_t i = 0; i < s.sources.size(); i++) {
+        for (auto &o : s.targets) project_source_object(s, s.sources[i], s.srcObjAmp[i], o);
+        if (fieldObjects)
+            for (auto &o : s.fields) project_source_object(s, s.sources[i], s.srcObjAmp[i], o);
+    }
     return 0;
 }
 
@@ -585,7 +639,7 @@ void orc_get_field_points(void *h, double *out) {
 }
 
 static Object *obj(Scene &s, int type, int idx) {
-    std::vector<Object> &v = (type == OBJ_FIELD) ? s.fields : s.targets;
+    std::vector<Object> &v = (type == OBJ_FIELD) ? s.fields : (type == OBJ_SOURCE ? s.sources : s.targets);
     if (idx < 0 || idx >= (int)v.size()) return nullptr;
     return &v[idx];
 }

@@ -50,10 +50,26 @@ for name, args, res in [
     ("orc_anyhit", [_v, _fp, _fp, _fp, _fp, ctypes.c_long, _u8p, _i, _ip, _ip], None),
     ("orc_vis_params", [_v, _fp], None), ("orc_kf", [_v], _f),
     ("orc_set_near_quadrature", [_v, _i, _f], None),
+    ("orc_set_source_object_amplitude", [_v, _i, ctypes.c_double, ctypes.c_double], None),
+    ("orc_near_pair", [ctypes.c_double, ctypes.c_double, _i, _f, _f, _fp, _fp, _fp, _fp, _fp, _fp, _dp], None),
 ]:
     fn = getattr(_lib, name)
     fn.argtypes = args
     fn.restype = res
+
+
+def near_pair(kr, atten, nsub, delta, area, pe, xa, ya, na, nb, pr):
+    """One pair of the near-field judge (nsub x nsub sub-fragment quadrature of the
+    pair kernel over the emitter's delta x delta cell); nb None: field destination."""
+    out = np.zeros(2)
+    a = [_f32(q).reshape(3) for q in (pe, xa, ya, na)]
+    nbp = None if nb is None else _f32(nb).reshape(3)
+    prp = _f32(pr).reshape(3)
+    _lib.orc_near_pair(float(kr), float(atten), int(nsub), float(delta), float(area),
+                       *[q.ctypes.data_as(_fp) for q in a],
+                       None if nbp is None else nbp.ctypes.data_as(_fp), prp.ctypes.data_as(_fp),
+                       out.ctypes.data_as(_dp))
+    return complex(out[0], out[1])
 
 
 def set_threads(n):
@@ -99,6 +115,9 @@ class Scene:
 
     def set_source_amplitude(self, i, v):
         _lib.orc_set_source_amplitude(self.h, i, float(np.real(v)), float(np.imag(v)))
+
+    def set_source_object_amplitude(self, i, v):
+        _lib.orc_set_source_object_amplitude(self.h, i, float(np.real(v)), float(np.imag(v)))
 
     def set_bruteforce(self, on):
         _lib.orc_set_bruteforce(self.h, 1 if on else 0)
@@ -207,6 +226,8 @@ def run_scene(scene, plane_source=None, bruteforce=False, near=None):
         s.load_geometry(tris, 0, d)
     for tris, d in scene.get("fields", []):
         s.load_geometry(tris, 2, d)
+    for tris, d in scene.get("source_objects", []):
+        s.load_geometry(tris, 1, d)
     src = scene["sources"] if plane_source is None else scene["sources"][plane_source:plane_source + 1]
     s.load_source_points(src)
     s.load_field_points(scene["points"])

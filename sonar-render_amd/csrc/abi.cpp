@@ -15,6 +15,7 @@
 #include "hostio.h"
 #include "nearfield.h"
 #include "batch.h"
+#include "render.h"
 
 namespace {
 
@@ -26,12 +27,11 @@ int gNextCtx = 1;
 thread_local sr::Model *tCur = nullptr;
 thread_local int tCurId = 0;
 sr::Model &cur() { return tCur ? *tCur : gModel; }
-// Render colour settings (ModelCuda.hpp:45-48 defaults), set by SoundVisualisationInit.
-float gMaxDb = -20.f, gMinDb = -90.f;
-bool gPhaseTarget = false, gPhaseField = false;
-int gStatus = 0;
-std::string gMsg;
-float gLastMs = 0.f;
+// Status of the calling thread's last call: each thread reads its own, so
+// threads working on different contexts never see (or free) each other's message.
+thread_local int gStatus = 0;
+thread_local std::string gMsg;
+thread_local float gLastMs = 0.f;
 
 template <class F>
 int guarded(const char *name, F &&f) {
@@ -129,6 +129,10 @@ int sr_set_source_amplitude(int s, float re, float im) {
     return guarded("set_source_amplitude", [&] { return sr::model_set_source_amplitude(cur(), s, re, im); });
 }
 int sr_num_planes(void) { return cur().nplanes; }
+int sr_set_source_object_amplitude(int index, float re, float im) {
+    return guarded("set_source_object_amplitude",
+                   [&] { return sr::model_set_source_object_amplitude(cur(), index, re, im); });
+}
 int sr_context_create(void) {
     return guarded("context_create", [&] {
         std::lock_guard<std::mutex> g(gCtxMu);
@@ -183,12 +187,12 @@ int sr_teardown(void) {
 int sr_object_count(int type) {
     if (type == sr::OBJ_TARGET) return (int)cur().targets.size();
     if (type == sr::OBJ_FIELD) return (int)cur().fields.size();
-    if (type == sr::OBJ_SOURCE) return cur().numSourceObjects;
+    if (type == sr::OBJ_SOURCE) return (int)cur().sources.size();
     return -2;
 }
 static sr::Object *obj(int type, int idx) {
-    auto &v = (type == sr::OBJ_FIELD) ? cur().fields : cur().targets;
-    if (type != sr::OBJ_FIELD && type != sr::OBJ_TARGET) return nullptr;
+    if (type != sr::OBJ_FIELD && type != sr::OBJ_TARGET && type != sr::OBJ_SOURCE) return nullptr;
+    auto &v = (type == sr::OBJ_FIELD) ? cur().fields : (type == sr::OBJ_SOURCE ? cur().sources : cur().targets);
     if (idx < 0 || idx >= (int)v.size()) return nullptr;
     return &v[idx];
 }
@@ -224,6 +228,7 @@ long sr_mask_words(int kind, int ta, int ia, int tb, int ib) {
     case 2: if (!A) return -2; ne = nr = A->nact; break;
     case 3: case 4: if (!A || !B) return -2; ne = A->nact; nr = B->nact; break;
     case 5: if (!A || !B) return -2; ne = B->nact; nr = A->nact; break;
+    case 6: A = obj(sr::OBJ_SOURCE, ia); if (!A || !B) return -2; ne = A->nact; nr = B->nact; break;
     default: return -2;
     }
     return ((nr + 63) / 64) * ne;
@@ -353,9 +358,94 @@ int sr_surface_rgba(int type, int idx, int plane, float *rgba, long n) {
         std::vector<double> p((size_t)2 * std::max<long>(n, 0));
         int rc = sr::model_get_object_pressure(cur(), type, idx, 0, plane, p.data(), n);
         if (rc) return rc;
-        const bool ph = (type == sr::OBJ_FIELD) ? gPhaseField : gPhaseTarget;
-        sr::colour_map(p.data(), n, gMinDb, gMaxDb, ph, rgba);
+        const sr::Model &m = cur();
+        const bool ph = (type == sr::OBJ_FIELD) ? m.renderPhaseField : m.renderPhaseTarget;
+        sr::colour_map(p.data(), n, m.renderMinDb, m.renderMaxDb, ph, rgba);
         return 0;
+    });
+}
+
+static sr::RenderSettings render_settings(const sr::Model &m) {
+    sr::RenderSettings rs;
+    rs.maxDb = m.renderMaxDb;
+    rs.minDb = m.renderMinDb;
+    rs.phaseTarget = m.renderPhaseTarget;
+    rs.phaseField = m.renderPhaseField;
+    return rs;
+}
+int sr_sound_visualisation_init(float maxDb, float minDb, int phaseTarget, int phaseField) {
+    return guarded("SoundVisualisationInit", [&] {
+        if (!(maxDb > minDb)) { sr::set_error("SoundVisualisationInit: maxDb must exceed minDb"); return -2; }
+        sr::Model &m = cur();
+        m.renderMaxDb = maxDb;
+        m.renderMinDb = minDb;
+        m.renderPhaseTarget = phaseTarget != 0;
+        m.renderPhaseField = phaseField != 0;
+        return 0;
+    });
+}
+int sr_render_rgb(int plane, int width, int height, const float *view9, unsigned char *rgb) {
+    return guarded("render_rgb", [&] {
+        if (!view9 || !rgb) { sr::set_error("render_rgb: null argument"); return -2; }
+        std::vector<uint8_t> img;
+        int rc = sr::render_rgb(cur(), plane, width, height, view9, render_settings(cur()), img);
+        if (rc) return rc;
+        std::copy(img.begin(), img.end(), rgb);
+        return 0;
+    });
+}
+int sr_render_png(const char *filename, int width, int height, const float *view9, int plane) {
+    return guarded("RenderOpenGL", [&] {
+        if (!filename || !view9) { sr::set_error("RenderOpenGL: null file name or view settings"); return -2; }
+        std::vector<uint8_t> img;
+        int rc = sr::render_rgb(cur(), plane, width, height, view9, render_settings(cur()), img);
+        if (rc) return rc;
+        std::string err;
+        rc = sr::write_png(filename, width, height, img.data(), err);
+        if (rc) sr::set_error(err);
+        return rc;
+    });
+}
+int sr_render_facets_rgb(const float *v9, int nf, float delta, const double *p2, long ncell, int phase, float minDb,
+                         float maxDb, int width, int height, const float *view9, unsigned char *rgb) {
+    return guarded("render_facets_rgb", [&] {
+        if (nf < 0 || (nf > 0 && !v9) || !(delta > 0.f) || !view9 || !rgb) {
+            sr::set_error("render_facets_rgb: bad argument");
+            return -2;
+        }
+        std::vector<sr::FacetFrame> frames;
+        std::vector<sr::FacetGrid> grids;
+        std::vector<long> off;
+        long tot = 0;
+        for (int f = 0; f < nf; f++) {
+            const float *t = v9 + (size_t)9 * f;
+            frames.push_back(sr::facet_frame(sr::Vec3{t[0], t[1], t[2]}, sr::Vec3{t[3], t[4], t[5]},
+                                             sr::Vec3{t[6], t[7], t[8]}));
+            grids.push_back(sr::facet_grid(frames.back(), delta));
+            off.push_back(tot);
+            tot += (long)grids.back().nX * grids.back().nY;
+        }
+        if (ncell != tot || (tot > 0 && !p2)) {
+            sr::set_error("render_facets_rgb: the pressure array must hold every cell of every facet");
+            return -2;
+        }
+        std::vector<float> col(4 * (size_t)tot);
+        sr::colour_map(p2, tot, minDb, maxDb, phase != 0, col.data());
+        std::vector<sr::RenderObject> objs{sr::RenderObject{&frames, &grids, &off, col.data()}};
+        std::vector<uint8_t> img;
+        int rc = sr::render_objects(objs, width, height, view9, img);
+        if (rc) return rc;
+        std::copy(img.begin(), img.end(), rgb);
+        return 0;
+    });
+}
+int sr_write_png(const char *path, int width, int height, const unsigned char *rgb) {
+    return guarded("write_png", [&] {
+        if (!path || !rgb || width <= 0 || height <= 0) { sr::set_error("write_png: bad argument"); return -2; }
+        std::string err;
+        int rc = sr::write_png(path, width, height, rgb, err);
+        if (rc) sr::set_error(err);
+        return rc;
     });
 }
 
@@ -401,18 +491,11 @@ void ProjectTargetToFieldObjects(void) { report("ProjectTargetToFieldObjects", s
 void ProjectTargetToFieldPoints(void) { report("ProjectTargetToFieldPoints", sr_project_target_to_field_points()); }
 void ProjectTargetToTargetObjects(int n) { report("ProjectTargetToTargetObjects", sr_project_target_to_target_objects(n)); }
 void GetFieldPointPressures(sr_dcomplex *out, int n) { report("GetFieldPointPressures", sr_get_field_point_pressures(out, n)); }
-void RenderOpenGL(int, int, char *, float *) {
-    int rc = guarded("RenderOpenGL", [&] {
-        sr::set_error("the OpenGL/PNG render runs on the host outside this library");
-        return -2;
-    });
-    report("RenderOpenGL", rc);
+void RenderOpenGL(int width, int height, char *filename, float viewSettings[9]) {
+    report("RenderOpenGL", sr_render_png(filename, width, height, viewSettings, 0));
 }
 void SoundVisualisationInit(float maxDb, float minDb, bool phaseTarget, bool phaseField) {
-    gMaxDb = maxDb;
-    gMinDb = minDb;
-    gPhaseTarget = phaseTarget;
-    gPhaseField = phaseField;
+    report("SoundVisualisationInit", sr_sound_visualisation_init(maxDb, minDb, phaseTarget, phaseField));
 }
 void TearDownCuda(void) { report("TearDownCuda", sr_teardown()); }
 

@@ -135,7 +135,7 @@ void release_all(Batch &b) {
     b.scratchElems = 0;
     for (auto &m : b.scenes) {
         m->stream = nullptr;  // the batch owns the stream
-        model_teardown(*m);
+        model_release(*m);    // also the scene's side stream, events and pinned words
     }
     b.scenes.clear();
     b.prepared = false;
@@ -408,18 +408,29 @@ int batch_prepare() {
         }
         b.scenes.push_back(std::move(m));
     }
+    // a failure from here on releases everything prepared so far
+    auto fail = [&]() {
+        release_all(b);
+        return -1;
+    };
     for (auto &mp : b.scenes) {
         for (int kind : {0, 2, 1}) {
             MaskTask tk;
             size_t words;
-            if (mask_task(*mp, kind, 0, 0, 0, 0, tk, words)) return -1;
+            if (mask_task(*mp, kind, 0, 0, 0, 0, tk, words)) return fail();
             uint64_t *p = nullptr;
-            SR_HIP(hipMalloc((void **)&p, words * sizeof(uint64_t)));
+            if (hipMalloc((void **)&p, words * sizeof(uint64_t)) != hipSuccess) {
+                set_error("batch_prepare: out of device memory for the visibility words");
+                return fail();
+            }
             b.maskBufs.push_back(p);
         }
     }
-    SR_HIP(hipMalloc((void **)&b.counters, (3 + b.bounces) * sizeof(int)));
-    if (build_tables(b)) return -1;
+    if (hipMalloc((void **)&b.counters, (3 + b.bounces) * sizeof(int)) != hipSuccess) {
+        set_error("batch_prepare: out of device memory");
+        return fail();
+    }
+    if (build_tables(b)) return fail();
     b.prepared = true;
     return 0;
 }

@@ -58,6 +58,7 @@ FacetFrame facet_frame(Vec3 v1, Vec3 v2, Vec3 v3) {
     f.xAxis = unit(edge);
     f.yAxis = unit(up);
     f.area = (f.baseLength * f.height) / 2.0f;
+    f.longest = longest;
     return f;
 }
 

@@ -11,6 +11,7 @@ struct FacetFrame {
     Vec3 v1, v2, v3;
     Vec3 normal, xAxis, yAxis, base;
     float baseLength, height, lenNeg, lenPos, area;
+    int longest;  // which edge is the base: 0 v1v2, 1 v1v3, 2 v2v3
 };
 
 struct FacetGrid {

@@ -21,8 +21,9 @@ int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipS
 
 namespace {
 
-std::mutex gErrMu;
-std::string gErr;
+// The error text of the calling thread's current call (one per thread, like
+// the status the C-ABI reports from it).
+thread_local std::string gErr;
 
 template <class T>
 int dev_alloc(T **p, size_t n) {
@@ -37,10 +38,13 @@ void dev_free(T *&p) {
     p = nullptr;
 }
 
-enum MaskKind { MK_P2F = 0, MK_F2P = 1, MK_SELF = 2, MK_NORESP = 3, MK_RESP_FWD = 4, MK_RESP_REC = 5 };
+enum MaskKind {
+    MK_P2F = 0, MK_F2P = 1, MK_SELF = 2, MK_NORESP = 3, MK_RESP_FWD = 4, MK_RESP_REC = 5,
+    MK_SRCOBJ = 6  // source object (ta = 1, ia) -> target or field object (tb, ib)
+};
 
 Object *object_at(Model &m, int type, int idx) {
-    std::vector<Object> &v = (type == OBJ_FIELD) ? m.fields : m.targets;
+    std::vector<Object> &v = (type == OBJ_FIELD) ? m.fields : (type == OBJ_SOURCE ? m.sources : m.targets);
     if (idx < 0 || idx >= (int)v.size()) return nullptr;
     return &v[idx];
 }
@@ -147,6 +151,14 @@ void free_object(Object &o) {
     free_transfer(o);
 }
 
+// Every cell of a source object at its amplitude (host copy; set-up time only).
+int fill_pressure(Object &o, float2 amp) {
+    if (!o.ncell) return 0;
+    std::vector<float2> h(o.ncell, amp);
+    SR_HIP(hipMemcpy(o.P, h.data(), o.ncell * sizeof(float2), hipMemcpyHostToDevice));
+    return 0;
+}
+
 int ensure_scratch(Model &m, size_t elems) {
     if (elems <= m.scratchElems) return 0;
     dev_free(m.scratchL);
@@ -232,14 +244,8 @@ int word_wait(Model &m, int slot, int &out) {
 
 int settle(Model &m);
 
-void set_error(const std::string &msg) {
-    std::lock_guard<std::mutex> g(gErrMu);
-    gErr = msg;
-}
-std::string last_error() {
-    std::lock_guard<std::mutex> g(gErrMu);
-    return gErr;
-}
+void set_error(const std::string &msg) { gErr = msg; }
+std::string last_error() { return gErr; }
 void clear_error() { set_error(""); }
 
 // ---------------------------------------------------------------------------
@@ -258,7 +264,6 @@ int model_load_geometry(Model &m, const float *v, int nf, int type, float res) {
     if (m.rendered) { set_error("load_geometry: scene already rendered; call TearDownCuda first"); return -3; }
     if (nf < 0 || (nf > 0 && !v)) { set_error("load_geometry: bad facet array"); return -2; }
     if (!(res > 0.f)) { set_error("load_geometry: resolution must be > 0"); return -2; }
-    if (type == OBJ_SOURCE) { m.numSourceObjects++; return 0; }  // stored, never used (Model.cpp:33-35)
     Object o;
     o.type = type;
     o.delta = res;
@@ -266,6 +271,11 @@ int model_load_geometry(Model &m, const float *v, int nf, int type, float res) {
     for (int i = 0; i < nf; i++) {
         const float *t = v + (size_t)9 * i;
         o.frames.push_back(facet_frame(Vec3{t[0], t[1], t[2]}, Vec3{t[3], t[4], t[5]}, Vec3{t[6], t[7], t[8]}));
+    }
+    if (type == OBJ_SOURCE) {
+        m.sources.push_back(std::move(o));
+        m.srcObjAmp.push_back(make_float2(1.f, 0.f));
+        return 0;
     }
     (type == OBJ_TARGET ? m.targets : m.fields).push_back(std::move(o));
     return 0;
@@ -302,9 +312,17 @@ int model_render(Model &m) {
     *(volatile int *)m.errHost = 0;
     m.errPending = false;
     const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
+    if (m.batchedSources && !m.sources.empty()) {
+        set_error("source objects radiate into every incidence: use them with unbatched source points");
+        return -3;
+    }
     m.nplanes = m.batchedSources ? std::max(1, ns) : 1;
     for (auto &o : m.targets) if (upload_object(m, o)) return -1;
     for (auto &o : m.fields) if (upload_object(m, o)) return -1;
+    for (size_t i = 0; i < m.sources.size(); i++) {
+        if (upload_object(m, m.sources[i])) return -1;
+        if (fill_pressure(m.sources[i], m.srcObjAmp[i])) return -1;
+    }
 
     // Occluders: every target triangle in load order (OptiX.cpp:273-292).
     std::vector<float4> tris;
@@ -333,8 +351,9 @@ int model_render(Model &m) {
             upd(a.x, a.y, a.z); upd(b.x, b.y, b.z); upd(c.x, c.y, c.z);
         }
     }
-    for (auto &o : m.fields)
-        for (auto &F : o.frames) { upd(F.v1.x, F.v1.y, F.v1.z); upd(F.v2.x, F.v2.y, F.v2.z); upd(F.v3.x, F.v3.y, F.v3.z); }
+    for (auto *v : {&m.fields, &m.sources})  // never occluders: only targets are (OptiX.cpp:273-292)
+        for (auto &o : *v)
+            for (auto &F : o.frames) { upd(F.v1.x, F.v1.y, F.v1.z); upd(F.v2.x, F.v2.y, F.v2.z); upd(F.v3.x, F.v3.y, F.v3.z); }
     for (size_t i = 0; i < m.srcHost.size(); i += 3) upd(m.srcHost[i], m.srcHost[i + 1], m.srcHost[i + 2]);
     for (size_t i = 0; i < m.fldHost.size(); i += 3) upd(m.fldHost[i], m.fldHost[i + 1], m.fldHost[i + 2]);
     m.vc.scale = scale;
@@ -391,7 +410,7 @@ int model_set_source_amplitude(Model &m, int i, float re, float im) {
 int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, size_t &words) {
     tk = MaskTask{};
     tk.emTriBase = tk.rcTriBase = -1;
-    Object *A = (kind == MK_P2F) ? nullptr : object_at(m, ta, ia);
+    Object *A = (kind == MK_P2F || kind == MK_SRCOBJ) ? nullptr : object_at(m, ta, ia);
     Object *B = (kind == MK_P2F || kind >= MK_NORESP) ? object_at(m, tb, ib) : nullptr;
     const int ns = (int)(m.srcHost.size() / 3), nq = (int)(m.fldHost.size() / 3);
     switch (kind) {
@@ -431,6 +450,16 @@ int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, 
         tk.emTriBase = A->triBase;
         tk.rcTriBase = B->triBase;
         tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 1; tk.useNormalRc = 1;
+        tk.tmin = m.vc.tminFrag;
+        break;
+    case MK_SRCOBJ:
+        A = object_at(m, OBJ_SOURCE, ia);
+        if (!A || !B) { set_error("source-object mask: no such object"); return -2; }
+        tk.em = PointSet{A->actP, A->actN, A->nact};
+        tk.rc = PointSet{B->actP, B->actN, B->nact};
+        tk.emTriBase = -1;  // source objects never occlude
+        tk.rcTriBase = B->type == OBJ_TARGET ? B->triBase : -1;
+        tk.origin = ORIGIN_EMITTER; tk.useNormalEm = 1; tk.useNormalRc = (B->type == OBJ_TARGET);
         tk.tmin = m.vc.tminFrag;
         break;
     case MK_RESP_REC:
@@ -498,6 +527,8 @@ int settle(Model &m) {
     if (*err) {
         *err = 0;
         for (auto &kv : m.masks) kv.second.valid = false;
+        // transfer matrices built from those words are rebuilt with them
+        for (auto &o : m.targets) o.tc.valid = false;
         set_error("visibility traversal stack overflow in a projection since the last result read");
         return -1;
     }
@@ -509,14 +540,16 @@ static int check_rendered(const Model &m) {
     return 0;
 }
 
+static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, ReduceTask rt, int nplanes,
+                         long evalStride, long gvalStride, long outStride, long pcellStride, double pairs);
+
 // ProjectSrcPointsToObjects (ModelCuda.cpp:242-278)
 int model_project_src(Model &m, bool fieldObjects) {
     if (int rc = check_rendered(m)) return rc;
     const int ns = (int)(m.srcHost.size() / 3);
-    if (ns == 0) return 0;
     Phys ph = make_phys(m, 0.f, 1.f);
     auto run = [&](Object &o, int type, int idx) -> int {
-        if (o.ncell == 0) return 0;
+        if (o.ncell == 0 || ns == 0) return 0;
         uint64_t *mk;
         if (get_mask(m, MK_P2F, 0, 0, type, idx, &mk)) return -1;
         P2FTask t{o.cellP, o.cellN, (int)o.ncell, o.type == OBJ_TARGET, m.srcP, m.srcAmp, ns, mk,
@@ -527,7 +560,40 @@ int model_project_src(Model &m, bool fieldObjects) {
     for (int i = 0; i < (int)m.targets.size(); i++) if (run(m.targets[i], OBJ_TARGET, i)) return -1;
     if (fieldObjects)
         for (int i = 0; i < (int)m.fields.size(); i++) if (run(m.fields[i], OBJ_FIELD, i)) return -1;
+    // Source objects: each cell at the object's coherent amplitude, summed into the
+    // receivers' total pressure P by the fragment->fragment kernel without
+    // reciprocity (FACET_NO_RESP form, OptiX.cu:192-406; receivers at their fragment points).
+    for (int si = 0; si < (int)m.sources.size(); si++) {
+        Object &S = m.sources[si];
+        if (!S.nact) continue;
+        auto emit = [&](Object &B, int type, int idx) -> int {
+            if (!B.nact) return 0;
+            uint64_t *mk;
+            if (get_mask(m, MK_SRCOBJ, OBJ_SOURCE, si, type, idx, &mk)) return -1;
+            Phys pf = make_phys(m, S.delta, 0.9f);
+            GatherTask t{};
+            t.ep = S.actP; t.en = S.actN; t.ecell = S.actCell; t.eval = S.P; t.ne = S.nact;
+            t.rp = B.actP; t.rn = B.actN; t.nr = B.nact; t.mask = mk;
+            t.gateMode = GATE_NONE;
+            ReduceTask r{};
+            r.gateMode = GATE_NONE;
+            r.pcell = B.P; r.rowCell = B.actCell;
+            const int kind = type == OBJ_FIELD ? KIND_F2F_FIELD : KIND_F2F;
+            return gather_planes(m, t, pf, kind, r, m.nplanes, S.ncell, 0, 0, B.ncell, (double)S.nact * B.nact);
+        };
+        for (int i = 0; i < (int)m.targets.size(); i++) if (emit(m.targets[i], OBJ_TARGET, i)) return -1;
+        if (fieldObjects)
+            for (int i = 0; i < (int)m.fields.size(); i++) if (emit(m.fields[i], OBJ_FIELD, i)) return -1;
+    }
     return finish_projection(m);
+}
+
+int model_set_source_object_amplitude(Model &m, int i, float re, float im) {
+    if (i < 0 || i >= (int)m.sources.size()) { set_error("source object index out of range"); return -2; }
+    m.srcObjAmp[i] = make_float2(re, im);
+    if (!m.rendered) return 0;
+    if (int rc = settle(m)) return rc;  // queued projections still read the cells
+    return fill_pressure(m.sources[i], m.srcObjAmp[i]);
 }
 
 // One gather + reduce over all planes in groups of <= 16.
@@ -746,7 +812,7 @@ static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, Reduc
 static int ensure_transfer(Model &m, int i) {
     Object &o = m.targets[i];
     auto &t = o.tc;
-    if (t.valid) return 0;
+    if (t.valid && (m.nearMode != 2 || t.nfGen == nearfield().gen)) return 0;
     uint64_t *mk;
     if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
     const int n = o.nact;
@@ -912,8 +978,9 @@ static int ensure_transfer(Model &m, int i) {
         ph.mlp = nf.P;
     }
     int counts[2] = {0, 0};
-    const double key[4] = {(double)m.nearMode, (double)m.nearRadiusWl, (double)m.nearQuadDeltas, (double)m.kf};
-    const bool known = std::equal(key, key + 4, t.nearKey);
+    const double key[5] = {(double)m.nearMode, (double)m.nearRadiusWl, (double)m.nearQuadDeltas, (double)m.kf,
+                           (double)o.delta};
+    const bool known = std::equal(key, key + 5, t.nearKey);
     for (int pass = 0; pass < 2; pass++) {
         if (m.nearMode != 0) {
             if (!t.nearCount && dev_alloc(&t.nearCount, 2)) return -1;
@@ -939,7 +1006,7 @@ static int ensure_transfer(Model &m, int i) {
         SR_HIP(hipMemcpyAsync(counts, t.nearCount, 2 * sizeof(int), hipMemcpyDeviceToHost, m.stream));
         SR_HIP(hipStreamSynchronize(m.stream));
         if (counts[0] <= t.nearCap && counts[1] <= t.nearCap) {
-            std::copy(key, key + 4, t.nearKey);
+            std::copy(key, key + 5, t.nearKey);
             t.nearKnown[0] = counts[0];
             t.nearKnown[1] = counts[1];
             break;
@@ -954,6 +1021,7 @@ static int ensure_transfer(Model &m, int i) {
         t.nearClear = true;  // cleared by the near-pair kernels, or never counted into
     }
     t.valid = true;
+    t.nfGen = nearfield().gen;
     return 0;
 }
 
@@ -1318,9 +1386,11 @@ void model_teardown(Model &m) {
     if (m.side) (void)hipStreamSynchronize(m.side);
     for (auto &o : m.targets) free_object(o);
     for (auto &o : m.fields) free_object(o);
+    for (auto &o : m.sources) free_object(o);
     m.targets.clear();
     m.fields.clear();
-    m.numSourceObjects = 0;
+    m.sources.clear();
+    m.srcObjAmp.clear();
     for (auto &kv : m.masks) (void)hipFree(kv.second.p);
     m.masks.clear();
     m.transferMode = 0;

@@ -68,7 +68,8 @@ struct Object {
         // counts depend only on geometry, visibility and these, so a rebuild with
         // the same settings needs no read-back
         int nearKnown[2] = {0, 0};
-        double nearKey[4] = {-1, 0, 0, 0};
+        double nearKey[5] = {-1, 0, 0, 0, 0};
+        unsigned nfGen = 0;     // learned mode: the network generation the near terms come from
         float *colScale = nullptr, *colInv = nullptr;
         float sS = 1.f;
         double aBytesRead = 0;  // bytes of S one GEMM pass reads (same-facet steps are skipped)
@@ -86,7 +87,13 @@ struct Model {
     float cp = 1480.f, freq = 0.f, atten = 0.f, density = 0.f;
     float kf = 0.f;
     std::vector<Object> targets, fields;
-    int numSourceObjects = 0;
+    // source objects (type 1): piston-like emitters, every cell at one coherent
+    // amplitude (README.md:2; stored but never used by the reference, Model.cpp:33-35)
+    std::vector<Object> sources;
+    std::vector<float2> srcObjAmp;
+    // surface colour settings of the render (SoundVisualisationInit; ModelCuda.hpp:45-48 defaults)
+    float renderMaxDb = -20.f, renderMinDb = -90.f;
+    bool renderPhaseTarget = false, renderPhaseField = false;
     std::vector<float> srcHost, fldHost;  // xyz triples
     bool rendered = false;
     bool batchedSources = false;

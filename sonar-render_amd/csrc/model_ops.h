@@ -18,6 +18,7 @@ int model_set_batched_sources(Model &m, int on);
 int model_render(Model &m);
 int model_set_source_amplitude(Model &m, int i, float re, float im);
 int model_project_src(Model &m, bool fieldObjects);
+int model_set_source_object_amplitude(Model &m, int i, float re, float im);
 int model_project_t2t(Model &m, int nref);
 int model_project_t2field_objects(Model &m);
 int model_project_t2field_points(Model &m);

@@ -65,6 +65,7 @@ int nf_init(uint32_t seed) {
     std::vector<float> P = init_params(seed);
     SR_HIP(hipMemcpy(nf.P, P.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
     nf.trained = false;
+    nf.gen++;
     return 0;
 }
 
@@ -75,6 +76,7 @@ int nf_set_params(const float *in, int n) {
     SR_HIP(hipDeviceSynchronize());  // queued projections may still read the parameters
     SR_HIP(hipMemcpy(nf.P, in, n * sizeof(float), hipMemcpyHostToDevice));
     nf.trained = true;
+    nf.gen++;
     return 0;
 }
 
@@ -138,6 +140,9 @@ int nf_train(int nsamples, int steps, int batch, float lr, uint32_t seed, float 
     if (nf_ensure(nf)) return -1;
     SR_HIP(hipDeviceSynchronize());  // queued projections may still read the parameters
     if (nsamples < batch || batch <= 0 || steps <= 0) { set_error("nf_train: bad sizes"); return -2; }
+    // the parameters change in place from here on: not usable until training succeeds
+    nf.trained = false;
+    nf.gen++;
     const int nval = 16384;
     float *X, *Y, *Xv, *Yv, *H1, *H2, *O, *G3, *D2, *D1, *g, *m, *v, *acc;
     if (dalloc(&X, (size_t)nsamples * kMlpIn) || dalloc(&Y, (size_t)nsamples * 2) || dalloc(&Xv, (size_t)nval * kMlpIn) ||

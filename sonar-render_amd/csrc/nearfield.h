@@ -13,6 +13,9 @@ struct NearField {
     float *P = nullptr;   // device parameters (kMlpParams FP32)
     bool trained = false;
     float trainErr = -1.f, valErr = -1.f;
+    // bumped whenever the parameters change: transfer matrices holding learned
+    // near terms record the generation they were built with
+    unsigned gen = 0;
 };
 
 NearField &nearfield();

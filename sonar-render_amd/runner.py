@@ -24,6 +24,8 @@ def load_scene(scene, batched=None, transfer_mode=0):
         api.load_stl_mesh_to_cuda(tris, 0, d)
     for tris, d in scene.get("fields", []):
         api.load_stl_mesh_to_cuda(tris, 2, d)
+    for tris, d in scene.get("source_objects", []):
+        api.load_stl_mesh_to_cuda(tris, 1, d)
     api.render_cuda()
     api.set_transfer_mode(transfer_mode)
 

@@ -135,6 +135,29 @@ def config_sphere(f=10e3, c=C_WATER, frac=7.5, subdiv=4, n_points=360, surface_c
         bounces=0, project_field_objects=False)
 
 
+def config_piston(f=10e3, c=C_WATER, frac=7.5, n_points=91, field_surface=False):
+    """A 0.2 m square piston source object 2 m above a 1 m plate, 20 degrees off
+    its normal, facing it; field points on a 10 m arc; optional field plane."""
+    lam = c / f
+    a = math.radians(20.0)
+    # piston: a small plate rotated to face the origin from (2 sin a, 0, 2 cos a)
+    pis = make_rectangle(0.2, 0.2)[:, :, :] * np.float32(1.0)
+    pis = pis[:, [0, 2, 1]]                       # reversed winding: normal -z
+    rot = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    centre = np.array([2 * math.sin(a), 0.0, 2 * math.cos(a)])
+    pis = (pis.reshape(-1, 3) @ rot.T + centre).reshape(-1, 3, 3).astype(np.float32)
+    fields = []
+    if field_surface:
+        fields = [(make_rectangle(1.5, 1.5, xy_plane=False) + np.float32([0.0, 0.0, 1.0]), lam / frac)]
+    return dict(
+        name="piston_plate_10kHz", f=f, c=c, atten=0.0,
+        targets=[(make_rectangle(1.0, 1.0), lam / frac)], fields=fields,
+        source_objects=[(pis, lam / frac)],
+        sources=np.zeros((0, 3), np.float32),
+        points=generate_field_points(10.0, np.linspace(-90, 90, n_points)),
+        bounces=0, project_field_objects=field_surface)
+
+
 def config_trihedral(f=20e3, c=C_WATER, frac=7.5, n_side=8, bounces=3, radius=15.0):
     """Trihedral corner reflector, 1 m legs, batched octant incidences, monostatic."""
     lam = c / f

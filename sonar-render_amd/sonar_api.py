@@ -77,6 +77,13 @@ _sig("sr_object_cell_count", [_i, _i], _l)
 _sig("sr_mask_words", [_i, _i, _i, _i, _i], _l)
 _sig("sr_fragment_facet", [_fp, _f, _ip, _fp, _fp, _fp, _l], _l)
 _sig("sr_last_call_ms", [], ctypes.c_float)
+_u8p = ctypes.POINTER(ctypes.c_ubyte)
+_sig("sr_sound_visualisation_init", [_f, _f, _i, _i])
+_sig("sr_set_source_object_amplitude", [_i, _f, _f])
+_sig("sr_render_png", [ctypes.c_char_p, _i, _i, _fp, _i])
+_sig("sr_render_rgb", [_i, _i, _i, _fp, _u8p])
+_sig("sr_render_facets_rgb", [_fp, _i, _f, _dp, _l, _i, _f, _f, _i, _i, _fp, _u8p])
+_sig("sr_write_png", [ctypes.c_char_p, _i, _i, _u8p])
 
 
 class SonarRenderError(RuntimeError):
@@ -133,8 +140,11 @@ def GetFieldPoints(NumFieldPnts):
 
 
 def render_openGL(width, height, viewSettings, filename="output"):
-    """The OpenGL/PNG render is host-side and outside this library."""
-    raise NotImplementedError("render_openGL: the OpenGL/PNG render is not part of the GPU engine")
+    """Writes ``filename + ".png"``: the scene's surfaces seen from viewSettings
+    (eye, look-at point, up vector), as api.py:70-79 (rendered on the host)."""
+    view = _f32(viewSettings).reshape(9)
+    _check(cpp_lib.sr_render_png((filename + ".png").encode("utf-8"), int(width), int(height), _ptr(view), 0),
+           "RenderOpenGL")
 
 
 def TearDownCuda():
@@ -158,7 +168,37 @@ def project_target_to_target_objects(number_reflections):
 
 
 def sound_visualisation_init(max_db, min_db, render_phase_target, render_phase_field):
-    raise NotImplementedError("sound_visualisation_init: render settings belong to the host renderer")
+    """Colour range in dB and phase flags of the render (python_interface.cpp:103-107)."""
+    _check(cpp_lib.sr_sound_visualisation_init(float(max_db), float(min_db), 1 if render_phase_target else 0,
+                                               1 if render_phase_field else 0), "SoundVisualisationInit")
+
+
+def render_rgb(width, height, viewSettings, plane=0):
+    """The render as a (height, width, 3) uint8 array, top row first."""
+    view = _f32(viewSettings).reshape(9)
+    out = np.zeros((height, width, 3), np.uint8)
+    _check(cpp_lib.sr_render_rgb(int(plane), int(width), int(height), _ptr(view),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))), "render_rgb")
+    return out
+
+
+def render_facets_rgb(tris, delta, pressures, width, height, viewSettings, phase=False, min_db=-90.0, max_db=-20.0):
+    """Host-only render of one object (triangles, fragment edge, complex per-cell pressures)."""
+    v = _f32(tris).reshape(-1, 9)
+    p = np.ascontiguousarray(np.stack([np.real(pressures), np.imag(pressures)], 1).astype(np.float64))
+    view = _f32(viewSettings).reshape(9)
+    out = np.zeros((height, width, 3), np.uint8)
+    _check(cpp_lib.sr_render_facets_rgb(_ptr(v), len(v), float(delta), _ptr(p, _dp), len(p), 1 if phase else 0,
+                                        float(min_db), float(max_db), int(width), int(height), _ptr(view),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))), "render_facets_rgb")
+    return out
+
+
+def write_png(path, rgb):
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    h, w = rgb.shape[:2]
+    _check(cpp_lib.sr_write_png(path.encode("utf-8"), w, h, rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))),
+           "write_png")
 
 
 # ---------------------------------------------------------------------------
@@ -187,6 +227,11 @@ def field_points_c64(n):
 
 def set_source_amplitude(i, value):
     _check(cpp_lib.sr_set_source_amplitude(i, float(np.real(value)), float(np.imag(value))), "set_source_amplitude")
+
+
+def set_source_object_amplitude(i, value):
+    _check(cpp_lib.sr_set_source_object_amplitude(i, float(np.real(value)), float(np.imag(value))),
+           "set_source_object_amplitude")
 
 
 def prepare_resp(target):

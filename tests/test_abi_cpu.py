@@ -52,14 +52,53 @@ def test_error_paths_return_status(api):
     assert api.cpp_lib.sr_last_status() == -2
     with pytest.raises(api.SonarRenderError):
         api.load_stl_mesh_to_cuda(tri[None], 5, 0.1)
-    with pytest.raises(NotImplementedError):
-        api.render_openGL(8, 8, [0.0] * 9)
+    # the render needs a rendered scene (status -3), not a GPU call
+    with pytest.raises(api.SonarRenderError, match="-3"):
+        api.render_openGL(8, 8, [0.0, 0, 3, 0, 0, 0, 0, 1, 0])
+    with pytest.raises(api.SonarRenderError):
+        api.sound_visualisation_init(-90.0, -20.0, False, False)  # max below min
 
 
-def test_source_objects_are_stored_not_used(api):
+def test_source_objects_are_counted(api):
     tri = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
     api.load_stl_mesh_to_cuda(tri, 1, 0.1)
     assert api.object_count(1) == 1 and api.object_count(0) == 0
+
+
+def test_status_is_per_thread():
+    """Two threads, each on its own context, fail concurrently with different
+    errors; each reads back its own status and message (sr_last_error)."""
+    import threading
+    import sonar_api as api
+    lib = api.cpp_lib
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    bad = []
+    barrier = threading.Barrier(2)
+
+    def worker(kind):
+        h = lib.sr_context_create()
+        assert h > 0 and lib.sr_context_select(h) == 0
+        barrier.wait()
+        for _ in range(2000):
+            if kind == 0:
+                rc = lib.sr_load_geometry(tri.ctypes.data_as(api._fp), 1, 7, 0.1)
+                want, rcw = "invalid object type", -2
+            else:
+                rc = lib.sr_project_src_points_to_objects(0)
+                want, rcw = "render_cuda", -3
+            msg = lib.sr_last_error().decode()
+            if rc != rcw or lib.sr_last_status() != rcw or want not in msg:
+                bad.append((kind, rc, msg))
+                break
+        lib.sr_context_select(0)
+        lib.sr_context_destroy(h)
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in (0, 1)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not bad, bad
 
 
 def test_fragment_probe_shapes(api):
