@@ -268,6 +268,21 @@ double sr_batch_pairs(void);
 int sr_batch_get_tune(int *out4);
 int sr_batch_set_tune(const int *in4);
 
+/* ProjectTargetToFieldPoints that allocates nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops). Its split-K partial sums go to a fixed
+ * 64 MiB device array per GPU (two halves of 4 Mi complex FP32 values) instead of
+ * the library's scratch; the field-point visibility words it needs are allocated
+ * by render_cuda. It uses fewer emitter splits when the partials would not fit,
+ * and fails with -2 when not even one split fits (more than 65,536 field points
+ * in the 64-plane form). The array is shared by every call on that GPU and keeps
+ * the last call's contents: calls that overlap on one GPU (other threads'
+ * contexts) race and must not run concurrently. Results equal
+ * ProjectTargetToFieldPoints' whenever its partial sums fit the array. */
+int sr_project_target_to_field_points_noalloc(void);
+/* Device allocations the scene model has made so far in this process. */
+long sr_device_allocations(void);
+
 /* hipDeviceSynchronize on the library's runtime. */
 int sr_device_synchronize(void);
 

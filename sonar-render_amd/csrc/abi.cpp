@@ -16,6 +16,7 @@
 #include "nearfield.h"
 #include "batch.h"
 #include "render.h"
+#include "noalloc.h"
 
 namespace {
 
@@ -96,6 +97,15 @@ int sr_project_target_to_field_objects(void) {
 int sr_project_target_to_field_points(void) {
     return timed("ProjectTargetToFieldPoints", [&] { return sr::model_project_t2field_points(cur()); });
 }
+int sr_project_target_to_field_points_noalloc(void) {
+    return timed("ProjectTargetToFieldPoints (allocation-free)", [&] {
+        float2 *scr = nullptr;
+        size_t elems = 0;
+        if (sr::noalloc_scratch(&scr, &elems)) return -1;
+        return sr::model_project_t2field_points_noalloc(cur(), scr, elems);
+    });
+}
+long sr_device_allocations(void) { return sr::model_device_allocations(); }
 int sr_project_target_to_target_objects(int n) {
     return timed("ProjectTargetToTargetObjects", [&] { return sr::model_project_t2t(cur(), n); });
 }

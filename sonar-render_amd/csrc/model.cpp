@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 
 #include "launch.h"
@@ -25,10 +26,14 @@ namespace {
 // the status the C-ABI reports from it).
 thread_local std::string gErr;
 
+// Device allocations made by the model (sr_device_allocations reports the count).
+std::atomic<long> gDevAllocs{0};
+
 template <class T>
 int dev_alloc(T **p, size_t n) {
     *p = nullptr;
     if (n == 0) return 0;
+    gDevAllocs++;
     SR_HIP(hipMalloc((void **)p, n * sizeof(T)));
     return 0;
 }
@@ -159,7 +164,18 @@ int fill_pressure(Object &o, float2 amp) {
     return 0;
 }
 
+// The partial-sum buffers of the current call: the library's, or in an
+// allocation-free call the fixed ones its entry point provides.
+float2 *scr_l(Model &m) { return m.noAlloc ? m.fixL : m.scratchL; }
+float2 *scr_u(Model &m) { return m.noAlloc ? m.fixU : m.scratchU; }
+
 int ensure_scratch(Model &m, size_t elems) {
+    if (m.noAlloc) {
+        if (elems <= m.fixElems) return 0;
+        set_error("partial sums exceed the allocation-free scratch (" + std::to_string(m.fixElems) +
+                  " complex values per half)");
+        return -2;
+    }
     if (elems <= m.scratchElems) return 0;
     dev_free(m.scratchL);
     dev_free(m.scratchU);
@@ -190,7 +206,7 @@ struct ProfScope {
     }
     ProfScope(Model &mm, int k, double p, double f, double by = 0.0, hipStream_t st = nullptr)
         : m(mm), kind(k), pairs(p), flops(f), bytes(by), s(st ? st : mm.stream) {
-        if (!m.profiling) return;
+        if (!m.profiling || m.noAlloc) return;  // allocation-free calls record no events
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
         (void)hipEventRecord(a, s);
     }
@@ -391,6 +407,19 @@ int model_render(Model &m) {
         SR_HIP(hipMemcpy(m.fldP, fp.data(), nq * sizeof(float4), hipMemcpyHostToDevice));
         SR_HIP(hipMemset(m.fldVal, 0, (size_t)nq * m.nplanes * sizeof(double2)));
     }
+    // The field-point visibility words of every target are allocated here, so that
+    // field-point projections (the allocation-free one included) allocate nothing.
+    if (nq)
+        for (int i = 0; i < (int)m.targets.size(); i++) {
+            if (!m.targets[i].nact) continue;
+            MaskTask tk{};
+            size_t words = 0;
+            if (mask_task(m, MK_F2P, OBJ_TARGET, i, 0, 0, tk, words)) return -1;
+            Model::MaskBuf &mb = m.masks[std::make_tuple((int)MK_F2P, (int)OBJ_TARGET, i, 0, 0)];
+            if (dev_alloc(&mb.p, words)) return -1;
+            mb.words = words;
+            mb.valid = false;
+        }
     SR_HIP(hipDeviceSynchronize());
     m.rendered = true;
     return 0;
@@ -488,6 +517,13 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
     MaskTask tk{};
     size_t words = 0;
     if (mask_task(m, kind, ta, ia, tb, ib, tk, words)) return -1;
+    if (m.noAlloc) {  // the buffer must exist already (render_cuda allocates the field-point words)
+        auto f = m.masks.find(key);
+        if (f == m.masks.end() || f->second.words != words) {
+            set_error("allocation-free call: its visibility words are not allocated");
+            return -3;
+        }
+    }
     Model::MaskBuf &mb = m.masks[key];
     if (mb.words != words) {
         dev_free(mb.p);
@@ -641,7 +677,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         g.eval = t.eval + b0 * evalStride;
         g.gval = nullptr;
         g.gateMode = GATE_NONE;
-        g.partL = m.scratchL;
+        g.partL = scr_l(m);
         g.partU = nullptr;
         {
             // flops: the complex multiply-accumulates executed on the matrix cores, 8 per
@@ -652,7 +688,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
             if (f2f_mfma_launch(g, ph, KIND_F2F, m.stream)) return -1;
         }
         ReduceTask r = rt;
-        r.partL = m.scratchL;
+        r.partL = scr_l(m);
         r.partU = nullptr;
         r.nsplit = g.nsplit;
         r.nbStride = 64;
@@ -696,7 +732,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
     GatherTask c = t;
     c.evalStride = evalStride;
     c.rp = m.gPc; c.rn = m.gNc; c.nr = ngated; c.mask = m.gMc;
-    c.gval = nullptr; c.gateMode = GATE_NONE; c.partL = m.scratchL; c.partU = nullptr;
+    c.gval = nullptr; c.gateMode = GATE_NONE; c.partL = scr_l(m); c.partU = nullptr;
     // ~512 workgroups: enough to fill the chip, few partial sums per entry
     const int cgroups = (ngated + 127) / 128;
     int cs = std::max(1, std::min((512 + cgroups - 1) / cgroups, t.ne / 64));
@@ -706,7 +742,7 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
         set_error("gated MFMA gather: scratch bound");
         return -2;
     }
-    c.partL = overlap ? m.scratchC : m.scratchL;
+    c.partL = overlap ? m.scratchC : scr_l(m);
     for (int b0 = 0; b0 < nplanes; b0 += 64) {
         const int nb = std::min(64, nplanes - b0);
         c.nrhs = nb;
@@ -729,7 +765,9 @@ static int gather_planes_mfma(Model &m, GatherTask t, const Phys &ph, ReduceTask
 // emitters' total pressures are read through the emitter->cell map as the tiles
 // are loaded; partial sums are FP32 within a split of 32 emitters and FP64 across splits.
 static int gather_points_mfma(Model &m, GatherTask t, const Phys &ph, int nplanes, long evalStride, double pairs) {
-    const int nsplit = std::max(1, std::min(2048, (t.ne + 31) / 32));
+    int nsplit = std::max(1, std::min(2048, (t.ne + 31) / 32));
+    if (m.noAlloc)  // as many splits as the fixed scratch holds
+        nsplit = (int)std::max<long>(1, std::min<long>(nsplit, (long)(m.fixElems / ((size_t)64 * std::max(1, t.nr)))));
     t.emPerSplit = ((t.ne + nsplit - 1) / nsplit + 31) / 32 * 32;
     t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
     if (ensure_scratch(m, (size_t)t.nsplit * 64 * t.nr)) return -1;  // the partials
@@ -739,14 +777,14 @@ static int gather_points_mfma(Model &m, GatherTask t, const Phys &ph, int nplane
         g.nrhs = nb;
         g.eval = t.eval + b0 * evalStride;
         g.evalStride = evalStride;
-        g.partL = m.scratchL;
+        g.partL = scr_l(m);
         g.partU = nullptr;
         {
             ProfScope ps(m, PK_F2P, pairs * nb, pairs * 8.0 * nb);
             if (f2f_mfma_launch(g, ph, KIND_F2P, m.stream)) return -1;
         }
         ProfScope pr(m, PK_REDUCE, 0.0, 0.0);
-        if (reduce_points_launch(m.scratchL, nullptr, g.nsplit, 64, nb, t.nr, m.fldVal + (size_t)b0 * t.nr, t.nr,
+        if (reduce_points_launch(scr_l(m), nullptr, g.nsplit, 64, nb, t.nr, m.fldVal + (size_t)b0 * t.nr, t.nr,
                                  m.stream))
             return -1;
     }
@@ -764,6 +802,9 @@ static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, Reduc
     // few receivers (field points): spread the emitters thinly across many workgroups
     const int minPer = (t.nr <= 256) ? 16 : 256;
     t.nsplit = choose_splits(nrb, t.ne, minPer);
+    if (m.noAlloc)  // as many splits as the fixed scratch holds (16-plane groups)
+        t.nsplit = (int)std::max<long>(1, std::min<long>(t.nsplit, (long)(m.fixElems /
+                                        ((size_t)gather_nb_stride(std::min(16, nplanes)) * std::max(1, t.nr)))));
     t.emPerSplit = (t.ne + t.nsplit - 1) / t.nsplit;
     t.nsplit = (t.ne + t.emPerSplit - 1) / t.emPerSplit;
     for (int b0 = 0; b0 < nplanes; b0 += 16) {
@@ -776,8 +817,8 @@ static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, Reduc
         g.gvalStride = gvalStride;
         g.eval = t.eval + b0 * evalStride;
         if (t.gval) g.gval = t.gval + b0 * gvalStride;
-        g.partL = m.scratchL;
-        g.partU = m.scratchU;
+        g.partL = scr_l(m);
+        g.partU = scr_u(m);
         {
             const int pk = kind == KIND_F2F ? PK_F2F : (kind == KIND_F2F_FIELD ? PK_F2F_FIELD : PK_F2P);
             ProfScope ps(m, pk, pairs * nb, pairs * (37.0 + 8.0 * nb));
@@ -785,13 +826,13 @@ static int gather_planes(Model &m, GatherTask t, const Phys &ph, int kind, Reduc
         }
         ProfScope pr(m, PK_REDUCE, 0.0, 0.0);
         if (kind == KIND_F2P) {
-            if (reduce_points_launch(m.scratchL, m.scratchU, g.nsplit, nbs, nb, t.nr, m.fldVal + (size_t)b0 * t.nr,
+            if (reduce_points_launch(scr_l(m), scr_u(m), g.nsplit, nbs, nb, t.nr, m.fldVal + (size_t)b0 * t.nr,
                                      t.nr, m.stream))
                 return -1;
         } else {
             ReduceTask r = rt;
-            r.partL = m.scratchL;
-            r.partU = m.scratchU;
+            r.partL = scr_l(m);
+            r.partU = scr_u(m);
             r.nsplit = g.nsplit;
             r.nbStride = nbs;
             r.nrhs = nb;
@@ -1167,6 +1208,26 @@ int model_project_t2field_points(Model &m) {
     }
     return finish_projection(m);
 }
+
+// FIELD_POINTS without any allocation: the split-K partial sums go to the fixed
+// buffer `scratch` (two halves of `elems` complex values) instead of the
+// library's scratch, profiling records nothing, and the visibility words must
+// already be allocated (render_cuda does that). Rejects work whose partial sums
+// do not fit even in one split.
+int model_project_t2field_points_noalloc(Model &m, float2 *scratch, size_t elems) {
+    if (int rc = check_rendered(m)) return rc;
+    struct Guard {
+        Model &m;
+        ~Guard() { m.noAlloc = false; m.fixL = m.fixU = nullptr; m.fixElems = 0; }
+    } guard{m};
+    m.noAlloc = true;
+    m.fixElems = elems / 2;
+    m.fixL = scratch;
+    m.fixU = scratch + m.fixElems;
+    return model_project_t2field_points(m);
+}
+
+long model_device_allocations() { return gDevAllocs.load(); }
 
 // FACET_RESP between two target objects (OptiX.cu:427-430; never dispatched by
 // the reference's orchestration). Accumulates into both objects' P_out.

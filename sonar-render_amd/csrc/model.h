@@ -139,6 +139,11 @@ struct Model {
     float nearQuadDeltas = kNfRMinDeltas;  // learned mode: pairs closer than this many fragment sizes use the quadrature
     float2 *scratchL = nullptr, *scratchU = nullptr;
     size_t scratchElems = 0;
+    // Allocation-free calls: partial sums go to a fixed buffer the caller's entry
+    // point provides (two halves of fixElems each), and nothing is allocated.
+    bool noAlloc = false;
+    float2 *fixL = nullptr, *fixU = nullptr;
+    size_t fixElems = 0;
     // the last transfer-form bounce product, for sr_time_transfer_gemm
     TransferGemm lastGemm{};
     bool haveLastGemm = false;

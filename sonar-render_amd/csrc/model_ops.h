@@ -22,6 +22,9 @@ int model_set_source_object_amplitude(Model &m, int i, float re, float im);
 int model_project_t2t(Model &m, int nref);
 int model_project_t2field_objects(Model &m);
 int model_project_t2field_points(Model &m);
+int model_project_t2field_points_noalloc(Model &m, float2 *scratch, size_t elems);
+// Device allocations the model has made so far (process-wide count).
+long model_device_allocations();
 int model_project_resp(Model &m, int ia, int ib);
 int model_prepare_resp(Model &m, int it);
 int model_get_field_points(Model &m, double *out, long n, int plane);

@@ -80,6 +80,8 @@ _sig("sr_last_call_ms", [], ctypes.c_float)
 _u8p = ctypes.POINTER(ctypes.c_ubyte)
 _sig("sr_sound_visualisation_init", [_f, _f, _i, _i])
 _sig("sr_set_source_object_amplitude", [_i, _f, _f])
+_sig("sr_project_target_to_field_points_noalloc", [])
+_sig("sr_device_allocations", [], _l)
 _sig("sr_render_png", [ctypes.c_char_p, _i, _i, _fp, _i])
 _sig("sr_render_rgb", [_i, _i, _i, _fp, _u8p])
 _sig("sr_render_facets_rgb", [_fp, _i, _f, _dp, _l, _i, _f, _f, _i, _i, _fp, _u8p])
@@ -161,6 +163,16 @@ def project_target_to_field_objects():
 
 def project_target_to_field_points():
     _check(cpp_lib.sr_project_target_to_field_points(), "ProjectTargetToFieldPoints")
+
+
+def project_target_to_field_points_noalloc():
+    """ProjectTargetToFieldPoints without any device allocation (fixed scratch)."""
+    _check(cpp_lib.sr_project_target_to_field_points_noalloc(), "ProjectTargetToFieldPoints (allocation-free)")
+
+
+def device_allocations():
+    """Device allocations the scene model has made so far in this process."""
+    return int(cpp_lib.sr_device_allocations())
 
 
 def project_target_to_target_objects(number_reflections):
