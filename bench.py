@@ -7,22 +7,24 @@ grid over the octant, 3 bounces with shadow rays, monostatic returns.
 
 One step = one whole scene for all 64 incidences with the geometry resident in
 HBM: zero the pressures, rebuild every visibility word, point->fragment, the
-3 bounces (rebuilding the transfer matrix in the mixed form), fragment->point.
+3 bounces (rebuilding the transfer matrix in the transfer forms),
+fragment->point.
 
-Three forms are timed back to back:
+Forms timed back to back:
   plain_fp32   the pair kernel evaluated on the fly in FP32 every bounce (its
                64-incidence complex products on the FP32 matrix cores);
   mixed_fp16   the pair kernel stored once as a complex FP16 matrix, every
                bounce an FP16 MFMA GEMM with FP32 accumulation;
-  learned      the same scene fragmented at lambda/4 (3.5x fewer fragments),
-               pairs with 3 delta <= r < 2 lambda corrected by the learned
-               near-field network (trained once on the GPU before timing),
-               closer pairs by the 16x16 quadrature, mixed form.
-plain and mixed are checked against the FP64 oracle (point rule, lambda/7.5)
-on the incidences the CPU baseline runs; learned against the FP64 oracle with
-the 16x16 sub-fragment quadrature judge at lambda/4, and its field points
-against the lambda/7.5 oracle. `value` is the Gpairs/s of the fastest form
-whose rel-L2 is <= 1e-3.
+  learned      the scene re-fragmented at lambda/4, near pairs corrected by the
+               learned network (trained on the GPU before timing) and the 16x16
+               quadrature, mixed form. A different discretisation.
+
+Accuracy gate, the same for every form: rel-L2 of the monostatic field points
+of the sampled incidences against the FP64 oracle of the workload as defined
+(the reference's point rule at lambda/7.5). `value` is the Gpairs/s of the
+fastest form whose error is <= 1e-3; a form that misses the bar is reported
+but never chosen. The learned form's error against its own lambda/4 judge (the
+oracle with 16x16 near-field quadrature) is reported beside it.
 
 Prints one JSON line. --gpus > 1 is not supported (one GPU per process).
 The engine is HIP code behind a C-ABI; timing brackets are device-wide
@@ -32,6 +34,7 @@ PyTorch is not loaded: its wheel carries a second copy of the HIP runtime.
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -45,11 +48,13 @@ FP16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-3
 
-# HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (FETCH_SIZE x 2 + WRITE_SIZE,
-# the factor 2 from a 1 GiB copy calibration), per workload; the figures and commands are in
-# profiles/transfer_gemm_hbm_bytes.md.
+# HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (calibrated
+# FETCH_SIZE and WRITE_SIZE, see profiles/transfer_gemm_hbm_bytes.md), per workload.
 PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 823.8e6,
                ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 108.9e6}
+# Vector-issue fraction of the visibility kernel (SQ_INSTS_VALU x 4 / SIMD-cycles of its
+# traced duration), per workload, from profiles/vis_mask_counters.md.
+VIS_VALU_ISSUE = {}
 
 
 def _configs():
@@ -62,11 +67,26 @@ def _configs():
     }
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
 def cpu_baseline(scene, sample_planes):
     """The FP64 OpenMP oracle on this host's cores, on a bounded sample of incidences."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = os.cpu_count() or 1
+    threads = cpu_threads()
     oracle.set_threads(threads)
     outs = []
     t0 = time.perf_counter()
@@ -99,16 +119,14 @@ def time_form(scene, mode, steps, warmup, near=None):
         step()
     api.device_synchronize()
     dt = time.perf_counter() - t0
-    # per-kernel times from HIP events, in separate steps: the event records
-    # add host work between launches and are kept out of the timed region
+    # per-kernel times from HIP events on the library's stream, in separate steps:
+    # the event records add host work between launches and stay out of the timed region
     api.profile(True)
     for _ in range(steps):
         step()
     api.device_synchronize()
     stats = {k: api.kernel_stats(k) for k in api.KERNEL_FAMILIES}
     api.profile(False)
-    if mode == 1:  # the bounce product alone, replayed back to back (no per-launch event overhead)
-        stats["gemm"]["replay_ms"] = api.time_transfer_gemm(20)
     res = runner.read_results(scene)
     api.TearDownCuda()
     return dt, stats, res
@@ -118,30 +136,42 @@ def roofline(stats, mode, workload):
     if mode == 0:
         st = stats["f2f"]
         ach = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
-        # achieved counts the conventional 8 flops per complex multiply-accumulate the kernel
-        # executed (it skips emitter pairs that no receiver of a wave sees; the count comes
-        # from the kernel itself); it issues 6 (three real products, Gauss) on the matrix cores
         return {"bound": "mfma", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "flops_counted": "8 per complex multiply-accumulate executed (emitter pairs a wave's receivers "
+                                 "all miss are skipped and not counted); the matrix cores issue 6 of them "
+                                 "(three real products per complex term)",
                 "mfma_issued_tflops": round(ach * 0.75, 3),
-                "kernel": "k_f2f_mfma (complex products on FP32 MFMA 32x32x2 as three real products; 8 flops per executed pair and plane counted, Green's function on the VALU)",
+                "kernel": "k_f2f_mfma (complex products on FP32 MFMA 32x32x2, the pair kernel on the vector units)",
                 "avg_launch_ms": round(st["ms"] / max(1, st["launches"]), 4), "launches": st["launches"]}
     st = stats["gemm"]
     n = max(1, st["launches"])
     algo, fl = st["bytes"] / n, st["flops"] / n
-    # per-launch time: the back-to-back replay of the same launch; the event
-    # brackets of the scene steps add ~4 us of dispatch per launch
-    ms = st.get("replay_ms") or st["ms"] / n
+    ms = st["ms"] / n  # in-scene launches, HIP events on the library's stream
     ach = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     tf = fl / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
     traffic = PMC_TRAFFIC.get((workload, "k_transfer_gemm"))
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": round(algo),
-            "kernel": "k_transfer_gemm<0> (FP16 MFMA 32x32x16, streams the FP16 transfer matrix once per bounce)",
-            "mfma_tflops": round(tf, 2), "mfma_frac": round(tf / FP16_PEAK_TFLOPS, 4),
-            "avg_launch_ms": round(ms, 4), "avg_launch_ms_in_scene": round(st["ms"] / n, 4),
-            "launches": st["launches"]}
+            "algorithmic_bytes": "FP16 complex S of every ordered cross-facet pair (4 B) + FP16 complex input "
+                                 "(4 B per cell and incidence) + FP32 complex P_out (8 B per cell and incidence)",
+            "kernel": "k_transfer_gemm (FP16 MFMA 32x32x16, streams the FP16 transfer matrix once per bounce)",
+            "mfma_tflops_executed": round(tf, 2), "mfma_frac": round(tf / FP16_PEAK_TFLOPS, 4),
+            "avg_launch_ms": round(ms, 4), "launches": st["launches"]}
+
+
+def vis_roofline(stats, workload, steps):
+    st = stats["vis"]
+    n = max(1, st["launches"])
+    ms = st["ms"] / n
+    out = {"kernel": "k_vis_mask (shadow rays + cosine culls -> one visibility word per 64 receivers and emitter)",
+           "launches_per_scene": st["launches"] // max(1, steps), "avg_launch_ms": round(ms, 4),
+           "pairs_per_s_G": round(st["pairs"] / max(st["ms"], 1e-12) / 1e6, 2),
+           "words_GBs": round(st["bytes"] / max(st["ms"], 1e-12) / 1e6, 1), "bound": "valu"}
+    v = VIS_VALU_ISSUE.get(workload)
+    out["valu_issue_frac"] = v
+    return out
 
 
 def main():
@@ -151,7 +181,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="trihedral", choices=sorted(_configs()))
     ap.add_argument("--forms", default="plain,mixed,learned", help="comma list of plain, mixed, learned")
-    ap.add_argument("--cpu-sample", type=int, default=1, help="incidences the CPU baseline runs")
+    ap.add_argument("--cpu-sample", type=int, default=4, help="incidences the CPU baseline and the gate run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nf-steps", type=int, default=60000,
                     help="training steps of the learned near field (fewer for profiler runs)")
@@ -180,9 +210,7 @@ def main():
     want = args.forms.split(",")
     forms = [("plain_fp32", 0, None, scene, "plain"), ("mixed_fp16_mfma", 1, None, scene, "mixed")]
     learned = None
-    # the near-field correction is defined for attenuation-free media only
-    if "learned" in want and scene.get("bounces", 0) and not scene.get("atten"):
-        import scenes
+    if "learned" in want and scene.get("bounces", 0):
         import sonar_api as api
         lam = scene["c"] / scene["f"]
         coarse = dict(scene, name=scene["name"] + "_lambda4",
@@ -205,30 +233,30 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        sample = list(range(min(args.cpu_sample, n_planes)))
+        sample = sorted(set(np.linspace(0, n_planes - 1, min(args.cpu_sample, n_planes)).round().astype(int).tolist()))
         cdt, threads, outs = cpu_baseline(scene, sample)
         frac = len(sample) / n_planes
         cpu = {"value": round(pairs * frac / cdt / 1e9, 5), "unit": "Gpairs/s", "cores": threads, "kind": "port",
-               "sample": f"{len(sample)} of {n_planes} incidences of the same scene, FP64 OpenMP oracle; "
-                         f"{cdt:.2f} s -> {cdt / frac:.1f} s/scene extrapolated"}
+               "cpu_model": cpu_model(),
+               "sample": f"incidences {sample} of {n_planes} of the same scene, FP64 OpenMP oracle on "
+                         f"{threads} threads; {cdt:.2f} s -> {cdt / frac:.1f} s/scene extrapolated linearly"}
         import oracle
+        ref_pts = np.concatenate([o["points"] for o in outs])
         for name, r in results.items():
-            errs, perrs = [], []
-            for p, o in zip(sample, outs):
-                if r["scene"] is scene:
-                    a = np.concatenate([r["res"]["points"][p], r["res"]["targets"][0][p]])
-                    b = np.concatenate([o["points"], o["targets"][0]])
-                else:  # learned: the judge at lambda/4, and the field points of the lambda/7.5 oracle
-                    j = oracle.run_scene(r["scene"], plane_source=p if scene.get("batched") else None,
-                                         near=(16, 2.0 * learned["lam"]))
-                    a = np.concatenate([r["res"]["points"][p], r["res"]["targets"][0][p]])
-                    b = np.concatenate([j["points"], j["targets"][0]])
-                    q = r["res"]["points"][p]
-                    perrs.append(float(np.linalg.norm(q - o["points"]) / np.linalg.norm(o["points"])))
-                errs.append(float(np.linalg.norm(a - b) / np.linalg.norm(b)))
-            r["err"] = max(errs)
-            if perrs:
-                r["err_points_vs_lambda7p5"] = max(perrs)
+            got = np.concatenate([r["res"]["points"][p] for p in sample])
+            # the gate: monostatic field points against the workload's own FP64 oracle, every form alike
+            r["err"] = float(np.linalg.norm(got - ref_pts) / np.linalg.norm(ref_pts))
+            if r["scene"] is scene:
+                a = np.concatenate([r["res"]["targets"][0][p] for p in sample])
+                b = np.concatenate([o["targets"][0] for o in outs])
+                r["err_surface"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            else:  # the learned form: also against its own discretisation's judge (16x16 near quadrature)
+                js = [oracle.run_scene(r["scene"], plane_source=p if scene.get("batched") else None,
+                                       near=(16, 2.0 * learned["lam"])) for p in sample[:1]]
+                a = np.concatenate([r["res"]["points"][p] for p in sample[:1]] +
+                                   [r["res"]["targets"][0][p] for p in sample[:1]])
+                b = np.concatenate([j["points"] for j in js] + [j["targets"][0] for j in js])
+                r["err_own_judge"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     per_form = {}
     for name, r in results.items():
@@ -238,22 +266,27 @@ def main():
             "pairs_per_scene": r["pairs"], "rel_l2_vs_fp64": r.get("err"),
             "kernel_ms_per_scene": {k: round(v["ms"] / args.steps, 4) for k, v in r["stats"].items() if v["launches"]},
             "roofline": roofline(r["stats"], r["mode"], r["scene"]["name"]),
+            "visibility": vis_roofline(r["stats"], r["scene"]["name"], args.steps),
         }
+        if "err_surface" in r:
+            per_form[name]["rel_l2_surface_vs_fp64"] = r["err_surface"]
     valid = [n for n in per_form if per_form[n]["rel_l2_vs_fp64"] is not None and per_form[n]["rel_l2_vs_fp64"] <= TOL]
-    # without a CPU check, the fastest form at the reference's own fragment size
+    # without a CPU check: the fastest form at the workload's own fragment size
     primary = min(valid or [n for n in per_form if results[n]["scene"] is scene] or list(per_form),
                   key=lambda n: per_form[n]["ms_per_scene"])
     P = per_form[primary]
-    speedup = speedup_l = None
-    if "plain_fp32" in per_form and "mixed_fp16_mfma" in per_form:
-        speedup = round(per_form["plain_fp32"]["ms_per_scene"] / per_form["mixed_fp16_mfma"]["ms_per_scene"], 2)
+    speedup = {}
+    if "plain_fp32" in per_form:
+        for n in per_form:
+            if n != "plain_fp32":
+                speedup[n] = round(per_form["plain_fp32"]["ms_per_scene"] / per_form[n]["ms_per_scene"], 2)
     if "learned_mixed_fp16_lambda4" in per_form:
         L = per_form["learned_mixed_fp16_lambda4"]
-        L["rel_l2_vs"] = "FP64 oracle with the 16x16 quadrature judge, same lambda/4 fragments"
-        L["rel_l2_points_vs_lambda7p5_fp64"] = results["learned_mixed_fp16_lambda4"].get("err_points_vs_lambda7p5")
+        L["discretisation"] = ("lambda/4 fragments (4,680), not the workload's lambda/7.5: its gate error is the "
+                               "distance between the two discretisations' monostatic returns")
+        L["rel_l2_vs_own_lambda4_judge"] = results["learned_mixed_fp16_lambda4"].get("err_own_judge")
+        L["meets_gate"] = "learned_mixed_fp16_lambda4" in valid
         L["training"] = {k: round(float(learned[k]), 4) for k in ("train_s", "train_err", "heldout_err")}
-        if "plain_fp32" in per_form:
-            speedup_l = round(per_form["plain_fp32"]["ms_per_scene"] / L["ms_per_scene"], 2)
 
     line = {
         "metric": "Gpairs/s & s/scene at <=1e-3 rel-L2 vs FP64: learned+mixed vs plain FP32",
@@ -266,8 +299,10 @@ def main():
                    "bounces": scene.get("bounces", 0), "pairs_per_scene": P["pairs_per_scene"],
                    "fragment": "lambda/4" if primary.startswith("learned") else "lambda/7.5",
                    "parallelism": "single GPU"},
-        "rel_l2_vs_fp64": P["rel_l2_vs_fp64"], "speedup_mixed_vs_plain": speedup,
-        "speedup_learned_vs_plain": speedup_l,
+        "rel_l2_vs_fp64": P["rel_l2_vs_fp64"],
+        "gate": "rel-L2 of the monostatic field points of the sampled incidences vs the FP64 oracle of the "
+                "workload (lambda/7.5 point rule), the same for every form; forms above 1e-3 are never chosen",
+        "speedup_vs_plain": speedup,
         "forms": per_form, "roofline": P["roofline"], "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

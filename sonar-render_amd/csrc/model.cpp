@@ -534,7 +534,9 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
     uint64_t *mk = mb.p;
     tk.mask = mk;
     {
-        ProfScope ps(m, PK_VIS, (double)tk.em.count * tk.rc.count, 0.0);
+        // pairs: the (emitter, receiver) pairs the words cover (a symmetric set traces half
+        // of them); bytes: the words written, the kernel's only output
+        ProfScope ps(m, PK_VIS, (double)tk.em.count * tk.rc.count, 0.0, (double)words * 8.0);
         if (vis_mask_launch(tk, bvh_of(m), m.vc, m.stream)) return -1;
     }
     mb.valid = true;  // the traversal-overflow flag is checked when the projection finishes
@@ -1117,8 +1119,11 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             const double pairs = ((double)n * n - o.sameFacetPairs) * np;
             // flops of the GEMM as executed: 2 * rows * 128 columns * 2 * K (padded)
             const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
-            // algorithmic HBM bytes: S read once, B read once, partial sums written
-            const double by = t.aBytesRead + (double)t.nKs * 256 * 16 + t.partRows * np * 8;
+            // algorithmic HBM bytes of the bounce product: the FP16 complex S entry of every
+            // ordered cross-facet pair (4 B), the FP16 complex input of every active cell and
+            // plane (4 B) and the FP32 complex output P_out (8 B); partial sums, padding and
+            // re-reads are traffic, not work
+            const double by = ((double)n * n - o.sameFacetPairs) * 4.0 + (double)n * np * 4.0 + (double)n * np * 8.0;
             ProfScope ps(m, PK_GEMM, pairs, fl, by);
             if (transfer_gemm(gm, m.stream)) return -1;
         }
