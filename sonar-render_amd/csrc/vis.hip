@@ -11,8 +11,8 @@
 // must round exactly as the host oracle does.
 
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <cstdio>
+#include <utility>
 #include <vector>
 
 #include "sr_types.h"
@@ -47,6 +47,121 @@ __global__ void k_morton(const float4 *__restrict__ tris, int n, float3 lo, floa
     unsigned uz = (unsigned)fminf(fz * 1024.0f, 1023.0f);
     keys[i] = (expand_bits10(ux) << 2) | (expand_bits10(uy) << 1) | expand_bits10(uz);
     ids[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (Morton key, triangle index) pairs, 8 bits a pass.
+// A tile of kSortTile keys belongs to one wave. Per pass: every tile counts its
+// digits, one workgroup turns the [digit][tile] counts into output offsets, and
+// every tile scatters its keys in order: a key's rank among equal digits in its
+// 64-key step comes from eight ballots (the lanes with the same digit), plus the
+// count of that digit in the tile's earlier steps, which keeps the sort stable.
+constexpr int kSortTile = 2048;
+constexpr int kSortWaves = 4;
+
+__device__ inline unsigned sort_digit(unsigned key, int shift) { return (key >> shift) & 0xffu; }
+
+__global__ __launch_bounds__(64 * kSortWaves) void k_sort_hist(const unsigned *__restrict__ keys, int n, int shift,
+                                                              int tiles, unsigned *__restrict__ hist) {
+    __shared__ unsigned cnt[kSortWaves][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int d = lane; d < 256; d += 64) cnt[wave][d] = 0u;
+    __syncthreads();
+    const int tile = blockIdx.x * kSortWaves + wave;
+    if (tile < tiles) {
+        const int i0 = tile * kSortTile, i1 = min(n, i0 + kSortTile);
+        for (int i = i0 + lane; i < i1; i += 64) atomicAdd(&cnt[wave][sort_digit(keys[i], shift)], 1u);
+    }
+    __syncthreads();
+    if (tile < tiles)
+        for (int d = lane; d < 256; d += 64) hist[(size_t)d * tiles + tile] = cnt[wave][d];
+}
+
+// Exclusive prefix sum over the [digit][tile] counts (one workgroup of 1024).
+__global__ __launch_bounds__(1024) void k_sort_scan(unsigned *__restrict__ hist, int total) {
+    __shared__ unsigned part[1024];
+    const int t = threadIdx.x;
+    const int per = (total + 1023) / 1024;
+    const int a = min(total, t * per), b = min(total, a + per);
+    unsigned sum = 0u;
+    for (int i = a; i < b; i++) sum += hist[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the segment sums
+        const unsigned v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned run = t > 0 ? part[t - 1] : 0u;
+    for (int i = a; i < b; i++) {
+        const unsigned c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(64 * kSortWaves) void k_sort_scatter(const unsigned *__restrict__ keys,
+                                                                 const int *__restrict__ vals, int n, int shift,
+                                                                 int tiles, const unsigned *__restrict__ offs,
+                                                                 unsigned *__restrict__ keysOut,
+                                                                 int *__restrict__ valsOut) {
+    __shared__ unsigned base[kSortWaves][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * kSortWaves + wave;
+    if (tile >= tiles) return;  // uniform over the wave; no barrier below
+    for (int d = lane; d < 256; d += 64) base[wave][d] = offs[(size_t)d * tiles + tile];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const int i0 = tile * kSortTile, i1 = min(n, i0 + kSortTile);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int c = i0; c < i1; c += 64) {
+        const int i = c + lane;
+        const bool ok = i < i1;
+        const unsigned key = ok ? keys[i] : 0u;
+        const unsigned d = sort_digit(key, shift);
+        unsigned long long same = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const unsigned long long m = __ballot(ok && ((d >> b) & 1u));
+            same &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const unsigned pos = ok ? base[wave][d] + (unsigned)__popcll(same & below) : 0u;
+        if (ok) {
+            keysOut[pos] = key;
+            valsOut[pos] = vals[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the last lane of each digit group advances that digit's base for the next step
+        if (ok && (same >> lane) == 1ull) base[wave][d] += (unsigned)__popcll(same);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    }
+}
+
+// Sorts keys/vals (n pairs, keys below 2^bits) with ping-pong buffers; the
+// sorted pairs end in keys/vals (an even number of passes). hist: 256 x tiles words.
+int radix_sort_pairs(unsigned *keys, int *vals, unsigned *keys2, int *vals2, unsigned *hist, int n, int bits,
+                     hipStream_t st) {
+    if (n <= 1) return 0;
+    const int tiles = (n + kSortTile - 1) / kSortTile;
+    const int blocks = (tiles + kSortWaves - 1) / kSortWaves;
+    int passes = (bits + 7) / 8;
+    if (passes & 1) passes++;  // the result lands back in keys/vals
+    unsigned *kin = keys, *kout = keys2;
+    int *vin = vals, *vout = vals2;
+    for (int p = 0; p < passes; p++) {
+        const int shift = 8 * p;
+        k_sort_hist<<<blocks, 64 * kSortWaves, 0, st>>>(kin, n, shift, tiles, hist);
+        SR_HIP(hipGetLastError());
+        k_sort_scan<<<1, 1024, 0, st>>>(hist, 256 * tiles);
+        SR_HIP(hipGetLastError());
+        k_sort_scatter<<<blocks, 64 * kSortWaves, 0, st>>>(kin, vin, n, shift, tiles, hist, kout, vout);
+        SR_HIP(hipGetLastError());
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    return 0;
 }
 
 __global__ void k_gather_tris(const float4 *__restrict__ src, const int *__restrict__ order, int n,
@@ -413,8 +528,8 @@ int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 
     int *ids = nullptr, *ids2 = nullptr, *child = nullptr, *pInt = nullptr, *pLeaf = nullptr, *cnt = nullptr;
     int *err = nullptr;
     float4 *boxes = nullptr;
-    void *tmp = nullptr;
-    size_t tmpBytes = 0;
+    unsigned *hist = nullptr;
+    const int sortTiles = (n + kSortTile - 1) / kSortTile;
     SR_HIP(hipMalloc(&keys, n * sizeof(unsigned)));
     SR_HIP(hipMalloc(&keys2, n * sizeof(unsigned)));
     SR_HIP(hipMalloc(&ids, n * sizeof(int)));
@@ -425,19 +540,18 @@ int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 
     SR_HIP(hipMalloc(&cnt, (n - 1) * sizeof(int)));
     SR_HIP(hipMalloc(&err, sizeof(int)));
     SR_HIP(hipMalloc(&boxes, 2 * (n - 1) * sizeof(float4)));
+    SR_HIP(hipMalloc(&hist, (size_t)256 * sortTiles * sizeof(unsigned)));
     const int B = 256, G = (n + B - 1) / B;
     k_morton<<<G, B, 0, st>>>(dTrisUnsorted, n, lo, invExt, keys, ids);
     SR_HIP(hipGetLastError());
-    SR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, keys, keys2, ids, ids2, n, 0, 30, st));
-    SR_HIP(hipMalloc(&tmp, tmpBytes));
-    SR_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, keys2, ids, ids2, n, 0, 30, st));
-    k_gather_tris<<<G, B, 0, st>>>(dTrisUnsorted, ids2, n, dTrisSorted);
+    if (radix_sort_pairs(keys, ids, keys2, ids2, hist, n, 30, st)) return -1;  // 30-bit Morton keys
+    k_gather_tris<<<G, B, 0, st>>>(dTrisUnsorted, ids, n, dTrisSorted);
     SR_HIP(hipGetLastError());
     SR_HIP(hipMemsetAsync(pInt, 0xff, (n - 1) * sizeof(int), st));
     SR_HIP(hipMemsetAsync(cnt, 0, (n - 1) * sizeof(int), st));
     SR_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
     const int Gi = (n - 1 + B - 1) / B;
-    k_hierarchy<<<Gi, B, 0, st>>>(keys2, n, child, pInt, pLeaf);
+    k_hierarchy<<<Gi, B, 0, st>>>(keys, n, child, pInt, pLeaf);
     SR_HIP(hipGetLastError());
     k_refit<<<G, B, 0, st>>>(dTrisSorted, n, child, pInt, pLeaf, boxes, cnt, err);
     SR_HIP(hipGetLastError());
@@ -446,7 +560,7 @@ int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 
     int herr = 0;
     SR_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
     SR_HIP(hipStreamSynchronize(st));
-    void *bufs[] = {keys, keys2, ids, ids2, child, pInt, pLeaf, cnt, err, boxes, tmp};
+    void *bufs[] = {keys, keys2, ids, ids2, child, pInt, pLeaf, cnt, err, boxes, hist};
     for (void *b : bufs) (void)hipFree(b);
     if (herr) { set_error("LBVH refit exceeded the depth limit"); return -1; }
     return 0;
