@@ -174,6 +174,19 @@ def vis_roofline(stats, workload, steps):
     return out
 
 
+def field_surface_render(steps, warmup):
+    """The hull's one field-surface render: one incidence, sources -> hull, two bounces,
+    hull -> a 512 x 512-fragment field plane (FACET_NO_RESP), plain form."""
+    import runner
+    import scenes
+    sc = dict(scenes.config_hull(n_angles=1, field_cells=512), batched=False, project_field_objects=False)
+    dt, stats, _ = time_form(sc, 0, steps, warmup)
+    pairs = runner.pair_count(sc)
+    ms = dt / steps * 1e3
+    return {"ms_per_render": round(ms, 3), "pairs_per_render": pairs, "gpairs_per_s": round(pairs / (ms * 1e-3) / 1e9, 2),
+            "kernel_ms_per_render": {k: round(v["ms"] / steps, 4) for k, v in stats.items() if v["launches"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +301,10 @@ def main():
         L["meets_gate"] = "learned_mixed_fp16_lambda4" in valid
         L["training"] = {k: round(float(learned[k]), 4) for k in ("train_s", "train_err", "heldout_err")}
 
+    extra = {}
+    if args.config == "hull":
+        extra["field_surface_512"] = field_surface_render(args.steps, args.warmup)
+
     line = {
         "metric": "Gpairs/s & s/scene at <=1e-3 rel-L2 vs FP64: learned+mixed vs plain FP32",
         "value": P["gpairs_per_s"], "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -305,6 +322,7 @@ def main():
         "speedup_vs_plain": speedup,
         "forms": per_form, "roofline": P["roofline"], "cpu_baseline": cpu,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

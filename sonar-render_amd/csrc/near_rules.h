@@ -16,15 +16,21 @@ namespace sr {
 
 // Kernel at emitter sub-point offset (receiver at (vx, vy, vz) from it), in
 // the emitter's frame (z = emitter normal); cosine culls as the reference's.
-__device__ inline float2 near_kernel_at(float vx, float vy, float vz, float3 nr, float k) {
+// k: Re k; alpha: Im k (the attenuation, src/Model.cpp:52-53), so that
+// -e^{ikr} ik / (4 pi r) = e^{-alpha r} / (4 pi r) [(alpha cos + k sin) + i (alpha sin - k cos)].
+__device__ inline float2 near_kernel_at(float vx, float vy, float vz, float3 nr, float k, float alpha = 0.0f) {
     const float r = sqrtf(vx * vx + vy * vy + vz * vz);
     const float cs = vz / r, ci = -(vx * nr.x + vy * nr.y + vz * nr.z) / r;
     if (!(cs >= 1e-6f) || !(ci >= 1e-6f)) return make_float2(0.f, 0.f);
     float s, c;
     sincosf(k * r, &s, &c);
-    const float a = k * 0.0795774715459476678f / r;
     const float w = cs * ci * 0.9f;
-    return make_float2(w * (a * s + 1.0f / (r * r)), -w * a * c);
+    if (alpha == 0.0f) {
+        const float a = k * 0.0795774715459476678f / r;
+        return make_float2(w * (a * s + 1.0f / (r * r)), -w * a * c);
+    }
+    const float a = expf(-alpha * r) * 0.0795774715459476678f / r;
+    return make_float2(w * (a * (alpha * c + k * s) + 1.0f / (r * r)), w * a * (alpha * s - k * c));
 }
 
 // rv, nr: receiver point (relative to the emitter point) and receiver normal,
@@ -44,11 +50,11 @@ __device__ inline float2 near_gauss2(float3 rv, float3 nr, float d, float k) {
 
 // Points 2 half and 2 half + 1 of the same rule, summed, unscaled: the two
 // halves' sum times 1/4 is near_gauss2 (up to the order of the additions).
-__device__ inline float2 near_gauss2_half(float3 rv, float3 nr, float d, float k, int half) {
+__device__ inline float2 near_gauss2_half(float3 rv, float3 nr, float d, float k, int half, float alpha = 0.0f) {
     const float o = 0.288675134594812882f * d;
     const float oy = half ? o : -o;
-    const float2 a = near_kernel_at(rv.x + o, rv.y - oy, rv.z, nr, k);
-    const float2 b = near_kernel_at(rv.x - o, rv.y - oy, rv.z, nr, k);
+    const float2 a = near_kernel_at(rv.x + o, rv.y - oy, rv.z, nr, k, alpha);
+    const float2 b = near_kernel_at(rv.x - o, rv.y - oy, rv.z, nr, k, alpha);
     return make_float2(a.x + b.x, a.y + b.y);
 }
 

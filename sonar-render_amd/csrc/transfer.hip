@@ -57,7 +57,9 @@ __device__ inline uint32_t pack_h2(float a, float b) {
 // emitter's cell, unnormalised. The phase of each sub-point uses the hardware
 // sine/cosine in revolutions (r < 2 lambda here, so r k / 2 pi < 2 keeps
 // ~1e-7 of a revolution).
-__device__ float2 quad16_row(float3 rv, float3 nr, float d, float k, int sy) {
+// alpha: the attenuation (Im k); the kernel then carries e^{-alpha r} as the
+// reference's thrust::exp(i k r) does (OptiX.cu:350).
+__device__ float2 quad16_row(float3 rv, float3 nr, float d, float k, float alpha, int sy) {
     const int n = 16;
     const float h = d / n;
     const float krev = k * 0.159154943091895336f;
@@ -71,16 +73,16 @@ __device__ float2 quad16_row(float3 rv, float3 nr, float d, float k, int sy) {
             const float cs = vz / r, ci = -(vx * nr.x + vy * nr.y + vz * nr.z) / r;
             if (!(cs >= 1e-6f) || !(ci >= 1e-6f)) continue;
             float re, im;
-            if (r < h) {
-                re = 0.f;
+            if (r < h) {  // ik / (4 pi), k = kr + i alpha
+                re = -alpha * kInv4PiT;
                 im = k * kInv4PiT;
             } else {
                 float x = r * krev;
                 x -= rintf(x);
                 const float s = __builtin_amdgcn_sinf(x), c = __builtin_amdgcn_cosf(x);
-                const float a = k * kInv4PiT / r;
-                re = a * s + 1.0f / (r * r);
-                im = -a * c;
+                const float a = kInv4PiT / r * (alpha != 0.0f ? __expf(-alpha * r) : 1.0f);
+                re = a * fmaf(alpha, c, k * s) + 1.0f / (r * r);
+                im = -a * fmaf(k, c, -alpha * s);
             }
             const float w = cs * ci * 0.9f;
             acc.x += w * re;
@@ -131,7 +133,7 @@ __device__ inline void near_quad_block(const TransferGeom &g, const TransferPhys
     if (valid) {
         pr = list[i];
         const NearGeom q = near_geom(g, ph, pr.x, pr.y);
-        v = quad16_row(q.rv, q.nr, ph.deltaReg, ph.kr, sy);
+        v = quad16_row(q.rv, q.nr, ph.deltaReg, ph.kr, ph.alpha, sy);
     }
 #pragma unroll
     for (int m = 8; m >= 1; m >>= 1) {
@@ -212,10 +214,14 @@ __device__ inline void near_mlp_blocks(const TransferGeom &g, const TransferPhys
         if (valid) {
             pr = list[i];
             const NearGeom q = near_geom(g, ph, pr.x, pr.y);
-            Kh = near_gauss2_half(q.rv, q.nr, d, ph.kr, h);
+            Kh = near_gauss2_half(q.rv, q.nr, d, ph.kr, h, ph.alpha);
             if (h == 0) {
                 const float r = q.r;
+                // the network's correction is the attenuation-free one; in an attenuating
+                // medium it is carried by e^{-alpha r} at the pair's distance (the factor's
+                // variation across the cell, alpha delta, is ~1e-3 of the correction)
                 M = 0.9f * (ph.kr * kInv4PiT / r + 1.0f / (r * r)) * (1.0f / kNfTargetGain);
+                if (ph.alpha != 0.0f) M *= expf(-ph.alpha * r);
                 sincosf(ph.kr * r, &sn_kr, &cs_kr);
                 float f[kMlpIn];
                 near_features(r, lam, d, make_float3(q.rv.x / r, q.rv.y / r, q.rv.z / r), q.nr, f);
@@ -774,14 +780,18 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
     const long waves = (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave);
     const long blocks = (waves + 3) / 4;
     if (blocks <= 0) return 0;
-    if (ph.nearMode != 0 && ph.alpha != 0.0f) {
-        set_error("near-field correction is defined for attenuation-free media only");
-        return -2;
+    const bool att = ph.alpha != 0.0f;
+    if (ph.nearMode == 1) {
+        if (att) k_build_s<true, 1><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+        else k_build_s<false, 1><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    } else if (ph.nearMode == 2) {
+        if (att) k_build_s<true, 2><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+        else k_build_s<false, 2><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    } else if (att) {
+        k_build_s<true, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
+    } else {
+        k_build_s<false, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
     }
-    if (ph.nearMode == 1) k_build_s<false, 1><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    else if (ph.nearMode == 2) k_build_s<false, 2><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    else if (ph.alpha != 0.0f) k_build_s<true, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    else k_build_s<false, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
     SR_HIP(hipGetLastError());
     return 0;
 }
