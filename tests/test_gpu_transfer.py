@@ -52,7 +52,7 @@ def test_mixed_many_small_facets(api):
         assert rel_l2(c, b) < 2e-4
 
 
-def test_mixed_equals_plain_at_20khz_subset(api):
+def test_mixed_equals_plain_12khz_4_incidences(api):
     sc = scenes.config_trihedral(f=12e3, n_side=2, bounces=3)
     _banner("mixed vs plain: trihedral 12 kHz, 4 incidences")
     a = runner.run_scene(sc, transfer_mode=1)
