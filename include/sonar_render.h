@@ -260,6 +260,11 @@ int sr_batch_clear(void);
 int sr_batch_add_scene(const float *tris9, int ntri, float delta, float cp, float frequency, float attenuation,
                        const float *src, int nsrc, const float *pts, int npts);
 int sr_batch_set_bounces(int n);
+/* Near-field terms of the batched bounces (before sr_batch_prepare): mode 0 the
+ * reference's point rule (default), 1 the 16x16 sub-fragment quadrature, 2 the
+ * learned correction (needs sr_nf_train; pairs closer than quadDeltas fragment
+ * sizes use the quadrature), for visible pairs closer than radiusWavelengths. */
+int sr_batch_set_near(int mode, float radiusWavelengths, float quadDeltas);
 int sr_batch_prepare(void);
 int sr_batch_run(int useGraph, float *ms);
 int sr_batch_autotune(float *report, int nmax);

@@ -470,6 +470,9 @@ int sr_batch_add_scene(const float *tris, int ntri, float delta, float cp, float
     return rc < 0 ? rc : idx;
 }
 int sr_batch_set_bounces(int n) { return guarded("batch_set_bounces", [&] { return sr::batch_set_bounces(n); }); }
+int sr_batch_set_near(int mode, float radiusWavelengths, float quadDeltas) {
+    return guarded("batch_set_near", [&] { return sr::batch_set_near(mode, radiusWavelengths, quadDeltas); });
+}
 int sr_batch_prepare(void) { return guarded("batch_prepare", [&] { return sr::batch_prepare(); }); }
 int sr_batch_run(int useGraph, float *ms) { return guarded("batch_run", [&] { return sr::batch_run(useGraph, ms); }); }
 int sr_batch_autotune(float *report, int n) {

@@ -25,9 +25,13 @@
 #include "model.h"
 #include "model_ops.h"
 #include "sums.h"
+#include "batch_near.h"
+#include "learned.h"
+#include "nearfield.h"
 
 namespace sr {
 
+int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st);
 int vis_mask_multi_launch(const MaskTask *tasks, const Bvh *bvhs, const VisConst *vcs, const int4 *items, int nitems,
                           int emPerWave, int *counter, int grid, hipStream_t st);
 int mask_transpose_multi_launch(const MaskTask *tasks, const int2 *items, int nitems, hipStream_t st);
@@ -65,6 +69,17 @@ struct Batch {
     bool prepared = false;
     hipStream_t st = nullptr;
     BatchTune tune;
+    // near-field correction of the bounces: 0 off (the reference's point rule),
+    // 1 16x16 quadrature, 2 learned (quadrature below nearQuadDeltas fragment sizes)
+    int nearMode = 0;
+    float nearRadiusWl = 2.0f, nearQuadDeltas = kNfRMinDeltas;
+    // per scene: CSR of the near pairs (row offsets fixed at prepare), their values
+    std::vector<int *> nearRowStart, nearRows, nearCols;
+    std::vector<float2 *> nearVals;
+    std::vector<long> nearTotal;
+    DevVec<NearBatchTask> nearTasks;
+    DevVec<int2> nearRowItems, nearMlpItems, nearQuadItems, nearApplyItems;
+    std::vector<DevVec<NearApplyTask>> nearApply;
     // per-scene mask buffers
     std::vector<uint64_t *> maskBufs;
     float2 *scratchL = nullptr, *scratchU = nullptr;
@@ -121,10 +136,40 @@ void release_tables(Batch &b) {
     b.selfTasks.clear(); b.redTasks.clear();
     b.selfPhys.release(); b.selfItems.release(); b.redItems.release();
     b.f2pTasks.release(); b.f2pPhys.release(); b.f2pItems.release(); b.prTasks.release(); b.prItems.release();
+    b.nearTasks.release(); b.nearRowItems.release(); b.nearMlpItems.release(); b.nearQuadItems.release();
+    b.nearApplyItems.release();
+    for (auto &t : b.nearApply) t.release();
+    b.nearApply.clear();
+}
+
+void release_near(Batch &b) {
+    for (auto *v : {&b.nearRowStart, &b.nearRows, &b.nearCols})
+        for (int *p : *v) if (p) (void)hipFree(p);
+    for (float2 *p : b.nearVals) if (p) (void)hipFree(p);
+    b.nearRowStart.clear(); b.nearRows.clear(); b.nearCols.clear(); b.nearVals.clear(); b.nearTotal.clear();
+}
+
+// The near-pair task of scene s (fill mode when its CSR exists).
+NearBatchTask near_task(const Batch &b, int s, uint64_t *mSelf, int *rowCount) {
+    const Model &m = *b.scenes[s];
+    const Object &o = m.targets[0];
+    NearBatchTask t{};
+    t.p = o.actP; t.nrm = o.actN; t.frames = o.frameTab; t.mask = mSelf; t.n = o.nact;
+    const double lam = 2.0 * M_PI / (double)m.kf;
+    const double R = b.nearRadiusWl * lam, Rq = (b.nearMode == 2 ? b.nearQuadDeltas : 0.0) * o.delta;
+    t.radius2 = (float)(R * R);
+    t.quadR2 = (float)(Rq * Rq);
+    t.kr = m.kf; t.alpha = m.atten; t.delta = o.delta; t.mode = b.nearMode;
+    if (!b.nearRowStart.empty()) {
+        t.rowStart = b.nearRowStart[s]; t.rows = b.nearRows[s]; t.cols = b.nearCols[s]; t.vals = b.nearVals[s];
+    }
+    t.rowCount = rowCount;
+    return t;
 }
 
 void release_all(Batch &b) {
     release_tables(b);
+    release_near(b);
     for (auto *p : b.maskBufs) (void)hipFree(p);
     b.maskBufs.clear();
     if (b.scratchL) (void)hipFree(b.scratchL);
@@ -173,6 +218,9 @@ int build_tables(Batch &b) {
     std::vector<int4> si, fi;
     std::vector<GatherTask> ft;
     std::vector<PointsReduceTask> prt;
+    std::vector<std::vector<NearApplyTask>> na(b.bounces);
+    std::vector<NearBatchTask> nt;
+    std::vector<int2> nri, nmi, nqi, nai;
     size_t scratchSelf = 0, scratchF2P = 0;
     std::vector<size_t> offSelf(S), offF2P(S);
     std::vector<int> nsSelf(S), nsF2P(S), perSelf(S), perF2P(S);
@@ -196,7 +244,7 @@ int build_tables(Batch &b) {
         perF2P[s] = (o.nact + nsF2P[s] - 1) / nsF2P[s];
         nsF2P[s] = std::max(1, (o.nact + perF2P[s] - 1) / perF2P[s]);
         offSelf[s] = scratchSelf;
-        scratchSelf += (size_t)nsSelf[s] * o.nact;
+        scratchSelf += (size_t)(nsSelf[s] + (b.nearMode ? 1 : 0)) * o.nact;  // + the near-field split
         offF2P[s] = scratchF2P;
         scratchF2P += (size_t)nsF2P[s] * nq;
         (void)nrb;
@@ -241,6 +289,16 @@ int build_tables(Batch &b) {
                 for (long p = 0; p < (long)nrbk * (nrbk - 1) / 2; p++) ti.push_back(make_int2(task, (int)p));
         }
         uint64_t *mP2F = vt[vt.size() - 3].mask, *mSelf = vt[vt.size() - 2].mask, *mF2P = vt[vt.size() - 1].mask;
+        if (b.nearMode) {  // near pairs: fill the CSR, evaluate, one apply task per bounce (below)
+            const int task = (int)nt.size();
+            nt.push_back(near_task(b, s, mSelf, nullptr));
+            for (int rb = 0; rb < (o.nact + 63) / 64; rb++) nri.push_back(make_int2(task, rb));
+            const long tot = b.nearTotal[s];
+            for (long k = 0; k < tot; k += 16) nqi.push_back(make_int2(task, (int)k));
+            if (b.nearMode == 2)
+                for (long k = 0; k < tot; k += 128) nmi.push_back(make_int2(task, (int)k));
+            for (long e = 0; e < o.nact; e += 256) nai.push_back(make_int2(task, (int)e));
+        }
         // pressure resets (P, field values)
         {
             const int z0 = (int)zt.size();
@@ -276,7 +334,16 @@ int build_tables(Batch &b) {
             g.partL = b.scratchL + offSelf[s]; g.partU = b.scratchU + offSelf[s];
             st[bo].push_back(g);
             ReduceTask r{};
-            r.partL = g.partL; r.partU = g.partU; r.nsplit = g.nsplit; r.nbStride = 1; r.nrhs = 1; r.nr = o.nact;
+            r.partL = g.partL; r.partU = g.partU; r.nsplit = g.nsplit + (b.nearMode ? 1 : 0);
+            r.nbStride = 1; r.nrhs = 1; r.nr = o.nact;
+            if (b.nearMode) {  // the near-field correction is the last split's partial
+                NearApplyTask a{};
+                a.rowStart = b.nearRowStart[s]; a.cols = b.nearCols[s]; a.vals = b.nearVals[s];
+                a.p = o.actP; a.nrm = o.actN; a.pin = pin; a.n = o.nact;
+                a.partL = g.partL + (size_t)g.nsplit * o.nact;
+                a.partU = g.partU + (size_t)g.nsplit * o.nact;
+                na[bo].push_back(a);
+            }
             r.gval = pin; r.gvalStride = o.nact; r.gateMode = GATE_UPPER;
             r.out = pout; r.outStride = o.nact; r.pcell = o.P; r.pcellStride = o.ncell; r.rowCell = o.actCell;
             rt[bo].push_back(r);
@@ -312,7 +379,75 @@ int build_tables(Batch &b) {
         return -1;
     for (int bo = 0; bo < b.bounces; bo++)
         if (b.selfTasks[bo].upload(st[bo]) || b.redTasks[bo].upload(rt[bo])) return -1;
+    if (b.nearMode) {
+        b.nearApply.resize(b.bounces);
+        if (b.nearTasks.upload(nt) || b.nearRowItems.upload(nri) || b.nearMlpItems.upload(nmi) ||
+            b.nearQuadItems.upload(nqi) || b.nearApplyItems.upload(nai))
+            return -1;
+        for (int bo = 0; bo < b.bounces; bo++)
+            if (b.nearApply[bo].upload(na[bo])) return -1;
+    }
     return 0;
+}
+
+// Counts every scene's near pairs on the prepared visibility words (one pass at
+// prepare; the counts depend only on geometry and the near settings) and
+// allocates each scene's CSR.
+int near_prepare(Batch &b) {
+    const int S = (int)b.scenes.size();
+    std::vector<int *> counts(S, nullptr);
+    std::vector<NearBatchTask> nt;
+    std::vector<int2> items;
+    auto fail = [&](int rc) {
+        for (int *p : counts) if (p) (void)hipFree(p);
+        return rc;
+    };
+    for (int s = 0; s < S; s++) {
+        Model &m = *b.scenes[s];
+        const Object &o = m.targets[0];
+        MaskTask tk;
+        size_t words;
+        if (mask_task(m, 2, 0, 0, 0, 0, tk, words)) return fail(-1);
+        tk.mask = b.maskBufs[3 * (size_t)s + 1];
+        tk.errFlag = b.counters + 2 + b.bounces;
+        if (vis_mask_launch(tk, Bvh{m.trisSorted, m.nodes, m.ntri}, m.vc, b.st)) return fail(-1);
+        if (hipMalloc((void **)&counts[s], std::max(1, o.nact) * sizeof(int)) != hipSuccess) {
+            set_error("batch near field: out of device memory");
+            return fail(-1);
+        }
+        nt.push_back(near_task(b, s, tk.mask, counts[s]));
+        for (int rb = 0; rb < (o.nact + 63) / 64; rb++) items.push_back(make_int2(s, rb));
+    }
+    DevVec<NearBatchTask> dt;
+    DevVec<int2> di;
+    if (dt.upload(nt) || di.upload(items)) return fail(-1);
+    int rc = near_rows_multi_launch(dt.p, di.p, (int)di.n, b.st);
+    if (!rc && hipStreamSynchronize(b.st) != hipSuccess) { set_error("batch near field: count pass failed"); rc = -1; }
+    for (int s = 0; !rc && s < S; s++) {
+        const int n = b.scenes[s]->targets[0].nact;
+        std::vector<int> c(n), start(n + 1, 0);
+        if (n && hipMemcpy(c.data(), counts[s], n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = -1;
+            break;
+        }
+        for (int r = 0; r < n; r++) start[r + 1] = start[r] + c[r];
+        const long tot = start[n];
+        int *rs = nullptr, *rw = nullptr, *cl = nullptr;
+        float2 *vl = nullptr;
+        if (hipMalloc((void **)&rs, (n + 1) * sizeof(int)) != hipSuccess ||
+            hipMalloc((void **)&rw, std::max(1L, tot) * sizeof(int)) != hipSuccess ||
+            hipMalloc((void **)&cl, std::max(1L, tot) * sizeof(int)) != hipSuccess ||
+            hipMalloc((void **)&vl, std::max(1L, tot) * sizeof(float2)) != hipSuccess ||
+            hipMemcpy(rs, start.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("batch near field: out of device memory");
+            rc = -1;
+        }
+        b.nearRowStart.push_back(rs); b.nearRows.push_back(rw); b.nearCols.push_back(cl); b.nearVals.push_back(vl);
+        b.nearTotal.push_back(tot);
+    }
+    dt.release();
+    di.release();
+    return fail(rc);
 }
 
 // The whole batch as one launch sequence on the batch stream.
@@ -325,11 +460,20 @@ int enqueue(Batch &b) {
                               b.tune.visEmPerWave, b.counters + 0, b.tune.visGrid, s))
         return -1;
     if (mask_transpose_multi_launch(b.visTasks.p, b.trItems.p, (int)b.trItems.n, s)) return -1;
+    if (b.nearMode) {  // the near pairs of every scene, in CSR order, and their values
+        if (near_rows_multi_launch(b.nearTasks.p, b.nearRowItems.p, (int)b.nearRowItems.n, s) ||
+            near_vals_multi_launch(b.nearTasks.p, nearfield().P, b.nearMlpItems.p, (int)b.nearMlpItems.n,
+                                   b.nearQuadItems.p, (int)b.nearQuadItems.n, s))
+            return -1;
+    }
     if (p2f_multi_launch(b.p2fTasks.p, b.p2fPhys.p, b.p2fItems.p, (int)b.p2fItems.n, b.att, s)) return -1;
     if (prime_multi_launch(b.primeTasks.p, b.primeItems.p, (int)b.primeItems.n, s)) return -1;
     for (int bo = 0; bo < b.bounces; bo++) {
         if (gather_multi_launch(b.selfTasks[bo].p, b.selfPhys.p, b.selfItems.p, (int)b.selfItems.n,
                                 b.counters + 2 + bo, KIND_F2F, 1, b.att, b.tune.gatherGrid, s))
+            return -1;
+        if (b.nearMode &&
+            near_apply_multi_launch(b.nearApply[bo].p, b.nearApplyItems.p, (int)b.nearApplyItems.n, s))
             return -1;
         if (reduce_multi_launch(b.redTasks[bo].p, b.redItems.p, (int)b.redItems.n, s)) return -1;
     }
@@ -382,6 +526,19 @@ int batch_add_scene(const float *tris, int ntri, float delta, float cp, float f,
     return (int)b.in.size() - 1;
 }
 
+int batch_set_near(int mode, float radiusWavelengths, float quadDeltas) {
+    Batch &b = batch();
+    if (b.prepared) { set_error("batch already prepared"); return -3; }
+    if (mode < 0 || mode > 2 || !(radiusWavelengths > 0.f) || !(quadDeltas >= 1.f) || quadDeltas > 16.f) {
+        set_error("batch near field: mode 0-2, radius > 0 wavelengths, quadrature radius 1-16 fragment sizes");
+        return -2;
+    }
+    b.nearMode = mode;
+    b.nearRadiusWl = radiusWavelengths;
+    b.nearQuadDeltas = quadDeltas;
+    return 0;
+}
+
 int batch_set_bounces(int n) {
     Batch &b = batch();
     if (n < 0) { set_error("bounces must be >= 0"); return -2; }
@@ -429,6 +586,13 @@ int batch_prepare() {
     if (hipMalloc((void **)&b.counters, (3 + b.bounces) * sizeof(int)) != hipSuccess) {
         set_error("batch_prepare: out of device memory");
         return fail();
+    }
+    if (b.nearMode) {
+        if (b.nearMode == 2 && !nearfield().trained) {
+            set_error("batch learned near field needs a trained network (sr_nf_train)");
+            return fail();
+        }
+        if (near_prepare(b)) return fail();
     }
     if (build_tables(b)) return fail();
     b.prepared = true;

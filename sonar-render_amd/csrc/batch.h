@@ -16,6 +16,7 @@ int batch_clear();
 int batch_add_scene(const float *tris, int ntri, float delta, float cp, float f, float atten, const float *src,
                     int nsrc, const float *pts, int npts);
 int batch_set_bounces(int n);
+int batch_set_near(int mode, float radiusWavelengths, float quadDeltas);
 int batch_prepare();
 int batch_run(int useGraph, float *ms);
 int batch_autotune(float *report, int nreport);

@@ -32,13 +32,11 @@
 #include "transfer.h"
 #include "learned.h"
 #include "near_rules.h"
+#include "near_eval.h"
 
 namespace sr {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-constexpr float kInv4PiT = 0.0795774715459476678f;
 
 __device__ inline uint32_t pack_h2(float a, float b) {
     // saturate to the FP16 range (near-field terms can exceed the |S| bound)
@@ -49,57 +47,13 @@ __device__ inline uint32_t pack_h2(float a, float b) {
 }
 
 // ---------------------------------------------------------------------------
-// Near-field values (per unit emitter area) for a pair closer than nearRadius.
-// rv, nr: receiver point (relative to the emitter point) and normal, in the
-// emitter's frame (x = xAxis, y = yAxis, z = normal).
-
-// One row (sy) of the 16 x 16 midpoint quadrature of the pair kernel over the
-// emitter's cell, unnormalised. The phase of each sub-point uses the hardware
-// sine/cosine in revolutions (r < 2 lambda here, so r k / 2 pi < 2 keeps
-// ~1e-7 of a revolution).
-// alpha: the attenuation (Im k); the kernel then carries e^{-alpha r} as the
-// reference's thrust::exp(i k r) does (OptiX.cu:350).
-__device__ float2 quad16_row(float3 rv, float3 nr, float d, float k, float alpha, int sy) {
-    const int n = 16;
-    const float h = d / n;
-    const float krev = k * 0.159154943091895336f;
-    float2 acc = make_float2(0.f, 0.f);
-    {
-#pragma unroll 4
-        for (int sx = 0; sx < n; sx++) {
-            const float vx = rv.x - ((sx + 0.5f) * h - 0.5f * d), vy = rv.y - ((sy + 0.5f) * h - 0.5f * d);
-            const float vz = rv.z;
-            const float r = sqrtf(vx * vx + vy * vy + vz * vz);
-            const float cs = vz / r, ci = -(vx * nr.x + vy * nr.y + vz * nr.z) / r;
-            if (!(cs >= 1e-6f) || !(ci >= 1e-6f)) continue;
-            float re, im;
-            if (r < h) {  // ik / (4 pi), k = kr + i alpha
-                re = -alpha * kInv4PiT;
-                im = k * kInv4PiT;
-            } else {
-                float x = r * krev;
-                x -= rintf(x);
-                const float s = __builtin_amdgcn_sinf(x), c = __builtin_amdgcn_cosf(x);
-                const float a = kInv4PiT / r * (alpha != 0.0f ? __expf(-alpha * r) : 1.0f);
-                re = a * fmaf(alpha, c, k * s) + 1.0f / (r * r);
-                im = -a * fmaf(k, c, -alpha * s);
-            }
-            const float w = cs * ci * 0.9f;
-            acc.x += w * re;
-            acc.y += w * im;
-        }
-    }
-    return acc;
-}
+// Near-field values (per unit emitter area) for a pair closer than nearRadius:
+// the evaluators of near_eval.h, read from the (row, emitter) lists the S build
+// fills and written into the tiled FP16 matrix.
 
 // Geometry of one near pair in the emitter's frame (x = xAxis, y = yAxis,
 // z = emitter normal): receiver point relative to the emitter point, and the
 // receiver normal.
-struct NearGeom {
-    float3 rv, nr;
-    float r;
-};
-
 __device__ inline NearGeom near_geom(const TransferGeom &g, const TransferPhys &ph, int row, int j) {
     const float4 rp = g.p[row], rn = g.nrm[row], ep = g.p[j], en = g.nrm[j];
     const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
@@ -121,172 +75,24 @@ __device__ inline void store_pair(uint4 *A, int nKs, int row, int j, uint32_t w)
     q[j & 3] = w;
 }
 
-// Near pairs by 16x16 sub-fragment quadrature: 16 lanes per pair, one row of
-// sub-points each, summed across the lanes.
-__device__ inline void near_quad_block(const TransferGeom &g, const TransferPhys &ph, const int2 *__restrict__ list,
-                                       int count, uint4 *__restrict__ A, int bid) {
-    const long i = ((long)bid * blockDim.x + threadIdx.x) >> 4;
-    const int sy = threadIdx.x & 15;
-    const bool valid = i < count;  // uniform over the pair's 16 lanes
-    int2 pr = make_int2(0, 0);
-    float2 v = make_float2(0.f, 0.f);
-    if (valid) {
-        pr = list[i];
-        const NearGeom q = near_geom(g, ph, pr.x, pr.y);
-        v = quad16_row(q.rv, q.nr, ph.deltaReg, ph.kr, ph.alpha, sy);
-    }
-#pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) {
-        v.x += __shfl_xor(v.x, m);
-        v.y += __shfl_xor(v.y, m);
-    }
-    const float sc = ph.scale * (1.0f / 256.0f);
-    if (valid && sy == 0)
-        store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(v.x * sc, v.y * sc));
-}
 
-// Near pairs by the learned correction: value = G2 + (M / g) t e^{ikr} with G2 the
-// 2 x 2 Gauss base rule (near_rules.h), t from the
-// 10-64-64-2 tanh MLP in BF16 (weights and activations rounded to BF16, FP32
-// accumulation) on the BF16 matrix cores (v_mfma_f32_32x32x16_bf16). A wave
-// evaluates 32 pairs: features -> LDS -> A operand of layer 1; each hidden
-// layer's accumulators go through LDS (tanh, BF16) to become the next A
-// operand; the 64 -> 2 output layer is a per-lane dot product. The layers'
-// B operands are built once per workgroup in fragment order in LDS.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-constexpr int kHP = 72;  // bf16 per activation row in LDS (64 + 8: 16-byte aligned rows)
-
-// tanh x = 1 - 2 / (e^{2x} + 1) on the hardware exp2 and reciprocal (a few ulp
-// in FP32, far below the BF16 rounding of the activations it feeds); +-1 at the
-// extremes (e^{2x} -> inf or 0), no cancellation issue at the BF16 scale.
-__device__ inline float tanh_hw(float x) {
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(x * 2.8853900817779268f) + 1.0f);
-}
-
-__device__ inline void near_mlp_blocks(const TransferGeom &g, const TransferPhys &ph, const int2 *__restrict__ list,
-                                       int count, uint4 *__restrict__ A, int bid, int nblk) {
-    __shared__ bf16x8 sW1[2][64];
-    __shared__ bf16x8 sW2[4][2][64];
-    __shared__ float sB1[64], sB2[64], sW3[2][64], sB3[2];
-    __shared__ __attribute__((aligned(16))) __bf16 sH[4][32][kHP];
-    const float *P = ph.mlp;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int h = lane >> 5, j = lane & 31;
-    // layer B fragments: lane l holds B[k 8 (l/32) .. +7][col l % 32] of its tile
-    for (int idx = threadIdx.x; idx < 2 * 64; idx += 256) {
-        const int c = idx >> 6, l = idx & 63, col = 32 * c + (l & 31), k0 = 8 * (l >> 5);
-        bf16x8 v;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = (__bf16)(k0 + q < kMlpIn ? P[col * kMlpIn + k0 + q] : 0.0f);
-        sW1[c][l] = v;
+// The near lists' pairs as a source for the evaluators: each value goes to its
+// FP16 word of S (scaled as S is).
+struct TransferNearSrc {
+    TransferGeom g;
+    TransferPhys ph;
+    const int2 *list;
+    uint4 *A;
+    __device__ NearGeom geom(long i) const {
+        const int2 pr = list[i];
+        return near_geom(g, ph, pr.x, pr.y);
     }
-    for (int idx = threadIdx.x; idx < 4 * 2 * 64; idx += 256) {
-        const int st = idx >> 7, c = (idx >> 6) & 1, l = idx & 63, col = 32 * c + (l & 31), k0 = 16 * st + 8 * (l >> 5);
-        bf16x8 v;
-#pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = (__bf16)P[kOffW2 + col * 64 + k0 + q];
-        sW2[st][c][l] = v;
+    __device__ bool want(long) const { return true; }
+    __device__ void put(long i, float re, float im) const {
+        const int2 pr = list[i];
+        store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
     }
-    for (int idx = threadIdx.x; idx < 64; idx += 256) {
-        sB1[idx] = (float)(__bf16)P[kOffB1 + idx];
-        sB2[idx] = (float)(__bf16)P[kOffB2 + idx];
-        sW3[0][idx] = (float)(__bf16)P[kOffW3 + idx];
-        sW3[1][idx] = (float)(__bf16)P[kOffW3 + 64 + idx];
-    }
-    if (threadIdx.x < 2) sB3[threadIdx.x] = (float)(__bf16)P[kOffB3 + threadIdx.x];
-    const float lam = 2.0f * 3.14159265358979f / ph.kr;
-    const float d = ph.deltaReg;
-    __bf16 (*H)[kHP] = sH[wave];
-    // workgroup-uniform trip count: tiles of 128 pairs (32 per wave)
-    for (int base = bid * 128; base < count; base += nblk * 128) {
-        const int i = base + wave * 32 + j;
-        const bool valid = i < count;
-        int2 pr = make_int2(0, 0);
-        float2 K = make_float2(0.f, 0.f);
-        float M = 0.f, cs_kr = 1.f, sn_kr = 0.f;
-        __syncthreads();  // the previous tile's reads of H are done
-        // both halves of the wave take pair j: each evaluates two points of the
-        // base rule the network corrects; half 0 also the features and e^{ikr}
-        bf16x8 f0, f1;
-#pragma unroll
-        for (int q = 0; q < 8; q++) { f0[q] = (__bf16)0.0f; f1[q] = (__bf16)0.0f; }
-        float2 Kh = make_float2(0.f, 0.f);
-        if (valid) {
-            pr = list[i];
-            const NearGeom q = near_geom(g, ph, pr.x, pr.y);
-            Kh = near_gauss2_half(q.rv, q.nr, d, ph.kr, h, ph.alpha);
-            if (h == 0) {
-                const float r = q.r;
-                // the network's correction is the attenuation-free one; in an attenuating
-                // medium it is carried by e^{-alpha r} at the pair's distance (the factor's
-                // variation across the cell, alpha delta, is ~1e-3 of the correction)
-                M = 0.9f * (ph.kr * kInv4PiT / r + 1.0f / (r * r)) * (1.0f / kNfTargetGain);
-                if (ph.alpha != 0.0f) M *= expf(-ph.alpha * r);
-                sincosf(ph.kr * r, &sn_kr, &cs_kr);
-                float f[kMlpIn];
-                near_features(r, lam, d, make_float3(q.rv.x / r, q.rv.y / r, q.rv.z / r), q.nr, f);
-#pragma unroll
-                for (int qq = 0; qq < 8; qq++) f0[qq] = (__bf16)f[qq];
-#pragma unroll
-                for (int qq = 8; qq < kMlpIn; qq++) f1[qq - 8] = (__bf16)f[qq];
-            }
-        }
-        K = make_float2(0.25f * (Kh.x + __shfl_xor(Kh.x, 32)), 0.25f * (Kh.y + __shfl_xor(Kh.y, 32)));
-        if (h == 0) {
-            *reinterpret_cast<bf16x8 *>(&H[j][0]) = f0;
-            *reinterpret_cast<bf16x8 *>(&H[j][8]) = f1;
-        }
-        __syncthreads();
-        // layer 1: [32 x 16] x [16 x 64]
-        f32x16 acc[2];
-        {
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&H[j][8 * h]);
-#pragma unroll
-            for (int c = 0; c < 2; c++) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, sW1[c][lane], f32x16{}, 0, 0, 0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int row = (e >> 2) * 8 + h * 4 + (e & 3), col = 32 * c + j;
-                H[row][col] = (__bf16)tanh_hw(acc[c][e] + sB1[col]);
-            }
-        __syncthreads();
-        // layer 2: [32 x 64] x [64 x 64]
-#pragma unroll
-        for (int c = 0; c < 2; c++) acc[c] = f32x16{};
-#pragma unroll
-        for (int st = 0; st < 4; st++) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&H[j][16 * st + 8 * h]);
-#pragma unroll
-            for (int c = 0; c < 2; c++) acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, sW2[st][c][lane], acc[c], 0, 0, 0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 2; c++)
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int row = (e >> 2) * 8 + h * 4 + (e & 3), col = 32 * c + j;
-                H[row][col] = (__bf16)tanh_hw(acc[c][e] + sB2[col]);
-            }
-        __syncthreads();
-        // layer 3: lane (j, h) -> output h of pair j
-        float o = sB3[h];
-#pragma unroll
-        for (int k8 = 0; k8 < 8; k8++) {
-            const bf16x8 hv = *reinterpret_cast<const bf16x8 *>(&H[j][8 * k8]);
-#pragma unroll
-            for (int q = 0; q < 8; q++) o = fmaf(sW3[h][8 * k8 + q], (float)hv[q], o);
-        }
-        const float o1 = __shfl(o, j + 32);
-        if (h == 0 && valid) {
-            // G2 + (M / g) t e^{ikr}
-            const float re = K.x + M * (o * cs_kr - o1 * sn_kr), im = K.y + M * (o * sn_kr + o1 * cs_kr);
-            store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
-        }
-    }
-}
+};
 
 // Both near-pair lists in one launch: blocks [0, quadBlocks) take the quadrature
 // list (16 lanes per pair), the rest the learned list (128 pairs per block and
@@ -294,8 +100,12 @@ __device__ inline void near_mlp_blocks(const TransferGeom &g, const TransferPhys
 __global__ __launch_bounds__(256) void k_near(TransferGeom g, TransferPhys ph, int nQuad, int nMlp, int quadBlocks,
                                               uint4 *__restrict__ A) {
     if (blockIdx.x == 0 && threadIdx.x < 2) ph.nearCount[threadIdx.x] = 0;
-    if ((int)blockIdx.x < quadBlocks) near_quad_block(g, ph, ph.nearQ, nQuad, A, blockIdx.x);
-    else near_mlp_blocks(g, ph, ph.nearM, nMlp, A, blockIdx.x - quadBlocks, gridDim.x - quadBlocks);
+    if ((int)blockIdx.x < quadBlocks) {
+        near_quad_block(TransferNearSrc{g, ph, ph.nearQ, A}, nQuad, ph.deltaReg, ph.kr, ph.alpha, blockIdx.x);
+    } else {
+        near_mlp_blocks(TransferNearSrc{g, ph, ph.nearM, A}, nMlp, ph.mlp, ph.deltaReg, ph.kr, ph.alpha,
+                        blockIdx.x - quadBlocks, gridDim.x - quadBlocks);
+    }
 }
 
 // Near pairs are staged per wave in LDS and flushed to the global lists with

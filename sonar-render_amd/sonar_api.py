@@ -82,6 +82,7 @@ _sig("sr_sound_visualisation_init", [_f, _f, _i, _i])
 _sig("sr_set_source_object_amplitude", [_i, _f, _f])
 _sig("sr_project_target_to_field_points_noalloc", [])
 _sig("sr_device_allocations", [], _l)
+_sig("sr_batch_set_near", [_i, _f, _f])
 _sig("sr_render_png", [ctypes.c_char_p, _i, _i, _fp, _i])
 _sig("sr_render_rgb", [_i, _i, _i, _fp, _u8p])
 _sig("sr_render_facets_rgb", [_fp, _i, _f, _dp, _l, _i, _f, _f, _i, _i, _fp, _u8p])
@@ -540,6 +541,11 @@ def batch_add_scene(tris, delta, f, sources, points, c=1480.0, atten=0.0):
 
 def batch_set_bounces(n):
     _check(cpp_lib.sr_batch_set_bounces(int(n)), "batch_set_bounces")
+
+
+def batch_set_near(mode, radius_wavelengths=2.0, quad_deltas=3.0):
+    """Near field of the batched bounces: 0 point rule, 1 16x16 quadrature, 2 learned."""
+    _check(cpp_lib.sr_batch_set_near(int(mode), float(radius_wavelengths), float(quad_deltas)), "batch_set_near")
 
 
 def batch_prepare():
