@@ -187,12 +187,87 @@ def field_surface_render(steps, warmup):
             "kernel_ms_per_render": {k: round(v["ms"] / steps, 4) for k, v in stats.items() if v["launches"]}}
 
 
+def bench_batch(args):
+    """Batched uneven scenes: 256 seeded scenes (random convex polyhedra of 8-40 triangles
+    or plates, 0.3-1.5 m, f uniform in 2-20 kHz, one source, 32 field points, 2 bounces),
+    fragments at lambda/4 with the learned near-field correction, all scenes in one
+    sequence of persistent multi-scene launches replayed as one hipGraph, tiles chosen
+    by the autotuner. Beside it: the quadrature near field at lambda/4 and the
+    reference's point rule at lambda/7.5. Each form is checked on every scene against
+    the FP64 oracle of its own discretisation (the lambda/4 forms against the oracle
+    with its 16x16 near-field judge)."""
+    import scenes
+    import sonar_api as api
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    base = scenes.config_batch()
+    api.nf_init(1)
+    t0 = time.perf_counter()
+    tr, va = api.nf_train(nsamples=262144, steps=args.nf_steps, batch=4096, lr=3e-3, seed=7)
+    api.device_synchronize()
+    train = {"train_s": round(time.perf_counter() - t0, 3), "train_err": round(float(tr), 4),
+             "heldout_err": round(float(va), 4)}
+    forms = [("learned_lambda4", 4.0, 2), ("quadrature_lambda4", 4.0, 1), ("plain_lambda7p5", 7.5, 0)]
+    out, cpu = {}, None
+    threads = cpu_threads()
+    oracle.set_threads(threads)
+    for name, frac, near in forms:
+        batch = [dict(sc, targets=[(sc["targets"][0][0], sc["c"] / sc["f"] / frac)]) for sc in base]
+        api.batch_clear()
+        api.batch_set_bounces(2)
+        api.batch_set_near(near)
+        for sc in batch:
+            tris, d = sc["targets"][0]
+            api.batch_add_scene(tris, d, sc["f"], sc["sources"], sc["points"], c=sc["c"], atten=sc["atten"])
+        api.batch_prepare()
+        rep = api.batch_autotune()
+        for _ in range(args.warmup):
+            api.batch_run(use_graph=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            api.batch_run(use_graph=True)
+        dt = (time.perf_counter() - t0) / args.steps
+        pairs = api.batch_pairs()
+        got = [api.batch_field_points(i, len(sc["points"])) for i, sc in enumerate(batch)]
+        api.batch_clear()
+        t0 = time.perf_counter()
+        refs = [oracle.run_scene(sc, near=(16, 2 * sc["c"] / sc["f"]) if near else None)["points"] for sc in batch]
+        cdt = time.perf_counter() - t0
+        err = float(np.linalg.norm(np.concatenate(got) - np.concatenate(refs)) / np.linalg.norm(np.concatenate(refs)))
+        worst = max(float(np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-300)) for g, r in zip(got, refs))
+        out[name] = {"ms_per_batch": round(dt * 1e3, 3), "gpairs_per_s": round(pairs / dt / 1e9, 3),
+                     "pairs_per_batch": pairs, "rel_l2_vs_fp64": err, "worst_scene_rel_l2": worst,
+                     "oracle": "FP64 oracle, 16x16 near-field judge" if near else "FP64 oracle, point rule",
+                     "autotune_best": dict(zip(("ms", "vis_grid", "gather_grid", "vis_emitters_per_item",
+                                                "gather_min_emitters_per_split"),
+                                               [float(x) for x in rep[np.argmin(rep[:, 0])]]))}
+        if name == "learned_lambda4":
+            cpu = {"value": round(pairs / cdt / 1e9, 5), "unit": "Gpairs/s", "cores": threads, "kind": "port",
+                   "cpu_model": cpu_model(),
+                   "sample": f"the whole batch of {len(batch)} scenes, FP64 OpenMP oracle with the 16x16 "
+                             f"near-field judge on {threads} threads: {cdt:.2f} s"}
+        print(f"[bench] {name}: {dt * 1e3:.3f} ms/batch, rel-L2 {err:.2e}", file=sys.stderr, flush=True)
+    P = out["learned_lambda4"]
+    line = {
+        "metric": "Gpairs/s & s/scene at <=1e-3 rel-L2 vs FP64: learned+mixed vs plain FP32",
+        "value": P["gpairs_per_s"], "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": P["ms_per_batch"], "s_per_scene": round(P["ms_per_batch"] / 1e3 / len(base), 8),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch_256_uneven_scenes", "form": "learned_lambda4", "scenes": len(base),
+                   "bounces": 2, "fragment": "lambda/4", "graph": "one hipGraph replay per batch",
+                   "parallelism": "single GPU"},
+        "rel_l2_vs_fp64": P["rel_l2_vs_fp64"], "forms": out, "training": train, "cpu_baseline": cpu,
+        "roofline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="trihedral", choices=sorted(_configs()))
+    ap.add_argument("--config", default="trihedral", choices=sorted(list(_configs()) + ["batch"]))
     ap.add_argument("--forms", default="plain,mixed,learned", help="comma list of plain, mixed, learned")
     ap.add_argument("--cpu-sample", type=int, default=4, help="incidences the CPU baseline and the gate run")
     ap.add_argument("--no-cpu", action="store_true")
@@ -214,6 +289,10 @@ def main():
         print("bench.py: this engine runs one GPU per process; multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "batch":
+        bench_batch(args)
+        return
 
     import runner
 
