@@ -262,12 +262,52 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def study_convergence(args):
+    """How far apart the discretisations are: the trihedral at 20 kHz, 3 bounces, two
+    seeded incidences, run by the plain FP32 form (pinned to the FP64 oracle at lambda/7.5)
+    at fragment edges lambda/7.5 .. lambda/30, and by the learned lambda/4 form. Prints the
+    rel-L2 distance of every run's monostatic returns from the lambda/7.5 run and from the
+    finest run."""
+    import runner
+    import scenes
+    import sonar_api as api
+    base = scenes.config_trihedral()
+    pick = [5, 50]
+    src = base["sources"][pick]
+    runs = {}
+    for frac in (7.5, 10.0, 15.0, 20.0, 30.0):
+        sc = scenes.config_trihedral(frac=frac)
+        sc = dict(sc, sources=src, points=src.copy())
+        t0 = time.perf_counter()
+        out = runner.run_scene(sc)
+        runs[f"lambda/{frac:g}"] = (out["points"].ravel(), time.perf_counter() - t0, runner.pair_count(sc))
+        print(f"[study] lambda/{frac:g}: {runs[f'lambda/{frac:g}'][1]:.2f} s", file=sys.stderr, flush=True)
+    api.nf_init(1)
+    api.nf_train(nsamples=262144, steps=args.nf_steps, batch=4096, lr=3e-3, seed=7)
+    sc = scenes.config_trihedral(frac=4.0)
+    sc = dict(sc, sources=src, points=src.copy())
+    runner.load_scene(sc, transfer_mode=1)
+    api.set_near_mode(2, 2.0)
+    runner.project_scene(sc)
+    runs["lambda/4 learned"] = (runner.read_results(sc)["points"].ravel(), 0.0, runner.pair_count(sc))
+    api.TearDownCuda()
+    ref75, fine = runs["lambda/7.5"][0], runs["lambda/30"][0]
+
+    def d(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    line = {"study": "discretisation distance, trihedral 1 m, 20 kHz, 3 bounces, incidences "
+                     f"{pick} of the 64-incidence grid, monostatic returns",
+            "rows": {k: {"rel_l2_vs_lambda7p5": d(v[0], ref75), "rel_l2_vs_lambda30": d(v[0], fine),
+                         "pairs": v[2]} for k, v in runs.items()}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="trihedral", choices=sorted(list(_configs()) + ["batch"]))
+    ap.add_argument("--config", default="trihedral", choices=sorted(list(_configs()) + ["batch", "convergence"]))
     ap.add_argument("--forms", default="plain,mixed,learned", help="comma list of plain, mixed, learned")
     ap.add_argument("--cpu-sample", type=int, default=4, help="incidences the CPU baseline and the gate run")
     ap.add_argument("--no-cpu", action="store_true")
@@ -292,6 +332,9 @@ def main():
 
     if args.config == "batch":
         bench_batch(args)
+        return
+    if args.config == "convergence":
+        study_convergence(args)
         return
 
     import runner
