@@ -69,20 +69,29 @@ def test_colour_map_levels_and_wheel(api):
     assert np.argmax(rgba[3, :3]) == 2
 
 
-FIXTURE_TRIHEDRAL = os.path.join(os.path.dirname(__file__), "golden", "trihedral_reflector.stl")
+FIXTURE_TRIHEDRAL = os.path.join(os.path.dirname(__file__), "golden", "thick_trihedral.stl")
 
 
-def test_reference_trihedral_fixture(api):
-    """The reference's trihedral mesh (run_scripts/trihedral_reflector.stl, copied as a data fixture):
-    36 triangles of a closed, plate-thick trihedral; every edge is shared (the plates' common edge 4 times)."""
+def test_thick_trihedral_fixture(api):
+    """The plate-thick trihedral (tests/golden/make_thick_trihedral.py, the geometry of the reference's
+    run_scripts/trihedral_reflector.stl): 36 triangles, three closed 1 x 1 x 0.01 m boxes with outward
+    normals, total area 6.12 m^2; the STL reader returns what the generator wrote."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_thick_trihedral
     t = api.read_stl(FIXTURE_TRIHEDRAL).astype(np.float64)
     assert t.shape == (36, 3, 3)
+    assert np.array_equal(t.astype(np.float32), make_thick_trihedral.thick_trihedral())
     area = 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1).sum()
     assert abs(area - 6.12) < 1e-5
     assert np.allclose(np.abs(t).max(), np.sqrt(0.5), atol=1e-6)
     from collections import Counter
-    edges = Counter()
-    for tri in t:
-        for a, b in ((0, 1), (1, 2), (2, 0)):
-            edges[tuple(sorted([tuple(np.round(tri[a], 5)), tuple(np.round(tri[b], 5))]))] += 1
-    assert sorted(Counter(edges.values()).items()) == [(2, 52), (4, 1)]
+    for box in t.reshape(3, 12, 3, 3):
+        edges = Counter()
+        for tri in box:
+            for a, b in ((0, 1), (1, 2), (2, 0)):
+                edges[tuple(sorted([tuple(np.round(tri[a], 5)), tuple(np.round(tri[b], 5))]))] += 1
+        assert set(edges.values()) == {2}  # closed
+        # divergence theorem: outward normals give the box volume 1 x 1 x 0.01
+        vol = sum(np.dot(tri[0], np.cross(tri[1], tri[2])) for tri in box) / 6.0
+        assert abs(vol - 0.01) < 1e-6
