@@ -50,11 +50,14 @@ TOL = 1e-3
 
 # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (calibrated
 # FETCH_SIZE and WRITE_SIZE, see profiles/transfer_gemm_hbm_bytes.md), per workload.
-PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 823.8e6,
-               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 108.9e6}
-# Vector-issue fraction of the visibility kernel (SQ_INSTS_VALU x 4 / SIMD-cycles of its
-# traced duration), per workload, from profiles/vis_mask_counters.md.
-VIS_VALU_ISSUE = {}
+PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 824.9e6,
+               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 108.8e6}
+# Vector wave-instructions (SQ_INSTS_VALU, exact) of one scene's visibility masks
+# (SELF + sources -> fragments + fragments -> field points), per workload, from
+# profiles/vis_mask_counters.md. The masks are bound by vector issue: a wave64
+# instruction holds a 16-lane SIMD for 4 cycles, 1,024 SIMDs at 2.4 GHz.
+VIS_VALU_INSTR = {"trihedral_1m_20kHz": 290_967_406, "trihedral_1m_20kHz_lambda4": 30_859_537}
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9  # G wave-instructions/s
 
 
 def _configs():
@@ -164,13 +167,18 @@ def roofline(stats, mode, workload):
 def vis_roofline(stats, workload, steps):
     st = stats["vis"]
     n = max(1, st["launches"])
-    ms = st["ms"] / n
+    per_scene_ms = st["ms"] / max(1, steps)
     out = {"kernel": "k_vis_mask (shadow rays + cosine culls -> one visibility word per 64 receivers and emitter)",
-           "launches_per_scene": st["launches"] // max(1, steps), "avg_launch_ms": round(ms, 4),
+           "launches_per_scene": st["launches"] // max(1, steps), "avg_launch_ms": round(st["ms"] / n, 4),
+           "ms_per_scene": round(per_scene_ms, 4),
            "pairs_per_s_G": round(st["pairs"] / max(st["ms"], 1e-12) / 1e6, 2),
-           "words_GBs": round(st["bytes"] / max(st["ms"], 1e-12) / 1e6, 1), "bound": "valu"}
-    v = VIS_VALU_ISSUE.get(workload)
-    out["valu_issue_frac"] = v
+           "words_GBs": round(st["bytes"] / max(st["ms"], 1e-12) / 1e6, 1)}
+    instr = VIS_VALU_INSTR.get(workload)
+    if instr and per_scene_ms > 0:
+        ach = instr / (per_scene_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "valu", "achieved": round(ach, 1), "peak": VALU_ISSUE_PEAK,
+                           "unit": "G wave-instr/s", "frac": round(ach / VALU_ISSUE_PEAK, 4),
+                           "instructions_per_scene": instr}
     return out
 
 
