@@ -168,6 +168,14 @@ int sr_clear_vis_cache(void);
  * FP16 MFMA GEMM with FP32 accumulation over all incidence planes. */
 int sr_set_transfer_mode(int mode);
 int sr_transfer_mode(void);
+/* Split-K of the transfer form's bounce product: each 128-row group's
+ * non-zero K chunks are dealt out evenly over n splits (n workgroups per group,
+ * partial sums reduced after). 0 (default): chosen from the geometry and the
+ * GPU's CU count when the transfer matrix is prepared. Takes effect at the next
+ * preparation (a changed value rebuilds the matrix); n up to 4096, capped at
+ * the group's chunk count. */
+int sr_set_transfer_splits(int n);
+int sr_transfer_splits(void);
 
 /* Plain fragment->fragment sums (transfer mode 0). 0 (default): with 16 or
  * more incidence planes the pair kernel is evaluated in FP32 on the vector

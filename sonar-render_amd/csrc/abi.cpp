@@ -287,6 +287,10 @@ int sr_set_transfer_mode(int mode) {
     return guarded("set_transfer_mode", [&] { return sr::model_set_transfer_mode(cur(), mode); });
 }
 int sr_transfer_mode(void) { return cur().transferMode; }
+int sr_set_transfer_splits(int n) {
+    return guarded("set_transfer_splits", [&] { return sr::model_set_transfer_splits(cur(), n); });
+}
+int sr_transfer_splits(void) { return cur().transferSplits; }
 int sr_set_gather_mode(int mode) {
     return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(cur(), mode); });
 }

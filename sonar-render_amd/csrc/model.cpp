@@ -877,7 +877,7 @@ static int ensure_transfer(Model &m, int i) {
     const bool gateIn = big && padded <= n + n / 4;
     const int nPad = gateIn ? (int)padded : n;
     const int nRt = ((nPad + 127) / 128) * 4;
-    if (!t.A || t.nKs != nKs || t.nRt != nRt || t.gateInGemm != gateIn) {
+    if (!t.A || t.nKs != nKs || t.nRt != nRt || t.gateInGemm != gateIn || t.splitsReq != m.transferSplits) {
         free_transfer(o);
         const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
@@ -889,6 +889,7 @@ static int ensure_transfer(Model &m, int i) {
         t.nRt = nRt;
         t.nPad = nPad;
         t.gateInGemm = gateIn;
+        t.splitsReq = m.transferSplits;
         {
             std::vector<int> fs(n);
             for (int r = 0; r < n; r++)
@@ -953,31 +954,41 @@ static int ensure_transfer(Model &m, int i) {
         }
         SR_HIP(hipMemcpy(t.wgSkip, skip.data(), nWg * sizeof(int2), hipMemcpyHostToDevice));
         t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
-        // Split K in whole chunks of 8 steps, at least 32 steps a split. The product
-        // holds two workgroups per CU (64 KiB of LDS each), so the grid runs in rounds
-        // of 2 x CUs workgroups: pick the split count with the least estimated time,
-        // rounds x (chunks per split + ~2 chunks of start-up and partial-sum store),
-        // which keeps the last round from running a few workgroups alone.
+        // Split each group's non-zero chunks evenly (split_walk), at least 4 chunks a
+        // split. Time in units of one workgroup streaming one chunk (32 KiB of S) while
+        // the chip is full; the product holds two workgroups per CU (64 KiB of LDS
+        // each), so the grid runs in rounds of 2 x CUs workgroups. Estimate, per split
+        // count: the rounds x (the most chunks a split walks + ~2 chunks of start-up and
+        // its 64 KiB partial-sum store), or the chip's bytes (every chunk, and each
+        // split's partials written and read back by the reduction) at full rate,
+        // whichever is longer; take the least.
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
-        const long slots = 2L * cus, chunks = (nKs + 7) / 8;
+        const long slots = 2L * cus;
+        long maxChunks = 0, sumChunks = 0;
+        for (int w = 0; w < nWg; w++) {
+            const long c = (nKs - (skip[w].y - skip[w].x)) / 8;
+            maxChunks = std::max(maxChunks, c);
+            sumChunks += c;
+        }
         int nsplit = 1;
         double best = 1e300;
-        for (int sp = 1; sp <= std::max(1, nKs / 32); sp++) {
-            const double est = (double)((nWg * (long)sp + slots - 1) / slots) * ((chunks + sp - 1) / sp + 2);
+        for (int sp = 1; sp <= std::max(1L, maxChunks / 4); sp++) {
+            const double rounds = (double)((nWg * (long)sp + slots - 1) / slots) * ((maxChunks + sp - 1) / sp + 2);
+            const double bytes = ((double)sumChunks + 4.0 * nWg * sp) / slots;
+            const double est = std::max(rounds, bytes);
             if (est < best) { best = est; nsplit = sp; }
         }
-        t.ksPer = ((nKs + nsplit - 1) / nsplit + 7) / 8 * 8;
-        t.nsplit = (nKs + t.ksPer - 1) / t.ksPer;
+        if (m.transferSplits > 0) nsplit = (int)std::min<long>(m.transferSplits, std::max(1L, maxChunks));
+        t.nsplit = nsplit;
         if (gateIn) {
             // which partials each (group, split) writes, by the product's own walk over K
             std::vector<unsigned char> cls((size_t)nWg * t.nsplit, 0);
             for (int w = 0; w < nWg; w++)
                 for (int s = 0; s < t.nsplit; s++) {
-                    const int ks0 = s * t.ksPer, ks1 = std::min(nKs, ks0 + t.ksPer);
-                    auto next = [&](int kb) { return (kb >= skip[w].x && kb < skip[w].y) ? skip[w].y : kb; };
+                    const SplitWalk sw = split_walk(skip[w], nKs, t.nsplit, s);
                     unsigned char c = 0;
-                    for (int kb = next(ks0); kb < ks1; kb = next(kb + 8)) c |= kb < kB[w] ? 1 : 2;
+                    for (int o = sw.o0; o < sw.o1; o++) c |= sw.k(o) < kB[w] ? 1 : 2;
                     cls[(size_t)w * t.nsplit + s] = c;
                 }
             // partial-sum rows one pass writes per plane: each partial a split writes, over its group's active rows
@@ -1080,7 +1091,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     const int n = o.nact;
-    const int nsplit = t.nsplit, ksPer = t.ksPer;  // split K, chosen with the transfer matrix
+    const int nsplit = t.nsplit;  // split K, chosen with the transfer matrix
     // Bounce state, two sets used in turn: this bounce's set, zero or holding the
     // column maxima of P_in that the previous bounce's reduction found; the other
     // set is cleared here for the next bounce.
@@ -1111,7 +1122,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
                                  t.colInv, t.B, m.stream, clr ? nxt : nullptr, clr ? stWords : 0))
                 return -1;
         }
-        TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, ksPer, t.wgSkip, m.scratchL,
+        TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, t.wgSkip, m.scratchL,
                         t.gateInGemm ? t.kB : nullptr, m.scratchU, t.gateInGemm ? nullptr : t.rowFs};
         m.lastGemm = gm;
         m.haveLastGemm = true;
@@ -1364,6 +1375,12 @@ int model_set_near_quadrature_radius(Model &m, float deltas) {
 int model_set_gather_mode(Model &m, int mode) {
     if (mode != 0 && mode != 1) { set_error("gather mode: 0 FP32 MFMA products from 16 planes on, 1 VALU only"); return -2; }
     m.gatherMode = mode;
+    return 0;
+}
+
+int model_set_transfer_splits(Model &m, int n) {
+    if (n < 0 || n > 4096) { set_error("transfer splits: 0 (automatic) to 4096"); return -2; }
+    m.transferSplits = n;
     return 0;
 }
 

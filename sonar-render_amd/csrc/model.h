@@ -59,7 +59,8 @@ struct Object {
         int nPad = 0;              // rows of S (padded)
         int *rowAct = nullptr, *actRow = nullptr, *kB = nullptr;
         unsigned char *cls = nullptr;
-        int nsplit = 1, ksPer = 8; // split-K of the bounce product
+        int nsplit = 1;             // split-K of the bounce product (split_walk)
+        int splitsReq = 0;          // the split request (Model::transferSplits) nsplit was chosen for
         int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
         int *nearCount = nullptr;
         bool nearClear = false;    // nearCount is known to be zero (the near-pair kernels clear it)
@@ -126,6 +127,7 @@ struct Model {
     struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; };
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
+    int transferSplits = 0;  // splits of the transfer product: 0 automatic
     int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
     int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
     int gatedCap = 0;

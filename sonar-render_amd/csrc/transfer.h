@@ -35,11 +35,25 @@ struct TransferPhys {
     float quadRadius;      // learned mode: closer pairs use the quadrature (metres, >= delta)
 };
 
+// Split s of nsplit over the K steps of one 128-row group whose zero chunks are
+// [skip.x, skip.y) (chunk-aligned K steps): the group's non-zero chunks of 8
+// steps, numbered in K order, dealt out evenly over the splits, so that every
+// (group, split) streams the same number of chunks whatever its facet. The split
+// walks the chunk numbers [o0, o1); chunk number o starts at K step k(o).
+struct SplitWalk {
+    int below, skipEnd, o0, o1;
+    __host__ __device__ int k(int o) const { return o < below ? o * 8 : (o - below) * 8 + skipEnd; }
+};
+__host__ __device__ inline SplitWalk split_walk(int2 skip, int nKs, int nsplit, int s) {
+    const int below = skip.x / 8, total = nKs / 8 - (skip.y - skip.x) / 8;
+    return SplitWalk{below, skip.y, (int)((long)s * total / nsplit), (int)((long)(s + 1) * total / nsplit)};
+}
+
 struct TransferGemm {
     const uint4 *A, *B;
     int n, nKs, nRt;
     int nplanes;        // planes of this group (<= 64)
-    int nsplit, ksPerSplit;
+    int nsplit;         // splits of each group's non-zero chunks (split_walk)
     const int2 *wgSkip;                   // per 4-row-tile group: K steps [x, y) that hold only zeros
     float2 *partL;                        // [split][64][n]
     // Zero-input gate inside the product: per 128-row group, the first K step of

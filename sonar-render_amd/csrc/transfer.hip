@@ -337,7 +337,8 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
 // the 4 waves; A fragments stream from HBM straight into registers. Both are
 // requested one chunk ahead of the MFMAs that use them (two LDS buffers, two
 // A register sets). Chunks whose emitters all lie on the workgroup's single
-// facet (or its run of facets with one normal) hold only zeros and are skipped.
+// facet (or its run of facets with one normal) hold only zeros and are skipped;
+// the splits (blockIdx.y) share out each group's remaining chunks evenly.
 // The zero-input gate needs each row's lower-facet terms (emitters below its
 // own facet) apart from the rest, in one of two ways:
 //   kB (rows of S padded so that each 128-row group is single-facet): the
@@ -350,23 +351,42 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
 constexpr int kKC = 8;
 static_assert(kKC == 8, "K steps and split sizes are padded to multiples of 8");
 
+// Workgroup barrier without __syncthreads' memory fence, whose wait for every
+// outstanding global load would also hold the A chunk requested ahead. What the
+// barrier orders here needs no fence: each wave waits (s_waitcnt) for its own
+// B loads into LDS before it, and its LDS reads of the other buffer have been
+// consumed by issued MFMAs. The empty asm keeps the compiler from moving memory
+// accesses across it.
+__device__ inline void workgroup_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // ROLE 0: one accumulator (the kB gate, or none); 2: DUAL.
 template <int ROLE>
 __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     constexpr bool DUAL = ROLE == 2;
-    __shared__ uint4 sB[2][kKC * 4 * 64];
+    // two separately named buffers, so that the compiler can tell the LDS reads of
+    // one from the asynchronous global -> LDS copies into the other
+    __shared__ uint4 sB0[kKC * 4 * 64], sB1[kKC * 4 * 64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int rt = blockIdx.x * 4 + wave;
-    const int split = blockIdx.y;
-    const int ks0 = split * t.ksPerSplit;
-    const int ks1 = min(t.nKs, ks0 + t.ksPerSplit);
+    // Workgroup L = group * nsplit + split. Workgroups are dealt round-robin to the
+    // 8 XCDs, so with nsplit dividing 8 every XCD runs one split of every group it
+    // holds: its L2 caches that split's part of B rather than all of it.
+    const int grp = blockIdx.x / t.nsplit;
+    const int split = blockIdx.x - grp * t.nsplit;
+    const int rt = grp * 4 + wave;
     const bool rtValid = rt < t.nRt;
-    const int2 skip = t.wgSkip[blockIdx.x];  // chunk-aligned K steps [x, y) that hold only zeros
+    // this split's run of the group's non-zero chunks (chunk-aligned K steps
+    // [skip.x, skip.y) hold only zeros and are never read)
+    const SplitWalk sw = split_walk(t.wgSkip[grp], t.nKs, t.nsplit, split);
+    const int nch = sw.o1 - sw.o0;
     const uint4 *Arow = t.A + (size_t)(rtValid ? rt : 0) * t.nKs * 64 + lane;
     const uint4 *Bt = t.B + threadIdx.x;     // thread x stages element x of every K step
     // in-product gate: K steps from kBnd on are the group's higher-facet emitters
-    const int kBnd = (!DUAL && t.kB) ? t.kB[blockIdx.x] : 0x7fffffff;
+    const int kBnd = (!DUAL && t.kB) ? t.kB[grp] : 0x7fffffff;
     int kLast = -1;  // the last chunk summed
     // DUAL: this lane's row's facet start, and the wave's least and greatest
     int fsRow = 0, fsMin = 0, fsMax = 0;
@@ -405,17 +425,17 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
         }
     };
     // one K step's four products into an accumulator set
-    auto mfma_step = [&](f32x16 *ac, uint4 a, int bb, int s) {
+    auto mfma_step = [&](f32x16 *ac, uint4 a, const uint4 *sb, int s) {
         const half8 av = __builtin_bit_cast(half8, a);
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const half8 bv = __builtin_bit_cast(half8, sB[bb][(s * 4 + c) * 64 + lane]);
+            const half8 bv = __builtin_bit_cast(half8, sb[(s * 4 + c) * 64 + lane]);
             ac[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, ac[c], 0, 0, 0);
         }
     };
     // DUAL: the chunk's steps, each split at the row's facet start (lane (j, h)
     // holds emitters 8 k + 4 h .. + 3 of row j, one complex FP16 word each)
-    auto dual_chunk = [&](const uint4 *a, int kb, int bb) {
+    auto dual_chunk = [&](const uint4 *a, int kb, const uint4 *bb) {
         if ((kb + kKC) * 8 <= fsMin) {  // below every row's facet: lower-facet terms
 #pragma unroll
             for (int s = 0; s < kKC; s++) mfma_step(acc, a[s], bb, s);
@@ -436,49 +456,59 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
         }
     };
 
-    auto next_chunk = [&](int kb) { return (kb >= skip.x && kb < skip.y) ? skip.y : kb; };
+    // K step of the split's i-th chunk; past the end the last chunk again, so that
+    // the loop's one-ahead requests are unconditional (and those re-reads hit L2)
+    auto kat = [&](int i) { return sw.k(sw.o0 + min(i, nch - 1)); };
     // B goes global -> LDS directly (global_load_lds_dwordx4: lane l of wave w
     // writes element 64 w + l of a K step); A alternates between two register
     // sets with the two LDS buffers. Both are requested one chunk ahead.
     typedef __attribute__((address_space(3))) void lds_void;
-    auto stage_b = [&](int k, int bb) {
+    auto stage_b = [&](int k, uint4 *bb) {
 #pragma unroll
         for (int s = 0; s < kKC; s++)
             __builtin_amdgcn_global_load_lds((const void *)(Bt + (size_t)(k + s) * 256),
-                                             (lds_void *)&sB[bb][s * 256 + wave * 64], 16, 0, 0);
+                                             (lds_void *)&bb[s * 256 + wave * 64], 16, 0, 0);
     };
+    // s_waitcnt immediate (gfx9 encoding): vector-memory count <= kKC, the export and
+    // LDS/scalar counts at their maxima (not waited on)
+    constexpr int kWaitAllButNextA = (kKC & 0xf) | ((kKC >> 4) << 14) | (0x7 << 4) | (0xf << 8);
     uint4 a0[kKC], a1[kKC];
-    int kb = next_chunk(ks0);
-    if (kb < ks1) {
-        stage_b(kb, 0);
-#pragma unroll
-        for (int s = 0; s < kKC; s++) a0[s] = Arow[(size_t)(kb + s) * 64];
-    }
-#define SR_GEMM_PHASE(ACUR, ANEXT, BUF)                                                              \
+    // One chunk per phase: the next chunk's A is requested first and stays in
+    // flight across the wait for this chunk's A and B and the barrier; then the
+    // next chunk's B is staged into the other buffer and this chunk's products
+    // run. Every request is unconditional (a phase past the end skips only its
+    // products), which keeps the two A register sets fixed across the loop.
+#define SR_GEMM_PHASE(ACUR, ANEXT, BUF, NBUF, I)                                                     \
     {                                                                                                \
-        const int kn = next_chunk(kb + kKC);                                                         \
-        __builtin_amdgcn_s_waitcnt(0); /* this wave's share of sB[BUF] has landed */                 \
-        __syncthreads(); /* sB[BUF] complete; every wave is done reading sB[BUF ^ 1] */              \
-        if (kn < ks1) {                                                                              \
-            stage_b(kn, BUF ^ 1);                                                                    \
-            _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = Arow[(size_t)(kn + s) * 64];  \
-        }                                                                                            \
-        if (DUAL) {                                                                                  \
-            dual_chunk(ACUR, kb, BUF);                                                               \
-        } else {                                                                                     \
-            _Pragma("unroll") for (int s = 0; s < kKC; s++) mfma_step(acc, ACUR[s], BUF, s);        \
-            kLast = kb;                                                                              \
-            if (kb < kBnd && kn >= kBnd && kn < ks1) { /* lower-facet part done, higher follows */   \
-                store_part(t.partL, acc);                                                            \
-                _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                     \
+        const int kc = kat(I), kn = kat((I) + 1);                                                    \
+        _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = Arow[(size_t)(kn + s) * 64];      \
+        __builtin_amdgcn_s_waitcnt(kWaitAllButNextA); /* all but those kKC loads: A, B of kc */      \
+        workgroup_barrier(); /* BUF complete; every wave is done reading NBUF */                     \
+        stage_b(kn, NBUF);                                                                           \
+        if ((I) < nch) {                                                                             \
+            if (DUAL) {                                                                              \
+                dual_chunk(ACUR, kc, BUF);                                                           \
+            } else {                                                                                 \
+                _Pragma("unroll") for (int s = 0; s < kKC; s++) mfma_step(acc, ACUR[s], BUF, s);    \
+                kLast = kc;                                                                          \
+                if (kc < kBnd && (I) + 1 < nch && kn >= kBnd) { /* lower-facet part done */          \
+                    store_part(t.partL, acc);                                                        \
+                    _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                 \
+                }                                                                                    \
             }                                                                                        \
         }                                                                                            \
-        kb = kn;                                                                                     \
     }
-    while (kb < ks1) {
-        SR_GEMM_PHASE(a0, a1, 0)
-        if (kb >= ks1) break;
-        SR_GEMM_PHASE(a1, a0, 1)
+    if (nch > 0) {
+        stage_b(kat(0), sB0);
+#pragma unroll
+        for (int s = 0; s < kKC; s++) a0[s] = Arow[(size_t)(kat(0) + s) * 64];
+        for (int i = 0; i < nch; i += 2) {
+            SR_GEMM_PHASE(a0, a1, sB0, sB1, i)
+            SR_GEMM_PHASE(a1, a0, sB1, sB0, i + 1)
+        }
+        // the last phase's requests (re-reads of the last chunk) land before the
+        // workgroup ends: its LDS may go to another workgroup next
+        __builtin_amdgcn_s_waitcnt(0);
     }
 #undef SR_GEMM_PHASE
     if (DUAL) {  // both partials of every split, zero where it summed nothing
@@ -639,17 +669,18 @@ int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int
 
 int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     if (t.n <= 0) return 0;
-    if (t.nKs % kKC || t.ksPerSplit % kKC || t.nRt % 4 || (long)t.nsplit * t.ksPerSplit < t.nKs) {
-        set_error("transfer GEMM: K steps, split size and row tiles must be padded (8, 8, 4)");
+    if (t.nKs % kKC || t.nRt % 4 || t.nsplit < 1) {
+        set_error("transfer GEMM: K steps and row tiles must be padded (8, 4), at least one split");
         return -2;
     }
     if (t.rowFs && (t.kB || !t.partU)) {
         set_error("transfer GEMM: two accumulators need partU and exclude kB");
         return -2;
     }
-    dim3 grid((t.nRt + 3) / 4, t.nsplit);
-    if (t.rowFs) k_transfer_gemm<2><<<grid, 256, 0, st>>>(t);
-    else k_transfer_gemm<0><<<grid, 256, 0, st>>>(t);
+    const long grid = (long)((t.nRt + 3) / 4) * t.nsplit;
+    if (grid > 0x7fffffffL) { set_error("transfer GEMM: grid too large"); return -2; }
+    if (t.rowFs) k_transfer_gemm<2><<<(unsigned)grid, 256, 0, st>>>(t);
+    else k_transfer_gemm<0><<<(unsigned)grid, 256, 0, st>>>(t);
     SR_HIP(hipGetLastError());
     return 0;
 }

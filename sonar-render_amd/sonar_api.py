@@ -375,6 +375,15 @@ def set_transfer_mode(mode):
     _check(cpp_lib.sr_set_transfer_mode(int(mode)), "set_transfer_mode")
 
 
+_sig("sr_set_transfer_splits", [_i])
+_sig("sr_transfer_splits", [])
+
+
+def set_transfer_splits(n):
+    """Split-K of the transfer product (0: automatic); applies at the next preparation."""
+    _check(cpp_lib.sr_set_transfer_splits(int(n)), "set_transfer_splits")
+
+
 _sig("sr_read_stl", [ctypes.c_char_p, _fp, _l], _l)
 _sig("sr_write_stl", [ctypes.c_char_p, _fp, _i])
 _sig("sr_load_stl", [ctypes.c_char_p, _i, _f])

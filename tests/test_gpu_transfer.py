@@ -31,6 +31,31 @@ def test_mixed_trihedral_vs_oracle_gated(api):
     assert worst < TOL_MIXED
 
 
+def test_transfer_split_counts_agree(api):
+    """The product's split-K (each 128-row group's non-zero chunks dealt out evenly
+    over the splits, the splits' partials reduced after) gives the same pressures
+    for any split count: only the FP32 summation order changes. The 12 kHz
+    trihedral's facets each skip a third of K, so the splits' K ranges jump the
+    skipped chunks and the boundary of the higher facets falls inside a split."""
+    sc = scenes.config_trihedral(f=12e3, n_side=2, bounces=3)
+    outs = {}
+    for n in (0, 1, 3, 7, 64):
+        _banner(f"transfer form, 12 kHz trihedral, {n} splits (0: automatic)")
+        runner.load_scene(sc, transfer_mode=1)
+        api.set_transfer_splits(n)
+        runner.project_scene(sc)
+        outs[n] = runner.read_results(sc)
+        api.TearDownCuda()
+    base = np.concatenate([outs[0]["points"].ravel(), outs[0]["targets"][0].ravel()])
+    for n, o in outs.items():
+        got = np.concatenate([o["points"].ravel(), o["targets"][0].ravel()])
+        print("splits", n, "rel-L2 vs automatic", rel_l2(got, base))
+        assert rel_l2(got, base) < 1e-5
+    ref = oracle.run_scene(sc, plane_source=1)
+    got = np.concatenate([outs[7]["points"][1], outs[7]["targets"][0][1]])
+    assert rel_l2(got, np.concatenate([ref["points"], ref["targets"][0]])) < TOL_MIXED
+
+
 def test_mixed_many_small_facets(api):
     """Facets with fewer than 8 fragments force K steps that straddle facets."""
     tris = scenes.trihedral(1.0)
