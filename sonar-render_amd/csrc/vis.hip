@@ -291,23 +291,26 @@ __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane
 
 // Returns this lane's "occluded" flag. active: lane has a ray to trace.
 // skipA, skipB: the triangles this lane's end points lie on (-1: none); never tested.
-// ldsTris: the triangle records staged in LDS when there are at most kBruteTris
-// of them (null: read them from global memory).
+// FEW (at most kBruteTris occluders): test every triangle, no tree, from the
+// records staged in LDS (ldsTris).
+template <bool FEW>
 __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk, bool active, F3 o, F3 d,
                                    float tmin, float tmax, int *errFlag, int skipA, int skipB,
-                                   const float4 *ldsTris = nullptr) {
+                                   const float4 *ldsTris) {
     if (bvh.ntri <= 0) return false;
     bool hit = false;
     unsigned long long live = ballot64(active);
     if (live == 0) return false;
-    if (bvh.ntri <= kBruteTris) {
-        // a handful of occluders: test every triangle, no tree
-        const float4 *tr = ldsTris ? ldsTris : bvh.tris;
+    if (FEW) {
+        // a triangle no lane needs is passed over by the whole wave
+        const float4 *tr = ldsTris;
         for (int ti = 0; ti < bvh.ntri; ti++) {
             const float4 q3 = tr[4 * ti + 3];
             const int id = __float_as_int(q3.w);
+            const bool need = active && !hit && id != skipA && id != skipB;
+            if (ballot64(need) == 0) continue;
             TriRec t = tri_unpack(tr[4 * ti], tr[4 * ti + 1], tr[4 * ti + 2], q3);
-            if (active && !hit && id != skipA && id != skipB) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
+            if (need) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
             if (ballot64(active && !hit) == 0) break;
         }
         return hit;
@@ -378,9 +381,11 @@ __device__ inline float4 lane_float4(float4 v, int k) {
                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), k)));
 }
 
+// FEW: at most kBruteTris occluders, their records staged in LDS at ldsTris.
+template <bool FEW>
 __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &tk, const Bvh &bvh,
                                                                const VisConst &vc, int rb, int ec, int emPerWave,
-                                                               int *stk, const float4 *ldsTris = nullptr) {
+                                                               int *stk, const float4 *ldsTris) {
     const int lane = __lane_id();
     const int r = rb * 64 + lane;
     const bool rvalid = r < tk.rc.count;
@@ -437,7 +442,8 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
         }
         const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
         const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
-        bool occ = wave_anyhit(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc, ldsTris);
+        bool occ = wave_anyhit<FEW>(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc,
+                                    ldsTris);
         unsigned long long word = ballot64(pair && !occ);
         if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
     }
@@ -457,8 +463,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
     const int nEmChunks = (tk.em.count + emPerWave - 1) / emPerWave;
     const long gw = (long)blockIdx.x * WAVES + wave;
     if (gw >= (long)nrb * nEmChunks) return;
-    vis_wave(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave],
-             brute ? sTri : nullptr);
+    if (brute)
+        vis_wave<true>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], sTri);
+    else
+        vis_wave<false>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], nullptr);
 }
 
 // Persistent form over many tasks (scenes): each wave pulls (task, receiver
@@ -470,6 +478,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *_
                                                                const int4 *__restrict__ items, int nitems,
                                                                int emPerWave, int *counter) {
     __shared__ int stkAll[WAVES][kStack];
+    __shared__ float4 sTriW[WAVES][4 * kBruteTris];  // few occluders: this wave's scene's records
     const int wave = threadIdx.x >> 6;
     for (;;) {
         int it = 0;
@@ -477,7 +486,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *_
         it = __builtin_amdgcn_readfirstlane(it);
         if (it >= nitems) break;
         const int4 w = items[it];
-        vis_wave(tasks[w.x], bvhs[w.x], vcs[w.x], w.y, w.z, emPerWave, stkAll[wave]);
+        const Bvh &b = bvhs[w.x];
+        if (b.ntri > 0 && b.ntri <= kBruteTris) {
+            // staged by this wave alone: its LDS accesses run in order, so the
+            // records it reads next are the ones it wrote (after its own last reads)
+            if (__lane_id() < 4 * b.ntri) sTriW[wave][__lane_id()] = b.tris[__lane_id()];
+            vis_wave<true>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], sTriW[wave]);
+        } else {
+            vis_wave<false>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], nullptr);
+        }
     }
 }
 
