@@ -135,18 +135,22 @@ __device__ inline void near_stage(int2 *entries, int &n, bool pred, int2 *list, 
 // ---------------------------------------------------------------------------
 // S build: a wave = 64 consecutive rows of S (two 32-row tiles), one lane per
 // row, all 8 emitters of each K step. The step's emitter records and
-// visibility words are then the same for every lane: they are read through the
-// scalar cache (constant address space: the kernel never writes them) and the
-// vector units take them as scalar operands. Each lane stores its row's two
+// visibility words are then the same for every lane: the wave stages them in
+// LDS once and reads each with a broadcast. Each lane stores its row's two
 // fragments (emitters 0-3, 4-7) into the MFMA A layout of its tile. Near pairs
 // (modes 1, 2) are left zero and listed for the near-field kernels.
-typedef const __attribute__((address_space(4))) float const_f32;
-typedef const __attribute__((address_space(4))) uint64_t const_u64;
+
+struct EmRec {
+    float4 p, n;      // point (area), normal (facet id bits)
+    uint64_t w0, w1;  // visibility words of the wave's two receiver blocks
+};
+constexpr int kBuildEm = 16 * 8;  // emitters per wave: at most 16 K steps
 
 template <bool ATT, int NEAR>
 __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
                                                  uint4 *__restrict__ A, int ksPerWave) {
     __shared__ int2 stage[NEAR != 0 ? 4 : 1][2][NEAR != 0 ? kNearStage : 1];
+    __shared__ EmRec sEmAll[4][kBuildEm];
     int2 *stQ = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][0], *stM = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][1];
     int nQ = 0, nM = 0;
     // wave-uniform by construction; readfirstlane lets the compiler see it
@@ -183,8 +187,19 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
     const int b1 = min(b0 + 1, nrbM - 1);
     const bool upper = rvalid && (row >> 6) != b0;  // this lane's words are block b0 + 1's
     const int bit = row & 63;
-    const const_f32 *P = (const const_f32 *)g.p, *N = (const const_f32 *)g.nrm;  // float4 records
-    const const_u64 *M0 = (const const_u64 *)(mask + (size_t)b0 * g.n), *M1 = (const const_u64 *)(mask + (size_t)b1 * g.n);
+    // This wave's emitters (at most kBuildEm: its K steps' 8 each) staged in LDS
+    // with coalesced loads, one per lane; every lane then reads each emitter's
+    // record at one address (a broadcast). An emitter past the last is the last
+    // one again (never selected). The wave alone writes and then reads its slots,
+    // and its LDS accesses run in order.
+    EmRec *sEm = sEmAll[threadIdx.x >> 6];
+    for (int u = lane; u < (ks1 - ks0) * 8; u += 64) {
+        const int jj = min(ks0 * 8 + u, g.n - 1);
+        sEm[u].p = g.p[jj];
+        sEm[u].n = g.nrm[jj];
+        sEm[u].w0 = mask[(size_t)b0 * g.n + jj];
+        sEm[u].w1 = mask[(size_t)b1 * g.n + jj];
+    }
     for (int ks = ks0; ks < ks1; ks++) {
         if (ks >= skip.x && ks < skip.y) continue;  // uniform
         uint32_t w[8];
@@ -192,11 +207,10 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int j = ks * 8 + q;
-            const int jj = min(j, g.n - 1);
-            const float4 ep = make_float4(P[4 * jj], P[4 * jj + 1], P[4 * jj + 2], 0.f);
-            const float4 en = make_float4(N[4 * jj], N[4 * jj + 1], N[4 * jj + 2], 0.f);
-            const uint64_t w0 = M0[jj], w1 = M1[jj];
-            const uint64_t word = upper ? w1 : w0;
+            const int jl = j - ks0 * 8;  // the emitter's slot in this wave's staged records
+            const float4 ep = sEm[jl].p;
+            const float4 en = sEm[jl].n;
+            const uint64_t word = upper ? sEm[jl].w1 : sEm[jl].w0;
             // every lane evaluates the far-field form and selects (the excluded
             // cases' values, even non-finite ones at r = 0, are never selected)
             const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
@@ -615,7 +629,7 @@ __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st) {
     // runs of 16 K steps per wave (64 rows each), shorter when that leaves fewer than ~8k waves
     if (g.nRt % 4) { set_error("S build: row tiles must be padded to 4"); return -2; }
-    int ksPerWave = 16;
+    int ksPerWave = kBuildEm / 8;  // the kernel stages that many K steps' emitters per wave
     while (ksPerWave > 4 && (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave) < 8192) ksPerWave /= 2;
     const long waves = (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave);
     const long blocks = (waves + 3) / 4;
