@@ -372,20 +372,14 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
 }
 
 // One wave's share: receiver block rb, emitters [ec*emPerWave, (ec+1)*emPerWave).
-// The emitters' records are read 64 at a time, one per lane, and handed to
-// the whole wave by lane reads, so no iteration waits on memory for them.
-__device__ inline float4 lane_float4(float4 v, int k) {
-    return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), k)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), k)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), k)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), k)));
-}
-
+// The emitters' records are read 64 at a time, one per lane, into the wave's LDS
+// slots and read back by the whole wave at one address each.
 // FEW: at most kBruteTris occluders, their records staged in LDS at ldsTris.
+// sEm: this wave's LDS slots for 64 emitters' points and normals.
 template <bool FEW>
 __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &tk, const Bvh &bvh,
                                                                const VisConst &vc, int rb, int ec, int emPerWave,
-                                                               int *stk, const float4 *ldsTris) {
+                                                               int *stk, const float4 *ldsTris, float4 *sEm) {
     const int lane = __lane_id();
     const int r = rb * 64 + lane;
     const bool rvalid = r < tk.rc.count;
@@ -395,23 +389,28 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
     int e0 = ec * emPerWave;
     const int e1 = min(tk.em.count, e0 + emPerWave);
     if (tk.upperOnly) e0 = max(e0, rb * 64);
-    float4 epRun = make_float4(0.f, 0.f, 0.f, 0.f), enRun = epRun;
     for (int e = e0; e < e1; e++) {
         const int k = (e - e0) & 63;
-        if (k == 0) {  // the next 64 emitters, one per lane
+        if (k == 0) {  // the next 64 emitters, one per lane, into the wave's LDS slots
             const int el = min(e + lane, e1 - 1);
-            epRun = tk.em.p[el];
-            enRun = tk.em.n ? tk.em.n[el] : make_float4(0.f, 0.f, 0.f, 0.f);
+            // the wave's reads of the previous 64 precede these writes (its LDS
+            // accesses run in order), as these precede the reads below
+            sEm[lane] = tk.em.p[el];
+            sEm[64 + lane] = tk.em.n ? tk.em.n[el] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        const float4 ep = lane_float4(epRun, k);
-        const float4 en = lane_float4(enRun, k);
+        const float4 ep = sEm[k];  // one address for the whole wave: a broadcast
+        const float4 en = sEm[64 + k];
         bool pair = rvalid;
         // an emitter on every lane's own facet pairs with none of them: no ray set-up
         if (tk.origin == ORIGIN_LOWER_FACET && ballot64(rvalid && __float_as_int(en.w) != rfacet) == 0ull) {
             if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = 0ull;
             continue;
         }
-        // Orientation: which end the ray leaves from and which normal culls where.
+        // Orientation: which end the ray leaves from. The unit direction and the
+        // length are formed once, emitter -> receiver; the ray leaving the
+        // receiver has exactly the negated direction (its difference vector is
+        // the exact negation, so the same length and reciprocal), and the two
+        // cosine culls are the same values whichever end it leaves from.
         bool fromEm = true;
         if (tk.origin == ORIGIN_RECEIVER) fromEm = false;
         if (tk.origin == ORIGIN_LOWER_FACET) {
@@ -419,27 +418,12 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
             if (ef == rfacet) pair = false;
             fromEm = ef < rfacet;
         }
-        F3 o, t, no, nt;
-        bool useNo, useNt;
-        if (fromEm) {
-            o = F3{ep.x, ep.y, ep.z}; t = F3{rp.x, rp.y, rp.z};
-            no = F3{en.x, en.y, en.z}; nt = F3{rn.x, rn.y, rn.z};
-            useNo = tk.useNormalEm; useNt = tk.useNormalRc;
-        } else {
-            o = F3{rp.x, rp.y, rp.z}; t = F3{ep.x, ep.y, ep.z};
-            no = F3{rn.x, rn.y, rn.z}; nt = F3{en.x, en.y, en.z};
-            useNo = tk.useNormalRc; useNt = tk.useNormalEm;
-        }
         F3 uv; float rr;
-        ray_setup(o, t, uv, rr);
-        if (useNo) {
-            float c = f3dot(uv, no);
-            if (!(c >= 1e-6f)) pair = false;
-        }
-        if (useNt) {
-            float c = f3dot(F3{-uv.x, -uv.y, -uv.z}, nt);
-            if (!(c >= 1e-6f)) pair = false;
-        }
+        ray_setup(F3{ep.x, ep.y, ep.z}, F3{rp.x, rp.y, rp.z}, uv, rr);
+        if (tk.useNormalEm && !(f3dot(uv, F3{en.x, en.y, en.z}) >= 1e-6f)) pair = false;
+        if (tk.useNormalRc && !(f3dot(F3{-uv.x, -uv.y, -uv.z}, F3{rn.x, rn.y, rn.z}) >= 1e-6f)) pair = false;
+        const F3 o = fromEm ? F3{ep.x, ep.y, ep.z} : F3{rp.x, rp.y, rp.z};
+        if (!fromEm) uv = F3{-uv.x, -uv.y, -uv.z};
         const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
         const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
         bool occ = wave_anyhit<FEW>(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc,
@@ -453,6 +437,7 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, VisConst vc, int emPerWave) {
     __shared__ int stkAll[WAVES][kStack];
+    __shared__ float4 sEmAll[WAVES][128];
     __shared__ float4 sTri[4 * kBruteTris];  // few occluders: their records, staged once per block
     const bool brute = bvh.ntri > 0 && bvh.ntri <= kBruteTris;
     if (brute)
@@ -464,9 +449,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
     const long gw = (long)blockIdx.x * WAVES + wave;
     if (gw >= (long)nrb * nEmChunks) return;
     if (brute)
-        vis_wave<true>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], sTri);
+        vis_wave<true>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], sTri,
+                       sEmAll[wave]);
     else
-        vis_wave<false>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], nullptr);
+        vis_wave<false>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], nullptr,
+                        sEmAll[wave]);
 }
 
 // Persistent form over many tasks (scenes): each wave pulls (task, receiver
@@ -478,6 +465,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *_
                                                                const int4 *__restrict__ items, int nitems,
                                                                int emPerWave, int *counter) {
     __shared__ int stkAll[WAVES][kStack];
+    __shared__ float4 sEmAll[WAVES][128];
     __shared__ float4 sTriW[WAVES][4 * kBruteTris];  // few occluders: this wave's scene's records
     const int wave = threadIdx.x >> 6;
     for (;;) {
@@ -491,9 +479,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *_
             // staged by this wave alone: its LDS accesses run in order, so the
             // records it reads next are the ones it wrote (after its own last reads)
             if (__lane_id() < 4 * b.ntri) sTriW[wave][__lane_id()] = b.tris[__lane_id()];
-            vis_wave<true>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], sTriW[wave]);
+            vis_wave<true>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], sTriW[wave], sEmAll[wave]);
         } else {
-            vis_wave<false>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], nullptr);
+            vis_wave<false>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], nullptr, sEmAll[wave]);
         }
     }
 }
