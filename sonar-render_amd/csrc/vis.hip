@@ -419,7 +419,7 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
             fromEm = ef < rfacet;
         }
         F3 uv; float rr;
-        ray_setup(F3{ep.x, ep.y, ep.z}, F3{rp.x, rp.y, rp.z}, uv, rr);
+        ray_setup_dev(F3{ep.x, ep.y, ep.z}, F3{rp.x, rp.y, rp.z}, uv, rr);
         if (tk.useNormalEm && !(f3dot(uv, F3{en.x, en.y, en.z}) >= 1e-6f)) pair = false;
         if (tk.useNormalRc && !(f3dot(F3{-uv.x, -uv.y, -uv.z}, F3{rn.x, rn.y, rn.z}) >= 1e-6f)) pair = false;
         const F3 o = fromEm ? F3{ep.x, ep.y, ep.z} : F3{rp.x, rp.y, rp.z};

@@ -32,6 +32,46 @@ __host__ __device__ inline void ray_setup(F3 o, F3 t, F3 &uv, float &r) {
     uv = F3{v.x * inv, v.y * inv, v.z * inv};
 }
 
+// The same ray on the GPU with IEEE sqrtf and 1/r evaluated without the
+// special-case paths that cannot apply to these lengths: for r^2 >= 2^-96 the
+// square root needs no denormal pre-scaling, and for 2^-60 <= r <= 2^60 the
+// division 1/r needs no range scaling (v_div_scale and v_div_fmas then act as
+// the identity and a plain FMA) nor a fix-up (finite, normal operands). What
+// remains is the correctly rounded evaluation itself: the hardware square root
+// corrected by its two one-ulp residual tests, and the reciprocal refined by
+// Newton-Raphson steps with FMA residuals. Lanes outside those ranges (r^2
+// below 2^-96: coincident points) take the full IEEE operations.
+__device__ inline float sqrt_ieee_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __int_as_float(__float_as_int(s) - 1), sp = __int_as_float(__float_as_int(s) + 1);
+    const float rm = fmaf(-sm, s, x), rp = fmaf(-sp, s, x);
+    const float t = rm <= 0.0f ? sm : s;
+    return rp > 0.0f ? sp : t;
+}
+
+__device__ inline float rcp_ieee_normal(float r) {
+    float y = __builtin_amdgcn_rcpf(r);
+    y = fmaf(fmaf(-r, y, 1.0f), y, y);
+    float q = y;
+    q = fmaf(fmaf(-r, q, 1.0f), y, q);
+    return fmaf(fmaf(-r, q, 1.0f), y, q);
+}
+
+__device__ inline void ray_setup_dev(F3 o, F3 t, F3 &uv, float &r) {
+    F3 v = f3sub(t, o);
+    const float r2 = f3dot(v, v);
+    const bool normal = r2 >= 0x1p-96f && r2 <= 0x1p100f;
+    float rr = sqrt_ieee_normal(r2), inv = rcp_ieee_normal(rr);
+    if (__builtin_expect(__ballot(!normal) != 0ull, 0)) {
+        if (!normal) {
+            rr = sqrtf(r2);
+            inv = 1.0f / rr;
+        }
+    }
+    r = rr;
+    uv = F3{v.x * inv, v.y * inv, v.z * inv};
+}
+
 struct TriRec {
     F3 v0, e1, e2, bmin, bmax;
 };
