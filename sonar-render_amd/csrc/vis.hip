@@ -310,7 +310,7 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
             const bool need = active && !hit && id != skipA && id != skipB;
             if (ballot64(need) == 0) continue;
             TriRec t = tri_unpack(tr[4 * ti], tr[4 * ti + 1], tr[4 * ti + 2], q3);
-            if (need) hit = mt_hit(t, vc.padTri, o, d, tmin, tmax);
+            if (need) hit = mt_hit_dev(t, vc.padTri, o, d, tmin, tmax);
             if (ballot64(active && !hit) == 0) break;
         }
         return hit;
@@ -334,7 +334,7 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
             const float4 q3 = bvh.tris[4 * ti + 3];
             const int id = __float_as_int(q3.w);
             TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], q3);
-            if (hl && id != skipA && id != skipB && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
+            if (hl && id != skipA && id != skipB && mt_hit_dev(t, vc.padTri, o, d, tmin, tmax)) hit = true;
             bl = 0;
         }
         if (cr < 0 && br) {
@@ -342,7 +342,7 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
             const float4 q3 = bvh.tris[4 * ti + 3];
             const int id = __float_as_int(q3.w);
             TriRec t = tri_unpack(bvh.tris[4 * ti], bvh.tris[4 * ti + 1], bvh.tris[4 * ti + 2], q3);
-            if (hr && !hit && id != skipA && id != skipB && mt_hit(t, vc.padTri, o, d, tmin, tmax)) hit = true;
+            if (hr && !hit && id != skipA && id != skipB && mt_hit_dev(t, vc.padTri, o, d, tmin, tmax)) hit = true;
             br = 0;
         }
         unsigned long long remaining = ballot64(active && !hit);
