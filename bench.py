@@ -50,13 +50,13 @@ TOL = 1e-3
 
 # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (calibrated
 # FETCH_SIZE and WRITE_SIZE, see profiles/transfer_gemm_hbm_bytes.md), per workload.
-PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 824.9e6,
-               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 108.8e6}
+PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 758.3e6,
+               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 124.5e6}
 # Vector wave-instructions (SQ_INSTS_VALU, exact) of one scene's visibility masks
 # (SELF + sources -> fragments + fragments -> field points), per workload, from
 # profiles/vis_mask_counters.md. The masks are bound by vector issue: a wave64
 # instruction holds a 16-lane SIMD for 4 cycles, 1,024 SIMDs at 2.4 GHz.
-VIS_VALU_INSTR = {"trihedral_1m_20kHz": 290_967_406, "trihedral_1m_20kHz_lambda4": 30_859_537}
+VIS_VALU_INSTR = {"trihedral_1m_20kHz": 235_351_531, "trihedral_1m_20kHz_lambda4": 25_375_463}
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9  # G wave-instructions/s
 
 
@@ -135,7 +135,7 @@ def time_form(scene, mode, steps, warmup, near=None):
     return dt, stats, res
 
 
-def roofline(stats, mode, workload):
+def roofline(stats, mode, workload, stream_gbs=None):
     if mode == 0:
         st = stats["f2f"]
         ach = st["flops"] / (st["ms"] * 1e-3) / 1e12 if st["ms"] > 0 else 0.0
@@ -161,7 +161,12 @@ def roofline(stats, mode, workload):
                                  "(4 B per cell and incidence) + FP32 complex P_out (8 B per cell and incidence)",
             "kernel": "k_transfer_gemm (FP16 MFMA 32x32x16, streams the FP16 transfer matrix once per bounce)",
             "mfma_tflops_executed": round(tf, 2), "mfma_frac": round(tf / FP16_PEAK_TFLOPS, 4),
-            "avg_launch_ms": round(ms, 4), "launches": st["launches"]}
+            "avg_launch_ms": round(ms, 4), "launches": st["launches"],
+            "attainable": None if not stream_gbs else {
+                "stream_read_GBs": round(stream_gbs, 1),
+                "traffic_GBs": round(traffic / (ms * 1e-3) / 1e9, 1) if traffic and ms > 0 else None,
+                "note": "a plain streaming read of 1 GiB on this GPU (sr_measure_stream_read); traffic_GBs = the "
+                        "PMC-measured bytes per launch over the launch time"}}
 
 
 def vis_roofline(stats, workload, steps):
@@ -401,6 +406,11 @@ def main():
                 b = np.concatenate([j["points"] for j in js] + [j["targets"][0] for j in js])
                 r["err_own_judge"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
+    # the attainable read bandwidth of this GPU (a plain streaming read), beside the HBM roofline
+    import sonar_api as api
+    stream_gbs = api.measure_stream_read(1024, 20)
+    print(f"[bench] plain streaming read: {stream_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+
     per_form = {}
     for name, r in results.items():
         ms = r["dt"] / args.steps * 1e3
@@ -408,7 +418,7 @@ def main():
             "gpairs_per_s": round(r["pairs"] * args.steps / r["dt"] / 1e9, 3), "ms_per_scene": round(ms, 4),
             "pairs_per_scene": r["pairs"], "rel_l2_vs_fp64": r.get("err"),
             "kernel_ms_per_scene": {k: round(v["ms"] / args.steps, 4) for k, v in r["stats"].items() if v["launches"]},
-            "roofline": roofline(r["stats"], r["mode"], r["scene"]["name"]),
+            "roofline": roofline(r["stats"], r["mode"], r["scene"]["name"], stream_gbs),
             "visibility": vis_roofline(r["stats"], r["scene"]["name"], args.steps),
         }
         if "err_surface" in r:

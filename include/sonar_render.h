@@ -229,6 +229,11 @@ int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double 
  * times back to back and returns the mean time per launch, free of the
  * per-launch event overhead of sr_kernel_stats. Overwrites only scratch. */
 int sr_time_transfer_gemm(int reps, double *msPerLaunch);
+/* Measurement hook with no counterpart in the reference: the attainable HBM
+ * read bandwidth of this GPU, a plain streaming read of `bytes` (>= 1 MiB,
+ * its own buffer, allocated and freed here) timed over `reps` passes after two
+ * warm-up passes; gbs = bytes / mean pass time in GB/s. */
+int sr_measure_stream_read(long bytes, int reps, double *gbs);
 
 /* STL meshes (binary or ASCII in, binary out): 9 floats per triangle. */
 long sr_read_stl(const char *path, float *out9, long cap);

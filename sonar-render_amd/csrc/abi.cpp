@@ -17,6 +17,7 @@
 #include "batch.h"
 #include "render.h"
 #include "noalloc.h"
+#include "measure.h"
 
 namespace {
 
@@ -330,6 +331,9 @@ int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double 
 
 int sr_time_transfer_gemm(int reps, double *msPerLaunch) {
     return guarded("time_transfer_gemm", [&] { return sr::model_time_transfer_gemm(cur(), reps, msPerLaunch); });
+}
+int sr_measure_stream_read(long bytes, int reps, double *gbs) {
+    return guarded("measure_stream_read", [&] { return sr::measure_stream_read(bytes, reps, gbs); });
 }
 
 long sr_read_stl(const char *path, float *out, long cap) {

@@ -1,0 +1,10 @@
+// Measurement of the GPU itself (no counterpart in the reference).
+#pragma once
+
+namespace sr {
+
+// Attainable HBM read bandwidth: a plain streaming read of `bytes` (>= 1 MiB) timed
+// over `reps` passes after two warm-up passes; gbs = bytes / mean pass time in GB/s.
+int measure_stream_read(long bytes, int reps, double *gbs);
+
+}  // namespace sr

@@ -1,0 +1,64 @@
+// Attainable HBM read bandwidth on this GPU: a plain streaming read (16-byte
+// loads, grid-stride over the buffer, one XOR per load so nothing is dropped),
+// the ceiling against which the memory-bound kernels' rates are read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "launch.h"
+#include "measure.h"
+
+namespace sr {
+
+__global__ __launch_bounds__(256) void k_stream_read(const uint4 *__restrict__ src, long n, unsigned *__restrict__ out) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    unsigned acc = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint4 v = src[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    out[(long)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
+// Reads `bytes` (rounded down to 16) `reps` times after two warm-up passes on its
+// own stream and buffers (allocated and freed here); gbs = bytes / mean time.
+int measure_stream_read(long bytes, int reps, double *gbs) {
+    if (bytes < (1L << 20) || reps < 1 || !gbs) { set_error("stream read: at least 1 MiB and one repetition"); return -2; }
+    const long n = bytes / 16;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
+    const int grid = 4 * cus;  // four 256-thread workgroups per CU: 16 waves each
+    uint4 *buf = nullptr;
+    unsigned *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto run = [&]() -> int {
+        SR_HIP(hipMalloc((void **)&buf, (size_t)n * 16));
+        SR_HIP(hipMalloc((void **)&out, (size_t)grid * 256 * sizeof(unsigned)));
+        SR_HIP(hipStreamCreate(&st));
+        SR_HIP(hipEventCreate(&e0));
+        SR_HIP(hipEventCreate(&e1));
+        SR_HIP(hipMemsetAsync(buf, 1, (size_t)n * 16, st));
+        for (int r = 0; r < 2; r++) k_stream_read<<<grid, 256, 0, st>>>(buf, n, out);
+        SR_HIP(hipGetLastError());
+        SR_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) k_stream_read<<<grid, 256, 0, st>>>(buf, n, out);
+        SR_HIP(hipGetLastError());
+        SR_HIP(hipEventRecord(e1, st));
+        SR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *gbs = (double)n * 16.0 / ((double)ms / reps * 1e-3) / 1e9;
+        return 0;
+    };
+    rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (out) (void)hipFree(out);
+    if (buf) (void)hipFree(buf);
+    return rc;
+}
+
+}  // namespace sr

@@ -356,6 +356,16 @@ def time_transfer_gemm(reps=20):
     return ms.value
 
 
+_sig("sr_measure_stream_read", [_l, _i, ctypes.POINTER(ctypes.c_double)])
+
+
+def measure_stream_read(mib=1024, reps=20):
+    """Attainable HBM read bandwidth (GB/s): a plain streaming read of `mib` MiB."""
+    gbs = ctypes.c_double()
+    _check(cpp_lib.sr_measure_stream_read(int(mib) << 20, int(reps), ctypes.byref(gbs)), "measure_stream_read")
+    return gbs.value
+
+
 _sig("sr_set_transfer_mode", [_i])
 _sig("sr_transfer_mode", [])
 KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9, "near": 10})
