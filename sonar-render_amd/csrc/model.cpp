@@ -1026,6 +1026,7 @@ static int ensure_transfer(Model &m, int i) {
                     (double)m.kf / (2.0 * M_PI), m.nearMode,
                     (float)(m.nearRadiusWl * 2.0 * M_PI / (double)m.kf), o.frameTab, nullptr,
                     nullptr, nullptr, nullptr, 0, m.nearQuadDeltas * o.delta};
+    ph.streamS = (double)nRt * 32 * nKs * 16 >= kStreamSBytes;
     if (m.nearMode == 2) {
         NearField &nf = nearfield();
         if (!nf.trained) { set_error("learned near-field mode needs a trained network (sr_nf_train)"); return -3; }
@@ -1124,6 +1125,7 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
         }
         TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, t.wgSkip, m.scratchL,
                         t.gateInGemm ? t.kB : nullptr, m.scratchU, t.gateInGemm ? nullptr : t.rowFs};
+        gm.streamS = (double)t.nRt * 32 * t.nKs * 16 >= kStreamSBytes;
         m.lastGemm = gm;
         m.haveLastGemm = true;
         {

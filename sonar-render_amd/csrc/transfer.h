@@ -33,6 +33,7 @@ struct TransferPhys {
     int *nearCount;        // [2] device counters: zero before a build; the near-pair kernel clears them
     int nearCap;           // capacity of each list
     float quadRadius;      // learned mode: closer pairs use the quadrature (metres, >= delta)
+    int streamS = 0;       // write S non-temporally (see kStreamSBytes)
 };
 
 // Split s of nsplit over the K steps of one 128-row group whose zero chunks are
@@ -65,7 +66,15 @@ struct TransferGemm {
     // first emitter of its own facet; the product sends the terms below it to
     // partL and the rest to partU, both for every split (null: one accumulator).
     const int *rowFs = nullptr;
+    // S larger than the GPU's memory-side cache: stream it (non-temporal loads)
+    // rather than let each bounce's pass evict the cached B and partial sums
+    // for data no later access in the pass reuses.
+    int streamS = 0;
 };
+
+// S of at least this many bytes is written and read non-temporally; a smaller
+// one stays in the memory-side cache (256 MiB on MI355X) between the passes.
+constexpr double kStreamSBytes = 160.0 * 1024 * 1024;
 
 // Reduction of a gated product: per (plane, active row), the lower-facet
 // partials always and the higher-facet ones where the row's own input is not

@@ -132,6 +132,17 @@ __device__ inline void near_stage(int2 *entries, int &n, bool pred, int2 *list, 
     n += cnt;
 }
 
+// Non-temporal 16-byte accesses (streamed past the caches), for an S larger
+// than every cache level: written once per build and read once per bounce.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ inline void store_nt(uint4 *p, uint4 v) {
+    __builtin_nontemporal_store(u32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4 *>(p));
+}
+__device__ inline uint4 load_nt(const uint4 *p) {
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // ---------------------------------------------------------------------------
 // S build: a wave = 64 consecutive rows of S (two 32-row tiles), one lane per
 // row, all 8 emitters of each K step. The step's emitter records and
@@ -261,8 +272,13 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
         }
         // fragment h of row tl of the tile: emitters 4h .. 4h + 3
         uint4 *dst = A + ((size_t)tile * g.nKs + ks) * 64 + tl;
-        dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-        dst[32] = make_uint4(w[4], w[5], w[6], w[7]);
+        if (ph.streamS) {
+            store_nt(dst, make_uint4(w[0], w[1], w[2], w[3]));
+            store_nt(dst + 32, make_uint4(w[4], w[5], w[6], w[7]));
+        } else {
+            dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+            dst[32] = make_uint4(w[4], w[5], w[6], w[7]);
+        }
     }
     if (NEAR != 0) {
         near_flush(stQ, nQ, ph.nearQ, ph.nearCount, ph.nearCap);
@@ -377,8 +393,8 @@ __device__ inline void workgroup_barrier() {
     asm volatile("" ::: "memory");
 }
 
-// ROLE 0: one accumulator (the kB gate, or none); 2: DUAL.
-template <int ROLE>
+// ROLE 0: one accumulator (the kB gate, or none); 2: DUAL. NT: S read non-temporally.
+template <int ROLE, bool NT>
 __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     constexpr bool DUAL = ROLE == 2;
     // two separately named buffers, so that the compiler can tell the LDS reads of
@@ -470,6 +486,7 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
         }
     };
 
+    auto load_a = [](const uint4 *p) { return NT ? load_nt(p) : *p; };
     // K step of the split's i-th chunk; past the end the last chunk again, so that
     // the loop's one-ahead requests are unconditional (and those re-reads hit L2)
     auto kat = [&](int i) { return sw.k(sw.o0 + min(i, nch - 1)); };
@@ -495,7 +512,7 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
 #define SR_GEMM_PHASE(ACUR, ANEXT, BUF, NBUF, I)                                                     \
     {                                                                                                \
         const int kc = kat(I), kn = kat((I) + 1);                                                    \
-        _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = Arow[(size_t)(kn + s) * 64];      \
+        _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = load_a(Arow + (size_t)(kn + s) * 64);  \
         __builtin_amdgcn_s_waitcnt(kWaitAllButNextA); /* all but those kKC loads: A, B of kc */      \
         workgroup_barrier(); /* BUF complete; every wave is done reading NBUF */                     \
         stage_b(kn, NBUF);                                                                           \
@@ -515,7 +532,7 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     if (nch > 0) {
         stage_b(kat(0), sB0);
 #pragma unroll
-        for (int s = 0; s < kKC; s++) a0[s] = Arow[(size_t)(kat(0) + s) * 64];
+        for (int s = 0; s < kKC; s++) a0[s] = load_a(Arow + (size_t)(kat(0) + s) * 64);
         for (int i = 0; i < nch; i += 2) {
             SR_GEMM_PHASE(a0, a1, sB0, sB1, i)
             SR_GEMM_PHASE(a1, a0, sB1, sB0, i + 1)
@@ -693,8 +710,13 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     }
     const long grid = (long)((t.nRt + 3) / 4) * t.nsplit;
     if (grid > 0x7fffffffL) { set_error("transfer GEMM: grid too large"); return -2; }
-    if (t.rowFs) k_transfer_gemm<2><<<(unsigned)grid, 256, 0, st>>>(t);
-    else k_transfer_gemm<0><<<(unsigned)grid, 256, 0, st>>>(t);
+    if (t.rowFs) {
+        if (t.streamS) k_transfer_gemm<2, true><<<(unsigned)grid, 256, 0, st>>>(t);
+        else k_transfer_gemm<2, false><<<(unsigned)grid, 256, 0, st>>>(t);
+    } else {
+        if (t.streamS) k_transfer_gemm<0, true><<<(unsigned)grid, 256, 0, st>>>(t);
+        else k_transfer_gemm<0, false><<<(unsigned)grid, 256, 0, st>>>(t);
+    }
     SR_HIP(hipGetLastError());
     return 0;
 }
