@@ -176,6 +176,14 @@ int sr_transfer_mode(void);
  * the group's chunk count. */
 int sr_set_transfer_splits(int n);
 int sr_transfer_splits(void);
+/* How the transfer matrix S is built (no counterpart in the reference, whose
+ * pair kernel has no stored matrix). 1 (default): where the rows and emitters
+ * are padded alike and no near-field pairs are listed, only the 64 x 64 blocks
+ * on and above the diagonal are evaluated and each is also stored transposed
+ * (S is symmetric bit for bit); elsewhere, and with 0, every block is
+ * evaluated. Both give the same matrix; a changed value rebuilds it. */
+int sr_set_transfer_symmetric(int on);
+int sr_transfer_symmetric(void);
 
 /* Plain fragment->fragment sums (transfer mode 0). 0 (default): with 16 or
  * more incidence planes the pair kernel is evaluated in FP32 on the vector
@@ -220,7 +228,8 @@ int sr_nf_gradient(const float *params, const float *X, const float *Y, int n, f
  * transfer matrix (quadrature / learned network; pairs = near pairs). Reports summed kernel time,
  * launches, ordered pairs (x planes), flops (vector gathers: 37 FP32
  * flops per pair plus 8 per pair and plane; matrix-core fragment->fragment:
- * the 8 per pair and plane that run on the matrix cores; transfer GEMM: 2 M N K as executed) and
+ * the 8 per pair and plane that run on the matrix cores; transfer GEMM: 2 M N K as executed;
+ * transfer-matrix build: the pair evaluations executed, padding included) and
  * algorithmic HBM bytes (transfer GEMM: S, B and the partial sums). */
 int sr_profile(int on);
 int sr_kernel_stats(int kind, double *ms, long *launches, double *pairs, double *flops, double *bytes);

@@ -292,6 +292,10 @@ int sr_set_transfer_splits(int n) {
     return guarded("set_transfer_splits", [&] { return sr::model_set_transfer_splits(cur(), n); });
 }
 int sr_transfer_splits(void) { return cur().transferSplits; }
+int sr_set_transfer_symmetric(int on) {
+    return guarded("set_transfer_symmetric", [&] { return sr::model_set_transfer_symmetric(cur(), on); });
+}
+int sr_transfer_symmetric(void) { return cur().transferSymmetric; }
 int sr_set_gather_mode(int mode) {
     return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(cur(), mode); });
 }

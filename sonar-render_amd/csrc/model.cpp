@@ -145,7 +145,7 @@ void free_transfer(Object &o) {
     auto &t = o.tc;
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
     dev_free(t.state); dev_free(t.rowFs);
-    dev_free(t.rowAct); dev_free(t.actRow); dev_free(t.kB); dev_free(t.cls);
+    dev_free(t.rowAct); dev_free(t.actRow); dev_free(t.kB); dev_free(t.cls); dev_free(t.symItems);
     dev_free(t.nearQ); dev_free(t.nearM); dev_free(t.nearCount);
     t = Object::Transfer{};
 }
@@ -859,11 +859,12 @@ static int ensure_transfer(Model &m, int i) {
     uint64_t *mk;
     if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
     const int n = o.nact;
-    const int nKs = ((n + 7) / 8 + 7) / 8 * 8;  // K steps of 8 emitters, whole chunks of 8 steps
     // Facet ranges of the active rows (sorted by facet). When every facet holds at
     // least 128 rows and padding each to whole 128-row groups adds at most a
     // quarter, the rows of S are padded so: every group is then single-facet and
-    // the zero-input gate is applied inside the product.
+    // the zero-input gate is applied inside the product. The emitters (K) are then
+    // padded the same way (K position k is row k), which makes every facet's K range
+    // whole chunks and the transpose of an aligned 64 x 64 block of S another one.
     const auto &fa = o.actFacetHost;
     std::vector<int2> fr;  // [fs, fe) per facet present
     for (int r = 0; r < n; r++)
@@ -877,6 +878,8 @@ static int ensure_transfer(Model &m, int i) {
     const bool gateIn = big && padded <= n + n / 4;
     const int nPad = gateIn ? (int)padded : n;
     const int nRt = ((nPad + 127) / 128) * 4;
+    // K steps of 8 emitters, whole chunks of 8 steps
+    const int nKs = gateIn ? nPad / 8 : ((n + 7) / 8 + 7) / 8 * 8;
     if (!t.A || t.nKs != nKs || t.nRt != nRt || t.gateInGemm != gateIn || t.splitsReq != m.transferSplits) {
         free_transfer(o);
         const int nWg = nRt / 4;
@@ -910,24 +913,30 @@ static int ensure_transfer(Model &m, int i) {
             for (size_t k = q; k < e; k++) run[k] = make_int2(fr[q].x, fr[e - 1].y);
             q = e;
         }
-        // Per 128-row group: its facet range (single-facet groups only) and that
-        // facet's run, the K chunks wholly inside the run (zero: never read) and,
-        // with the gate inside the product, the first K step of the higher facets.
+        // Per 128-row group: its facet's range of K positions (single-facet groups
+        // only) and that facet's run's, the K chunks wholly inside the run (zero:
+        // never read) and, with the gate inside the product, the first K step of
+        // the higher facets. With padded rows these ranges are in the padded
+        // layout (whole 128-position groups), otherwise they are active cells.
         std::vector<int> rowAct(gateIn ? (size_t)nRt * 32 : 0, -1), actRow(gateIn ? n : 0);
         std::vector<int2> grp(nWg, make_int2(-1, -1)), grpRun(nWg, make_int2(-1, -1));
         if (gateIn) {
-            int base = 0;
+            std::vector<int> pb(fr.size() + 1, 0);  // padded start of each facet
+            for (size_t q = 0; q < fr.size(); q++) pb[q + 1] = pb[q] + ((fr[q].y - fr[q].x) + 127) / 128 * 128;
             for (size_t q = 0; q < fr.size(); q++) {
                 const int2 f = fr[q];
                 for (int r = f.x; r < f.y; r++) {
-                    rowAct[base + (r - f.x)] = r;
-                    actRow[r] = base + (r - f.x);
+                    rowAct[pb[q] + (r - f.x)] = r;
+                    actRow[r] = pb[q] + (r - f.x);
                 }
-                for (int w = base / 128; w < (base + (f.y - f.x) + 127) / 128; w++) {
-                    grp[w] = f;
-                    grpRun[w] = run[q];
+                // the run's first and last facets (runs are consecutive facets)
+                size_t q0 = q, q1 = q;
+                while (q0 > 0 && run[q0 - 1].x == run[q].x) q0--;
+                while (q1 + 1 < fr.size() && run[q1 + 1].x == run[q].x) q1++;
+                for (int w = pb[q] / 128; w < pb[q + 1] / 128; w++) {
+                    grp[w] = make_int2(pb[q], pb[q + 1]);
+                    grpRun[w] = make_int2(pb[q0], pb[q1 + 1]);
                 }
-                base += ((f.y - f.x) + 127) / 128 * 128;
             }
         } else {
             for (int w = 0; w < nWg; w++) {
@@ -948,7 +957,8 @@ static int ensure_transfer(Model &m, int i) {
                 const int cLo = ((rs + 7) / 8 + 7) / 8, cHi = (re / 8) / 8;  // chunks wholly inside [rs, re)
                 if (cHi > cLo) skip[w] = make_int2(cLo * 8, cHi * 8);
             }
-            // the chunk holding the group's facet's tail (zero) and the higher facets' start
+            // the higher facets' first K step (used with padded rows only, where it
+            // starts a chunk)
             if (grp[w].x >= 0) kB[w] = ((grp[w].y / 8) / 8) * 8;
             steps += 4.0 * (nKs - (skip[w].y - skip[w].x));
         }
@@ -1005,6 +1015,24 @@ static int ensure_transfer(Model &m, int i) {
             SR_HIP(hipMemcpy(t.actRow, actRow.data(), n * sizeof(int), hipMemcpyHostToDevice));
             SR_HIP(hipMemcpy(t.kB, kB.data(), nWg * sizeof(int), hipMemcpyHostToDevice));
             SR_HIP(hipMemcpy(t.cls, cls.data(), cls.size(), hipMemcpyHostToDevice));
+            // Symmetric build items: block (rb, cb), rb <= cb, whose K chunk the row
+            // group keeps; its transpose (cb, rb) must then be kept too (and a
+            // skipped block's transpose skipped), which holds when the runs are
+            // whole padded facets.
+            auto kept = [&](int rb, int cb) { return cb * 8 < skip[rb >> 1].x || cb * 8 >= skip[rb >> 1].y; };
+            const int nb = nKs / 8;  // 64-row blocks = K chunks
+            std::vector<int2> items;
+            bool symOk = nb * 64 == nPad;
+            for (int rb = 0; rb < nb && symOk; rb++)
+                for (int cb = rb; cb < nb; cb++) {
+                    if (kept(rb, cb) != kept(cb, rb)) { symOk = false; break; }
+                    if (kept(rb, cb)) items.push_back(make_int2(rb, cb));
+                }
+            if (symOk && !items.empty()) {
+                if (dev_alloc(&t.symItems, items.size())) return -1;
+                SR_HIP(hipMemcpy(t.symItems, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+                t.nSym = (int)items.size();
+            }
         } else {
             // two accumulators: every split writes both partials of every row
             t.partRows = 2.0 * t.nsplit * n;
@@ -1049,8 +1077,13 @@ static int ensure_transfer(Model &m, int i) {
             t.nearClear = false;  // the build counts into it
         }
         {
-            ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, 0.0);
-            if (transfer_build_s(g, ph, mk, t.A, m.stream)) return -1;
+            const bool sym = m.nearMode == 0 && m.transferSymmetric && t.nSym > 0;
+            // pair evaluations executed: every kept block, or those on and above the diagonal
+            const double evals = sym ? 4096.0 * t.nSym : t.aBytesRead / 1024.0 * 256.0;
+            ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, evals);
+            if (sym ? transfer_build_s_sym(g, ph, mk, t.A, t.symItems, t.nSym, m.stream)
+                    : transfer_build_s(g, ph, mk, t.A, m.stream))
+                return -1;
         }
         if (m.nearMode == 0) break;
         if (known) {  // same lists as the last build, which fitted
@@ -1120,7 +1153,8 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             const bool clr = haveMax && g0 == 0;
             ProfScope ps(m, PK_BUILD_B, 0.0, 0.0);
             if (transfer_build_b(o.actP, pin, n, n, t.nKs, np, colMax + (size_t)g0 * kColStride, t.sS, t.colScale,
-                                 t.colInv, t.B, m.stream, clr ? nxt : nullptr, clr ? stWords : 0))
+                                 t.colInv, t.B, m.stream, clr ? nxt : nullptr, clr ? stWords : 0,
+                                 t.gateInGemm ? t.rowAct : nullptr))
                 return -1;
         }
         TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, t.wgSkip, m.scratchL,
@@ -1386,6 +1420,14 @@ int model_set_transfer_splits(Model &m, int n) {
     return 0;
 }
 
+int model_set_transfer_symmetric(Model &m, int on) {
+    if (on != 0 && on != 1) { set_error("transfer symmetric build: 0 every block, 1 half and transposes"); return -2; }
+    if (on != m.transferSymmetric)
+        for (auto &o : m.targets) o.tc.valid = false;  // built the other way: rebuild
+    m.transferSymmetric = on;
+    return 0;
+}
+
 int model_set_transfer_mode(Model &m, int mode) {
     if (mode != 0 && mode != 1) { set_error("transfer mode: 0 plain FP32, 1 FP16 MFMA transfer matrix"); return -2; }
     m.transferMode = mode;
@@ -1479,6 +1521,8 @@ void model_teardown(Model &m) {
     for (auto &kv : m.masks) (void)hipFree(kv.second.p);
     m.masks.clear();
     m.transferMode = 0;
+    m.transferSplits = 0;
+    m.transferSymmetric = 1;
     m.haveLastGemm = false;
     m.nearMode = 0;
     m.nearQuadDeltas = kNfRMinDeltas;

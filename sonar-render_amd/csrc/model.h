@@ -61,6 +61,11 @@ struct Object {
         unsigned char *cls = nullptr;
         int nsplit = 1;             // split-K of the bounce product (split_walk)
         int splitsReq = 0;          // the split request (Model::transferSplits) nsplit was chosen for
+        // symmetric S build (padded rows): the 64 x 64 blocks (row block, K chunk) on
+        // or above the diagonal that are not skipped; none when the skipped chunks
+        // are not symmetric
+        int2 *symItems = nullptr;
+        int nSym = 0;
         int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
         int *nearCount = nullptr;
         bool nearClear = false;    // nearCount is known to be zero (the near-pair kernels clear it)
@@ -128,6 +133,7 @@ struct Model {
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
     int transferSplits = 0;  // splits of the transfer product: 0 automatic
+    int transferSymmetric = 1;  // S build: 1 half the blocks and their transposes where possible, 0 every block
     int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
     int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
     int gatedCap = 0;

@@ -9,12 +9,13 @@ struct TransferGeom {
     const float4 *p;    // active cells: point, area
     const float4 *nrm;  // normal, facet id bits
     int n;              // active cells
-    int nKs;            // K steps of 8 emitters (ceil(n/8), padded to a multiple of 8)
+    int nKs;            // K steps of 8 emitters (padded to a multiple of 8)
     int nRt;            // row tiles of 32 (padded to a multiple of 4)
     const int2 *wgSkip; // per 4-row-tile group: K steps [x, y) that hold only zeros (not written)
     // Rows of S may be padded so that every facet starts a 128-row group: rowAct
-    // maps a row of S to its active cell (-1: padding), actRow the other way.
-    // Both null: row = active cell.
+    // maps a row of S to its active cell (-1: padding), actRow the other way. The
+    // emitters (K positions) are then padded the same way: K position k is the
+    // emitter rowAct[k]. Both null: row = K position = active cell.
     const int *rowAct = nullptr;
     const int *actRow = nullptr;
 };
@@ -102,6 +103,10 @@ struct ReduceGateTask {
 };
 
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
+// The same S from the blocks on and above the diagonal (items: (row block, K chunk)
+// of 64), each stored with its transpose; rows and emitters padded alike, near mode 0.
+int transfer_build_s_sym(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A,
+                         const int2 *items, int nItems, hipStream_t st);
 // Writes the listed near pairs' values into S (counts read back from nearCount)
 // and clears nearCount for the next build.
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st);
@@ -114,7 +119,7 @@ int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int
 // ints zeroed (the next bounce's state set, when no column-max pass runs).
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
                      const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st,
-                     int *clearState = nullptr, int clearWords = 0);
+                     int *clearState = nullptr, int clearWords = 0, const int *kAct = nullptr);
 int transfer_gemm(const TransferGemm &t, hipStream_t st);
 int transfer_reduce_gate(const ReduceGateTask &t, hipStream_t st);
 // Lists the rows with an exactly zero input pressure in some plane (count: device int,

@@ -68,7 +68,12 @@ __device__ inline NearGeom near_geom(const TransferGeom &g, const TransferPhys &
     return q;
 }
 
-// The FP16 word of pair (row, emitter j) in the tiled transfer matrix.
+// The emitter (active cell) at K position k, -1 for a padding position.
+__device__ inline int emitter_at(const TransferGeom &g, int k) {
+    return g.rowAct ? g.rowAct[k] : (k < g.n ? k : -1);
+}
+
+// The FP16 word of the pair (row, K position j) in the tiled transfer matrix.
 __device__ inline void store_pair(uint4 *A, int nKs, int row, int j, uint32_t w) {
     const size_t lane = (size_t)(row & 31) + 32 * ((j >> 2) & 1);
     uint32_t *q = reinterpret_cast<uint32_t *>(A + ((size_t)(row >> 5) * nKs + (j >> 3)) * 64 + lane);
@@ -90,7 +95,8 @@ struct TransferNearSrc {
     __device__ bool want(long) const { return true; }
     __device__ void put(long i, float re, float im) const {
         const int2 pr = list[i];
-        store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, pr.y, pack_h2(re * ph.scale, im * ph.scale));
+        store_pair(A, g.nKs, g.actRow ? g.actRow[pr.x] : pr.x, g.actRow ? g.actRow[pr.y] : pr.y,
+                   pack_h2(re * ph.scale, im * ph.scale));
     }
 };
 
@@ -157,6 +163,52 @@ struct EmRec {
 };
 constexpr int kBuildEm = 16 * 8;  // emitters per wave: at most 16 K steps
 
+// The FP16 word of the pair (receiver rp, rn; emitter ep, en; vis: visible),
+// and whether it is a near pair (its word left zero, listed for the near-field
+// kernels) and which rule takes it. Every lane evaluates the far-field form and
+// selects (the excluded cases' values, even non-finite ones at r = 0, are never
+// selected). Exchanging receiver and emitter negates the difference vector
+// exactly, which leaves r2 and r as they are and turns the two cosine dot
+// products into each other's exact negations: the word of (j, i) is the word of
+// (i, j), bit for bit.
+template <bool ATT, int NEAR>
+__device__ __forceinline__ uint32_t pair_word(const TransferPhys &ph, const float4 &rp, const float4 &rn,
+                                              const float4 &ep, const float4 &en, bool vis, bool &isNear,
+                                              bool &isQuad) {
+    const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
+    const float r2 = vx * vx + vy * vy + vz * vz;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float de = vx * en.x + vy * en.y + vz * en.z;
+    const float dr = vx * rn.x + vy * rn.y + vz * rn.z;
+    const float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
+    // FP64 reduction of k r to one revolution, then FP32 sin/cos.
+    const double rev = (double)r * ph.krevD;
+    const float x = (float)(rev - rint(rev));
+    const float c = __builtin_amdgcn_cosf(x), sn = __builtin_amdgcn_sinf(x);
+    float a = rinv * kInv4PiT;
+    if (ATT) a *= __builtin_amdgcn_exp2f(-ph.alphaLog2e * r);
+    float Gre, Gim;
+    if (ATT) {
+        Gre = a * fmaf(ph.alpha, c, ph.kr * sn) + rinv * rinv;
+        Gim = -a * fmaf(ph.kr, c, -ph.alpha * sn);
+    } else {
+        Gre = (a * ph.kr) * sn + rinv * rinv;
+        Gim = -(a * ph.kr) * c;
+    }
+    // near pairs: evaluated afterwards from the near lists (k_near)
+    isNear = NEAR != 0 && vis && r < ph.nearRadius;
+    // the learned range starts at r >= delta
+    isQuad = isNear && (NEAR == 1 || r < ph.quadRadius);
+    const bool reg = r < ph.deltaReg;
+    float re = reg ? wgt * (-ph.alpha * kInv4PiT) : wgt * Gre;
+    float im = reg ? wgt * (ph.kr * kInv4PiT) : wgt * Gim;
+    const bool keep = vis && !isNear;
+    re = keep ? re : 0.f;
+    im = keep ? im : 0.f;
+    return pack_h2(re, im);
+}
+
 template <bool ATT, int NEAR>
 __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
                                                  uint4 *__restrict__ A, int ksPerWave) {
@@ -205,11 +257,13 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
     // and its LDS accesses run in order.
     EmRec *sEm = sEmAll[threadIdx.x >> 6];
     for (int u = lane; u < (ks1 - ks0) * 8; u += 64) {
-        const int jj = min(ks0 * 8 + u, g.n - 1);
-        sEm[u].p = g.p[jj];
-        sEm[u].n = g.nrm[jj];
-        sEm[u].w0 = mask[(size_t)b0 * g.n + jj];
-        sEm[u].w1 = mask[(size_t)b1 * g.n + jj];
+        const int jj = emitter_at(g, ks0 * 8 + u);
+        const int jc = jj >= 0 ? jj : 0;
+        sEm[u].p = g.p[jc];
+        sEm[u].n = g.nrm[jc];
+        // a padding position: no visibility (never selected)
+        sEm[u].w0 = jj >= 0 ? mask[(size_t)b0 * g.n + jj] : 0ull;
+        sEm[u].w1 = jj >= 0 ? mask[(size_t)b1 * g.n + jj] : 0ull;
     }
     for (int ks = ks0; ks < ks1; ks++) {
         if (ks >= skip.x && ks < skip.y) continue;  // uniform
@@ -217,46 +271,11 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
         unsigned nearBits = 0, quadBits = 0;  // per emitter: listed near pair, by quadrature
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            const int j = ks * 8 + q;
-            const int jl = j - ks0 * 8;  // the emitter's slot in this wave's staged records
-            const float4 ep = sEm[jl].p;
-            const float4 en = sEm[jl].n;
+            const int jl = ks * 8 + q - ks0 * 8;  // the emitter's slot in this wave's staged records
             const uint64_t word = upper ? sEm[jl].w1 : sEm[jl].w0;
-            // every lane evaluates the far-field form and selects (the excluded
-            // cases' values, even non-finite ones at r = 0, are never selected)
-            const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
-            const float r2 = vx * vx + vy * vy + vz * vz;
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * rinv;
-            const float de = vx * en.x + vy * en.y + vz * en.z;
-            const float dr = vx * rn.x + vy * rn.y + vz * rn.z;
-            const float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
-            // FP64 reduction of k r to one revolution, then FP32 sin/cos.
-            const double rev = (double)r * ph.krevD;
-            const float x = (float)(rev - rint(rev));
-            const float c = __builtin_amdgcn_cosf(x), sn = __builtin_amdgcn_sinf(x);
-            float a = rinv * kInv4PiT;
-            if (ATT) a *= __builtin_amdgcn_exp2f(-ph.alphaLog2e * r);
-            float Gre, Gim;
-            if (ATT) {
-                Gre = a * fmaf(ph.alpha, c, ph.kr * sn) + rinv * rinv;
-                Gim = -a * fmaf(ph.kr, c, -ph.alpha * sn);
-            } else {
-                Gre = (a * ph.kr) * sn + rinv * rinv;
-                Gim = -(a * ph.kr) * c;
-            }
-            const bool vis = rvalid && j < g.n && ((word >> bit) & 1ull);
-            // near pairs: evaluated afterwards from the near lists (k_near)
-            const bool isNear = NEAR != 0 && vis && r < ph.nearRadius;
-            // the learned range starts at r >= delta
-            const bool isQuad = isNear && (NEAR == 1 || r < ph.quadRadius);
-            const bool reg = r < ph.deltaReg;
-            float re = reg ? wgt * (-ph.alpha * kInv4PiT) : wgt * Gre;
-            float im = reg ? wgt * (ph.kr * kInv4PiT) : wgt * Gim;
-            const bool keep = vis && !isNear;
-            re = keep ? re : 0.f;
-            im = keep ? im : 0.f;
-            w[q] = pack_h2(re, im);
+            bool isNear, isQuad;
+            w[q] = pair_word<ATT, NEAR>(ph, rp, rn, sEm[jl].p, sEm[jl].n, rvalid && ((word >> bit) & 1ull), isNear,
+                                        isQuad);
             nearBits |= (unsigned)isNear << q;
             quadBits |= (unsigned)isQuad << q;
         }
@@ -264,7 +283,7 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
         if (NEAR != 0 && __ballot(nearBits != 0u)) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                const int j = ks * 8 + q;
+                const int j = emitter_at(g, ks * 8 + q);  // listed as active cells
                 const bool nr = (nearBits >> q) & 1u, qd = (quadBits >> q) & 1u;
                 near_stage(stQ, nQ, nr && qd, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
                 near_stage(stM, nM, nr && !qd, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
@@ -283,6 +302,96 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
     if (NEAR != 0) {
         near_flush(stQ, nQ, ph.nearQ, ph.nearCount, ph.nearCap);
         near_flush(stM, nM, ph.nearM, ph.nearCount + 1, ph.nearCap);
+    }
+}
+
+// Symmetric S build (rows and emitters padded alike, no near pairs listed). S is
+// symmetric bit for bit (pair_word; the visibility words are symmetric), so a
+// wave evaluates one 64 x 64 block, rows of block rb by K chunk cb with rb <= cb,
+// stores it, and stores its transpose as block (cb, rb): after each K step the
+// lanes' 8 words go through the wave's LDS tile and come back as transposed rows
+// (16-byte pieces of 4 consecutive K positions; 8 lanes store one row's 128
+// contiguous bytes of a K step). Items: the (rb, cb) pairs whose chunk is not
+// skipped, listed on the host. A block with no valid row or emitter is stored
+// as zeros without evaluating it.
+constexpr int kSymT = 72;  // LDS tile row stride in words: 8 transposed rows of 64 K positions, padded
+template <bool ATT>
+__global__ __launch_bounds__(256) void k_build_s_sym(TransferGeom g, TransferPhys ph,
+                                                     const uint64_t *__restrict__ mask, uint4 *__restrict__ A,
+                                                     const int2 *__restrict__ items, int nItems) {
+    __shared__ EmRec sEmAll[4][64];
+    __shared__ uint4 sTAll[4][8 * kSymT / 4];
+    const int item = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (item >= nItems) return;  // uniform
+    const int lane = threadIdx.x & 63;
+    const int2 bc = items[item];
+    const int rb = __builtin_amdgcn_readfirstlane(bc.x), cb = __builtin_amdgcn_readfirstlane(bc.y);
+    const int srow = rb * 64 + lane;
+    const int row = g.rowAct[srow];  // this lane's receiver (-1: padding)
+    const int em = g.rowAct[cb * 64 + lane];  // this lane's staged emitter (-1: padding)
+    const bool rvalid = row >= 0;
+    const float4 rp = rvalid ? g.p[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 rn = rvalid ? g.nrm[row] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint64_t valid = __ballot(rvalid);
+    const bool live = valid != 0ull && __ballot(em >= 0) != 0ull;  // uniform
+    // The rows are consecutive active cells of one facet: their visibility words
+    // come from two 64-row blocks at most (as in k_build_s).
+    const int first = live ? __builtin_amdgcn_readfirstlane(__ffsll((long long)valid) - 1) : 0;
+    const int b0 = live ? __builtin_amdgcn_readfirstlane(__shfl(row, first) >> 6) : 0;
+    const int b1 = min(b0 + 1, ((g.n + 63) >> 6) - 1);
+    const bool upper = rvalid && (row >> 6) != b0;
+    const int bit = row & 63;
+    EmRec *sEm = sEmAll[threadIdx.x >> 6];
+    uint32_t *sT = reinterpret_cast<uint32_t *>(sTAll[threadIdx.x >> 6]);
+    if (live) {
+        const int ec = em >= 0 ? em : 0;
+        sEm[lane].p = g.p[ec];
+        sEm[lane].n = g.nrm[ec];
+        sEm[lane].w0 = em >= 0 ? mask[(size_t)b0 * g.n + em] : 0ull;
+        sEm[lane].w1 = em >= 0 ? mask[(size_t)b1 * g.n + em] : 0ull;
+    }
+    const bool mirror = rb != cb;
+    // this lane's two transposed pieces per K step: row q of the step's 8, K positions
+    // 8 s2 + 4 h .. + 3 of block rb
+    const int q0 = lane & 7, h0 = (lane >> 3) & 1, s0 = lane >> 4;  // piece lane; the other is s0 + 4
+    for (int ks = 0; ks < 8; ks++) {
+        uint32_t w[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            w[q] = 0u;
+            if (live) {
+                const int jl = ks * 8 + q;
+                const uint64_t word = upper ? sEm[jl].w1 : sEm[jl].w0;
+                bool isNear, isQuad;
+                w[q] = pair_word<ATT, 0>(ph, rp, rn, sEm[jl].p, sEm[jl].n, rvalid && ((word >> bit) & 1ull), isNear,
+                                         isQuad);
+            }
+        }
+        uint4 *dst = A + ((size_t)(srow >> 5) * g.nKs + cb * 8 + ks) * 64 + (srow & 31);
+        const uint4 lo = make_uint4(w[0], w[1], w[2], w[3]), hi = make_uint4(w[4], w[5], w[6], w[7]);
+        if (ph.streamS) {
+            store_nt(dst, lo);
+            store_nt(dst + 32, hi);
+        } else {
+            dst[0] = lo;
+            dst[32] = hi;
+        }
+        if (!mirror) continue;  // uniform
+        // The wave's LDS accesses run in order: the last step's reads of the tile
+        // precede these writes, which precede the reads below.
+#pragma unroll
+        for (int q = 0; q < 8; q++) sT[q * kSymT + lane] = w[q];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int s2 = s0 + 4 * u;
+            const uint4 v = *reinterpret_cast<const uint4 *>(sT + q0 * kSymT + s2 * 8 + h0 * 4);
+            const int trow = cb * 64 + ks * 8 + q0;  // row of S: the emitter
+            uint4 *td = A + ((size_t)(trow >> 5) * g.nKs + rb * 8 + s2) * 64 + (trow & 31) + 32 * h0;
+            if (ph.streamS) store_nt(td, v);
+            else *td = v;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -329,7 +438,7 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
                                                  const unsigned *__restrict__ colMax, float sS,
                                                  float *__restrict__ scaleOut, float *__restrict__ invOut,
                                                  uint4 *__restrict__ B, int *__restrict__ clearState,
-                                                 int clearWords) {
+                                                 int clearWords, const int *__restrict__ kAct) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (clearState && t < clearWords) clearState[t] = 0;
     if (t < nplanes) {
@@ -347,9 +456,10 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
     uint32_t w[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const int j = ks * 8 + h * 4 + q;
+        const int k = ks * 8 + h * 4 + q;
+        const int j = kAct ? kAct[k] : (k < n ? k : -1);  // its active cell (-1: padding)
         float xr = 0.f, xi = 0.f;
-        if (j < n && b < nplanes) {
+        if (j >= 0 && b < nplanes) {
             float2 v = Pin[b * stride + j];
             float a = p[j].w * col_scale_of(colMax[b * kColStride]);
             xr = a * v.x;
@@ -667,6 +777,20 @@ int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64
     return 0;
 }
 
+int transfer_build_s_sym(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A,
+                         const int2 *items, int nItems, hipStream_t st) {
+    if (!g.rowAct || ph.nearMode != 0) {
+        set_error("symmetric S build: needs rows and emitters padded alike and no near pairs");
+        return -2;
+    }
+    if (nItems <= 0) return 0;
+    const unsigned blocks = (unsigned)((nItems + 3) / 4);
+    if (ph.alpha != 0.0f) k_build_s_sym<true><<<blocks, 256, 0, st>>>(g, ph, mask, A, items, nItems);
+    else k_build_s_sym<false><<<blocks, 256, 0, st>>>(g, ph, mask, A, items, nItems);
+    SR_HIP(hipGetLastError());
+    return 0;
+}
+
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st) {
     if (nQuad > ph.nearCap || nMlp > ph.nearCap) { set_error("near lists overflow"); return -2; }
     if (nMlp > 0 && !ph.mlp) { set_error("learned near field without a network"); return -2; }
@@ -689,11 +813,11 @@ int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int
 
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
                      const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st,
-                     int *clearState, int clearWords) {
+                     int *clearState, int clearWords, const int *kAct) {
     const long tot = std::max({(long)nKs * 256, (long)nplanes, clearState ? (long)clearWords : 0L});
     if (tot <= 0) return 0;
     k_build_b<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, colMax, sS, scale,
-                                                             invScale, B, clearState, clearWords);
+                                                             invScale, B, clearState, clearWords, kAct);
     SR_HIP(hipGetLastError());
     return 0;
 }

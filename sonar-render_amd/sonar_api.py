@@ -394,6 +394,16 @@ def set_transfer_splits(n):
     _check(cpp_lib.sr_set_transfer_splits(int(n)), "set_transfer_splits")
 
 
+_sig("sr_set_transfer_symmetric", [_i])
+_sig("sr_transfer_symmetric", [])
+
+
+def set_transfer_symmetric(on):
+    """1: S from the blocks on and above the diagonal and their transposes where the
+    layout allows (default); 0: every block evaluated. The same matrix either way."""
+    _check(cpp_lib.sr_set_transfer_symmetric(int(on)), "set_transfer_symmetric")
+
+
 _sig("sr_read_stl", [ctypes.c_char_p, _fp, _l], _l)
 _sig("sr_write_stl", [ctypes.c_char_p, _fp, _i])
 _sig("sr_load_stl", [ctypes.c_char_p, _i, _f])

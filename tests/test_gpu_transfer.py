@@ -56,6 +56,47 @@ def test_transfer_split_counts_agree(api):
     assert rel_l2(got, np.concatenate([ref["points"], ref["targets"][0]])) < TOL_MIXED
 
 
+def _symmetric_cases():
+    att = scenes.config_trihedral(f=12e3, n_side=2, bounces=3)
+    att["atten"] = 0.1
+    tris = scenes.halve_facets(scenes.halve_facets(scenes.trihedral(1.0)))
+    runs = dict(name="trihedral_48_facets_20kHz", f=20e3, c=1480.0, atten=0.0,
+                targets=[(tris, 1480 / 20e3 / 7.5)], fields=[], sources=scenes.octant_incidences(2, 15.0),
+                points=scenes.octant_incidences(2, 15.0), bounces=3, batched=True)
+    return [("trihedral 20 kHz", scenes.config_trihedral(f=20e3, n_side=2, bounces=3)),
+            ("trihedral 12 kHz, attenuation 0.1", att),
+            ("trihedral of 48 facets of 342 fragments (each padded to 384), 20 kHz", runs)]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_symmetric_s_build_matches_full(api, case):
+    """S is symmetric bit for bit: exchanging receiver and emitter negates the
+    difference vector exactly, so r and the two cosine products come out the
+    same, and the visibility words are symmetric. The build that evaluates only
+    the 64 x 64 blocks on and above the diagonal and stores each one transposed
+    as well must then give the same pressures, bit for bit, as evaluating every
+    block, with about half the pair evaluations. The cases cover the
+    attenuating pair kernel and many facets per plate, each padded (K chunks
+    holding padding rows and emitters, coplanar facets' chunks skipped)."""
+    name, sc = _symmetric_cases()[case]
+    outs, evals = {}, {}
+    for sym in (1, 0):
+        _banner(f"transfer form, {name}, {'symmetric' if sym else 'full'} S build")
+        runner.load_scene(sc, transfer_mode=1)
+        api.set_transfer_symmetric(sym)
+        api.profile(True)
+        runner.project_scene(sc)
+        evals[sym] = api.kernel_stats("build_s")["flops"]
+        api.profile(False)
+        outs[sym] = runner.read_results(sc)
+        api.TearDownCuda()
+    print("pair evaluations: symmetric", evals[1], "full", evals[0])
+    assert 0 < evals[1] < 0.55 * evals[0]
+    a, b = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]]) for o in (outs[1], outs[0]))
+    assert np.isfinite(a).all() and np.abs(a).max() > 0
+    assert a.tobytes() == b.tobytes()
+
+
 def test_mixed_many_small_facets(api):
     """Facets with fewer than 8 fragments force K steps that straddle facets."""
     tris = scenes.trihedral(1.0)
