@@ -284,10 +284,47 @@ __global__ void k_pack_nodes(const float4 *__restrict__ tris, int n, const int *
 
 constexpr int kStack = 128;
 constexpr int kBruteTris = 8;
+constexpr int kPreTris = 4;  // few enough occluders for their origin terms to be kept per ray origin
 
 __device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
 
 __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// The origin terms (origin_terms) of triangle ti of the LDS-staged records, for
+// ray origin o; zeros past the last triangle.
+__device__ inline float4 staged_origin_terms(const float4 *ldsTris, int ti, int ntri, F3 o) {
+    if (ti >= ntri) return make_float4(0.f, 0.f, 0.f, 0.f);
+    return origin_terms(tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], ldsTris[4 * ti + 3]), o);
+}
+
+// wave_anyhit<true> for at most kPreTris occluders with their origin terms given:
+// preE[64 ti] the emitter's (a broadcast LDS read), preR[ti] the receiver's; each
+// lane takes the terms of the end its ray leaves from. Same tests, same result.
+__device__ inline bool wave_anyhit_pre(int ntri, const VisConst &vc, bool active, F3 o, F3 d, float tmin, float tmax,
+                                       int skipA, int skipB, const float4 *ldsTris, const float4 *preE,
+                                       const float4 (&preR)[kPreTris], bool fromEm) {
+    bool hit = false;
+    if (ballot64(active) == 0) return false;
+    // the records are read from LDS at each use (an opaque zero offset keeps the
+    // compiler from holding all of them in registers across the emitter loop)
+    int off = 0;
+    asm volatile("" : "+v"(off));
+    ldsTris += off;
+#pragma unroll
+    for (int ti = 0; ti < kPreTris; ti++) {
+        if (ti >= ntri) break;
+        const float4 q3 = ldsTris[4 * ti + 3];
+        const int id = __float_as_int(q3.w);
+        const bool need = active && !hit && id != skipA && id != skipB;
+        if (ballot64(need) == 0) continue;
+        TriRec t = tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], q3);
+        const float4 pe = preE[64 * ti];
+        const float4 qt = fromEm ? pe : preR[ti];
+        if (need) hit = mt_hit_pre(t, vc.padTri, o, d, tmin, tmax, qt);
+        if (ballot64(active && !hit) == 0) break;
+    }
+    return hit;
+}
 
 // Returns this lane's "occluded" flag. active: lane has a ray to trace.
 // skipA, skipB: the triangles this lane's end points lie on (-1: none); never tested.
@@ -376,16 +413,24 @@ __device__ inline bool wave_anyhit(const Bvh &bvh, const VisConst &vc, int *stk,
 // slots and read back by the whole wave at one address each.
 // FEW: at most kBruteTris occluders, their records staged in LDS at ldsTris.
 // sEm: this wave's LDS slots for 64 emitters' points and normals.
-template <bool FEW>
+template <bool FEW, bool PRE = false>
 __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &tk, const Bvh &bvh,
                                                                const VisConst &vc, int rb, int ec, int emPerWave,
-                                                               int *stk, const float4 *ldsTris, float4 *sEm) {
+                                                               int *stk, const float4 *ldsTris, float4 *sEm,
+                                                               float4 *sPre = nullptr) {
     const int lane = __lane_id();
     const int r = rb * 64 + lane;
     const bool rvalid = r < tk.rc.count;
     float4 rp = rvalid ? tk.rc.p[r] : make_float4(0.f, 0.f, 0.f, 0.f);
     float4 rn = (rvalid && tk.rc.n) ? tk.rc.n[r] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int rfacet = __float_as_int(rn.w);
+    // PRE (at most kPreTris occluders): each triangle's origin terms for this
+    // lane's receiver as the ray origin, once; the emitters' are staged with them
+    float4 preR[kPreTris];
+    if (PRE) {
+#pragma unroll
+        for (int ti = 0; ti < kPreTris; ti++) preR[ti] = staged_origin_terms(ldsTris, ti, bvh.ntri, F3{rp.x, rp.y, rp.z});
+    }
     int e0 = ec * emPerWave;
     const int e1 = min(tk.em.count, e0 + emPerWave);
     if (tk.upperOnly) e0 = max(e0, rb * 64);
@@ -395,8 +440,14 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
             const int el = min(e + lane, e1 - 1);
             // the wave's reads of the previous 64 precede these writes (its LDS
             // accesses run in order), as these precede the reads below
-            sEm[lane] = tk.em.p[el];
+            const float4 epl = tk.em.p[el];
+            sEm[lane] = epl;
             sEm[64 + lane] = tk.em.n ? tk.em.n[el] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (PRE) {
+#pragma unroll
+                for (int ti = 0; ti < kPreTris; ti++)
+                    sPre[64 * ti + lane] = staged_origin_terms(ldsTris, ti, bvh.ntri, F3{epl.x, epl.y, epl.z});
+            }
         }
         const float4 ep = sEm[k];  // one address for the whole wave: a broadcast
         const float4 en = sEm[64 + k];
@@ -426,19 +477,41 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
         if (!fromEm) uv = F3{-uv.x, -uv.y, -uv.z};
         const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
         const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
-        bool occ = wave_anyhit<FEW>(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc,
-                                    ldsTris);
+        bool occ = PRE ? wave_anyhit_pre(bvh.ntri, vc, pair, o, uv, tk.tmin, rr - 1e-3f, triEm, triRc, ldsTris,
+                                         sPre + k, preR, fromEm)
+                       : wave_anyhit<FEW>(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc,
+                                          ldsTris);
         unsigned long long word = ballot64(pair && !occ);
         if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
     }
 }
 
-// One wave = one receiver block of 64 and a run of emitters.
-template <int WAVES>
+// Waves of a symmetric set (upperOnly, emitters = receivers) with runs of
+// emPerWave dividing 64: only the (receiver block rb, emitter run ec) pairs with
+// emitters at or above 64 rb, f = 64 / emPerWave runs fewer per block:
+// waves before block rb: P(rb) = rb C - f rb (rb - 1) / 2 (C runs per block).
+__host__ __device__ inline long upper_waves_before(long rb, long C, long f) { return rb * C - f * rb * (rb - 1) / 2; }
+__host__ __device__ inline bool compact_upper(const MaskTask &tk, int emPerWave) {
+    return tk.upperOnly && tk.em.count == tk.rc.count && emPerWave <= 64 && 64 % emPerWave == 0;
+}
+__device__ inline void upper_wave(long gw, int nrb, int C, int f, int &rb, int &ec) {
+    const double a = C + 0.5 * f;
+    long r = (long)((a - sqrt(fmax(0.0, a * a - 2.0 * f * (double)gw))) / f);
+    r = r < 0 ? 0 : (r > nrb - 1 ? nrb - 1 : r);
+    while (r > 0 && upper_waves_before(r, C, f) > gw) r--;
+    while (r + 1 < nrb && upper_waves_before(r + 1, C, f) <= gw) r++;
+    rb = (int)r;
+    ec = (int)(r * f + (gw - upper_waves_before(r, C, f)));
+}
+
+// One wave = one receiver block of 64 and a run of emitters. PRE: at most
+// kPreTris occluders (tested from LDS with their origin terms kept per origin).
+template <int WAVES, bool PRE>
 __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, VisConst vc, int emPerWave) {
-    __shared__ int stkAll[WAVES][kStack];
+    __shared__ int stkAll[WAVES][PRE ? 1 : kStack];
     __shared__ float4 sEmAll[WAVES][128];
     __shared__ float4 sTri[4 * kBruteTris];  // few occluders: their records, staged once per block
+    __shared__ float4 sPreAll[WAVES][PRE ? kPreTris * 64 : 1];  // the staged emitters' origin terms
     const bool brute = bvh.ntri > 0 && bvh.ntri <= kBruteTris;
     if (brute)
         for (int q = threadIdx.x; q < 4 * bvh.ntri; q += 64 * WAVES) sTri[q] = bvh.tris[q];
@@ -447,13 +520,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
     const int nrb = (tk.rc.count + 63) >> 6;
     const int nEmChunks = (tk.em.count + emPerWave - 1) / emPerWave;
     const long gw = (long)blockIdx.x * WAVES + wave;
-    if (gw >= (long)nrb * nEmChunks) return;
-    if (brute)
-        vis_wave<true>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], sTri,
-                       sEmAll[wave]);
+    const bool compact = compact_upper(tk, emPerWave);
+    const int f = 64 / emPerWave;
+    if (gw >= (compact ? upper_waves_before(nrb, nEmChunks, f) : (long)nrb * nEmChunks)) return;
+    int rb = (int)(gw / nEmChunks), ec = (int)(gw % nEmChunks);
+    if (compact) upper_wave(gw, nrb, nEmChunks, f, rb, ec);
+    if (PRE)
+        vis_wave<true, true>(tk, bvh, vc, rb, ec, emPerWave, nullptr, sTri, sEmAll[wave], sPreAll[wave]);
+    else if (brute)
+        vis_wave<true>(tk, bvh, vc, rb, ec, emPerWave, stkAll[wave], sTri, sEmAll[wave]);
     else
-        vis_wave<false>(tk, bvh, vc, (int)(gw / nEmChunks), (int)(gw % nEmChunks), emPerWave, stkAll[wave], nullptr,
-                        sEmAll[wave]);
+        vis_wave<false>(tk, bvh, vc, rb, ec, emPerWave, stkAll[wave], nullptr, sEmAll[wave]);
 }
 
 // Persistent form over many tasks (scenes): each wave pulls (task, receiver
@@ -577,16 +654,22 @@ int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipS
     const long nrb = (tk.rc.count + 63) / 64;
     // Enough waves to fill 256 CUs several times over, but runs of emitters long
     // enough that the receiver block's data is reused (shorter runs for small sets).
-    int emPerWave = 16;
-    while (emPerWave > 1 && nrb * ((tk.em.count + emPerWave - 1) / emPerWave) < 8192) emPerWave /= 2;
-    long waves = nrb * ((tk.em.count + emPerWave - 1) / emPerWave);
+    // A symmetric set launches only the waves with emitters (compact_upper).
+    auto waves_for = [&](int epw) {
+        const long C = (tk.em.count + epw - 1) / epw;
+        return compact_upper(tk, epw) ? upper_waves_before(nrb, C, 64 / epw) : nrb * C;
+    };
+    int emPerWave = 32;
+    while (emPerWave > 1 && waves_for(emPerWave) < 8192) emPerWave /= 2;
+    long waves = waves_for(emPerWave);
     while (waves > (1L << 22) && emPerWave < 1024) {
         emPerWave *= 2;
-        waves = nrb * ((tk.em.count + emPerWave - 1) / emPerWave);
+        waves = waves_for(emPerWave);
     }
     long blocks = (waves + W - 1) / W;
     if (blocks > 0x7fffffffL) { set_error("visibility grid too large"); return -1; }
-    k_vis_mask<W><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
+    if (bvh.ntri > 0 && bvh.ntri <= kPreTris) k_vis_mask<W, true><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
+    else k_vis_mask<W, false><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
     SR_HIP(hipGetLastError());
     if (tk.upperOnly && nrb > 1) {
         const long pairs = nrb * (nrb - 1) / 2;

@@ -129,6 +129,39 @@ __device__ inline bool mt_hit_dev(const TriRec &T, float padTri, F3 o, F3 d, flo
            h.y <= T.bmax.y + padTri && h.z >= T.bmin.z - padTri && h.z <= T.bmax.z + padTri;
 }
 
+// mt_hit_dev with the terms that depend only on the ray origin and the triangle
+// given: q = (o - v0) x e1 and tq = e2 . q, computed by the same operations
+// (origin_terms), so every value is the one mt_hit_dev computes. The hit is the
+// conjunction of the same tests; t (from tq) is tested first, before u and v,
+// so a wave whose rays all end before the triangle's plane skips those.
+__device__ inline float4 origin_terms(const TriRec &T, F3 o) {
+    const F3 s = f3sub(o, T.v0);
+    const F3 q = f3cross(s, T.e1);
+    return make_float4(q.x, q.y, q.z, f3dot(T.e2, q));
+}
+
+__device__ inline bool mt_hit_pre(const TriRec &T, float padTri, F3 o, F3 d, float tmin, float tmax, float4 qt) {
+    F3 p = f3cross(d, T.e2);
+    float det = f3dot(T.e1, p);
+    if (det == 0.0f) return false;
+    const bool normal = fabsf(det) >= 0x1p-60f && fabsf(det) <= 0x1p60f;
+    float inv = rcp_ieee_normal(det);
+    if (__builtin_expect(__ballot(!normal) != 0ull, 0)) {
+        if (!normal) inv = 1.0f / det;
+    }
+    float t = qt.w * inv;
+    if (!(t > tmin && t < tmax)) return false;
+    F3 s = f3sub(o, T.v0);
+    float u = f3dot(s, p) * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    F3 q{qt.x, qt.y, qt.z};
+    float v = f3dot(d, q) * inv;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+    F3 h{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+    return h.x >= T.bmin.x - padTri && h.x <= T.bmax.x + padTri && h.y >= T.bmin.y - padTri &&
+           h.y <= T.bmax.y + padTri && h.z >= T.bmin.z - padTri && h.z <= T.bmax.z + padTri;
+}
+
 __host__ __device__ inline float safe_inv(float d) {
     float dd = (fabsf(d) < 1e-20f) ? (d < 0.0f ? -1e-20f : 1e-20f) : d;
     return 1.0f / dd;
