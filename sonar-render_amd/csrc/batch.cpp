@@ -31,7 +31,8 @@
 
 namespace sr {
 
-int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st);
+int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st,
+                    bool deferLower = false);
 int vis_mask_multi_launch(const MaskTask *tasks, const Bvh *bvhs, const VisConst *vcs, const int4 *items, int nitems,
                           int emPerWave, int *counter, int grid, hipStream_t st);
 int mask_transpose_multi_launch(const MaskTask *tasks, const int2 *items, int nitems, hipStream_t st);

@@ -18,7 +18,9 @@ namespace sr {
 // Kernel launchers from vis.hip
 int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 *dTrisSorted, float4 *dNodes,
                hipStream_t st);
-int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st);
+int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st,
+                    bool deferLower = false);
+int vis_mask_fill_lower(uint64_t *mask, int n, hipStream_t st);
 
 namespace {
 
@@ -510,10 +512,25 @@ int mask_task(Model &m, int kind, int ta, int ia, int tb, int ib, MaskTask &tk, 
 }
 
 // Visibility words for one (emitter set, receiver set), built on first use.
-int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out) {
+// upperOk: the caller reads only the words at or above the diagonal blocks of a
+// symmetric set (emitter >= 64 x receiver block); the others are filled in by
+// transposition when a caller first needs them.
+int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out, bool upperOk = false) {
     auto key = std::make_tuple(kind, ta, ia, tb, ib);
     auto it = m.masks.find(key);
-    if (it != m.masks.end() && it->second.valid) { *out = it->second.p; return 0; }
+    if (it != m.masks.end() && it->second.valid) {
+        Model::MaskBuf &mb = it->second;
+        if (mb.lowerPending && !upperOk) {
+            MaskTask tk{};
+            size_t words = 0;
+            if (mask_task(m, kind, ta, ia, tb, ib, tk, words)) return -1;
+            ProfScope ps(m, PK_VIS, 0.0, 0.0, (double)words * 8.0);
+            if (vis_mask_fill_lower(mb.p, tk.rc.count, m.stream)) return -1;
+            mb.lowerPending = false;
+        }
+        *out = mb.p;
+        return 0;
+    }
     MaskTask tk{};
     size_t words = 0;
     if (mask_task(m, kind, ta, ia, tb, ib, tk, words)) return -1;
@@ -537,8 +554,9 @@ int get_mask(Model &m, int kind, int ta, int ia, int tb, int ib, uint64_t **out)
         // pairs: the (emitter, receiver) pairs the words cover (a symmetric set traces half
         // of them); bytes: the words written, the kernel's only output
         ProfScope ps(m, PK_VIS, (double)tk.em.count * tk.rc.count, 0.0, (double)words * 8.0);
-        if (vis_mask_launch(tk, bvh_of(m), m.vc, m.stream)) return -1;
+        if (vis_mask_launch(tk, bvh_of(m), m.vc, m.stream, upperOk)) return -1;
     }
+    mb.lowerPending = upperOk && tk.upperOnly;
     mb.valid = true;  // the traversal-overflow flag is checked when the projection finishes
     *out = mk;
     return 0;
@@ -856,8 +874,6 @@ static int ensure_transfer(Model &m, int i) {
     Object &o = m.targets[i];
     auto &t = o.tc;
     if (t.valid && (m.nearMode != 2 || t.nfGen == nearfield().gen)) return 0;
-    uint64_t *mk;
-    if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk)) return -1;
     const int n = o.nact;
     // Facet ranges of the active rows (sorted by facet). When every facet holds at
     // least 128 rows and padding each to whole 128-row groups adds at most a
@@ -1015,17 +1031,21 @@ static int ensure_transfer(Model &m, int i) {
             SR_HIP(hipMemcpy(t.actRow, actRow.data(), n * sizeof(int), hipMemcpyHostToDevice));
             SR_HIP(hipMemcpy(t.kB, kB.data(), nWg * sizeof(int), hipMemcpyHostToDevice));
             SR_HIP(hipMemcpy(t.cls, cls.data(), cls.size(), hipMemcpyHostToDevice));
-            // Symmetric build items: block (rb, cb), rb <= cb, whose K chunk the row
+            // Symmetric build items: block (rb, cb), rb < cb, whose K chunk the row
             // group keeps; its transpose (cb, rb) must then be kept too (and a
             // skipped block's transpose skipped), which holds when the runs are
-            // whole padded facets.
+            // whole padded facets. Every row of such a block precedes every
+            // emitter (the padding keeps the active cells' order), so the build
+            // reads only visibility words at or above the diagonal blocks. A kept
+            // diagonal block (none when every facet spans 128 rows) would need the
+            // others: no symmetric build then.
             auto kept = [&](int rb, int cb) { return cb * 8 < skip[rb >> 1].x || cb * 8 >= skip[rb >> 1].y; };
             const int nb = nKs / 8;  // 64-row blocks = K chunks
             std::vector<int2> items;
             bool symOk = nb * 64 == nPad;
             for (int rb = 0; rb < nb && symOk; rb++)
                 for (int cb = rb; cb < nb; cb++) {
-                    if (kept(rb, cb) != kept(cb, rb)) { symOk = false; break; }
+                    if (kept(rb, cb) != kept(cb, rb) || (cb == rb && kept(rb, cb))) { symOk = false; break; }
                     if (kept(rb, cb)) items.push_back(make_int2(rb, cb));
                 }
             if (symOk && !items.empty()) {
@@ -1060,6 +1080,9 @@ static int ensure_transfer(Model &m, int i) {
         if (!nf.trained) { set_error("learned near-field mode needs a trained network (sr_nf_train)"); return -3; }
         ph.mlp = nf.P;
     }
+    const bool sym = m.nearMode == 0 && m.transferSymmetric && t.nSym > 0;
+    uint64_t *mk;
+    if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk, sym)) return -1;
     int counts[2] = {0, 0};
     const double key[5] = {(double)m.nearMode, (double)m.nearRadiusWl, (double)m.nearQuadDeltas, (double)m.kf,
                            (double)o.delta};
@@ -1077,7 +1100,6 @@ static int ensure_transfer(Model &m, int i) {
             t.nearClear = false;  // the build counts into it
         }
         {
-            const bool sym = m.nearMode == 0 && m.transferSymmetric && t.nSym > 0;
             // pair evaluations executed: every kept block, or those on and above the diagonal
             const double evals = sym ? 4096.0 * t.nSym : t.aBytesRead / 1024.0 * 256.0;
             ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, evals);

@@ -129,7 +129,9 @@ struct Model {
     float4 *nodes = nullptr;
     int ntri = 0;
     VisConst vc{};
-    struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; };
+    // lowerPending: a symmetric set whose words below the diagonal blocks are not yet
+    // filled in by transposition (only the upper ones were needed so far)
+    struct MaskBuf { uint64_t *p = nullptr; size_t words = 0; bool valid = false; bool lowerPending = false; };
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
     int transferSplits = 0;  // splits of the transfer product: 0 automatic

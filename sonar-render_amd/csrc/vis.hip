@@ -648,7 +648,11 @@ int lbvh_build(const float4 *dTrisUnsorted, int n, float3 lo, float3 hi, float4 
     return 0;
 }
 
-int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st) {
+int vis_mask_fill_lower(uint64_t *mask, int n, hipStream_t st);
+
+// deferLower: leave a symmetric set's words below the diagonal blocks unfilled
+// (vis_mask_fill_lower fills them when a caller needs them).
+int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipStream_t st, bool deferLower) {
     if (tk.rc.count <= 0 || tk.em.count <= 0) return 0;
     constexpr int W = 4;
     const long nrb = (tk.rc.count + 63) / 64;
@@ -671,11 +675,17 @@ int vis_mask_launch(const MaskTask &tk, const Bvh &bvh, const VisConst &vc, hipS
     if (bvh.ntri > 0 && bvh.ntri <= kPreTris) k_vis_mask<W, true><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
     else k_vis_mask<W, false><<<(unsigned)blocks, 64 * W, 0, st>>>(tk, bvh, vc, emPerWave);
     SR_HIP(hipGetLastError());
-    if (tk.upperOnly && nrb > 1) {
-        const long pairs = nrb * (nrb - 1) / 2;
-        k_mask_transpose<<<(unsigned)((pairs * 64 + 255) / 256), 256, 0, st>>>(tk.mask, tk.rc.count);
-        SR_HIP(hipGetLastError());
-    }
+    if (tk.upperOnly && !deferLower) return vis_mask_fill_lower(tk.mask, tk.rc.count, st);
+    return 0;
+}
+
+// The words below the diagonal blocks of a symmetric set of n, from those above.
+int vis_mask_fill_lower(uint64_t *mask, int n, hipStream_t st) {
+    const long nrb = (n + 63) / 64;
+    if (nrb <= 1) return 0;
+    const long pairs = nrb * (nrb - 1) / 2;
+    k_mask_transpose<<<(unsigned)((pairs * 64 + 255) / 256), 256, 0, st>>>(mask, n);
+    SR_HIP(hipGetLastError());
     return 0;
 }
 

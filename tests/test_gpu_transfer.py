@@ -97,6 +97,39 @@ def test_symmetric_s_build_matches_full(api, case):
     assert a.tobytes() == b.tobytes()
 
 
+def test_self_words_filled_when_needed_after_symmetric_build(api):
+    """The symmetric S build reads only the SELF visibility words at or above the
+    diagonal blocks (its rows always precede its emitters), so the transposition
+    that fills in the rest waits until a caller needs them. Read back through the
+    C ABI, or used by the plain form's bounce in the same scene, they must then
+    be the full symmetric set: the same words and pressures as a fresh scene."""
+    sc = scenes.config_trihedral(f=12e3, n_side=2, bounces=3)
+
+    def project():
+        api.reset_pressures()
+        runner.project_scene(sc)
+        return runner.read_results(sc)
+
+    _banner("SELF words read after a symmetric S build vs a fresh scene")
+    runner.load_scene(sc, transfer_mode=1)
+    project()
+    words_after = api.mask_words(2, ta=0, ia=0)
+    runner.load_scene(sc, transfer_mode=0)
+    words_fresh = api.mask_words(2, ta=0, ia=0)
+    assert words_after.tobytes() == words_fresh.tobytes()
+
+    _banner("plain bounce after a symmetric S build in the same scene vs a fresh scene")
+    runner.load_scene(sc, transfer_mode=1)
+    project()
+    api.set_transfer_mode(0)
+    after = project()
+    fresh = runner.run_scene(sc, transfer_mode=0)
+    a, b = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]]) for o in (after, fresh))
+    assert np.abs(a).max() > 0
+    assert a.tobytes() == b.tobytes()
+    api.TearDownCuda()
+
+
 def test_mixed_many_small_facets(api):
     """Facets with fewer than 8 fragments force K steps that straddle facets."""
     tris = scenes.trihedral(1.0)
