@@ -297,14 +297,63 @@ __device__ inline float4 staged_origin_terms(const float4 *ldsTris, int ti, int 
     return origin_terms(tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], ldsTris[4 * ti + 3]), o);
 }
 
-// wave_anyhit<true> for at most kPreTris occluders with their origin terms given:
-// preE[64 ti] the emitter's (a broadcast LDS read), preR[ti] the receiver's; each
-// lane takes the terms of the end its ray leaves from. Same tests, same result.
-__device__ inline bool wave_anyhit_pre(int ntri, const VisConst &vc, bool active, F3 o, F3 d, float tmin, float tmax,
-                                       int skipA, int skipB, const float4 *ldsTris, const float4 *preE,
-                                       const float4 (&preR)[kPreTris], bool fromEm) {
+// Plane-side rule for few occluders. With N = e1 x e2 and side(p) = N.p - N.v0
+// (FP32, in this order), a ray whose two end points both lie beyond
+// m = 1e-4 scale |e1|_1 |e2|_1 on one side of the triangle's plane is never
+// accepted by the FP32 triangle test (mt_hit), so the test can be left out. The
+// test's t = (e2.q)/det estimates N.(v0 - o)/(N.d): with both ends on one side
+// beyond m, t lies outside (0, L) by a distance that needs errors of order m in
+// e2.q or m/L in det to close, while each is off by a few ulps of
+// scale |e1| |e2| (|o - v0| <= 2 sqrt(3) scale, |d| = 1), a factor of about 30
+// below m; the side values themselves are off by less still. The rule is
+// checked against the oracle's FP32 test on random, grazing and sliver cases on
+// the CPU.
+__device__ inline float plane_margin(const TriRec &T, float scale) {
+    const float l1 = (fabsf(T.e1.x) + fabsf(T.e1.y)) + fabsf(T.e1.z);
+    const float l2 = (fabsf(T.e2.x) + fabsf(T.e2.y)) + fabsf(T.e2.z);
+    return ((1e-4f * scale) * l1) * l2;
+}
+
+__device__ inline float plane_side(const TriRec &T, F3 p) {
+    const F3 n = f3cross(T.e1, T.e2);
+    return f3dot(n, p) - f3dot(n, T.v0);
+}
+
+// Per end point and triangle ti < ntri (at most kPreTris): bit ti: beyond the
+// margin above the plane, bit 4 + ti: below, bit 8 + ti: the triangle is the one
+// the point lies on (own: its index, -1 none). A pair's candidate triangles are
+// those with neither a shared side bit nor an own bit of either end.
+__device__ inline unsigned plane_bits(const float4 *ldsTris, int ntri, float scale, F3 p, int own) {
+    unsigned b = 0;
+#pragma unroll
+    for (int ti = 0; ti < kPreTris; ti++) {
+        if (ti >= ntri) break;
+        const float4 q3 = ldsTris[4 * ti + 3];
+        const TriRec T = tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], q3);
+        const float m = plane_margin(T, scale), sd = plane_side(T, p);
+        b |= ((unsigned)(sd > m) << ti) | ((unsigned)(sd < -m) << (ti + 4)) |
+             ((unsigned)(__float_as_int(q3.w) == own) << (ti + 8));
+    }
+    return b;
+}
+
+__device__ inline unsigned pair_candidates(unsigned bE, unsigned bR, int ntri) {
+    const unsigned same = bE & bR;
+    return ~(same | (same >> 4) | ((bE | bR) >> 8)) & ((1u << ntri) - 1u);
+}
+
+// wave_anyhit<true> for at most kPreTris occluders: the lane's candidate
+// triangles cand (pair_candidates; 0 for no ray), the origin terms preE[64 ti]
+// of the emitter (a broadcast LDS read) and preR[ti] of the receiver, the ray
+// leaving the emitter along uv (fromEm) or the receiver along -uv. The triangles
+// left out cannot be hit: the same result as testing every one.
+__device__ inline bool wave_anyhit_pre(unsigned cand, const VisConst &vc, float tmin, float tmax,
+                                       const float4 *ldsTris, const float4 *preE, const float4 (&preR)[kPreTris],
+                                       bool fromEm, F3 ep, F3 rp, F3 uv) {
+    if (ballot64(cand != 0u) == 0) return false;
+    const F3 o = fromEm ? ep : rp;
+    const F3 d = fromEm ? uv : F3{-uv.x, -uv.y, -uv.z};
     bool hit = false;
-    if (ballot64(active) == 0) return false;
     // the records are read from LDS at each use (an opaque zero offset keeps the
     // compiler from holding all of them in registers across the emitter loop)
     int off = 0;
@@ -312,16 +361,13 @@ __device__ inline bool wave_anyhit_pre(int ntri, const VisConst &vc, bool active
     ldsTris += off;
 #pragma unroll
     for (int ti = 0; ti < kPreTris; ti++) {
-        if (ti >= ntri) break;
-        const float4 q3 = ldsTris[4 * ti + 3];
-        const int id = __float_as_int(q3.w);
-        const bool need = active && !hit && id != skipA && id != skipB;
+        const bool need = ((cand >> ti) & 1u) && !hit;
         if (ballot64(need) == 0) continue;
-        TriRec t = tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], q3);
+        TriRec t = tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], ldsTris[4 * ti + 3]);
         const float4 pe = preE[64 * ti];
         const float4 qt = fromEm ? pe : preR[ti];
         if (need) hit = mt_hit_pre(t, vc.padTri, o, d, tmin, tmax, qt);
-        if (ballot64(active && !hit) == 0) break;
+        if (ballot64((cand >> (ti + 1)) != 0u && !hit) == 0) break;
     }
     return hit;
 }
@@ -417,7 +463,7 @@ template <bool FEW, bool PRE = false>
 __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &tk, const Bvh &bvh,
                                                                const VisConst &vc, int rb, int ec, int emPerWave,
                                                                int *stk, const float4 *ldsTris, float4 *sEm,
-                                                               float4 *sPre = nullptr) {
+                                                               float4 *sPre = nullptr, unsigned *sBits = nullptr) {
     const int lane = __lane_id();
     const int r = rb * 64 + lane;
     const bool rvalid = r < tk.rc.count;
@@ -427,9 +473,12 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
     // PRE (at most kPreTris occluders): each triangle's origin terms for this
     // lane's receiver as the ray origin, once; the emitters' are staged with them
     float4 preR[kPreTris];
+    unsigned bitsR = 0;
     if (PRE) {
 #pragma unroll
         for (int ti = 0; ti < kPreTris; ti++) preR[ti] = staged_origin_terms(ldsTris, ti, bvh.ntri, F3{rp.x, rp.y, rp.z});
+        bitsR = plane_bits(ldsTris, bvh.ntri, vc.scale, F3{rp.x, rp.y, rp.z},
+                           tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1);
     }
     int e0 = ec * emPerWave;
     const int e1 = min(tk.em.count, e0 + emPerWave);
@@ -441,12 +490,15 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
             // the wave's reads of the previous 64 precede these writes (its LDS
             // accesses run in order), as these precede the reads below
             const float4 epl = tk.em.p[el];
+            const float4 enl = tk.em.n ? tk.em.n[el] : make_float4(0.f, 0.f, 0.f, 0.f);
             sEm[lane] = epl;
-            sEm[64 + lane] = tk.em.n ? tk.em.n[el] : make_float4(0.f, 0.f, 0.f, 0.f);
+            sEm[64 + lane] = enl;
             if (PRE) {
 #pragma unroll
                 for (int ti = 0; ti < kPreTris; ti++)
                     sPre[64 * ti + lane] = staged_origin_terms(ldsTris, ti, bvh.ntri, F3{epl.x, epl.y, epl.z});
+                sBits[lane] = plane_bits(ldsTris, bvh.ntri, vc.scale, F3{epl.x, epl.y, epl.z},
+                                         tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(enl.w) : -1);
             }
         }
         const float4 ep = sEm[k];  // one address for the whole wave: a broadcast
@@ -473,14 +525,18 @@ __device__ __attribute__((always_inline)) inline void vis_wave(const MaskTask &t
         ray_setup_dev(F3{ep.x, ep.y, ep.z}, F3{rp.x, rp.y, rp.z}, uv, rr);
         if (tk.useNormalEm && !(f3dot(uv, F3{en.x, en.y, en.z}) >= 1e-6f)) pair = false;
         if (tk.useNormalRc && !(f3dot(F3{-uv.x, -uv.y, -uv.z}, F3{rn.x, rn.y, rn.z}) >= 1e-6f)) pair = false;
-        const F3 o = fromEm ? F3{ep.x, ep.y, ep.z} : F3{rp.x, rp.y, rp.z};
-        if (!fromEm) uv = F3{-uv.x, -uv.y, -uv.z};
-        const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
-        const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
-        bool occ = PRE ? wave_anyhit_pre(bvh.ntri, vc, pair, o, uv, tk.tmin, rr - 1e-3f, triEm, triRc, ldsTris,
-                                         sPre + k, preR, fromEm)
-                       : wave_anyhit<FEW>(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc,
-                                          ldsTris);
+        bool occ;
+        if (PRE) {
+            const unsigned cand = pair ? pair_candidates(sBits[k], bitsR, bvh.ntri) : 0u;
+            occ = wave_anyhit_pre(cand, vc, tk.tmin, rr - 1e-3f, ldsTris, sPre + k, preR, fromEm,
+                                  F3{ep.x, ep.y, ep.z}, F3{rp.x, rp.y, rp.z}, uv);
+        } else {
+            const F3 o = fromEm ? F3{ep.x, ep.y, ep.z} : F3{rp.x, rp.y, rp.z};
+            if (!fromEm) uv = F3{-uv.x, -uv.y, -uv.z};
+            const int triEm = tk.emTriBase >= 0 ? tk.emTriBase + __float_as_int(en.w) : -1;
+            const int triRc = tk.rcTriBase >= 0 ? tk.rcTriBase + rfacet : -1;
+            occ = wave_anyhit<FEW>(bvh, vc, stk, pair, o, uv, tk.tmin, rr - 1e-3f, tk.errFlag, triEm, triRc, ldsTris);
+        }
         unsigned long long word = ballot64(pair && !occ);
         if (lane == 0) tk.mask[(size_t)rb * tk.em.count + e] = word;
     }
@@ -512,6 +568,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
     __shared__ float4 sEmAll[WAVES][128];
     __shared__ float4 sTri[4 * kBruteTris];  // few occluders: their records, staged once per block
     __shared__ float4 sPreAll[WAVES][PRE ? kPreTris * 64 : 1];  // the staged emitters' origin terms
+    __shared__ unsigned sBitsAll[WAVES][PRE ? 64 : 1];           // ... and plane-side bits
     const bool brute = bvh.ntri > 0 && bvh.ntri <= kBruteTris;
     if (brute)
         for (int q = threadIdx.x; q < 4 * bvh.ntri; q += 64 * WAVES) sTri[q] = bvh.tris[q];
@@ -526,7 +583,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
     int rb = (int)(gw / nEmChunks), ec = (int)(gw % nEmChunks);
     if (compact) upper_wave(gw, nrb, nEmChunks, f, rb, ec);
     if (PRE)
-        vis_wave<true, true>(tk, bvh, vc, rb, ec, emPerWave, nullptr, sTri, sEmAll[wave], sPreAll[wave]);
+        vis_wave<true, true>(tk, bvh, vc, rb, ec, emPerWave, nullptr, sTri, sEmAll[wave], sPreAll[wave],
+                             sBitsAll[wave]);
     else if (brute)
         vis_wave<true>(tk, bvh, vc, rb, ec, emPerWave, stkAll[wave], sTri, sEmAll[wave]);
     else
