@@ -177,11 +177,14 @@ int sr_transfer_mode(void);
 int sr_set_transfer_splits(int n);
 int sr_transfer_splits(void);
 /* How the transfer matrix S is built (no counterpart in the reference, whose
- * pair kernel has no stored matrix). 1 (default): where the rows and emitters
- * are padded alike and no near-field pairs are listed, only the 64 x 64 blocks
- * on and above the diagonal are evaluated and each is also stored transposed
- * (S is symmetric bit for bit); elsewhere, and with 0, every block is
- * evaluated. Both give the same matrix; a changed value rebuilds it. */
+ * pair kernel has no stored matrix). 1: where the rows and emitters are padded
+ * alike and no near-field pairs are listed, only the 64 x 64 blocks on and
+ * above the diagonal are evaluated and each is also stored transposed (S is
+ * symmetric bit for bit); elsewhere, and with 0, every block is evaluated.
+ * 2 (default): as 1, and in a scene of at most 4 occluder triangles that
+ * build evaluates the object's own fragment-pair visibility itself (the same
+ * FP32 test as the visibility words, which are then not built for it). All
+ * give the same matrix; a changed value rebuilds it. */
 int sr_set_transfer_symmetric(int on);
 int sr_transfer_symmetric(void);
 

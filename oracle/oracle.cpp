@@ -463,6 +463,17 @@ void orc_near_pair(double kr, double atten, int nsub, float delta, float Ae, con
     out2[1] = r.imag();
 }
 
+// The oracle's cosine cull of count pairs outside any scene: kept[i] = 1 when
+// the cosine of v_i (the difference vector) with unit normal n_i, formed as the
+// sums form it (ray_dir, dotp: OptiX.cu:326-333), is not below 1e-6.
+void orc_cosine_kept(const float *v, const float *n, long count, unsigned char *kept) {
+    for (long i = 0; i < count; i++) {
+        float r; V3 uv;
+        ray_dir(V3{v[3 * i], v[3 * i + 1], v[3 * i + 2]}, r, uv);
+        kept[i] = dotp(uv, V3{n[3 * i], n[3 * i + 1], n[3 * i + 2]}) < 1e-6f ? 0 : 1;
+    }
+}
+
 void orc_set_near_quadrature(void *h, int nsub, float radius) {
     ((Scene *)h)->nearSub = nsub;
     ((Scene *)h)->nearRadius = radius;

@@ -52,6 +52,7 @@ for name, args, res in [
     ("orc_set_near_quadrature", [_v, _i, _f], None),
     ("orc_set_source_object_amplitude", [_v, _i, ctypes.c_double, ctypes.c_double], None),
     ("orc_near_pair", [ctypes.c_double, ctypes.c_double, _i, _f, _f, _fp, _fp, _fp, _fp, _fp, _fp, _dp], None),
+    ("orc_cosine_kept", [_fp, _fp, ctypes.c_long, _u8p], None),
 ]:
     fn = getattr(_lib, name)
     fn.argtypes = args
@@ -70,6 +71,15 @@ def near_pair(kr, atten, nsub, delta, area, pe, xa, ya, na, nb, pr):
                        None if nbp is None else nbp.ctypes.data_as(_fp), prp.ctypes.data_as(_fp),
                        out.ctypes.data_as(_dp))
     return complex(out[0], out[1])
+
+
+def cosine_kept(v, n):
+    """The oracle's cosine cull of pairs (difference vectors v, unit normals n, FP32):
+    True where the pair is kept (cos formed by ray_dir and dotp is not below 1e-6)."""
+    v, n = _f32(v).reshape(-1, 3), _f32(n).reshape(-1, 3)
+    out = np.zeros(len(v), np.uint8)
+    _lib.orc_cosine_kept(v.ctypes.data_as(_fp), n.ctypes.data_as(_fp), len(v), out.ctypes.data_as(_u8p))
+    return out.astype(bool)
 
 
 def set_threads(n):

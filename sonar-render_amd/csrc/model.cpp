@@ -1081,8 +1081,11 @@ static int ensure_transfer(Model &m, int i) {
         ph.mlp = nf.P;
     }
     const bool sym = m.nearMode == 0 && m.transferSymmetric && t.nSym > 0;
-    uint64_t *mk;
-    if (get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk, sym)) return -1;
+    // few occluders: the symmetric build evaluates the SELF visibility itself
+    const bool fused = sym && m.transferSymmetric == 2 && m.ntri >= 1 && m.ntri <= 4;
+    const SelfVis sv{m.trisSorted, m.ntri, o.triBase, m.vc.scale, m.vc.padTri, m.vc.tminFrag};
+    uint64_t *mk = nullptr;
+    if (!fused && get_mask(m, MK_SELF, OBJ_TARGET, i, OBJ_TARGET, i, &mk, sym)) return -1;
     int counts[2] = {0, 0};
     const double key[5] = {(double)m.nearMode, (double)m.nearRadiusWl, (double)m.nearQuadDeltas, (double)m.kf,
                            (double)o.delta};
@@ -1103,7 +1106,7 @@ static int ensure_transfer(Model &m, int i) {
             // pair evaluations executed: every kept block, or those on and above the diagonal
             const double evals = sym ? 4096.0 * t.nSym : t.aBytesRead / 1024.0 * 256.0;
             ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, evals);
-            if (sym ? transfer_build_s_sym(g, ph, mk, t.A, t.symItems, t.nSym, m.stream)
+            if (sym ? transfer_build_s_sym(g, ph, mk, t.A, t.symItems, t.nSym, m.stream, fused ? &sv : nullptr)
                     : transfer_build_s(g, ph, mk, t.A, m.stream))
                 return -1;
         }
@@ -1443,7 +1446,10 @@ int model_set_transfer_splits(Model &m, int n) {
 }
 
 int model_set_transfer_symmetric(Model &m, int on) {
-    if (on != 0 && on != 1) { set_error("transfer symmetric build: 0 every block, 1 half and transposes"); return -2; }
+    if (on < 0 || on > 2) {
+        set_error("transfer symmetric build: 0 every block, 1 half and transposes, 2 also the visibility inside");
+        return -2;
+    }
     if (on != m.transferSymmetric)
         for (auto &o : m.targets) o.tc.valid = false;  // built the other way: rebuild
     m.transferSymmetric = on;
@@ -1544,7 +1550,7 @@ void model_teardown(Model &m) {
     m.masks.clear();
     m.transferMode = 0;
     m.transferSplits = 0;
-    m.transferSymmetric = 1;
+    m.transferSymmetric = 2;
     m.haveLastGemm = false;
     m.nearMode = 0;
     m.nearQuadDeltas = kNfRMinDeltas;

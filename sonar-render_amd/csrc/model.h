@@ -135,7 +135,9 @@ struct Model {
     std::map<std::tuple<int, int, int, int, int>, MaskBuf> masks;
     int transferMode = 0;  // 0 plain FP32 gathers, 1 FP16 MFMA transfer matrix
     int transferSplits = 0;  // splits of the transfer product: 0 automatic
-    int transferSymmetric = 1;  // S build: 1 half the blocks and their transposes where possible, 0 every block
+    // S build: 0 every block; 1 half the blocks and their transposes where possible;
+    // 2 as 1, and with at most 4 occluder triangles the SELF visibility evaluated in that build
+    int transferSymmetric = 2;
     int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
     int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
     int gatedCap = 0;

@@ -102,11 +102,23 @@ struct ReduceGateTask {
     unsigned *colMaxNext;
 };
 
+// The visibility test of an object's own fragment pairs, for a scene of at most
+// 4 occluder triangles, evaluated inside the symmetric S build instead of read
+// from the SELF visibility words (the same bits: the visibility kernels' test on
+// the same FP32 arithmetic).
+struct SelfVis {
+    const float4 *tris;  // the scene's occluder records (Bvh::tris), ntri x 4 float4
+    int ntri;            // 1 .. 4
+    int triBase;         // occluder index of this object's facet 0
+    float scale, padTri, tmin;  // VisConst::scale, padTri, tminFrag
+};
+
 int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st);
 // The same S from the blocks on and above the diagonal (items: (row block, K chunk)
 // of 64), each stored with its transpose; rows and emitters padded alike, near mode 0.
+// sv: evaluate the visibility here (mask unused); null: read it from mask.
 int transfer_build_s_sym(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A,
-                         const int2 *items, int nItems, hipStream_t st);
+                         const int2 *items, int nItems, hipStream_t st, const SelfVis *sv = nullptr);
 // Writes the listed near pairs' values into S (counts read back from nearCount)
 // and clears nearCount for the next build.
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st);

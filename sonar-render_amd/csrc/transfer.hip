@@ -30,6 +30,7 @@
 #include "sums.h"
 #include "pair_kernel.h"
 #include "transfer.h"
+#include "transfer_dev.h"
 #include "learned.h"
 #include "near_rules.h"
 #include "near_eval.h"
@@ -37,14 +38,6 @@
 namespace sr {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-
-__device__ inline uint32_t pack_h2(float a, float b) {
-    // saturate to the FP16 range (near-field terms can exceed the |S| bound)
-    a = fminf(fmaxf(a, -65000.f), 65000.f);
-    b = fminf(fmaxf(b, -65000.f), 65000.f);
-    _Float16 ha = (_Float16)a, hb = (_Float16)b;
-    return (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
-}
 
 // ---------------------------------------------------------------------------
 // Near-field values (per unit emitter area) for a pair closer than nearRadius:
@@ -66,11 +59,6 @@ __device__ inline NearGeom near_geom(const TransferGeom &g, const TransferPhys &
                        rn.x * en.x + rn.y * en.y + rn.z * en.z);
     q.r = sqrtf(q.rv.x * q.rv.x + q.rv.y * q.rv.y + q.rv.z * q.rv.z);
     return q;
-}
-
-// The emitter (active cell) at K position k, -1 for a padding position.
-__device__ inline int emitter_at(const TransferGeom &g, int k) {
-    return g.rowAct ? g.rowAct[k] : (k < g.n ? k : -1);
 }
 
 // The FP16 word of the pair (row, K position j) in the tiled transfer matrix.
@@ -111,287 +99,6 @@ __global__ __launch_bounds__(256) void k_near(TransferGeom g, TransferPhys ph, i
     } else {
         near_mlp_blocks(TransferNearSrc{g, ph, ph.nearM, A}, nMlp, ph.mlp, ph.deltaReg, ph.kr, ph.alpha,
                         blockIdx.x - quadBlocks, gridDim.x - quadBlocks);
-    }
-}
-
-// Near pairs are staged per wave in LDS and flushed to the global lists with
-// one atomic per batch (called by the whole wave; counts are wave-uniform).
-constexpr int kNearStage = 256;
-
-// entries: this wave's LDS staging area; n: its fill (wave-uniform register)
-__device__ inline void near_flush(int2 *entries, int &n, int2 *list, int *count, int cap) {
-    if (!n) return;
-    int base = 0;
-    if (__lane_id() == 0) base = atomicAdd(count, n);
-    base = __shfl(base, 0);
-    for (int k = __lane_id(); k < n; k += 64)
-        if (base + k < cap) list[base + k] = entries[k];
-    n = 0;
-}
-
-__device__ inline void near_stage(int2 *entries, int &n, bool pred, int2 *list, int *count, int cap, int row, int j) {
-    const uint64_t m = __ballot(pred);
-    if (!m) return;
-    const int cnt = __popcll(m);
-    if (n + cnt > kNearStage) near_flush(entries, n, list, count, cap);
-    if (pred) entries[n + __popcll(m & ((1ull << __lane_id()) - 1ull))] = make_int2(row, j);
-    n += cnt;
-}
-
-// Non-temporal 16-byte accesses (streamed past the caches), for an S larger
-// than every cache level: written once per build and read once per bounce.
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-__device__ inline void store_nt(uint4 *p, uint4 v) {
-    __builtin_nontemporal_store(u32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4 *>(p));
-}
-__device__ inline uint4 load_nt(const uint4 *p) {
-    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
-    return make_uint4(v.x, v.y, v.z, v.w);
-}
-
-// ---------------------------------------------------------------------------
-// S build: a wave = 64 consecutive rows of S (two 32-row tiles), one lane per
-// row, all 8 emitters of each K step. The step's emitter records and
-// visibility words are then the same for every lane: the wave stages them in
-// LDS once and reads each with a broadcast. Each lane stores its row's two
-// fragments (emitters 0-3, 4-7) into the MFMA A layout of its tile. Near pairs
-// (modes 1, 2) are left zero and listed for the near-field kernels.
-
-struct EmRec {
-    float4 p, n;      // point (area), normal (facet id bits)
-    uint64_t w0, w1;  // visibility words of the wave's two receiver blocks
-};
-constexpr int kBuildEm = 16 * 8;  // emitters per wave: at most 16 K steps
-
-// The FP16 word of the pair (receiver rp, rn; emitter ep, en; vis: visible),
-// and whether it is a near pair (its word left zero, listed for the near-field
-// kernels) and which rule takes it. Every lane evaluates the far-field form and
-// selects (the excluded cases' values, even non-finite ones at r = 0, are never
-// selected). Exchanging receiver and emitter negates the difference vector
-// exactly, which leaves r2 and r as they are and turns the two cosine dot
-// products into each other's exact negations: the word of (j, i) is the word of
-// (i, j), bit for bit.
-template <bool ATT, int NEAR>
-__device__ __forceinline__ uint32_t pair_word(const TransferPhys &ph, const float4 &rp, const float4 &rn,
-                                              const float4 &ep, const float4 &en, bool vis, bool &isNear,
-                                              bool &isQuad) {
-    const float vx = rp.x - ep.x, vy = rp.y - ep.y, vz = rp.z - ep.z;
-    const float r2 = vx * vx + vy * vy + vz * vz;
-    const float rinv = __builtin_amdgcn_rsqf(r2);
-    const float r = r2 * rinv;
-    const float de = vx * en.x + vy * en.y + vz * en.z;
-    const float dr = vx * rn.x + vy * rn.y + vz * rn.z;
-    const float wgt = -(de * dr) * (rinv * rinv) * ph.damp * ph.scale;
-    // FP64 reduction of k r to one revolution, then FP32 sin/cos.
-    const double rev = (double)r * ph.krevD;
-    const float x = (float)(rev - rint(rev));
-    const float c = __builtin_amdgcn_cosf(x), sn = __builtin_amdgcn_sinf(x);
-    float a = rinv * kInv4PiT;
-    if (ATT) a *= __builtin_amdgcn_exp2f(-ph.alphaLog2e * r);
-    float Gre, Gim;
-    if (ATT) {
-        Gre = a * fmaf(ph.alpha, c, ph.kr * sn) + rinv * rinv;
-        Gim = -a * fmaf(ph.kr, c, -ph.alpha * sn);
-    } else {
-        Gre = (a * ph.kr) * sn + rinv * rinv;
-        Gim = -(a * ph.kr) * c;
-    }
-    // near pairs: evaluated afterwards from the near lists (k_near)
-    isNear = NEAR != 0 && vis && r < ph.nearRadius;
-    // the learned range starts at r >= delta
-    isQuad = isNear && (NEAR == 1 || r < ph.quadRadius);
-    const bool reg = r < ph.deltaReg;
-    float re = reg ? wgt * (-ph.alpha * kInv4PiT) : wgt * Gre;
-    float im = reg ? wgt * (ph.kr * kInv4PiT) : wgt * Gim;
-    const bool keep = vis && !isNear;
-    re = keep ? re : 0.f;
-    im = keep ? im : 0.f;
-    return pack_h2(re, im);
-}
-
-template <bool ATT, int NEAR>
-__global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph, const uint64_t *__restrict__ mask,
-                                                 uint4 *__restrict__ A, int ksPerWave) {
-    __shared__ int2 stage[NEAR != 0 ? 4 : 1][2][NEAR != 0 ? kNearStage : 1];
-    __shared__ EmRec sEmAll[4][kBuildEm];
-    int2 *stQ = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][0], *stM = stage[NEAR != 0 ? (threadIdx.x >> 6) : 0][1];
-    int nQ = 0, nM = 0;
-    // wave-uniform by construction; readfirstlane lets the compiler see it
-    const int wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
-    const int nKsChunks = (g.nKs + ksPerWave - 1) / ksPerWave;
-    const int rp64 = wave / nKsChunks;  // 64-row block of S
-    const int kc = wave % nKsChunks;
-    if (rp64 >= g.nRt / 2) return;  // uniform (nRt is a multiple of 4)
-    const int srow = rp64 * 64 + lane;  // row of S
-    const int row = g.rowAct ? g.rowAct[srow] : srow;  // its active cell (-1: padding)
-    const bool rvalid = row >= 0 && row < g.n;
-    const float4 rp = rvalid ? g.p[row] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 rn = rvalid ? g.nrm[row] : make_float4(0.f, 0.f, 0.f, 0.f);
-    // The wave's rows are consecutive active cells (one facet's, or the identity
-    // map), so their visibility words come from two 64-row blocks at most: b0
-    // (lane 0's, or the first valid lane's) and b0 + 1.
-    const int ks0 = kc * ksPerWave, ks1 = min(g.nKs, ks0 + ksPerWave);
-    const int2 skip = g.wgSkip ? g.wgSkip[rp64 >> 1] : make_int2(0, 0);  // never read by the GEMM
-    const int tile = srow >> 5, tl = srow & 31;
-    const uint64_t valid = __ballot(rvalid);
-    if (!valid) {  // padding rows only: zero fragments
-        for (int ks = ks0; ks < ks1; ks++) {
-            if (ks >= skip.x && ks < skip.y) continue;
-            uint4 *dst = A + ((size_t)tile * g.nKs + ks) * 64 + tl;
-            dst[0] = make_uint4(0u, 0u, 0u, 0u);
-            dst[32] = make_uint4(0u, 0u, 0u, 0u);
-        }
-        return;
-    }
-    const int first = __builtin_amdgcn_readfirstlane(__ffsll((long long)valid) - 1);
-    const int b0 = __builtin_amdgcn_readfirstlane(__shfl(row, first) >> 6);
-    const int nrbM = (g.n + 63) >> 6;
-    const int b1 = min(b0 + 1, nrbM - 1);
-    const bool upper = rvalid && (row >> 6) != b0;  // this lane's words are block b0 + 1's
-    const int bit = row & 63;
-    // This wave's emitters (at most kBuildEm: its K steps' 8 each) staged in LDS
-    // with coalesced loads, one per lane; every lane then reads each emitter's
-    // record at one address (a broadcast). An emitter past the last is the last
-    // one again (never selected). The wave alone writes and then reads its slots,
-    // and its LDS accesses run in order.
-    EmRec *sEm = sEmAll[threadIdx.x >> 6];
-    for (int u = lane; u < (ks1 - ks0) * 8; u += 64) {
-        const int jj = emitter_at(g, ks0 * 8 + u);
-        const int jc = jj >= 0 ? jj : 0;
-        sEm[u].p = g.p[jc];
-        sEm[u].n = g.nrm[jc];
-        // a padding position: no visibility (never selected)
-        sEm[u].w0 = jj >= 0 ? mask[(size_t)b0 * g.n + jj] : 0ull;
-        sEm[u].w1 = jj >= 0 ? mask[(size_t)b1 * g.n + jj] : 0ull;
-    }
-    for (int ks = ks0; ks < ks1; ks++) {
-        if (ks >= skip.x && ks < skip.y) continue;  // uniform
-        uint32_t w[8];
-        unsigned nearBits = 0, quadBits = 0;  // per emitter: listed near pair, by quadrature
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int jl = ks * 8 + q - ks0 * 8;  // the emitter's slot in this wave's staged records
-            const uint64_t word = upper ? sEm[jl].w1 : sEm[jl].w0;
-            bool isNear, isQuad;
-            w[q] = pair_word<ATT, NEAR>(ph, rp, rn, sEm[jl].p, sEm[jl].n, rvalid && ((word >> bit) & 1ull), isNear,
-                                        isQuad);
-            nearBits |= (unsigned)isNear << q;
-            quadBits |= (unsigned)isQuad << q;
-        }
-        // near pairs are rare: one ballot decides whether this step lists any
-        if (NEAR != 0 && __ballot(nearBits != 0u)) {
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int j = emitter_at(g, ks * 8 + q);  // listed as active cells
-                const bool nr = (nearBits >> q) & 1u, qd = (quadBits >> q) & 1u;
-                near_stage(stQ, nQ, nr && qd, ph.nearQ, ph.nearCount, ph.nearCap, row, j);
-                near_stage(stM, nM, nr && !qd, ph.nearM, ph.nearCount + 1, ph.nearCap, row, j);
-            }
-        }
-        // fragment h of row tl of the tile: emitters 4h .. 4h + 3
-        uint4 *dst = A + ((size_t)tile * g.nKs + ks) * 64 + tl;
-        if (ph.streamS) {
-            store_nt(dst, make_uint4(w[0], w[1], w[2], w[3]));
-            store_nt(dst + 32, make_uint4(w[4], w[5], w[6], w[7]));
-        } else {
-            dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-            dst[32] = make_uint4(w[4], w[5], w[6], w[7]);
-        }
-    }
-    if (NEAR != 0) {
-        near_flush(stQ, nQ, ph.nearQ, ph.nearCount, ph.nearCap);
-        near_flush(stM, nM, ph.nearM, ph.nearCount + 1, ph.nearCap);
-    }
-}
-
-// Symmetric S build (rows and emitters padded alike, no near pairs listed). S is
-// symmetric bit for bit (pair_word; the visibility words are symmetric), so a
-// wave evaluates one 64 x 64 block, rows of block rb by K chunk cb with rb <= cb,
-// stores it, and stores its transpose as block (cb, rb): after each K step the
-// lanes' 8 words go through the wave's LDS tile and come back as transposed rows
-// (16-byte pieces of 4 consecutive K positions; 8 lanes store one row's 128
-// contiguous bytes of a K step). Items: the (rb, cb) pairs whose chunk is not
-// skipped, listed on the host. A block with no valid row or emitter is stored
-// as zeros without evaluating it.
-constexpr int kSymT = 72;  // LDS tile row stride in words: 8 transposed rows of 64 K positions, padded
-template <bool ATT>
-__global__ __launch_bounds__(256) void k_build_s_sym(TransferGeom g, TransferPhys ph,
-                                                     const uint64_t *__restrict__ mask, uint4 *__restrict__ A,
-                                                     const int2 *__restrict__ items, int nItems) {
-    __shared__ EmRec sEmAll[4][64];
-    __shared__ uint4 sTAll[4][8 * kSymT / 4];
-    const int item = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (item >= nItems) return;  // uniform
-    const int lane = threadIdx.x & 63;
-    const int2 bc = items[item];
-    const int rb = __builtin_amdgcn_readfirstlane(bc.x), cb = __builtin_amdgcn_readfirstlane(bc.y);
-    const int srow = rb * 64 + lane;
-    const int row = g.rowAct[srow];  // this lane's receiver (-1: padding)
-    const int em = g.rowAct[cb * 64 + lane];  // this lane's staged emitter (-1: padding)
-    const bool rvalid = row >= 0;
-    const float4 rp = rvalid ? g.p[row] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 rn = rvalid ? g.nrm[row] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint64_t valid = __ballot(rvalid);
-    const bool live = valid != 0ull && __ballot(em >= 0) != 0ull;  // uniform
-    // The rows are consecutive active cells of one facet: their visibility words
-    // come from two 64-row blocks at most (as in k_build_s).
-    const int first = live ? __builtin_amdgcn_readfirstlane(__ffsll((long long)valid) - 1) : 0;
-    const int b0 = live ? __builtin_amdgcn_readfirstlane(__shfl(row, first) >> 6) : 0;
-    const int b1 = min(b0 + 1, ((g.n + 63) >> 6) - 1);
-    const bool upper = rvalid && (row >> 6) != b0;
-    const int bit = row & 63;
-    EmRec *sEm = sEmAll[threadIdx.x >> 6];
-    uint32_t *sT = reinterpret_cast<uint32_t *>(sTAll[threadIdx.x >> 6]);
-    if (live) {
-        const int ec = em >= 0 ? em : 0;
-        sEm[lane].p = g.p[ec];
-        sEm[lane].n = g.nrm[ec];
-        sEm[lane].w0 = em >= 0 ? mask[(size_t)b0 * g.n + em] : 0ull;
-        sEm[lane].w1 = em >= 0 ? mask[(size_t)b1 * g.n + em] : 0ull;
-    }
-    const bool mirror = rb != cb;
-    // this lane's two transposed pieces per K step: row q of the step's 8, K positions
-    // 8 s2 + 4 h .. + 3 of block rb
-    const int q0 = lane & 7, h0 = (lane >> 3) & 1, s0 = lane >> 4;  // piece lane; the other is s0 + 4
-    for (int ks = 0; ks < 8; ks++) {
-        uint32_t w[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            w[q] = 0u;
-            if (live) {
-                const int jl = ks * 8 + q;
-                const uint64_t word = upper ? sEm[jl].w1 : sEm[jl].w0;
-                bool isNear, isQuad;
-                w[q] = pair_word<ATT, 0>(ph, rp, rn, sEm[jl].p, sEm[jl].n, rvalid && ((word >> bit) & 1ull), isNear,
-                                         isQuad);
-            }
-        }
-        uint4 *dst = A + ((size_t)(srow >> 5) * g.nKs + cb * 8 + ks) * 64 + (srow & 31);
-        const uint4 lo = make_uint4(w[0], w[1], w[2], w[3]), hi = make_uint4(w[4], w[5], w[6], w[7]);
-        if (ph.streamS) {
-            store_nt(dst, lo);
-            store_nt(dst + 32, hi);
-        } else {
-            dst[0] = lo;
-            dst[32] = hi;
-        }
-        if (!mirror) continue;  // uniform
-        // The wave's LDS accesses run in order: the last step's reads of the tile
-        // precede these writes, which precede the reads below.
-#pragma unroll
-        for (int q = 0; q < 8; q++) sT[q * kSymT + lane] = w[q];
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int s2 = s0 + 4 * u;
-            const uint4 v = *reinterpret_cast<const uint4 *>(sT + q0 * kSymT + s2 * 8 + h0 * 4);
-            const int trow = cb * 64 + ks * 8 + q0;  // row of S: the emitter
-            uint4 *td = A + ((size_t)(trow >> 5) * g.nKs + rb * 8 + s2) * 64 + (trow & 31) + 32 * h0;
-            if (ph.streamS) store_nt(td, v);
-            else *td = v;
-        }
-        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -752,44 +459,6 @@ __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
 
 
 // ---------------------------------------------------------------------------
-
-int transfer_build_s(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A, hipStream_t st) {
-    // runs of 16 K steps per wave (64 rows each), shorter when that leaves fewer than ~8k waves
-    if (g.nRt % 4) { set_error("S build: row tiles must be padded to 4"); return -2; }
-    int ksPerWave = kBuildEm / 8;  // the kernel stages that many K steps' emitters per wave
-    while (ksPerWave > 4 && (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave) < 8192) ksPerWave /= 2;
-    const long waves = (long)(g.nRt / 2) * ((g.nKs + ksPerWave - 1) / ksPerWave);
-    const long blocks = (waves + 3) / 4;
-    if (blocks <= 0) return 0;
-    const bool att = ph.alpha != 0.0f;
-    if (ph.nearMode == 1) {
-        if (att) k_build_s<true, 1><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-        else k_build_s<false, 1><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    } else if (ph.nearMode == 2) {
-        if (att) k_build_s<true, 2><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-        else k_build_s<false, 2><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    } else if (att) {
-        k_build_s<true, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    } else {
-        k_build_s<false, 0><<<(unsigned)blocks, 256, 0, st>>>(g, ph, mask, A, ksPerWave);
-    }
-    SR_HIP(hipGetLastError());
-    return 0;
-}
-
-int transfer_build_s_sym(const TransferGeom &g, const TransferPhys &ph, const uint64_t *mask, uint4 *A,
-                         const int2 *items, int nItems, hipStream_t st) {
-    if (!g.rowAct || ph.nearMode != 0) {
-        set_error("symmetric S build: needs rows and emitters padded alike and no near pairs");
-        return -2;
-    }
-    if (nItems <= 0) return 0;
-    const unsigned blocks = (unsigned)((nItems + 3) / 4);
-    if (ph.alpha != 0.0f) k_build_s_sym<true><<<blocks, 256, 0, st>>>(g, ph, mask, A, items, nItems);
-    else k_build_s_sym<false><<<blocks, 256, 0, st>>>(g, ph, mask, A, items, nItems);
-    SR_HIP(hipGetLastError());
-    return 0;
-}
 
 int transfer_near(const TransferGeom &g, const TransferPhys &ph, int nQuad, int nMlp, uint4 *A, hipStream_t st) {
     if (nQuad > ph.nearCap || nMlp > ph.nearCap) { set_error("near lists overflow"); return -2; }

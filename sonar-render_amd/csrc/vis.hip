@@ -284,7 +284,6 @@ __global__ void k_pack_nodes(const float4 *__restrict__ tris, int n, const int *
 
 constexpr int kStack = 128;
 constexpr int kBruteTris = 8;
-constexpr int kPreTris = 4;  // few enough occluders for their origin terms to be kept per ray origin
 
 __device__ inline unsigned long long ballot64(bool p) { return __ballot(p); }
 
@@ -295,51 +294,6 @@ __device__ inline int uniform_int(int v) { return __builtin_amdgcn_readfirstlane
 __device__ inline float4 staged_origin_terms(const float4 *ldsTris, int ti, int ntri, F3 o) {
     if (ti >= ntri) return make_float4(0.f, 0.f, 0.f, 0.f);
     return origin_terms(tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], ldsTris[4 * ti + 3]), o);
-}
-
-// Plane-side rule for few occluders. With N = e1 x e2 and side(p) = N.p - N.v0
-// (FP32, in this order), a ray whose two end points both lie beyond
-// m = 1e-4 scale |e1|_1 |e2|_1 on one side of the triangle's plane is never
-// accepted by the FP32 triangle test (mt_hit), so the test can be left out. The
-// test's t = (e2.q)/det estimates N.(v0 - o)/(N.d): with both ends on one side
-// beyond m, t lies outside (0, L) by a distance that needs errors of order m in
-// e2.q or m/L in det to close, while each is off by a few ulps of
-// scale |e1| |e2| (|o - v0| <= 2 sqrt(3) scale, |d| = 1), a factor of about 30
-// below m; the side values themselves are off by less still. The rule is
-// checked against the oracle's FP32 test on random, grazing and sliver cases on
-// the CPU.
-__device__ inline float plane_margin(const TriRec &T, float scale) {
-    const float l1 = (fabsf(T.e1.x) + fabsf(T.e1.y)) + fabsf(T.e1.z);
-    const float l2 = (fabsf(T.e2.x) + fabsf(T.e2.y)) + fabsf(T.e2.z);
-    return ((1e-4f * scale) * l1) * l2;
-}
-
-__device__ inline float plane_side(const TriRec &T, F3 p) {
-    const F3 n = f3cross(T.e1, T.e2);
-    return f3dot(n, p) - f3dot(n, T.v0);
-}
-
-// Per end point and triangle ti < ntri (at most kPreTris): bit ti: beyond the
-// margin above the plane, bit 4 + ti: below, bit 8 + ti: the triangle is the one
-// the point lies on (own: its index, -1 none). A pair's candidate triangles are
-// those with neither a shared side bit nor an own bit of either end.
-__device__ inline unsigned plane_bits(const float4 *ldsTris, int ntri, float scale, F3 p, int own) {
-    unsigned b = 0;
-#pragma unroll
-    for (int ti = 0; ti < kPreTris; ti++) {
-        if (ti >= ntri) break;
-        const float4 q3 = ldsTris[4 * ti + 3];
-        const TriRec T = tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], q3);
-        const float m = plane_margin(T, scale), sd = plane_side(T, p);
-        b |= ((unsigned)(sd > m) << ti) | ((unsigned)(sd < -m) << (ti + 4)) |
-             ((unsigned)(__float_as_int(q3.w) == own) << (ti + 8));
-    }
-    return b;
-}
-
-__device__ inline unsigned pair_candidates(unsigned bE, unsigned bR, int ntri) {
-    const unsigned same = bE & bR;
-    return ~(same | (same >> 4) | ((bE | bR) >> 8)) & ((1u << ntri) - 1u);
 }
 
 // wave_anyhit<true> for at most kPreTris occluders: the lane's candidate

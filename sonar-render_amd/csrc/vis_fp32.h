@@ -184,4 +184,74 @@ __host__ __device__ inline bool slab_hit(F3 bmin, F3 bmax, float pad, F3 o, F3 i
     return tn <= tf && tf >= tmin - pad && tn <= tmax + pad;
 }
 
+constexpr int kPreTris = 4;  // few enough occluders for per-point plane-side bits and origin terms
+
+// Plane-side rule for few occluders. With N = e1 x e2 and side(p) = N.p - N.v0
+// (FP32, in this order), a ray whose two end points both lie beyond
+// m = 1e-4 scale |e1|_1 |e2|_1 on one side of the triangle's plane is never
+// accepted by the FP32 triangle test (mt_hit), so the test can be left out. The
+// test's t = (e2.q)/det estimates N.(v0 - o)/(N.d): with both ends on one side
+// beyond m, t lies outside (0, L) by a distance that needs errors of order m in
+// e2.q or m/L in det to close, while each is off by a few ulps of
+// scale |e1| |e2| (|o - v0| <= 2 sqrt(3) scale, |d| = 1), a factor of about 30
+// below m; the side values themselves are off by less still. The rule is
+// checked against the oracle's FP32 test on random, grazing and sliver cases on
+// the CPU.
+__device__ inline float plane_margin(const TriRec &T, float scale) {
+    const float l1 = (fabsf(T.e1.x) + fabsf(T.e1.y)) + fabsf(T.e1.z);
+    const float l2 = (fabsf(T.e2.x) + fabsf(T.e2.y)) + fabsf(T.e2.z);
+    return ((1e-4f * scale) * l1) * l2;
+}
+
+__device__ inline float plane_side(const TriRec &T, F3 p) {
+    const F3 n = f3cross(T.e1, T.e2);
+    return f3dot(n, p) - f3dot(n, T.v0);
+}
+
+// Per end point and triangle ti < ntri (at most kPreTris): bit ti: beyond the
+// margin above the plane, bit 4 + ti: below, bit 8 + ti: the triangle is the one
+// the point lies on (own: its index, -1 none). A pair's candidate triangles are
+// those with neither a shared side bit nor an own bit of either end.
+__device__ inline unsigned plane_bits(const float4 *ldsTris, int ntri, float scale, F3 p, int own) {
+    unsigned b = 0;
+#pragma unroll
+    for (int ti = 0; ti < kPreTris; ti++) {
+        if (ti >= ntri) break;
+        const float4 q3 = ldsTris[4 * ti + 3];
+        const TriRec T = tri_unpack(ldsTris[4 * ti], ldsTris[4 * ti + 1], ldsTris[4 * ti + 2], q3);
+        const float m = plane_margin(T, scale), sd = plane_side(T, p);
+        b |= ((unsigned)(sd > m) << ti) | ((unsigned)(sd < -m) << (ti + 4)) |
+             ((unsigned)(__float_as_int(q3.w) == own) << (ti + 8));
+    }
+    return b;
+}
+
+__device__ inline unsigned pair_candidates(unsigned bE, unsigned bR, int ntri) {
+    const unsigned same = bE & bR;
+    return ~(same | (same >> 4) | ((bE | bR) >> 8)) & ((1u << ntri) - 1u);
+}
+
+// Cosine cull decided without the ray set-up where rounding cannot change it.
+// The cull keeps a pair when fl(uv . n) >= 1e-6 with uv = v * (1 / sqrtf(v . v))
+// (ray_setup_dev, then f3dot; the receiver's cull is the same with -v). Given
+// x = v . n and r2 = v . v evaluated in FP32 (any order, fused or not), with
+// s = |v| and c = (v . n) / s exact, and |n| <= 1 + 2^-20:
+//   |x - v . n| <= 3.01 u s            (u = 2^-24; a three-term dot product)
+//   |fl(uv . n) - c| <= 7.6 u          (sqrt, reciprocal and the three products
+//                                       of uv: 4.5 u each term, then the dot)
+// so for r2 in [2^-60, 2^60] (no underflow in x^2 or K r2, no overflow):
+//   x <= 0                   -> c <= 3.01 u, fl(uv . n) <= 10.7 u < 1e-6: culled;
+//   x^2 < 1e-14 r2           -> c < 1e-7 + 3.01 u, fl(uv . n) < 7.3e-7: culled;
+//   x > 0, x^2 > 9e-12 r2    -> c > 3e-6 (1 - 3u) - 3.01 u, fl(uv . n) > 2.3e-6: kept.
+// Returns 1 kept, 0 culled, 2 undecided (cos within about [1e-7, 3e-6], or r2
+// out of range): only those need the exact set-up.
+__host__ __device__ inline int cull_margin(float x, float r2) {
+    if (!(r2 >= 0x1p-60f && r2 <= 0x1p60f)) return 2;
+    if (!(x > 0.0f)) return 0;
+    const float x2 = x * x;
+    if (x2 > 9e-12f * r2) return 1;
+    if (x2 < 1e-14f * r2) return 0;
+    return 2;
+}
+
 }  // namespace sr

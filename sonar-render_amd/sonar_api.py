@@ -399,8 +399,10 @@ _sig("sr_transfer_symmetric", [])
 
 
 def set_transfer_symmetric(on):
-    """1: S from the blocks on and above the diagonal and their transposes where the
-    layout allows (default); 0: every block evaluated. The same matrix either way."""
+    """2 (default): S from the blocks on and above the diagonal and their transposes
+    where the layout allows, with the object's own visibility evaluated in that build
+    when the scene has at most 4 occluder triangles; 1: the same, visibility always
+    from the SELF visibility words; 0: every block evaluated. The same matrix every way."""
     _check(cpp_lib.sr_set_transfer_symmetric(int(on)), "set_transfer_symmetric")
 
 

@@ -65,36 +65,77 @@ def _symmetric_cases():
                 points=scenes.octant_incidences(2, 15.0), bounces=3, batched=True)
     return [("trihedral 20 kHz", scenes.config_trihedral(f=20e3, n_side=2, bounces=3)),
             ("trihedral 12 kHz, attenuation 0.1", att),
-            ("trihedral of 48 facets of 342 fragments (each padded to 384), 20 kHz", runs)]
+            ("trihedral of 48 facets of 342 fragments (each padded to 384), 20 kHz", runs),
+            ("three stacked plates, the middle one shadowing, 10 kHz", _stacked_plates())]
 
 
-@pytest.mark.parametrize("case", range(3))
+def _stacked_plates():
+    """Three parallel plates (one triangle each): the lower one (z = 0, normal +z)
+    and the upper one (z = 1, normal -z) face each other; the middle one
+    (z = 0.5, normal +z, 0.4 m legs) faces the upper one and shadows part of
+    the lower-upper pairs, whose rays cross its plane."""
+    tris = np.array([[[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]],
+                     [[0.1, 0.1, 0.5], [0.5, 0.1, 0.5], [0.1, 0.5, 0.5]],
+                     [[0.0, 0.0, 1.0], [0.0, 1.0, 1.0], [1.0, 0.0, 1.0]]], np.float32)
+    # low elevations from the sides, between the plates: they light the lower
+    # plate (and the middle one, from above its plane) past the upper one
+    src = np.array([[-15.0, 0.3, 0.6], [0.4, -15.0, 0.3], [15.0, 0.2, 0.8]], np.float32)
+    return dict(name="stacked_plates", f=10e3, c=1480.0, atten=0.0, targets=[(tris, 1480 / 10e3 / 7.5)], fields=[],
+                sources=src, points=src, bounces=3, batched=True)
+
+
+@pytest.mark.parametrize("case", range(4))
 def test_symmetric_s_build_matches_full(api, case):
     """S is symmetric bit for bit: exchanging receiver and emitter negates the
     difference vector exactly, so r and the two cosine products come out the
     same, and the visibility words are symmetric. The build that evaluates only
     the 64 x 64 blocks on and above the diagonal and stores each one transposed
     as well must then give the same pressures, bit for bit, as evaluating every
-    block, with about half the pair evaluations. The cases cover the
-    attenuating pair kernel and many facets per plate, each padded (K chunks
-    holding padding rows and emitters, coplanar facets' chunks skipped)."""
+    block, with about half the pair evaluations, and so must that build with
+    the object's own visibility evaluated inside it (mode 2, scenes of at most 4
+    occluder triangles) rather than read from the visibility words. The cases
+    cover the attenuating pair kernel, many facets per plate, each padded (K
+    chunks holding padding rows and emitters, coplanar facets' chunks skipped;
+    48 triangles: no visibility inside the build), and an occluder whose plane
+    separates the two ends of many pairs (their triangle tests run inside the
+    build)."""
     name, sc = _symmetric_cases()[case]
     outs, evals = {}, {}
-    for sym in (1, 0):
-        _banner(f"transfer form, {name}, {'symmetric' if sym else 'full'} S build")
+    for sym in (2, 1, 0):
+        _banner(f"transfer form, {name}, S build mode {sym}")
         runner.load_scene(sc, transfer_mode=1)
         api.set_transfer_symmetric(sym)
         api.profile(True)
         runner.project_scene(sc)
         evals[sym] = api.kernel_stats("build_s")["flops"]
+        vis = api.kernel_stats("vis")["pairs"]
         api.profile(False)
         outs[sym] = runner.read_results(sc)
         api.TearDownCuda()
-    print("pair evaluations: symmetric", evals[1], "full", evals[0])
+        print("mode", sym, "pair evaluations", evals[sym], "visibility pairs", vis)
     assert 0 < evals[1] < 0.55 * evals[0]
-    a, b = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]]) for o in (outs[1], outs[0]))
+    assert evals[2] == evals[1]
+    a, b, c = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]])
+               for o in (outs[2], outs[1], outs[0]))
     assert np.isfinite(a).all() and np.abs(a).max() > 0
-    assert a.tobytes() == b.tobytes()
+    assert b.tobytes() == c.tobytes()
+    assert a.tobytes() == c.tobytes()
+
+
+def test_stacked_plates_shadowing_vs_oracle(api):
+    """The scene whose S build tests an occluder inside it, against the FP64
+    oracle (brute-force triangle tests) on every incidence."""
+    sc = _stacked_plates()
+    _banner("transfer form, three stacked plates, visibility inside the S build")
+    out = runner.run_scene(sc, transfer_mode=1)
+    worst = 0.0
+    for p in range(len(sc["sources"])):
+        ref = oracle.run_scene(sc, plane_source=p, bruteforce=True)
+        e = rel_l2(np.concatenate([out["points"][p], out["targets"][0][p]]),
+                   np.concatenate([ref["points"], ref["targets"][0]]))
+        print("plane", p, "rel-L2", e)
+        worst = max(worst, e)
+    assert worst < TOL_MIXED
 
 
 def test_self_words_filled_when_needed_after_symmetric_build(api):
