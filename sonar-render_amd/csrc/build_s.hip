@@ -221,17 +221,22 @@ __global__ __launch_bounds__(256) void k_build_s(TransferGeom g, TransferPhys ph
 // Visibility of a SELF pair inside the symmetric build (FUSED): the test the
 // visibility kernels apply to an object's own fragment pairs (vis.hip,
 // ORIGIN_LOWER_FACET, both cosine culls, at most kPreTris occluders), on the
-// same FP32 arithmetic, so each bit is the visibility word's bit.
-//   Occluders, first, per 64 x 64 block: from the plane bits of its rows and of
-//   its emitters (plane_bits), the special rows and emitters (special_points):
-//   every pair with a candidate triangle (pair_candidates) has a special end,
-//   so the wave tests the pairs of those rows and emitters alone,
-//   one row (or emitter) at a time across its 64 lanes, with the exact ray
-//   set-up and the FP32 triangle test (mt_hit_dev, the ray leaving the lower
-//   facet's end), and keeps one bit per occluded pair in LDS.
-//   Culls, then, per pair: cull_margin on de and -dr decides every pair whose
-//   cosines are not within about [1e-7, 3e-6]; the undecided ones (a ballot)
-//   take the visibility kernels' exact set-up and comparisons.
+// same FP32 arithmetic, so each bit is the visibility word's bit. Per 64 x 64
+// block (rows on one facet, emitters on another), before the block's pairs are
+// evaluated:
+//   Culls (block_culls): with one normal per side, the rows that lie well in
+//   front of the emitters' plane keep every pair by the emitter's cull, those
+//   behind it cull every pair, and likewise the emitters against the rows'
+//   plane; the rows and emitters in between are special.
+//   Occluders (special_points): from the plane bits of the rows and emitters
+//   (plane_bits), the rows and emitters a candidate triangle (pair_candidates)
+//   can involve are special too; no other pair has one.
+//   The pairs of the special rows and emitters are then decided exactly, one
+//   row (or emitter) at a time across the wave's 64 lanes (special_vis): both
+//   culls by cull_margin or, near the threshold, the visibility kernels' exact
+//   set-up and comparisons; candidate triangles by the exact ray and the FP32
+//   triangle test (mt_hit_dev, the ray leaving the lower facet's end). Their
+//   bits go to LDS; every other pair's bit is known from its row and emitter.
 
 __device__ __forceinline__ bool pair_culls(const PairGeom &q, const float4 &rp, const float4 &rn, const float4 &ep,
                                            const float4 &en, bool pairOk) {
@@ -272,16 +277,30 @@ __device__ inline bool pair_occluded(unsigned cand, const float4 *tris, const Se
     return hit;
 }
 
+// The whole visibility bit of one pair, exactly (the special pairs).
+__device__ inline bool pair_vis_exact(const SelfVis &sv, const float4 *tris, const float4 &rp, const float4 &rn,
+                                      unsigned bR, bool rv, const float4 &ep, const float4 &en, unsigned bE, bool ev) {
+    const int rf = __float_as_int(rn.w), ef = __float_as_int(en.w);
+    const bool pairOk = rv && ev && ef != rf;
+    bool vis = pair_culls(pair_geom(rp, rn, ep, en), rp, rn, ep, en, pairOk);
+    const unsigned cand = vis ? pair_candidates(bE, bR, sv.ntri) : 0u;
+    if (__ballot(cand != 0u) != 0ull) {
+        bool hit = false;
+        if (cand != 0u) hit = pair_occluded(cand, tris, sv, ep, rp, ef < rf);
+        vis = vis && !hit;
+    }
+    return vis;
+}
+
 // The block's points that can have a candidate triangle: per triangle, the side
 // (beyond the margin) that more of the block's points not lying on it are on;
 // a point is special when, for some triangle it does not lie on, it is not
 // beyond the margin on that side. Two points that are not special have no
 // candidate triangle (each triangle is one's own, or both are beyond the margin
-// on its majority side), so every pair with one has a special end. Ballots of
-// the special rows (bR, rv) and of the special emitters (bE, ev).
-__device__ inline void special_points(unsigned bR, bool rv, unsigned bE, bool ev, int ntri, uint64_t &rows,
-                                      uint64_t &ems) {
-    bool specR = false, specE = false;
+// on its majority side), so every pair with one has a special end.
+__device__ inline void special_points(unsigned bR, bool rv, unsigned bE, bool ev, int ntri, bool &specR,
+                                      bool &specE) {
+    specR = specE = false;
     for (int ti = 0; ti < ntri; ti++) {
         const bool nownR = rv && !((bR >> (ti + 8)) & 1u), nownE = ev && !((bE >> (ti + 8)) & 1u);
         const int nAbove = __popcll(__ballot(nownR && ((bR >> ti) & 1u))) + __popcll(__ballot(nownE && ((bE >> ti) & 1u)));
@@ -291,50 +310,84 @@ __device__ inline void special_points(unsigned bR, bool rv, unsigned bE, bool ev
         specR = specR || (nownR && !((bR >> side) & 1u));
         specE = specE || (nownE && !((bE >> side) & 1u));
     }
-    rows = __ballot(specR);
-    ems = __ballot(specE);
 }
 
-// Occlusion bits of one 64 x 64 block: bit r of the returned word of lane e says
-// the pair (row r, staged emitter e) is shadowed (the pairs of the special rows
-// and emitters tested, no other pair has a candidate triangle). Lanes hold their row (rp, rf,
-// bR, rv) and their staged emitter (ep, ef, bE, ev); sEm holds the staged
-// emitters' records.
-__device__ inline uint64_t block_occlusion(const SelfVis &sv, const float4 *tris, const EmRec *sEm, const float4 &rp,
-                                           int rf, unsigned bR, bool rv, const float4 &ep, int ef, unsigned bE,
-                                           bool ev) {
+__device__ inline float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// The cosine culls of a block by row and by emitter, when all its rows share one
+// normal nR and all its emitters another, nE. With a = nE . rp per row and
+// b = nE . ep per emitter (FP32, each within E = 4e-7 scale of the exact dot
+// product: |p| <= sqrt(3) scale), a pair's nE . (rp - ep) lies within
+// [a - max b - 2E, a - min b + 2E], and |rp - ep| <= D (the largest distances
+// of the rows and of the emitters from one row point, added). A row whose lower
+// end exceeds 3.5e-6 D keeps every pair by the emitter's cull (every exact
+// cosine, its set-up within 7.6 u of the true one, cull_margin, is above
+// 2.9e-6); a row whose upper end is below zero culls every pair. Likewise each
+// emitter with nR . (ep - rp) for the receiver's cull. Per lane: rowKept /
+// rowCulled for its row, emKept / emCulled for its staged emitter; with
+// differing normals every valid point is neither (special).
+__device__ inline void block_culls(const float4 &rp, const float4 &rn, bool rv, const float4 &ep, const float4 &en,
+                                   bool ev, float scale, bool &rowKept, bool &rowCulled, bool &emKept,
+                                   bool &emCulled) {
+    rowKept = rowCulled = emKept = emCulled = false;
+    const int fr = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(rv)) - 1);
+    const int fe = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(ev)) - 1);
+    const F3 nR{__shfl(rn.x, fr), __shfl(rn.y, fr), __shfl(rn.z, fr)};
+    const F3 nE{__shfl(en.x, fe), __shfl(en.y, fe), __shfl(en.z, fe)};
+    const bool sameR = !rv || (rn.x == nR.x && rn.y == nR.y && rn.z == nR.z);
+    const bool sameE = !ev || (en.x == nE.x && en.y == nE.y && en.z == nE.z);
+    if (__ballot(!(sameR && sameE)) != 0ull) return;
+    const F3 r3{rp.x, rp.y, rp.z}, e3{ep.x, ep.y, ep.z};
+    const F3 p0{__shfl(rp.x, fr), __shfl(rp.y, fr), __shfl(rp.z, fr)};
+    const float inf = __builtin_inff();
+    const float dR = rv ? sqrtf(f3dot(f3sub(r3, p0), f3sub(r3, p0))) : 0.f;
+    const float dE = ev ? sqrtf(f3dot(f3sub(e3, p0), f3sub(e3, p0))) : 0.f;
+    const float D = (wave_max(dR) + wave_max(dE)) * 1.0001f;
+    const float E2 = 8e-7f * scale, thr = 3.5e-6f * D;
+    const float a = f3dot(nE, r3), b = f3dot(nE, e3);  // emitter's cull: nE . (rp - ep)
+    const float c = f3dot(nR, e3), d = f3dot(nR, r3);  // receiver's cull: nR . (ep - rp)
+    const float minB = wave_min(ev ? b : inf), maxB = wave_max(ev ? b : -inf);
+    const float minD = wave_min(rv ? d : inf), maxD = wave_max(rv ? d : -inf);
+    rowCulled = rv && (a - minB) + E2 < 0.f;
+    rowKept = rv && (a - maxB) - E2 > thr;
+    emCulled = ev && (c - minD) + E2 < 0.f;
+    emKept = ev && (c - maxD) - E2 > thr;
+}
+
+// The special pairs' bits: lane e's word, bit r = visibility of (row r, staged
+// emitter e), for every row r in specRows and every pair of the emitters in
+// specEms (other bits zero). Lanes hold their row (rp, rn, bR, rv) and their
+// staged emitter (ep, en, bE, ev); sEm holds the staged emitters' records.
+__device__ inline uint64_t special_vis(const SelfVis &sv, const float4 *tris, const EmRec *sEm, uint64_t specRows,
+                                       uint64_t specEms, const float4 &rp, const float4 &rn, unsigned bR, bool rv,
+                                       const float4 &ep, const float4 &en, unsigned bE, bool ev) {
     const int lane = __lane_id();
-    uint64_t occ = 0;
-    uint64_t rowsC, emsC;
-    special_points(bR, rv, bE, ev, sv.ntri, rowsC, emsC);
-    // the candidate rows' pairs: lane e takes the pair (row r, emitter e)
-    for (uint64_t m = rowsC; m; m &= m - 1) {
+    uint64_t vis = 0;
+    // the special rows: lane e takes the pair (row r, emitter e)
+    for (uint64_t m = specRows; m; m &= m - 1) {
         const int r = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
         const float4 rpR = make_float4(__shfl(rp.x, r), __shfl(rp.y, r), __shfl(rp.z, r), 0.f);
-        const int rfR = __shfl(rf, r);
-        const unsigned bRr = __shfl(bR, r);
-        const bool fromEm = ef < rfR;
-        const unsigned cand = ev ? pair_candidates(bE, bRr, sv.ntri) : 0u;
-        if (__ballot(cand != 0u) == 0ull) continue;
-        bool hit = false;
-        if (cand != 0u) hit = pair_occluded(cand, tris, sv, ep, rpR, fromEm);
-        occ |= (uint64_t)hit << r;
+        const float4 rnR = make_float4(__shfl(rn.x, r), __shfl(rn.y, r), __shfl(rn.z, r), __shfl(rn.w, r));
+        const bool v = pair_vis_exact(sv, tris, rpR, rnR, __shfl(bR, r), true, ep, en, bE, ev);
+        vis |= (uint64_t)v << r;
     }
-    // the candidate emitters' pairs: lane r takes the pair (row r, emitter e)
-    for (uint64_t m = emsC; m; m &= m - 1) {
+    // the special emitters: lane r takes the pair (row r, emitter e)
+    for (uint64_t m = specEms; m; m &= m - 1) {
         const int e = __builtin_amdgcn_readfirstlane(__ffsll((long long)m) - 1);
-        const float4 epE = sEm[e].p;
-        const int efE = __float_as_int(sEm[e].n.w);
-        const unsigned bEe = (unsigned)sEm[e].w0;
-        const bool fromEm = efE < rf;
-        const unsigned cand = rv ? pair_candidates(bEe, bR, sv.ntri) : 0u;
-        if (__ballot(cand != 0u) == 0ull) continue;
-        bool hit = false;
-        if (cand != 0u) hit = pair_occluded(cand, tris, sv, epE, rp, fromEm);
-        const uint64_t hm = __ballot(hit);
-        if (lane == e) occ |= hm;
+        const bool v = pair_vis_exact(sv, tris, rp, rn, bR, rv, sEm[e].p, sEm[e].n, (unsigned)sEm[e].w0, true);
+        const uint64_t vm = __ballot(v);
+        if (lane == e) vis |= vm;
     }
-    return occ;
+    return vis;
 }
 
 // Symmetric S build (rows and emitters padded alike, no near pairs listed). S is
@@ -346,7 +399,8 @@ __device__ inline uint64_t block_occlusion(const SelfVis &sv, const float4 *tris
 // contiguous bytes of a K step). Items: the (rb, cb) pairs whose chunk is not
 // skipped, listed on the host. A block with no valid row or emitter is stored
 // as zeros without evaluating it. FUSED: the visibility bits are evaluated here
-// (block_occlusion, pair_culls) instead of read from the SELF visibility words.
+// (block_culls, special_points, special_vis) instead of read from the SELF
+// visibility words.
 constexpr int kSymT = 72;  // LDS tile row stride in words: 8 transposed rows of 64 K positions, padded
 template <bool ATT, bool FUSED>
 __global__ __launch_bounds__(256) void k_build_s_sym(TransferGeom g, TransferPhys ph,
@@ -355,7 +409,7 @@ __global__ __launch_bounds__(256) void k_build_s_sym(TransferGeom g, TransferPhy
     __shared__ EmRec sEmAll[4][64];
     __shared__ uint4 sTAll[4][8 * kSymT / 4];
     __shared__ float4 sTri[FUSED ? 4 * kPreTris : 1];
-    __shared__ uint64_t sOccAll[4][FUSED ? 64 : 1];
+    __shared__ uint64_t sVisAll[4][FUSED ? 64 : 1];
     if (FUSED) {
         for (int q = threadIdx.x; q < 4 * sv.ntri; q += 256) sTri[q] = sv.tris[q];
         __syncthreads();
@@ -382,31 +436,47 @@ __global__ __launch_bounds__(256) void k_build_s_sym(TransferGeom g, TransferPhy
     const int bit = row & 63;
     EmRec *sEm = sEmAll[threadIdx.x >> 6];
     uint32_t *sT = reinterpret_cast<uint32_t *>(sTAll[threadIdx.x >> 6]);
-    uint64_t *sOcc = sOccAll[threadIdx.x >> 6];
-    bool anyOcc = false;
+    uint64_t *sVis = sVisAll[threadIdx.x >> 6];
+    // FUSED: this lane's row is special or keeps its non-special pairs; per
+    // staged emitter (uniform masks): special, keeps its non-special pairs
+    bool rowSpec = false, rowOk = false;
+    uint64_t emSpec = 0, emOk = 0;
     if (live) {
         const int ec = em >= 0 ? em : 0;
         const float4 ep = g.p[ec], en = g.nrm[ec];
         sEm[lane].p = ep;
         sEm[lane].n = en;
         if (FUSED) {
+            const bool ev = em >= 0;
             const unsigned bE = plane_bits(sTri, sv.ntri, sv.scale, F3{ep.x, ep.y, ep.z},
                                            sv.triBase + __float_as_int(en.w));
             const unsigned bR = plane_bits(sTri, sv.ntri, sv.scale, F3{rp.x, rp.y, rp.z},
                                            sv.triBase + __float_as_int(rn.w));
             sEm[lane].w0 = bE;
             __builtin_amdgcn_wave_barrier();  // the staged records precede the reads of them
-            const uint64_t occ = block_occlusion(sv, sTri, sEm, rp, __float_as_int(rn.w), bR, rvalid, ep,
-                                                 __float_as_int(en.w), bE, em >= 0);
-            sOcc[lane] = occ;
-            anyOcc = __ballot(occ != 0ull) != 0ull;
+            bool rowKept, rowCulled, emKept, emCulled, occR, occE;
+            block_culls(rp, rn, rvalid, ep, en, ev, sv.scale, rowKept, rowCulled, emKept, emCulled);
+            special_points(bR, rvalid, bE, ev, sv.ntri, occR, occE);
+            // a pair with neither end special: both culls decided by its row and its
+            // emitter, no candidate triangle, and the two ends on different facets
+            // (the rows on one facet, the emitters on another: else every row is special)
+            const int rf = __float_as_int(rn.w), ef = __float_as_int(en.w);
+            const int rf0 = __shfl(rf, first);
+            const int ef0 = __shfl(ef, __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(ev)) - 1));
+            const bool twoFacets = rf0 != ef0 && __ballot(rvalid && rf != rf0) == 0ull &&
+                                   __ballot(ev && ef != ef0) == 0ull;
+            rowSpec = rvalid && (!twoFacets || (!rowCulled && (!rowKept || occR)));
+            rowOk = rvalid && rowKept && twoFacets;
+            emSpec = __ballot(ev && !emCulled && (!emKept || occE));
+            emOk = __ballot(ev && emKept);
+            const uint64_t rowsM = __ballot(rowSpec);
+            sVis[lane] = (rowsM | emSpec) ? special_vis(sv, sTri, sEm, rowsM, emSpec, rp, rn, bR, rvalid, ep, en, bE, ev)
+                                          : 0ull;
         } else {
             sEm[lane].w0 = em >= 0 ? mask[(size_t)b0 * g.n + em] : 0ull;
             sEm[lane].w1 = em >= 0 ? mask[(size_t)b1 * g.n + em] : 0ull;
         }
     }
-    // FUSED: whether each staged emitter is one (not padding), as bits of this wave
-    const uint64_t emValid = __ballot(em >= 0);
     const bool mirror = rb != cb;
     // this lane's two transposed pieces per K step: row q of the step's 8, K positions
     // 8 s2 + 4 h .. + 3 of block rb
@@ -419,19 +489,16 @@ __global__ __launch_bounds__(256) void k_build_s_sym(TransferGeom g, TransferPhy
             if (live) {
                 const int jl = ks * 8 + q;
                 const float4 ep = sEm[jl].p, en = sEm[jl].n;
-                const PairGeom pg = pair_geom(rp, rn, ep, en);
                 bool vis;
                 if (FUSED) {
-                    const bool pairOk = rvalid && ((emValid >> jl) & 1ull) &&
-                                        __float_as_int(en.w) != __float_as_int(rn.w);
-                    vis = pair_culls(pg, rp, rn, ep, en, pairOk);
-                    if (anyOcc) vis = vis && !((sOcc[jl] >> lane) & 1ull);
+                    const bool spec = rowSpec || ((emSpec >> jl) & 1ull);
+                    vis = spec ? ((sVis[jl] >> lane) & 1ull) : (rowOk && ((emOk >> jl) & 1ull));
                 } else {
                     const uint64_t word = upper ? sEm[jl].w1 : sEm[jl].w0;
                     vis = rvalid && ((word >> bit) & 1ull);
                 }
                 bool isNear, isQuad;
-                w[q] = pair_value<ATT, 0>(ph, pg, vis, isNear, isQuad);
+                w[q] = pair_value<ATT, 0>(ph, pair_geom(rp, rn, ep, en), vis, isNear, isQuad);
             }
         }
         uint4 *dst = A + ((size_t)(srow >> 5) * g.nKs + cb * 8 + ks) * 64 + (srow & 31);
