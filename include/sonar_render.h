@@ -246,6 +246,10 @@ int sr_time_transfer_gemm(int reps, double *msPerLaunch);
  * its own buffer, allocated and freed here) timed over `reps` passes after two
  * warm-up passes; gbs = bytes / mean pass time in GB/s. */
 int sr_measure_stream_read(long bytes, int reps, double *gbs);
+/* The same for writes (no counterpart in the reference): a streaming write of
+ * `bytes` with 16-byte stores, non-temporal (nontemporal = 1, past the caches)
+ * or ordinary (0). */
+int sr_measure_stream_write(long bytes, int reps, int nontemporal, double *gbs);
 
 /* STL meshes (binary or ASCII in, binary out): 9 floats per triangle. */
 long sr_read_stl(const char *path, float *out9, long cap);

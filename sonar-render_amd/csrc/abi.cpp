@@ -339,6 +339,9 @@ int sr_time_transfer_gemm(int reps, double *msPerLaunch) {
 int sr_measure_stream_read(long bytes, int reps, double *gbs) {
     return guarded("measure_stream_read", [&] { return sr::measure_stream_read(bytes, reps, gbs); });
 }
+int sr_measure_stream_write(long bytes, int reps, int nontemporal, double *gbs) {
+    return guarded("measure_stream_write", [&] { return sr::measure_stream_write(bytes, reps, nontemporal, gbs); });
+}
 
 long sr_read_stl(const char *path, float *out, long cap) {
     long rc = -1;

@@ -6,5 +6,7 @@ namespace sr {
 // Attainable HBM read bandwidth: a plain streaming read of `bytes` (>= 1 MiB) timed
 // over `reps` passes after two warm-up passes; gbs = bytes / mean pass time in GB/s.
 int measure_stream_read(long bytes, int reps, double *gbs);
+// Attainable HBM write bandwidth: 16-byte stores, non-temporal or ordinary.
+int measure_stream_write(long bytes, int reps, int nontemporal, double *gbs);
 
 }  // namespace sr

@@ -1,6 +1,7 @@
-// Attainable HBM read bandwidth on this GPU: a plain streaming read (16-byte
-// loads, grid-stride over the buffer, one XOR per load so nothing is dropped),
-// the ceiling against which the memory-bound kernels' rates are read.
+// Attainable HBM bandwidth on this GPU: a plain streaming read (16-byte loads,
+// grid-stride over the buffer, one XOR per load so nothing is dropped) and a
+// plain streaming write, the ceilings against which the memory-bound kernels'
+// rates are read.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,10 +21,24 @@ __global__ __launch_bounds__(256) void k_stream_read(const uint4 *__restrict__ s
     out[(long)blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
-// Reads `bytes` (rounded down to 16) `reps` times after two warm-up passes on its
+// A plain streaming write: 16-byte stores, grid-stride, non-temporal (past the
+// caches, as the S build writes a matrix larger than them) or ordinary.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_write(uint4 *__restrict__ dst, long n, unsigned v) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned w = v ^ (unsigned)i;
+        if (NT) __builtin_nontemporal_store(u4{w, w, w, w}, reinterpret_cast<u4 *>(dst + i));
+        else dst[i] = make_uint4(w, w, w, w);
+    }
+}
+
+// Streams `bytes` (rounded down to 16) `reps` times after two warm-up passes on its
 // own stream and buffers (allocated and freed here); gbs = bytes / mean time.
-int measure_stream_read(long bytes, int reps, double *gbs) {
-    if (bytes < (1L << 20) || reps < 1 || !gbs) { set_error("stream read: at least 1 MiB and one repetition"); return -2; }
+// kind 0: read, 1: non-temporal write, 2: ordinary write.
+static int measure_stream(int kind, long bytes, int reps, double *gbs) {
+    if (bytes < (1L << 20) || reps < 1 || !gbs) { set_error("stream measurement: at least 1 MiB and one repetition"); return -2; }
     const long n = bytes / 16;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
@@ -32,7 +47,11 @@ int measure_stream_read(long bytes, int reps, double *gbs) {
     unsigned *out = nullptr;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = 0;
+    auto pass = [&]() {
+        if (kind == 0) k_stream_read<<<grid, 256, 0, st>>>(buf, n, out);
+        else if (kind == 1) k_stream_write<true><<<grid, 256, 0, st>>>(buf, n, 0x3c003c00u);
+        else k_stream_write<false><<<grid, 256, 0, st>>>(buf, n, 0x3c003c00u);
+    };
     auto run = [&]() -> int {
         SR_HIP(hipMalloc((void **)&buf, (size_t)n * 16));
         SR_HIP(hipMalloc((void **)&out, (size_t)grid * 256 * sizeof(unsigned)));
@@ -40,10 +59,10 @@ int measure_stream_read(long bytes, int reps, double *gbs) {
         SR_HIP(hipEventCreate(&e0));
         SR_HIP(hipEventCreate(&e1));
         SR_HIP(hipMemsetAsync(buf, 1, (size_t)n * 16, st));
-        for (int r = 0; r < 2; r++) k_stream_read<<<grid, 256, 0, st>>>(buf, n, out);
+        for (int r = 0; r < 2; r++) pass();
         SR_HIP(hipGetLastError());
         SR_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; r++) k_stream_read<<<grid, 256, 0, st>>>(buf, n, out);
+        for (int r = 0; r < reps; r++) pass();
         SR_HIP(hipGetLastError());
         SR_HIP(hipEventRecord(e1, st));
         SR_HIP(hipEventSynchronize(e1));
@@ -52,13 +71,19 @@ int measure_stream_read(long bytes, int reps, double *gbs) {
         *gbs = (double)n * 16.0 / ((double)ms / reps * 1e-3) / 1e9;
         return 0;
     };
-    rc = run();
+    const int rc = run();
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
     if (out) (void)hipFree(out);
     if (buf) (void)hipFree(buf);
     return rc;
+}
+
+int measure_stream_read(long bytes, int reps, double *gbs) { return measure_stream(0, bytes, reps, gbs); }
+
+int measure_stream_write(long bytes, int reps, int nontemporal, double *gbs) {
+    return measure_stream(nontemporal ? 1 : 2, bytes, reps, gbs);
 }
 
 }  // namespace sr

@@ -982,15 +982,19 @@ static int ensure_transfer(Model &m, int i) {
         t.aBytesRead = steps * 64 * 16;  // A bytes one GEMM pass reads
         // Split each group's non-zero chunks evenly (split_walk), at least 4 chunks a
         // split. Time in units of one workgroup streaming one chunk (32 KiB of S) while
-        // the chip is full; the product holds two workgroups per CU (64 KiB of LDS
-        // each), so the grid runs in rounds of 2 x CUs workgroups. Estimate, per split
-        // count: the rounds x (the most chunks a split walks + ~2 chunks of start-up and
-        // its 64 KiB partial-sum store), or the chip's bytes (every chunk, and each
-        // split's partials written and read back by the reduction) at full rate,
-        // whichever is longer; take the least.
+        // the chip is full. One workgroup per CU already streams S at the rate two
+        // do (measured: 2 splits of the lambda/7.5 trihedral's 126 groups, 252
+        // workgroups, 0.139 ms per product; 4 splits 0.147 ms), so the grid runs in
+        // rounds of one workgroup per CU. Estimate, per split count: the rounds x (the
+        // most chunks a split walks + ~2 chunks of start-up and its 64 KiB partial-sum
+        // store), or the chip's bytes (every chunk, and each split's partials written
+        // and read back by the reduction) at full rate, whichever is longer; take the
+        // least. When B exceeds an XCD's L2, only split counts that divide 8 or that 8
+        // divides: the XCDs then each run one split of the groups they hold.
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
-        const long slots = 2L * cus;
+        const long slots = cus;
+        const bool bigB = (double)nKs * 4096.0 > 4.0 * 1024 * 1024;
         long maxChunks = 0, sumChunks = 0;
         for (int w = 0; w < nWg; w++) {
             const long c = (nKs - (skip[w].y - skip[w].x)) / 8;
@@ -1000,6 +1004,7 @@ static int ensure_transfer(Model &m, int i) {
         int nsplit = 1;
         double best = 1e300;
         for (int sp = 1; sp <= std::max(1L, maxChunks / 4); sp++) {
+            if (bigB && 8 % sp != 0 && sp % 8 != 0) continue;
             const double rounds = (double)((nWg * (long)sp + slots - 1) / slots) * ((maxChunks + sp - 1) / sp + 2);
             const double bytes = ((double)sumChunks + 4.0 * nWg * sp) / slots;
             const double est = std::max(rounds, bytes);

@@ -566,26 +566,39 @@ __global__ void k_any_zero(const float2 *v, long n, int *flag) {
 
 __global__ void k_reduce(ReduceTask t) { reduce_elem(t, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 
-// One wave per field-point value: lanes take every 64th split, then a fixed
-// butterfly adds the 64 lane sums (the order depends only on nsplit).
-__global__ __launch_bounds__(256) void k_reduce_points(const float2 *hi, const float2 *lo, int nsplit, int nbStride,
-                                                       int nrhs, int nq, double2 *fld, long fldStride) {
-    const long i = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (i >= (long)nrhs * nq) return;  // wave-uniform
-    const int b = (int)(i / nq), q = (int)(i % nq);
+// A workgroup per 64 consecutive field points of one plane (a lane each, so
+// every load of a split's partial is one coalesced 512 bytes): wave w sums splits
+// w, w + 16, ... in order in FP64, then the 16 wave sums are added in wave order.
+constexpr int kRpWaves = 16;
+__global__ __launch_bounds__(64 * kRpWaves) void k_reduce_points(const float2 *hi, const float2 *lo, int nsplit,
+                                                                 int nbStride, int nq, double2 *fld, long fldStride) {
+    __shared__ double2 part[kRpWaves][64];
+    const int qBlocks = (nq + 63) / 64;
+    const int b = blockIdx.x / qBlocks;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = (blockIdx.x - b * qBlocks) * 64 + lane;
     double sr = 0.0, si = 0.0;
-    for (int s = lane; s < nsplit; s += 64) {
-        const size_t o = ((size_t)s * nbStride + b) * nq + q;
-        sr += (double)hi[o].x + (lo ? (double)lo[o].x : 0.0);
-        si += (double)hi[o].y + (lo ? (double)lo[o].y : 0.0);
+    if (q < nq) {
+#pragma unroll 4
+        for (int s = w; s < nsplit; s += kRpWaves) {
+            const size_t o = ((size_t)s * nbStride + b) * nq + q;
+            const float2 h = hi[o];
+            sr += (double)h.x;
+            si += (double)h.y;
+            if (lo) {
+                const float2 l = lo[o];
+                sr += (double)l.x;
+                si += (double)l.y;
+            }
+        }
     }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        sr += __shfl_xor(sr, m);
-        si += __shfl_xor(si, m);
-    }
-    if (lane == 0) {
+    part[w][lane] = make_double2(sr, si);
+    __syncthreads();
+    if (w == 0 && q < nq) {
+        for (int k = 1; k < kRpWaves; k++) {
+            sr += part[k][lane].x;
+            si += part[k][lane].y;
+        }
         double2 &f = fld[b * fldStride + q];
         f.x += sr;
         f.y += si;
@@ -754,9 +767,9 @@ int reduce_launch(const ReduceTask &t, hipStream_t st) {
 
 int reduce_points_launch(const float2 *hi, const float2 *lo, int nsplit, int nbStride, int nrhs, int nq,
                          double2 *fld, long fldStride, hipStream_t st) {
-    const long n = (long)nrhs * nq;
-    if (n <= 0) return 0;
-    k_reduce_points<<<(unsigned)((n + 3) / 4), 256, 0, st>>>(hi, lo, nsplit, nbStride, nrhs, nq, fld, fldStride);
+    const long blocks = (long)nrhs * ((nq + 63) / 64);
+    if (blocks <= 0) return 0;
+    k_reduce_points<<<(unsigned)blocks, 64 * kRpWaves, 0, st>>>(hi, lo, nsplit, nbStride, nq, fld, fldStride);
     SR_HIP(hipGetLastError());
     return 0;
 }

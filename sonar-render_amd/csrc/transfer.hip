@@ -139,13 +139,18 @@ __device__ inline float col_scale_of(unsigned bits) {
 
 // B build for one group of up to 64 incidences.
 // Scales come from colMax (k_col_max); threads 0..nplanes-1 also store each
-// plane's scale and the inverse the reduction applies (1 / (scale sS)).
+// plane's scale and the inverse the reduction applies (1 / (scale sS)). A
+// workgroup builds one chunk of 8 K steps (64 K positions): it reads the 64
+// planes' inputs along K (coalesced, one plane per wave and pass) into LDS, then
+// writes the chunk's B fragments (one plane per lane) from there.
+constexpr int kBStride = 65;  // LDS row stride (float2) of one plane's 64 K positions, padded
 __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
                                                  long stride, int n, int nKs, int nplanes,
                                                  const unsigned *__restrict__ colMax, float sS,
                                                  float *__restrict__ scaleOut, float *__restrict__ invOut,
                                                  uint4 *__restrict__ B, int *__restrict__ clearState,
                                                  int clearWords, const int *__restrict__ kAct) {
+    __shared__ float2 sX[64 * kBStride];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (clearState && t < clearWords) clearState[t] = 0;
     if (t < nplanes) {
@@ -153,28 +158,44 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
         scaleOut[t] = sc;
         invOut[t] = 1.0f / (sc * sS);
     }
-    if (t >= (long)nKs * 4 * 64) return;
-    const int lane = (int)(t & 63);
-    const int ct = (int)((t >> 6) & 3);
-    const int ks = (int)(t >> 8);
-    const int c = ct * 32 + (lane & 31);
-    const int h = lane >> 5;
-    const int b = c & 63;
-    uint32_t w[4];
+    const int chunk = blockIdx.x;
+    if (chunk >= nKs / 8) return;  // uniform over the workgroup
+    const int kk = threadIdx.x & 63;
+    const int k = chunk * 64 + kk;
+    const int j = kAct ? kAct[k] : (k < n ? k : -1);  // its active cell (-1: padding)
+    const float area = j >= 0 ? p[j].w : 0.f;
+    // all 16 of this thread's loads in flight at once (the workgroup count is small)
+    float2 v[16];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int k = ks * 8 + h * 4 + q;
-        const int j = kAct ? kAct[k] : (k < n ? k : -1);  // its active cell (-1: padding)
+    for (int i = 0; i < 16; i++) {
+        const int b = (threadIdx.x >> 6) + 4 * i;
+        v[i] = (j >= 0 && b < nplanes) ? Pin[b * stride + j] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int b = (threadIdx.x >> 6) + 4 * i;
         float xr = 0.f, xi = 0.f;
         if (j >= 0 && b < nplanes) {
-            float2 v = Pin[b * stride + j];
-            float a = p[j].w * col_scale_of(colMax[b * kColStride]);
-            xr = a * v.x;
-            xi = a * v.y;
+            const float a = area * col_scale_of(colMax[b * kColStride]);
+            xr = a * v[i].x;
+            xi = a * v[i].y;
         }
-        w[q] = (c < 64) ? pack_h2(xr, -xi) : pack_h2(xi, xr);
+        sX[b * kBStride + kk] = make_float2(xr, xi);
     }
-    B[t] = make_uint4(w[0], w[1], w[2], w[3]);
+    __syncthreads();
+    // element idx of the chunk: K step idx / 256, column tile (idx / 64) % 4, lane idx % 64
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int idx = threadIdx.x + 256 * i;
+        const int s = idx >> 8, lane = idx & 63;
+        const int c = ((idx >> 6) & 3) * 32 + (lane & 31);
+        const int b = c & 63;
+        const float2 *x = sX + b * kBStride + s * 8 + (lane >> 5) * 4;
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) w[q] = (c < 64) ? pack_h2(x[q].x, -x[q].y) : pack_h2(x[q].y, x[q].x);
+        B[(size_t)chunk * 2048 + idx] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -212,7 +233,7 @@ __device__ inline void workgroup_barrier() {
 
 // ROLE 0: one accumulator (the kB gate, or none); 2: DUAL. NT: S read non-temporally.
 template <int ROLE, bool NT>
-__global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
+__global__ __launch_bounds__(256, ROLE == 2 ? 1 : 2) void k_transfer_gemm(TransferGemm t) {
     constexpr bool DUAL = ROLE == 2;
     // two separately named buffers, so that the compiler can tell the LDS reads of
     // one from the asynchronous global -> LDS copies into the other
@@ -305,11 +326,10 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
 
     auto load_a = [](const uint4 *p) { return NT ? load_nt(p) : *p; };
     // K step of the split's i-th chunk; past the end the last chunk again, so that
-    // the loop's one-ahead requests are unconditional (and those re-reads hit L2)
+    // the loop's requests ahead are unconditional (and those re-reads hit L2)
     auto kat = [&](int i) { return sw.k(sw.o0 + min(i, nch - 1)); };
     // B goes global -> LDS directly (global_load_lds_dwordx4: lane l of wave w
-    // writes element 64 w + l of a K step); A alternates between two register
-    // sets with the two LDS buffers. Both are requested one chunk ahead.
+    // writes element 64 w + l of a K step) into two buffers used in turn.
     typedef __attribute__((address_space(3))) void lds_void;
     auto stage_b = [&](int k, uint4 *bb) {
 #pragma unroll
@@ -320,39 +340,67 @@ __global__ __launch_bounds__(256) void k_transfer_gemm(TransferGemm t) {
     // s_waitcnt immediate (gfx9 encoding): vector-memory count <= kKC, the export and
     // LDS/scalar counts at their maxima (not waited on)
     constexpr int kWaitAllButNextA = (kKC & 0xf) | ((kKC >> 4) << 14) | (0x7 << 4) | (0xf << 8);
-    uint4 a0[kKC], a1[kKC];
-    // One chunk per phase: the next chunk's A is requested first and stays in
-    // flight across the wait for this chunk's A and B and the barrier; then the
-    // next chunk's B is staged into the other buffer and this chunk's products
-    // run. Every request is unconditional (a phase past the end skips only its
-    // products), which keeps the two A register sets fixed across the loop.
-#define SR_GEMM_PHASE(ACUR, ANEXT, BUF, NBUF, I)                                                     \
+    uint4 a0[kKC], a1[kKC], a2[kKC];
+    // One accumulator set: one chunk per phase, A requested two chunks ahead into
+    // three register sets.
+    // Phase I: wait until this chunk's A and B have landed (the vector-memory
+    // count completes in issue order, and the one request issued after this
+    // chunk's B, the next chunk's A, may stay in flight), barrier (every wave's
+    // part of B in LDS; every wave done reading the other buffer), stage the next
+    // chunk's B into the other buffer, request A two chunks ahead, then run this
+    // chunk's products while two chunks of A are in flight. Every request is
+    // unconditional (past the end the last chunk again), which keeps the register
+    // sets fixed across the loop; the loop stops after the last chunk's phase.
+#define SR_GEMM_PHASE(ACUR, AFAR, BUF, NBUF, I)                                                      \
+    {                                                                                                \
+        const int kc = kat(I), kn = kat((I) + 1), kf = kat((I) + 2);                                 \
+        __builtin_amdgcn_s_waitcnt(kWaitAllButNextA); /* all but the next chunk's A: A, B of kc */  \
+        workgroup_barrier(); /* BUF complete; every wave is done reading NBUF */                     \
+        stage_b(kn, NBUF);                                                                           \
+        _Pragma("unroll") for (int s = 0; s < kKC; s++) AFAR[s] = load_a(Arow + (size_t)(kf + s) * 64); \
+        _Pragma("unroll") for (int s = 0; s < kKC; s++) mfma_step(acc, ACUR[s], BUF, s);            \
+        kLast = kc;                                                                                  \
+        if (kc < kBnd && (I) + 1 < nch && kn >= kBnd) { /* lower-facet part done */                  \
+            store_part(t.partL, acc);                                                                \
+            _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                         \
+        }                                                                                            \
+        if ((I) + 1 >= nch) break;                                                                   \
+    }
+    // DUAL (two accumulator sets): one chunk ahead, two register sets
+#define SR_GEMM_PHASE1(ACUR, ANEXT, BUF, NBUF, I)                                                    \
     {                                                                                                \
         const int kc = kat(I), kn = kat((I) + 1);                                                    \
-        _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = load_a(Arow + (size_t)(kn + s) * 64);  \
+        _Pragma("unroll") for (int s = 0; s < kKC; s++) ANEXT[s] = load_a(Arow + (size_t)(kn + s) * 64); \
         __builtin_amdgcn_s_waitcnt(kWaitAllButNextA); /* all but those kKC loads: A, B of kc */      \
         workgroup_barrier(); /* BUF complete; every wave is done reading NBUF */                     \
         stage_b(kn, NBUF);                                                                           \
-        if ((I) < nch) {                                                                             \
-            if (DUAL) {                                                                              \
-                dual_chunk(ACUR, kc, BUF);                                                           \
-            } else {                                                                                 \
-                _Pragma("unroll") for (int s = 0; s < kKC; s++) mfma_step(acc, ACUR[s], BUF, s);    \
-                kLast = kc;                                                                          \
-                if (kc < kBnd && (I) + 1 < nch && kn >= kBnd) { /* lower-facet part done */          \
-                    store_part(t.partL, acc);                                                        \
-                    _Pragma("unroll") for (int c = 0; c < 4; c++) acc[c] = f32x16{};                 \
-                }                                                                                    \
-            }                                                                                        \
-        }                                                                                            \
+        if ((I) < nch) dual_chunk(ACUR, kc, BUF);                                                    \
     }
-    if (nch > 0) {
+    if (DUAL && nch > 0) {
         stage_b(kat(0), sB0);
 #pragma unroll
         for (int s = 0; s < kKC; s++) a0[s] = load_a(Arow + (size_t)(kat(0) + s) * 64);
         for (int i = 0; i < nch; i += 2) {
-            SR_GEMM_PHASE(a0, a1, sB0, sB1, i)
+            SR_GEMM_PHASE1(a0, a1, sB0, sB1, i)
+            SR_GEMM_PHASE1(a1, a0, sB1, sB0, i + 1)
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+#undef SR_GEMM_PHASE1
+    if (!DUAL && nch > 0) {
+        // issue order A(0), B(0), A(1): the first phase's wait leaves A(1) in flight
+#pragma unroll
+        for (int s = 0; s < kKC; s++) a0[s] = load_a(Arow + (size_t)(kat(0) + s) * 64);
+        stage_b(kat(0), sB0);
+#pragma unroll
+        for (int s = 0; s < kKC; s++) a1[s] = load_a(Arow + (size_t)(kat(1) + s) * 64);
+        for (int i = 0;; i += 6) {
+            SR_GEMM_PHASE(a0, a2, sB0, sB1, i)
             SR_GEMM_PHASE(a1, a0, sB1, sB0, i + 1)
+            SR_GEMM_PHASE(a2, a1, sB0, sB1, i + 2)
+            SR_GEMM_PHASE(a0, a2, sB1, sB0, i + 3)
+            SR_GEMM_PHASE(a1, a0, sB0, sB1, i + 4)
+            SR_GEMM_PHASE(a2, a1, sB1, sB0, i + 5)
         }
         // the last phase's requests (re-reads of the last chunk) land before the
         // workgroup ends: its LDS may go to another workgroup next
@@ -483,10 +531,12 @@ int transfer_col_max(const float4 *p, const float2 *Pin, long stride, int n, int
 int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int nKs, int nplanes,
                      const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st,
                      int *clearState, int clearWords, const int *kAct) {
-    const long tot = std::max({(long)nKs * 256, (long)nplanes, clearState ? (long)clearWords : 0L});
-    if (tot <= 0) return 0;
-    k_build_b<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, colMax, sS, scale,
-                                                             invScale, B, clearState, clearWords, kAct);
+    if (nKs % 8) { set_error("B build: K steps must be padded to whole chunks of 8"); return -2; }
+    const long blocks = std::max({(long)nKs / 8, ((long)nplanes + 255) / 256,
+                                  clearState ? ((long)clearWords + 255) / 256 : 0L});
+    if (blocks <= 0) return 0;
+    k_build_b<<<(unsigned)blocks, 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, colMax, sS, scale, invScale, B,
+                                                clearState, clearWords, kAct);
     SR_HIP(hipGetLastError());
     return 0;
 }

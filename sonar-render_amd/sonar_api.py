@@ -366,6 +366,17 @@ def measure_stream_read(mib=1024, reps=20):
     return gbs.value
 
 
+_sig("sr_measure_stream_write", [_l, _i, _i, ctypes.POINTER(ctypes.c_double)])
+
+
+def measure_stream_write(mib=1024, reps=20, nontemporal=True):
+    """Attainable HBM write bandwidth (GB/s): a plain streaming write of `mib` MiB."""
+    gbs = ctypes.c_double()
+    _check(cpp_lib.sr_measure_stream_write(int(mib) << 20, int(reps), 1 if nontemporal else 0, ctypes.byref(gbs)),
+           "measure_stream_write")
+    return gbs.value
+
+
 _sig("sr_set_transfer_mode", [_i])
 _sig("sr_transfer_mode", [])
 KERNEL_FAMILIES.update({"build_s": 6, "gemm": 7, "build_b": 8, "gated_fix": 9, "near": 10})
