@@ -50,13 +50,18 @@ TOL = 1e-3
 
 # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (calibrated
 # FETCH_SIZE and WRITE_SIZE, see profiles/transfer_gemm_hbm_bytes.md), per workload.
-PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 758.3e6,
-               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 124.5e6}
-# Vector wave-instructions (SQ_INSTS_VALU, exact) of one scene's visibility masks
-# (SELF + sources -> fragments + fragments -> field points), per workload, from
-# profiles/vis_mask_counters.md. The masks are bound by vector issue: a wave64
-# instruction holds a 16-lane SIMD for 4 cycles, 1,024 SIMDs at 2.4 GHz.
-VIS_VALU_INSTR = {"trihedral_1m_20kHz": 235_351_531, "trihedral_1m_20kHz_lambda4": 25_375_463}
+PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 764.5e6,
+               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 95.8e6}
+# Vector wave-instructions (SQ_INSTS_VALU, exact) of one scene's visibility words
+# (k_vis_mask: SELF, sources -> fragments, fragments -> field points; k_mask_transpose),
+# per workload and form (0 plain, 1 transfer: there the S build evaluates the SELF
+# visibility itself), and of one S build, from profiles/vis_mask_counters.md. Both
+# are read against the vector-instruction issue rate measured on the GPU itself
+# (sr_measure_valu_issue); the nominal rule of 4 cycles per wave64 instruction on
+# 1,024 SIMDs at 2.4 GHz (VALU_ISSUE_PEAK) is exceeded by simple integer instructions.
+VIS_VALU_INSTR = {("trihedral_1m_20kHz", 0): 233_059_639, ("trihedral_1m_20kHz", 1): 20_010_079,
+                  ("trihedral_1m_20kHz_lambda4", 1): 34_232_385}
+BUILD_S_VALU_INSTR = {("trihedral_1m_20kHz", 1): 89_202_954, ("trihedral_1m_20kHz_lambda4", 1): 13_602_360}
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9  # G wave-instructions/s
 
 
@@ -169,7 +174,31 @@ def roofline(stats, mode, workload, stream_gbs=None):
                         "PMC-measured bytes per launch over the launch time"}}
 
 
-def vis_roofline(stats, workload, steps):
+def build_s_roofline(stats, workload, mode, steps, write_gbs=None, issue_g=None):
+    st = stats["build_s"]
+    if not st["launches"]:
+        return None
+    n = st["launches"]
+    ms = st["ms"] / n
+    by = st["bytes"] / n
+    out = {"kernel": "k_build_s_sym / k_build_s (the transfer matrix S, FP16 complex; the symmetric build also "
+                     "evaluates the object's own visibility)",
+           "launches_per_scene": n // max(1, steps), "avg_launch_ms": round(ms, 4),
+           "bytes_written_per_launch": round(by),
+           "roofline": {"bound": "hbm write", "achieved": round(by / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "attainable_stream_write_GBs": round(write_gbs, 1) if write_gbs else None,
+                        "frac_of_attainable": round(by / (ms * 1e-3) / 1e9 / write_gbs, 4) if write_gbs else None}}
+    instr = BUILD_S_VALU_INSTR.get((workload, mode))
+    if instr and ms > 0:
+        ach = instr / (ms * 1e-3) / 1e9
+        peak = issue_g or VALU_ISSUE_PEAK
+        out["vector_issue"] = {"achieved": round(ach, 1), "peak": round(peak, 1), "unit": "G wave-instr/s",
+                               "frac": round(ach / peak, 4), "instructions_per_launch": instr}
+    return out
+
+
+def vis_roofline(stats, workload, mode, steps, issue_g=None):
     st = stats["vis"]
     n = max(1, st["launches"])
     per_scene_ms = st["ms"] / max(1, steps)
@@ -178,12 +207,17 @@ def vis_roofline(stats, workload, steps):
            "ms_per_scene": round(per_scene_ms, 4),
            "pairs_per_s_G": round(st["pairs"] / max(st["ms"], 1e-12) / 1e6, 2),
            "words_GBs": round(st["bytes"] / max(st["ms"], 1e-12) / 1e6, 1)}
-    instr = VIS_VALU_INSTR.get(workload)
+    instr = VIS_VALU_INSTR.get((workload, mode))
     if instr and per_scene_ms > 0:
         ach = instr / (per_scene_ms * 1e-3) / 1e9
-        out["roofline"] = {"bound": "valu", "achieved": round(ach, 1), "peak": VALU_ISSUE_PEAK,
-                           "unit": "G wave-instr/s", "frac": round(ach / VALU_ISSUE_PEAK, 4),
-                           "instructions_per_scene": instr}
+        peak = issue_g or VALU_ISSUE_PEAK
+        out["roofline"] = {"bound": "valu", "achieved": round(ach, 1), "peak": round(peak, 1),
+                           "unit": "G wave-instr/s", "frac": round(ach / peak, 4), "instructions_per_scene": instr,
+                           "nominal_4_cycle_rule": VALU_ISSUE_PEAK,
+                           "note": "peak: the vector-instruction issue rate measured on this GPU "
+                                   "(sr_measure_valu_issue: every SIMD full of independent integer vector "
+                                   "instructions); 1,024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction "
+                                   "would be 614.4"}
     return out
 
 
@@ -409,7 +443,10 @@ def main():
     # the attainable read bandwidth of this GPU (a plain streaming read), beside the HBM roofline
     import sonar_api as api
     stream_gbs = api.measure_stream_read(1024, 20)
-    print(f"[bench] plain streaming read: {stream_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    write_gbs = api.measure_stream_write(1024, 20, True)
+    issue_g = api.measure_valu_issue(5)
+    print(f"[bench] plain streaming read: {stream_gbs:.0f} GB/s, write {write_gbs:.0f} GB/s; vector issue "
+          f"{issue_g:.0f} G wave-instr/s", file=sys.stderr, flush=True)
 
     per_form = {}
     for name, r in results.items():
@@ -419,8 +456,11 @@ def main():
             "pairs_per_scene": r["pairs"], "rel_l2_vs_fp64": r.get("err"),
             "kernel_ms_per_scene": {k: round(v["ms"] / args.steps, 4) for k, v in r["stats"].items() if v["launches"]},
             "roofline": roofline(r["stats"], r["mode"], r["scene"]["name"], stream_gbs),
-            "visibility": vis_roofline(r["stats"], r["scene"]["name"], args.steps),
+            "visibility": vis_roofline(r["stats"], r["scene"]["name"], r["mode"], args.steps, issue_g),
         }
+        if r["mode"] == 1:
+            per_form[name]["s_build"] = build_s_roofline(r["stats"], r["scene"]["name"], r["mode"], args.steps,
+                                                         write_gbs, issue_g)
         if "err_surface" in r:
             per_form[name]["rel_l2_surface_vs_fp64"] = r["err_surface"]
     valid = [n for n in per_form if per_form[n]["rel_l2_vs_fp64"] is not None and per_form[n]["rel_l2_vs_fp64"] <= TOL]

@@ -250,6 +250,10 @@ int sr_measure_stream_read(long bytes, int reps, double *gbs);
  * `bytes` with 16-byte stores, non-temporal (nontemporal = 1, past the caches)
  * or ordinary (0). */
 int sr_measure_stream_write(long bytes, int reps, int nontemporal, double *gbs);
+/* The attainable vector-instruction issue rate of this GPU (no counterpart in
+ * the reference): every SIMD full of waves running independent integer vector
+ * instructions, timed over `reps` launches; ginstr in G wave-instructions/s. */
+int sr_measure_valu_issue(int reps, double *ginstr);
 
 /* STL meshes (binary or ASCII in, binary out): 9 floats per triangle. */
 long sr_read_stl(const char *path, float *out9, long cap);

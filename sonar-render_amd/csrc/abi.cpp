@@ -342,6 +342,9 @@ int sr_measure_stream_read(long bytes, int reps, double *gbs) {
 int sr_measure_stream_write(long bytes, int reps, int nontemporal, double *gbs) {
     return guarded("measure_stream_write", [&] { return sr::measure_stream_write(bytes, reps, nontemporal, gbs); });
 }
+int sr_measure_valu_issue(int reps, double *ginstr) {
+    return guarded("measure_valu_issue", [&] { return sr::measure_valu_issue(reps, ginstr); });
+}
 
 long sr_read_stl(const char *path, float *out, long cap) {
     long rc = -1;

@@ -8,5 +8,7 @@ namespace sr {
 int measure_stream_read(long bytes, int reps, double *gbs);
 // Attainable HBM write bandwidth: 16-byte stores, non-temporal or ordinary.
 int measure_stream_write(long bytes, int reps, int nontemporal, double *gbs);
+// Attainable vector-instruction issue rate (G wave-instructions/s) with every SIMD full.
+int measure_valu_issue(int reps, double *ginstr);
 
 }  // namespace sr

@@ -80,6 +80,63 @@ static int measure_stream(int kind, long bytes, int reps, double *gbs) {
     return rc;
 }
 
+// Attainable vector-instruction issue rate: every SIMD full of waves (eight per
+// SIMD), each running 16 independent single-cycle-class integer vector
+// instructions per iteration (eight add chains, eight xor chains; written as
+// inline asm so each is one instruction) around a scalar loop counter.
+constexpr int kIssueIters = 4096;
+constexpr int kIssuePerIter = 16;
+__global__ __launch_bounds__(256) void k_valu_issue(unsigned *__restrict__ out, unsigned seed) {
+    unsigned a[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) a[c] = seed ^ (threadIdx.x * 8u + c);
+    const unsigned b = seed * 3u + 1u;
+    for (int i = 0; i < kIssueIters; i++) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) asm volatile("v_add_u32 %0, %0, %1" : "+v"(a[c]) : "v"(b));
+#pragma unroll
+        for (int c = 0; c < 8; c++) asm volatile("v_xor_b32 %0, %0, %1" : "+v"(a[c]) : "v"(b));
+    }
+    unsigned r = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) r ^= a[c];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = r;
+}
+
+int measure_valu_issue(int reps, double *ginstr) {
+    if (reps < 1 || !ginstr) { set_error("vector issue measurement: at least one repetition"); return -2; }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
+    const int grid = cus * 8;  // 8 workgroups of 4 waves per CU: 8 waves per SIMD
+    unsigned *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto run = [&]() -> int {
+        SR_HIP(hipMalloc((void **)&out, (size_t)grid * 256 * sizeof(unsigned)));
+        SR_HIP(hipStreamCreate(&st));
+        SR_HIP(hipEventCreate(&e0));
+        SR_HIP(hipEventCreate(&e1));
+        k_valu_issue<<<grid, 256, 0, st>>>(out, 1u);
+        SR_HIP(hipGetLastError());
+        SR_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) k_valu_issue<<<grid, 256, 0, st>>>(out, (unsigned)r + 2u);
+        SR_HIP(hipGetLastError());
+        SR_HIP(hipEventRecord(e1, st));
+        SR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double instr = (double)grid * 4 * kIssueIters * kIssuePerIter;  // wave-instructions per launch
+        *ginstr = instr / ((double)ms / reps * 1e-3) / 1e9;
+        return 0;
+    };
+    const int rc = run();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 int measure_stream_read(long bytes, int reps, double *gbs) { return measure_stream(0, bytes, reps, gbs); }
 
 int measure_stream_write(long bytes, int reps, int nontemporal, double *gbs) {

@@ -1109,8 +1109,10 @@ static int ensure_transfer(Model &m, int i) {
         }
         {
             // pair evaluations executed: every kept block, or those on and above the diagonal
+            // pair evaluations executed, and the bytes of S written (every chunk the
+            // product reads, padding included: the build's only large output)
             const double evals = sym ? 4096.0 * t.nSym : t.aBytesRead / 1024.0 * 256.0;
-            ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, evals);
+            ProfScope ps(m, PK_BUILD_S, (double)n * n - o.sameFacetPairs, evals, t.aBytesRead);
             if (sym ? transfer_build_s_sym(g, ph, mk, t.A, t.symItems, t.nSym, m.stream, fused ? &sv : nullptr)
                     : transfer_build_s(g, ph, mk, t.A, m.stream))
                 return -1;

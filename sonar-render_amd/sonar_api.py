@@ -366,6 +366,16 @@ def measure_stream_read(mib=1024, reps=20):
     return gbs.value
 
 
+_sig("sr_measure_valu_issue", [_i, ctypes.POINTER(ctypes.c_double)])
+
+
+def measure_valu_issue(reps=5):
+    """Attainable vector-instruction issue rate (G wave-instructions/s), every SIMD full."""
+    g = ctypes.c_double()
+    _check(cpp_lib.sr_measure_valu_issue(int(reps), ctypes.byref(g)), "measure_valu_issue")
+    return g.value
+
+
 _sig("sr_measure_stream_write", [_l, _i, _i, ctypes.POINTER(ctypes.c_double)])
 
 
