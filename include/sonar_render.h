@@ -188,6 +188,16 @@ int sr_transfer_splits(void);
 int sr_set_transfer_symmetric(int on);
 int sr_transfer_symmetric(void);
 
+/* Point->fragment sums of the single-scene model with one incidence plane per
+ * source (no counterpart in the reference, whose sums read a visibility
+ * buffer filled beforehand: src/OptiX.cu:35-110). 1 (default): in a scene of
+ * at most 4 occluder triangles the sum evaluates each (source, fragment)
+ * pair's visibility itself, the same FP32 test as the sources->fragments
+ * visibility words, which are then not built; 0: always from those words.
+ * Both give the same result bit for bit. Reset to 1 by TearDownCuda. */
+int sr_set_inline_visibility(int on);
+int sr_inline_visibility(void);
+
 /* Plain fragment->fragment sums (transfer mode 0). 0 (default): with 16 or
  * more incidence planes the pair kernel is evaluated in FP32 on the vector
  * units and the complex products accumulate on the FP32 matrix cores, 64

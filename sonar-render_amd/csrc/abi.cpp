@@ -296,6 +296,10 @@ int sr_set_transfer_symmetric(int on) {
     return guarded("set_transfer_symmetric", [&] { return sr::model_set_transfer_symmetric(cur(), on); });
 }
 int sr_transfer_symmetric(void) { return cur().transferSymmetric; }
+int sr_set_inline_visibility(int on) {
+    return guarded("set_inline_visibility", [&] { return sr::model_set_inline_visibility(cur(), on); });
+}
+int sr_inline_visibility(void) { return cur().inlineVis; }
 int sr_set_gather_mode(int mode) {
     return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(cur(), mode); });
 }

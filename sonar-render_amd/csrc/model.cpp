@@ -606,10 +606,15 @@ int model_project_src(Model &m, bool fieldObjects) {
     Phys ph = make_phys(m, 0.f, 1.f);
     auto run = [&](Object &o, int type, int idx) -> int {
         if (o.ncell == 0 || ns == 0) return 0;
-        uint64_t *mk;
-        if (get_mask(m, MK_P2F, 0, 0, type, idx, &mk)) return -1;
+        // few occluders and one plane per source: the sum evaluates its visibility itself
+        const bool inl = m.inlineVis && m.batchedSources && m.ntri >= 1 && m.ntri <= 4;
+        uint64_t *mk = nullptr;
+        if (!inl && get_mask(m, MK_P2F, 0, 0, type, idx, &mk)) return -1;
         P2FTask t{o.cellP, o.cellN, (int)o.ncell, o.type == OBJ_TARGET, m.srcP, m.srcAmp, ns, mk,
                   m.batchedSources ? 1 : 0, o.P, o.ncell};
+        if (inl)
+            t.vis = P2FVis{m.trisSorted, m.ntri, o.type == OBJ_TARGET ? o.triBase : -1, m.vc.scale, m.vc.padTri,
+                           m.vc.tminPoint};
         ProfScope ps(m, PK_P2F, (double)ns * o.ncell, 30.0 * ns * o.ncell);
         return p2f_launch(t, ph, m.stream);
     };
@@ -1463,6 +1468,12 @@ int model_set_transfer_symmetric(Model &m, int on) {
     return 0;
 }
 
+int model_set_inline_visibility(Model &m, int on) {
+    if (on != 0 && on != 1) { set_error("inline visibility: 0 from the visibility words, 1 inside the sums"); return -2; }
+    m.inlineVis = on;
+    return 0;
+}
+
 int model_set_transfer_mode(Model &m, int mode) {
     if (mode != 0 && mode != 1) { set_error("transfer mode: 0 plain FP32, 1 FP16 MFMA transfer matrix"); return -2; }
     m.transferMode = mode;
@@ -1558,6 +1569,7 @@ void model_teardown(Model &m) {
     m.transferMode = 0;
     m.transferSplits = 0;
     m.transferSymmetric = 2;
+    m.inlineVis = 1;
     m.haveLastGemm = false;
     m.nearMode = 0;
     m.nearQuadDeltas = kNfRMinDeltas;

@@ -138,6 +138,9 @@ struct Model {
     // S build: 0 every block; 1 half the blocks and their transposes where possible;
     // 2 as 1, and with at most 4 occluder triangles the SELF visibility evaluated in that build
     int transferSymmetric = 2;
+    // 1: with at most 4 occluder triangles the point->fragment sum (one plane per
+    // source) evaluates its visibility itself; 0: always from the visibility words
+    int inlineVis = 1;
     int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
     int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
     int gatedCap = 0;

@@ -73,6 +73,16 @@ struct ReduceTask {
     const float *colInvScale; // optional: value *= colInvScale[plane]
 };
 
+// Visibility of point->fragment pairs evaluated inside the sum (scenes of at
+// most 4 occluder triangles): the sources->fragments test of the visibility
+// kernels. tris null: read the visibility words instead.
+struct P2FVis {
+    const float4 *tris;  // the scene's occluder records (Bvh::tris), ntri x 4 float4
+    int ntri;
+    int rcTriBase;       // occluder index of the receiving object's facet 0 (-1: not an occluder)
+    float scale, padTri, tmin;  // VisConst::scale, padTri, tminPoint
+};
+
 struct P2FTask {
     const float4 *cp;       // receiver (cell) positions
     const float4 *cn;       // cell normals
@@ -85,6 +95,7 @@ struct P2FTask {
     int batched;            // 1: source s writes plane s only
     float2 *P;
     long Pstride;
+    P2FVis vis{};           // batched only: the visibility evaluated inline (mask unused)
 };
 
 int gather_launch(const GatherTask &t, const Phys &ph, int kind, hipStream_t st);

@@ -23,6 +23,7 @@
 
 #include "sums.h"
 #include "pair_kernel.h"
+#include "p2f_term.h"
 #include "launch.h"
 
 namespace sr {
@@ -476,77 +477,6 @@ __device__ inline void reduce_points_elem(const float2 *hi, const float2 *lo, in
     f.y += si;
 }
 
-// One (cell, source) term of point->fragment: cos_inc * A_s * e^{ikr} / r.
-template <bool ATT>
-__device__ __attribute__((always_inline)) inline float2 p2f_term(const P2FTask &t, const Phys &ph, float4 cp, float4 cn,
-                                                                 int s) {
-    const float4 sp = t.sp[s];
-    // r and cos_inc exactly as the reference forms them (no contraction).
-    const float vx = __fsub_rn(cp.x, sp.x), vy = __fsub_rn(cp.y, sp.y), vz = __fsub_rn(cp.z, sp.z);
-    const float r = __fsqrt_rn(__fadd_rn(__fadd_rn(__fmul_rn(vx, vx), __fmul_rn(vy, vy)), __fmul_rn(vz, vz)));
-    const float inv = __fdiv_rn(1.0f, r);
-    float ci = 1.0f;
-    if (t.useNormal) {
-        const float ux = -__fmul_rn(vx, inv), uy = -__fmul_rn(vy, inv), uz = -__fmul_rn(vz, inv);
-        ci = __fadd_rn(__fadd_rn(__fmul_rn(ux, cn.x), __fmul_rn(uy, cn.y)), __fmul_rn(uz, cn.z));
-    }
-    const float x = phase_rev_split(r, ph.krevHi, ph.krevLo);
-    const float cc = hw_cos_rev(x), ss = hw_sin_rev(x);
-    float m = ci * inv;
-    if (ATT) m *= hw_exp2(-ph.alphaLog2e * r);
-    const float2 A = t.samp[s];
-    // m * (cc + i ss) * A
-    return make_float2(m * (cc * A.x - ss * A.y), m * (cc * A.y + ss * A.x));
-}
-
-// Batched sources (source s writes plane s only): one wave = 64 cells x one
-// source, so every (cell, source) update is independent.
-template <bool ATT>
-__global__ __launch_bounds__(256) void k_p2f_planes(P2FTask t, Phys ph) {
-    const long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int ncb = (t.ncell + 63) >> 6;
-    if (w >= (long)ncb * t.ns) return;
-    const int s = (int)(w / ncb), cb = (int)(w % ncb);
-    const int lane = threadIdx.x & 63;
-    const int c = cb * 64 + lane;
-    const uint64_t word = t.mask[(size_t)cb * t.ns + s];
-    if (c >= t.ncell || !((word >> lane) & 1ull)) return;
-    const float2 v = p2f_term<ATT>(t, ph, t.cp[c], t.cn[c], s);
-    float2 &P = t.P[(size_t)s * t.Pstride + c];
-    P.x += v.x;
-    P.y += v.y;
-}
-
-// point->fragment: lanes own cells; sources are few.
-template <bool ATT>
-__device__ __attribute__((always_inline)) inline void p2f_elem(const P2FTask &t, const Phys &ph, int c) {
-    const int lane = threadIdx.x & 63;
-    const int cb = c >> 6;
-    if (cb * 64 >= t.ncell) return;
-    const bool valid = c < t.ncell;
-    float4 cp = valid ? t.cp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 cn = valid ? t.cn[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float2 sum = make_float2(0.f, 0.f);
-    for (int s = 0; s < t.ns; s++) {
-        uint64_t word = t.mask[(size_t)cb * t.ns + s];
-        bool bit = (word >> lane) & 1ull;
-        float2 contrib = make_float2(0.f, 0.f);
-        if (bit) contrib = p2f_term<ATT>(t, ph, cp, cn, s);
-        if (t.batched) {
-            if (valid) {
-                float2 &P = t.P[s * t.Pstride + c];
-                P.x += contrib.x; P.y += contrib.y;
-            }
-        } else {
-            sum.x += contrib.x; sum.y += contrib.y;
-        }
-    }
-    if (!t.batched && valid) {
-        float2 &P = t.P[c];
-        P.x += sum.x; P.y += sum.y;
-    }
-}
-
 __device__ inline void prime_elem(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
                                   long outStride, long i) {
     if (i >= (long)nact * nrhs) return;
@@ -605,10 +535,6 @@ __global__ __launch_bounds__(64 * kRpWaves) void k_reduce_points(const float2 *h
     }
 }
 
-template <bool ATT>
-__global__ __launch_bounds__(256) void k_p2f(P2FTask t, Phys ph) {
-    p2f_elem<ATT>(t, ph, blockIdx.x * blockDim.x + threadIdx.x);
-}
 
 __global__ void k_prime(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout,
                         long outStride) {
@@ -774,22 +700,6 @@ int reduce_points_launch(const float2 *hi, const float2 *lo, int nsplit, int nbS
     return 0;
 }
 
-int p2f_launch(const P2FTask &t, const Phys &ph, hipStream_t st) {
-    if (t.ncell <= 0 || t.ns <= 0) return 0;
-    if (t.batched) {
-        const long waves = (long)((t.ncell + 63) / 64) * t.ns;
-        const unsigned gb = (unsigned)((waves + 3) / 4);
-        if (ph.alpha != 0.0f) k_p2f_planes<true><<<gb, 256, 0, st>>>(t, ph);
-        else k_p2f_planes<false><<<gb, 256, 0, st>>>(t, ph);
-        SR_HIP(hipGetLastError());
-        return 0;
-    }
-    const unsigned g = (unsigned)((t.ncell + 255) / 256);
-    if (ph.alpha != 0.0f) k_p2f<true><<<g, 256, 0, st>>>(t, ph);
-    else k_p2f<false><<<g, 256, 0, st>>>(t, ph);
-    SR_HIP(hipGetLastError());
-    return 0;
-}
 
 int prime_launch(const float2 *P, long pStride, const int *actCell, int nact, int nrhs, float2 *Pout, long outStride,
                  hipStream_t st) {

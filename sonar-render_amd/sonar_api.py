@@ -427,6 +427,17 @@ def set_transfer_symmetric(on):
     _check(cpp_lib.sr_set_transfer_symmetric(int(on)), "set_transfer_symmetric")
 
 
+_sig("sr_set_inline_visibility", [_i])
+_sig("sr_inline_visibility", [])
+
+
+def set_inline_visibility(on):
+    """1 (default): with one incidence plane per source and at most 4 occluder
+    triangles, the point->fragment sum evaluates its pairs' visibility itself;
+    0: from the sources->fragments visibility words. The same result either way."""
+    _check(cpp_lib.sr_set_inline_visibility(int(on)), "set_inline_visibility")
+
+
 _sig("sr_read_stl", [ctypes.c_char_p, _fp, _l], _l)
 _sig("sr_write_stl", [ctypes.c_char_p, _fp, _i])
 _sig("sr_load_stl", [ctypes.c_char_p, _i, _f])

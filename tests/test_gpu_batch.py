@@ -126,3 +126,34 @@ def test_batch_near_field_vs_oracle_judge(api, near):
     for i, sc in enumerate(batch):
         assert np.array_equal(api.batch_field_points(i, len(sc["points"])), eager[i])
     api.batch_clear()
+
+
+def test_batch_256_scenes_learned_lambda4(api):
+    """The batched workload as specified: 256 seeded uneven scenes (convex polyhedra of
+    8-40 triangles or plates, 0.3-1.5 m, 2-20 kHz, one source, 32 field points each, 2
+    bounces) at lambda/4 with the learned near-field correction, replayed as one hipGraph
+    after autotuning. A seeded subset of 16 scenes is checked against the FP64 oracle with
+    its 16x16 near-field judge (the whole batch is checked by bench.py --config batch)."""
+    batch = scenes.config_batch()
+    assert len(batch) == 256
+    _banner("256-scene batch, learned lambda/4 near field, hipGraph, vs the oracle's judge on 16 scenes")
+    api.nf_init(1)
+    print("train / held-out", api.nf_train(nsamples=262144, steps=20000, batch=4096, lr=3e-3, seed=7), flush=True)
+    _load(api, batch, near=2)
+    api.batch_autotune()
+    api.batch_run(use_graph=True)
+    pick = np.sort(np.random.default_rng(11).choice(len(batch), 16, replace=False))
+    got, want = [], []
+    for i in pick:
+        sc = batch[i]
+        lam = sc["c"] / sc["f"]
+        ref = oracle.run_scene(sc, near=(16, 2 * lam))["points"]
+        g = api.batch_field_points(int(i), len(sc["points"]))
+        print(sc["name"], f"{sc['f']:.0f} Hz", "rel-L2 vs judge", rel_l2(g, ref), flush=True)
+        assert rel_l2(g, ref) < 5e-3
+        got.append(g)
+        want.append(ref)
+    total = rel_l2(np.concatenate(got), np.concatenate(want))
+    print("subset rel-L2", total, flush=True)
+    assert total < 1e-3
+    api.batch_clear()

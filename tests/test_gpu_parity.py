@@ -212,3 +212,67 @@ def test_resp_between_two_targets(api):
         e = rel_l2(got[i], s.pressure(0, i, 2))
         print("P_out target", i, e)
         assert e < TOL_FP32
+
+
+def _plate_and_field_plane():
+    """A 1 m plate (two triangles) and a vertical field plane through it, reaching
+    below it; three incidences: above the plate (the plate shadows the field
+    cells below it), below it (every plate cell culled by its cosine) and
+    grazing."""
+    lam = 1480.0 / 6e3
+    src = np.array([[1.5, 0.3, 4.0], [-2.0, 0.5, -3.0], [6.0, -0.2, 0.05]], np.float32)
+    return dict(name="plate_and_field_plane", f=6e3, c=1480.0, atten=0.0,
+                targets=[(scenes.make_rectangle(1.0, 1.0), lam / 6)],
+                fields=[(scenes.make_rectangle(1.6, 1.2, xy_plane=False) + np.float32([0.1, 0.0, -0.2]), lam / 6)],
+                sources=src, points=scenes.generate_field_points(10.0, np.linspace(-80, 80, 17)), bounces=1,
+                project_field_objects=True, batched=True)
+
+
+def _inline_cases():
+    from test_gpu_transfer import _stacked_plates
+    return [("plate and field plane, 6 kHz", _plate_and_field_plane()),
+            ("trihedral 12 kHz", scenes.config_trihedral(f=12e3, n_side=2, bounces=2)),
+            ("three stacked plates, the middle one shadowing, 10 kHz", _stacked_plates())]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_p2f_inline_visibility_matches_words(api, case):
+    """With one incidence plane per source and at most 4 occluder triangles the
+    point->fragment sum evaluates each (source, fragment) pair's visibility itself
+    (the receiver's cosine cull, the plane-side candidates, the FP32 triangle
+    test of the sources->fragments words, which test_masks_sphere_exact pins bit
+    for bit against the oracle's): every pressure must be the same, bit for bit,
+    as with the words, with no sources->fragments words traced. The plate case
+    also checks the field object's incidence (no cosine cull, a shadow) against
+    the FP64 oracle."""
+    name, sc = _inline_cases()[case]
+    outs, vis = {}, {}
+    for inl in (1, 0):
+        _banner(f"{name}: point->fragment visibility {'inside the sum' if inl else 'from the words'}")
+        runner.load_scene(sc)
+        api.set_inline_visibility(inl)
+        api.profile(True)
+        api.project_source_points_to_objects(sc.get("project_field_objects", False))
+        vis[inl] = api.kernel_stats("vis")["pairs"]
+        api.profile(False)
+        if sc.get("bounces", 0):
+            api.project_target_to_target_objects(sc["bounces"])
+        if sc.get("fields"):
+            api.project_target_to_field_objects()
+        api.project_target_to_field_points()
+        outs[inl] = runner.read_results(sc)
+        api.TearDownCuda()
+        print("inline", inl, "visibility pairs of the source step", vis[inl])
+    assert vis[1] == 0 and vis[0] > 0
+    a, b = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]] +
+                           [f.ravel() for f in o["fields"]]) for o in (outs[1], outs[0]))
+    assert np.isfinite(a).all() and np.abs(a).max() > 0
+    assert a.tobytes() == b.tobytes()
+    if case == 0:
+        for p in range(len(sc["sources"])):
+            ref = oracle.run_scene(sc, plane_source=p)
+            for k, (x, y) in {"field": (outs[1]["fields"][0][p], ref["fields"][0]),
+                              "target": (outs[1]["targets"][0][p], ref["targets"][0])}.items():
+                e = rel_l2(x, y) if np.abs(y).max() > 0 else float(np.abs(x).max())
+                print("plane", p, k, e)
+                assert e < TOL_FP32
