@@ -904,7 +904,7 @@ static int ensure_transfer(Model &m, int i) {
     if (!t.A || t.nKs != nKs || t.nRt != nRt || t.gateInGemm != gateIn || t.splitsReq != m.transferSplits) {
         free_transfer(o);
         const int nWg = nRt / 4;
-        if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 256) ||
+        if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 128) ||
             dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
             dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes * kColStride)) ||
             dev_alloc(&t.rowFs, n))
@@ -999,7 +999,7 @@ static int ensure_transfer(Model &m, int i) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0) cus = 256;
         const long slots = cus;
-        const bool bigB = (double)nKs * 4096.0 > 4.0 * 1024 * 1024;
+        const bool bigB = (double)nKs * 2048.0 > 4.0 * 1024 * 1024;
         long maxChunks = 0, sumChunks = 0;
         for (int w = 0; w < nWg; w++) {
             const long c = (nKs - (skip[w].y - skip[w].x)) / 8;

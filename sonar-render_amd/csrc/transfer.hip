@@ -7,9 +7,11 @@
 // parts interleaved along K, scaled by one factor so |S| stays below 2^14),
 // and every bounce is one real GEMM on the matrix cores:
 //     C[i, 0..63]   = sum_j Re S_ij X_re[j,b] - Im S_ij X_im[j,b]
-//     C[i, 64..127] = sum_j Re S_ij X_im[j,b] + Im S_ij X_re[j,b]
-// with A = S (M x 2N) and B = [[X_re, X_im], [-X_im, X_re]] (2N x 128),
-// X = A_j P_in[j,b] scaled per incidence, FP32 accumulation
+//     C[i, 64..127] = sum_j Im S_ij X_re[j,b] + Re S_ij X_im[j,b]
+// with B = [X_re; X_im] (2N x 64, real and imaginary parts interleaved along K
+// as in S) and two A operands formed from each S fragment in registers: the
+// words (Re S, -Im S) for the real part and (Im S, Re S) for the imaginary
+// part, X = A_j P_in[j,b] scaled per incidence, FP32 accumulation
 // (v_mfma_f32_32x32x16_f16). The reference's zero-input gate
 // (OptiX.cu:241-245: terms from higher facets need P_in[i] != 0) is kept exact
 // by accumulating lower-facet and higher-facet emitters separately.
@@ -18,7 +20,8 @@
 // one coalesced 1 KiB per wave):
 //   A: [row tile of 32][K step of 8 emitters][lane][8 halves]
 //      lane l: row = tile*32 + l%32, emitters step*8 + (l/32)*4 + q, halves (Re, Im)
-//   B: [K step][column tile of 32][lane][8 halves], same K assignment, column l%32.
+//   B: [K step][column tile of 32 (2)][lane][8 halves], same K assignment, plane
+//      tile*32 + l%32.
 // The phase k r is reduced modulo one revolution in FP64 before the FP32
 // sin/cos (the single FP64 step of the form).
 
@@ -183,24 +186,23 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
         sX[b * kBStride + kk] = make_float2(xr, xi);
     }
     __syncthreads();
-    // element idx of the chunk: K step idx / 256, column tile (idx / 64) % 4, lane idx % 64
+    // element idx of the chunk: K step idx / 128, column tile (idx / 64) % 2, lane idx % 64
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < 4; i++) {
         const int idx = threadIdx.x + 256 * i;
-        const int s = idx >> 8, lane = idx & 63;
-        const int c = ((idx >> 6) & 3) * 32 + (lane & 31);
-        const int b = c & 63;
+        const int s = idx >> 7, lane = idx & 63;
+        const int b = ((idx >> 6) & 1) * 32 + (lane & 31);
         const float2 *x = sX + b * kBStride + s * 8 + (lane >> 5) * 4;
         uint32_t w[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) w[q] = (c < 64) ? pack_h2(x[q].x, -x[q].y) : pack_h2(x[q].y, x[q].x);
-        B[(size_t)chunk * 2048 + idx] = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int q = 0; q < 4; q++) w[q] = pack_h2(x[q].x, x[q].y);
+        B[(size_t)chunk * 1024 + idx] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
 // ---------------------------------------------------------------------------
 // GEMM: a workgroup = 4 waves = 128 rows x 128 columns; a wave = 32 rows.
-// K advances in chunks of 8 steps (8 emitters each). B chunks (32 KiB) are
+// K advances in chunks of 8 steps (8 emitters each). B chunks (16 KiB) are
 // copied global -> LDS without passing through registers and are shared by
 // the 4 waves; A fragments stream from HBM straight into registers. Both are
 // requested one chunk ahead of the MFMAs that use them (two LDS buffers, two
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(256, ROLE == 2 ? 1 : 2) void k_transfer_gemm(Transf
     constexpr bool DUAL = ROLE == 2;
     // two separately named buffers, so that the compiler can tell the LDS reads of
     // one from the asynchronous global -> LDS copies into the other
-    __shared__ uint4 sB0[kKC * 4 * 64], sB1[kKC * 4 * 64];
+    __shared__ uint4 sB0[kKC * 2 * 64], sB1[kKC * 2 * 64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     // Workgroup L = group * nsplit + split. Workgroups are dealt round-robin to the
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(256, ROLE == 2 ? 1 : 2) void k_transfer_gemm(Transf
     const SplitWalk sw = split_walk(t.wgSkip[grp], t.nKs, t.nsplit, split);
     const int nch = sw.o1 - sw.o0;
     const uint4 *Arow = t.A + (size_t)(rtValid ? rt : 0) * t.nKs * 64 + lane;
-    const uint4 *Bt = t.B + threadIdx.x;     // thread x stages element x of every K step
+    const uint4 *Bt = t.B + threadIdx.x;     // thread x stages elements x + 256 r of a chunk
     // in-product gate: K steps from kBnd on are the group's higher-facet emitters
     const int kBnd = (!DUAL && t.kB) ? t.kB[grp] : 0x7fffffff;
     int kLast = -1;  // the last chunk summed
@@ -292,13 +294,19 @@ __global__ __launch_bounds__(256, ROLE == 2 ? 1 : 2) void k_transfer_gemm(Transf
             }
         }
     };
-    // one K step's four products into an accumulator set
+    // one K step's four products into an accumulator set: the S words (Re, Im)
+    // as (Re, -Im) into the real parts (ac[0..1]) and as (Im, Re) into the
+    // imaginary parts (ac[2..3]), each against the plane tile's B fragment
     auto mfma_step = [&](f32x16 *ac, uint4 a, const uint4 *sb, int s) {
-        const half8 av = __builtin_bit_cast(half8, a);
+        const uint4 ar = make_uint4(a.x ^ 0x80000000u, a.y ^ 0x80000000u, a.z ^ 0x80000000u, a.w ^ 0x80000000u);
+        const uint4 ai = make_uint4(__builtin_amdgcn_alignbit(a.x, a.x, 16), __builtin_amdgcn_alignbit(a.y, a.y, 16),
+                                    __builtin_amdgcn_alignbit(a.z, a.z, 16), __builtin_amdgcn_alignbit(a.w, a.w, 16));
+        const half8 avr = __builtin_bit_cast(half8, ar), avi = __builtin_bit_cast(half8, ai);
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const half8 bv = __builtin_bit_cast(half8, sb[(s * 4 + c) * 64 + lane]);
-            ac[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, ac[c], 0, 0, 0);
+        for (int c = 0; c < 2; c++) {
+            const half8 bv = __builtin_bit_cast(half8, sb[(s * 2 + c) * 64 + lane]);
+            ac[c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(avr, bv, ac[c], 0, 0, 0);
+            ac[c + 2] = __builtin_amdgcn_mfma_f32_32x32x16_f16(avi, bv, ac[c + 2], 0, 0, 0);
         }
     };
     // DUAL: the chunk's steps, each split at the row's facet start (lane (j, h)
@@ -329,13 +337,14 @@ __global__ __launch_bounds__(256, ROLE == 2 ? 1 : 2) void k_transfer_gemm(Transf
     // the loop's requests ahead are unconditional (and those re-reads hit L2)
     auto kat = [&](int i) { return sw.k(sw.o0 + min(i, nch - 1)); };
     // B goes global -> LDS directly (global_load_lds_dwordx4: lane l of wave w
-    // writes element 64 w + l of a K step) into two buffers used in turn.
+    // writes element 256 r + 64 w + l of the chunk in request r) into two buffers
+    // used in turn; a chunk's 8 K steps are contiguous in B.
     typedef __attribute__((address_space(3))) void lds_void;
     auto stage_b = [&](int k, uint4 *bb) {
 #pragma unroll
-        for (int s = 0; s < kKC; s++)
-            __builtin_amdgcn_global_load_lds((const void *)(Bt + (size_t)(k + s) * 256),
-                                             (lds_void *)&bb[s * 256 + wave * 64], 16, 0, 0);
+        for (int r = 0; r < kKC / 2; r++)
+            __builtin_amdgcn_global_load_lds((const void *)(Bt + (size_t)k * 128 + r * 256),
+                                             (lds_void *)&bb[r * 256 + wave * 64], 16, 0, 0);
     };
     // s_waitcnt immediate (gfx9 encoding): vector-memory count <= kKC, the export and
     // LDS/scalar counts at their maxima (not waited on)
