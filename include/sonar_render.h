@@ -198,6 +198,18 @@ int sr_transfer_symmetric(void);
 int sr_set_inline_visibility(int on);
 int sr_inline_visibility(void);
 
+/* Split-K reduction of the transfer product (no counterpart in the reference,
+ * whose bounce sums each receiver's terms in one thread: src/OptiX.cu:297-354).
+ * 1: the last workgroup of each 128-row group to finish reduces the group's
+ * partial sums, applies the zero-input gate and writes P_out and P; 0
+ * (default): the same reduction as a pass of its own. Both sum the splits in
+ * the same order: the same pressures bit for bit. The handover between
+ * workgroups needs device-scope release and acquire fences, which on MI355X
+ * write back and invalidate an XCD's L2: slower than the pass. Reset to 0 by
+ * TearDownCuda. */
+int sr_set_reduce_in_product(int on);
+int sr_reduce_in_product(void);
+
 /* Plain fragment->fragment sums (transfer mode 0). 0 (default): with 16 or
  * more incidence planes the pair kernel is evaluated in FP32 on the vector
  * units and the complex products accumulate on the FP32 matrix cores, 64

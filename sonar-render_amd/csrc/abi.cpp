@@ -300,6 +300,10 @@ int sr_set_inline_visibility(int on) {
     return guarded("set_inline_visibility", [&] { return sr::model_set_inline_visibility(cur(), on); });
 }
 int sr_inline_visibility(void) { return cur().inlineVis; }
+int sr_set_reduce_in_product(int on) {
+    return guarded("set_reduce_in_product", [&] { return sr::model_set_reduce_in_product(cur(), on); });
+}
+int sr_reduce_in_product(void) { return cur().reduceInGemm; }
 int sr_set_gather_mode(int mode) {
     return guarded("set_gather_mode", [&] { return sr::model_set_gather_mode(cur(), mode); });
 }

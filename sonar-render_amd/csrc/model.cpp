@@ -148,6 +148,7 @@ void free_transfer(Object &o) {
     dev_free(t.A); dev_free(t.B); dev_free(t.wgSkip); dev_free(t.colScale); dev_free(t.colInv);
     dev_free(t.state); dev_free(t.rowFs);
     dev_free(t.rowAct); dev_free(t.actRow); dev_free(t.kB); dev_free(t.cls); dev_free(t.symItems);
+    dev_free(t.grpCount);
     dev_free(t.nearQ); dev_free(t.nearM); dev_free(t.nearCount);
     t = Object::Transfer{};
 }
@@ -905,10 +906,12 @@ static int ensure_transfer(Model &m, int i) {
         free_transfer(o);
         const int nWg = nRt / 4;
         if (dev_alloc(&t.A, (size_t)nRt * nKs * 64) || dev_alloc(&t.B, (size_t)nKs * 128) ||
-            dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.colScale, 64) || dev_alloc(&t.colInv, 64) ||
+            dev_alloc(&t.wgSkip, nWg) || dev_alloc(&t.grpCount, nWg) || dev_alloc(&t.colScale, 64) ||
+            dev_alloc(&t.colInv, 64) ||
             dev_alloc(&t.state, 2 * (1 + (size_t)m.nplanes * kColStride)) ||
             dev_alloc(&t.rowFs, n))
             return -1;
+        SR_HIP(hipMemsetAsync(t.grpCount, 0, (size_t)nWg * sizeof(unsigned), m.stream));
         t.nKs = nKs;
         t.nRt = nRt;
         t.nPad = nPad;
@@ -1197,7 +1200,15 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
         TransferGemm gm{t.A, t.B, t.nPad, t.nKs, t.nRt, np, nsplit, t.wgSkip, m.scratchL,
                         t.gateInGemm ? t.kB : nullptr, m.scratchU, t.gateInGemm ? nullptr : t.rowFs};
         gm.streamS = (double)t.nRt * 32 * t.nKs * 16 >= kStreamSBytes;
+        // the split-K reduction, the gate and (another bounce following) its column maxima,
+        // done by each row group's last workgroup of the product
+        gm.red = ReduceGateTask{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.rowAct, pin,
+                                o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
+                                o.actP,
+                                more ? reinterpret_cast<unsigned *>(nxt + 1) + (size_t)g0 * kColStride : nullptr};
+        gm.grpCount = m.reduceInGemm ? t.grpCount : nullptr;
         m.lastGemm = gm;
+        m.lastGemm.grpCount = nullptr;  // replays time the product alone
         m.haveLastGemm = true;
         {
             const double pairs = ((double)n * n - o.sameFacetPairs) * np;
@@ -1211,12 +1222,10 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             ProfScope ps(m, PK_GEMM, pairs, fl, by);
             if (transfer_gemm(gm, m.stream)) return -1;
         }
-        // another bounce follows: the reduction also finds its column maxima (into the next set)
-        ReduceGateTask r{m.scratchL, m.scratchU, t.cls, nsplit, t.nPad, np, n, t.rowAct, pin,
-                         o.Pout + (size_t)g0 * n, o.P + (size_t)g0 * o.ncell, o.ncell, o.actCell, t.colInv,
-                         o.actP, more ? reinterpret_cast<unsigned *>(nxt + 1) + (size_t)g0 * kColStride : nullptr};
-        ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
-        if (transfer_reduce_gate(r, m.stream)) return -1;
+        if (!gm.grpCount) {  // the same reduction as a pass of its own
+            ProfScope ps(m, PK_REDUCE, 0.0, 0.0);
+            if (transfer_reduce_gate(gm.red, m.stream)) return -1;
+        }
     }
     if (more) t.maxReady = true;
     else t.stateClear = true;
@@ -1468,6 +1477,12 @@ int model_set_transfer_symmetric(Model &m, int on) {
     return 0;
 }
 
+int model_set_reduce_in_product(Model &m, int on) {
+    if (on != 0 && on != 1) { set_error("reduce in product: 0 a pass of its own, 1 inside the product"); return -2; }
+    m.reduceInGemm = on;
+    return 0;
+}
+
 int model_set_inline_visibility(Model &m, int on) {
     if (on != 0 && on != 1) { set_error("inline visibility: 0 from the visibility words, 1 inside the sums"); return -2; }
     m.inlineVis = on;
@@ -1570,6 +1585,7 @@ void model_teardown(Model &m) {
     m.transferSplits = 0;
     m.transferSymmetric = 2;
     m.inlineVis = 1;
+    m.reduceInGemm = 0;
     m.haveLastGemm = false;
     m.nearMode = 0;
     m.nearQuadDeltas = kNfRMinDeltas;

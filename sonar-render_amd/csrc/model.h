@@ -59,6 +59,7 @@ struct Object {
         int nPad = 0;              // rows of S (padded)
         int *rowAct = nullptr, *actRow = nullptr, *kB = nullptr;
         unsigned char *cls = nullptr;
+        unsigned *grpCount = nullptr;  // per 128-row group: the product's workgroups done (zero between launches)
         int nsplit = 1;             // split-K of the bounce product (split_walk)
         int splitsReq = 0;          // the split request (Model::transferSplits) nsplit was chosen for
         // symmetric S build (padded rows): the 64 x 64 blocks (row block, K chunk) on
@@ -141,6 +142,12 @@ struct Model {
     // 1: with at most 4 occluder triangles the point->fragment sum (one plane per
     // source) evaluates its visibility itself; 0: always from the visibility words
     int inlineVis = 1;
+    // 1: the transfer product's last workgroup of each row group reduces its split-K
+    // partials (gate, P_out, P, next column maxima); 0: a pass of its own after it.
+    // 0 by default: the agent-scope release and acquire the handover needs write back
+    // and invalidate an XCD's L2 each (the cached B with it), which costs more than
+    // the pass (measured 0.266 against 0.133 + 0.015 ms per bounce at lambda/7.5)
+    int reduceInGemm = 0;
     int gatherMode = 0;    // plain fragment->fragment: 0 FP32 MFMA products from 16 planes on, 1 VALU only
     int *gatedList = nullptr, *gatedCount = nullptr;  // plain form: rows a zero input gates
     int gatedCap = 0;

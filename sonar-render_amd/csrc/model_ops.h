@@ -36,6 +36,7 @@ int model_set_transfer_mode(Model &m, int mode);
 int model_set_transfer_splits(Model &m, int n);
 int model_set_transfer_symmetric(Model &m, int on);
 int model_set_inline_visibility(Model &m, int on);
+int model_set_reduce_in_product(Model &m, int on);
 int model_set_gather_mode(Model &m, int mode);
 int model_set_near_quadrature_radius(Model &m, float deltas);
 int model_set_near_mode(Model &m, int mode, float radiusWl);

@@ -51,32 +51,6 @@ __host__ __device__ inline SplitWalk split_walk(int2 skip, int nKs, int nsplit, 
     return SplitWalk{below, skip.y, (int)((long)s * total / nsplit), (int)((long)(s + 1) * total / nsplit)};
 }
 
-struct TransferGemm {
-    const uint4 *A, *B;
-    int n, nKs, nRt;
-    int nplanes;        // planes of this group (<= 64)
-    int nsplit;         // splits of each group's non-zero chunks (split_walk)
-    const int2 *wgSkip;                   // per 4-row-tile group: K steps [x, y) that hold only zeros
-    float2 *partL;                        // [split][64][n]
-    // Zero-input gate inside the product: per 128-row group, the first K step of
-    // the higher-facet emitters (whole chunks). A workgroup writes what it sums
-    // below that step to partL and from it on to partU (null: partL only).
-    const int *kB = nullptr;
-    float2 *partU = nullptr;
-    // Zero-input gate by two accumulators (any facet layout): per row of S, the
-    // first emitter of its own facet; the product sends the terms below it to
-    // partL and the rest to partU, both for every split (null: one accumulator).
-    const int *rowFs = nullptr;
-    // S larger than the GPU's memory-side cache: stream it (non-temporal loads)
-    // rather than let each bounce's pass evict the cached B and partial sums
-    // for data no later access in the pass reuses.
-    int streamS = 0;
-};
-
-// S of at least this many bytes is written and read non-temporally; a smaller
-// one stays in the memory-side cache (256 MiB on MI355X) between the passes.
-constexpr double kStreamSBytes = 160.0 * 1024 * 1024;
-
 // Reduction of a gated product: per (plane, active row), the lower-facet
 // partials always and the higher-facet ones where the row's own input is not
 // zero (src/OptiX.cu:241-245); cls[group][split] bit 0/1 says which of the
@@ -101,6 +75,39 @@ struct ReduceGateTask {
     const float4 *actP;
     unsigned *colMaxNext;
 };
+
+struct TransferGemm {
+    const uint4 *A, *B;
+    int n, nKs, nRt;
+    int nplanes;        // planes of this group (<= 64)
+    int nsplit;         // splits of each group's non-zero chunks (split_walk)
+    const int2 *wgSkip;                   // per 4-row-tile group: K steps [x, y) that hold only zeros
+    float2 *partL;                        // [split][64][n]
+    // Zero-input gate inside the product: per 128-row group, the first K step of
+    // the higher-facet emitters (whole chunks). A workgroup writes what it sums
+    // below that step to partL and from it on to partU (null: partL only).
+    const int *kB = nullptr;
+    float2 *partU = nullptr;
+    // Zero-input gate by two accumulators (any facet layout): per row of S, the
+    // first emitter of its own facet; the product sends the terms below it to
+    // partL and the rest to partU, both for every split (null: one accumulator).
+    const int *rowFs = nullptr;
+    // S larger than the GPU's memory-side cache: stream it (non-temporal loads)
+    // rather than let each bounce's pass evict the cached B and partial sums
+    // for data no later access in the pass reuses.
+    int streamS = 0;
+    // The reduction inside the product (grpCount non-null; off by default, as its
+    // device-scope fences cost more than the separate pass): each workgroup counts
+    // itself done on its group's counter (zero before the launch; the last one
+    // resets it) and the last of a group's splits reduces the group's rows as
+    // k_reduce_gate would (red: the same task), the splits summed in split order.
+    ReduceGateTask red{};
+    unsigned *grpCount = nullptr;
+};
+
+// S of at least this many bytes is written and read non-temporally; a smaller
+// one stays in the memory-side cache (256 MiB on MI355X) between the passes.
+constexpr double kStreamSBytes = 160.0 * 1024 * 1024;
 
 // The visibility test of an object's own fragment pairs, for a scene of at most
 // 4 occluder triangles, evaluated inside the symmetric S build instead of read

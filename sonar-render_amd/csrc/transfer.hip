@@ -143,17 +143,19 @@ __device__ inline float col_scale_of(unsigned bits) {
 // B build for one group of up to 64 incidences.
 // Scales come from colMax (k_col_max); threads 0..nplanes-1 also store each
 // plane's scale and the inverse the reduction applies (1 / (scale sS)). A
-// workgroup builds one chunk of 8 K steps (64 K positions): it reads the 64
-// planes' inputs along K (coalesced, one plane per wave and pass) into LDS, then
-// writes the chunk's B fragments (one plane per lane) from there.
-constexpr int kBStride = 65;  // LDS row stride (float2) of one plane's 64 K positions, padded
+// workgroup builds one quarter of a chunk of 8 K steps: one column tile (32
+// planes) by one half (4 K steps, 32 K positions). It reads those planes' inputs
+// along K (coalesced runs of 32 K positions) into LDS, then writes the quarter's
+// B fragments (one plane per lane) from there: four workgroups per chunk, so
+// that even a small B fills the GPU.
+constexpr int kBStride = 33;  // LDS row stride (float2) of one plane's 32 K positions, padded
 __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, const float2 *__restrict__ Pin,
                                                  long stride, int n, int nKs, int nplanes,
                                                  const unsigned *__restrict__ colMax, float sS,
                                                  float *__restrict__ scaleOut, float *__restrict__ invOut,
                                                  uint4 *__restrict__ B, int *__restrict__ clearState,
                                                  int clearWords, const int *__restrict__ kAct) {
-    __shared__ float2 sX[64 * kBStride];
+    __shared__ float2 sX[32 * kBStride];
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (clearState && t < clearWords) clearState[t] = 0;
     if (t < nplanes) {
@@ -161,43 +163,39 @@ __global__ __launch_bounds__(256) void k_build_b(const float4 *__restrict__ p, c
         scaleOut[t] = sc;
         invOut[t] = 1.0f / (sc * sS);
     }
-    const int chunk = blockIdx.x;
+    const int unit = blockIdx.x;  // (chunk, column tile, half)
+    const int chunk = unit >> 2, tile = (unit >> 1) & 1, half = unit & 1;
     if (chunk >= nKs / 8) return;  // uniform over the workgroup
-    const int kk = threadIdx.x & 63;
-    const int k = chunk * 64 + kk;
+    const int kk = threadIdx.x & 31;
+    const int k = chunk * 64 + half * 32 + kk;
     const int j = kAct ? kAct[k] : (k < n ? k : -1);  // its active cell (-1: padding)
     const float area = j >= 0 ? p[j].w : 0.f;
-    // all 16 of this thread's loads in flight at once (the workgroup count is small)
-    float2 v[16];
+    // all 4 of this thread's loads in flight at once
+    float2 v[4];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const int b = (threadIdx.x >> 6) + 4 * i;
+    for (int i = 0; i < 4; i++) {
+        const int b = tile * 32 + (threadIdx.x >> 5) + 8 * i;
         v[i] = (j >= 0 && b < nplanes) ? Pin[b * stride + j] : make_float2(0.f, 0.f);
     }
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const int b = (threadIdx.x >> 6) + 4 * i;
+    for (int i = 0; i < 4; i++) {
+        const int bl = (threadIdx.x >> 5) + 8 * i, b = tile * 32 + bl;
         float xr = 0.f, xi = 0.f;
         if (j >= 0 && b < nplanes) {
             const float a = area * col_scale_of(colMax[b * kColStride]);
             xr = a * v[i].x;
             xi = a * v[i].y;
         }
-        sX[b * kBStride + kk] = make_float2(xr, xi);
+        sX[bl * kBStride + kk] = make_float2(xr, xi);
     }
     __syncthreads();
-    // element idx of the chunk: K step idx / 128, column tile (idx / 64) % 2, lane idx % 64
+    // fragment: K step chunk*8 + half*4 + s, column tile `tile`, lane (plane tile*32 + lane%32)
+    const int s = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float2 *x = sX + (lane & 31) * kBStride + s * 8 + (lane >> 5) * 4;
+    uint32_t w[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int idx = threadIdx.x + 256 * i;
-        const int s = idx >> 7, lane = idx & 63;
-        const int b = ((idx >> 6) & 1) * 32 + (lane & 31);
-        const float2 *x = sX + b * kBStride + s * 8 + (lane >> 5) * 4;
-        uint32_t w[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) w[q] = pack_h2(x[q].x, x[q].y);
-        B[(size_t)chunk * 1024 + idx] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
+    for (int q = 0; q < 4; q++) w[q] = pack_h2(x[q].x, x[q].y);
+    B[((size_t)chunk * 8 + half * 4 + s) * 128 + tile * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,6 +230,8 @@ __device__ inline void workgroup_barrier() {
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
 }
+
+__device__ void reduce_group_last(const TransferGemm &t, int grp);
 
 // ROLE 0: one accumulator (the kB gate, or none); 2: DUAL. NT: S read non-temporally.
 template <int ROLE, bool NT>
@@ -424,6 +424,7 @@ __global__ __launch_bounds__(256, ROLE == 2 ? 1 : 2) void k_transfer_gemm(Transf
     } else if (kLast >= 0) {
         store_part(kLast < kBnd ? t.partL : t.partU, acc);
     }
+    if (t.grpCount) reduce_group_last(t, grp);
 }
 
 // Rows whose own input pressure is exactly zero in some plane (the zero-input
@@ -492,6 +493,40 @@ __device__ inline float2 reduce_gate_row(const ReduceGateTask &t, int b, int sr,
     return v;
 }
 
+// The reduction inside the product, run by each of its workgroups after its
+// partial-sum stores: the group's last workgroup to finish (its counter, zero
+// before the launch) reduces the group's 128 rows of S for every plane
+// (reduce_gate_row: the splits in split order, as k_reduce_gate), each wave's
+// 64 rows also giving one column maximum per plane, and sets the counter back
+// to zero for the next launch. The fences order every workgroup's partial sums
+// before its count, and the last workgroup's reads after the counts.
+__device__ void reduce_group_last(const TransferGemm &t, int grp) {
+    __shared__ unsigned sLast;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) sLast = atomicAdd(&t.grpCount[grp], 1u) == (unsigned)t.nsplit - 1u;
+    __syncthreads();
+    if (!sLast) return;  // uniform
+    __threadfence();
+    const ReduceGateTask &rt = t.red;
+    const int sr = grp * 128 + (threadIdx.x & 127);
+    const int lane = threadIdx.x & 63;
+    for (int b = threadIdx.x >> 7; b < rt.nrhs; b += 2) {  // uniform over each wave
+        int r;
+        const float2 v = reduce_gate_row(rt, b, sr, r);
+        if (!rt.colMaxNext) continue;
+        float m = 0.f;
+        if (r >= 0) {
+            const float a = rt.actP[r].w;
+            m = fmaxf(fabsf(a * v.x), fabsf(a * v.y));
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0 && m > 0.f) atomicMax(&rt.colMaxNext[b * kColStride], __float_as_uint(m));
+    }
+    if (threadIdx.x == 0) t.grpCount[grp] = 0u;
+}
+
 // Gated reduction: a workgroup = 256 rows of S for one plane (blockIdx.y).
 __global__ __launch_bounds__(256) void k_reduce_gate(ReduceGateTask t) {
     const int b = blockIdx.y;
@@ -541,7 +576,7 @@ int transfer_build_b(const float4 *p, const float2 *Pin, long stride, int n, int
                      const unsigned *colMax, float sS, float *scale, float *invScale, uint4 *B, hipStream_t st,
                      int *clearState, int clearWords, const int *kAct) {
     if (nKs % 8) { set_error("B build: K steps must be padded to whole chunks of 8"); return -2; }
-    const long blocks = std::max({(long)nKs / 8, ((long)nplanes + 255) / 256,
+    const long blocks = std::max({(long)nKs / 8 * 4, ((long)nplanes + 255) / 256,
                                   clearState ? ((long)clearWords + 255) / 256 : 0L});
     if (blocks <= 0) return 0;
     k_build_b<<<(unsigned)blocks, 256, 0, st>>>(p, Pin, stride, n, nKs, nplanes, colMax, sS, scale, invScale, B,
@@ -558,6 +593,11 @@ int transfer_gemm(const TransferGemm &t, hipStream_t st) {
     }
     if (t.rowFs && (t.kB || !t.partU)) {
         set_error("transfer GEMM: two accumulators need partU and exclude kB");
+        return -2;
+    }
+    if (t.grpCount && (t.red.nPadRows != t.n || t.red.nsplit != t.nsplit || t.red.nrhs != t.nplanes ||
+                       (t.red.colMaxNext && !t.red.actP) || (!t.red.rowAct && t.red.nPadRows != t.red.n))) {
+        set_error("transfer GEMM: the reduction inside it needs the product's rows, splits and planes");
         return -2;
     }
     const long grid = (long)((t.nRt + 3) / 4) * t.nsplit;

@@ -438,6 +438,17 @@ def set_inline_visibility(on):
     _check(cpp_lib.sr_set_inline_visibility(int(on)), "set_inline_visibility")
 
 
+_sig("sr_set_reduce_in_product", [_i])
+_sig("sr_reduce_in_product", [])
+
+
+def set_reduce_in_product(on):
+    """1: the transfer product's last workgroup of each row group reduces the split-K
+    partial sums; 0 (default): a reduction pass of its own (faster on MI355X). The
+    same result."""
+    _check(cpp_lib.sr_set_reduce_in_product(int(on)), "set_reduce_in_product")
+
+
 _sig("sr_read_stl", [ctypes.c_char_p, _fp, _l], _l)
 _sig("sr_write_stl", [ctypes.c_char_p, _fp, _i])
 _sig("sr_load_stl", [ctypes.c_char_p, _i, _f])

@@ -303,3 +303,47 @@ def test_mixed_dihedral_parallel_facet_runs(api):
         print("plane", p, "mixed", rel_l2(a, b), "plain", rel_l2(c, b))
         assert rel_l2(a, b) < TOL_MIXED
         assert rel_l2(c, b) < 2e-4
+
+
+def _reduce_cases():
+    tris = scenes.trihedral(1.0)
+    for _ in range(3):
+        tris = scenes.halve_facets(tris)
+    small = dict(name="trihedral_192_facets", f=3e3, c=1480.0, atten=0.1, targets=[(tris, 1480 / 3e3 / 7.5)],
+                 fields=[], sources=scenes.octant_incidences(2, 15.0), points=scenes.octant_incidences(2, 15.0),
+                 bounces=3, batched=True)
+    two = scenes.config_trihedral(f=5e3, n_side=9, bounces=3)
+    two["sources"] = two["sources"][:80]
+    two["points"] = two["sources"].copy()
+    return [("trihedral 20 kHz, 4 incidences (padded single-facet groups, gate at kB)",
+             scenes.config_trihedral(f=20e3, n_side=2, bounces=3)),
+            ("192-facet trihedral, attenuation (two accumulators per row)", small),
+            ("trihedral 5 kHz, 80 incidences (two plane groups per bounce)", two)]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_reduce_in_product_matches_separate_pass(api, case):
+    """The split-K reduction done by each row group's last workgroup of the product
+    (its counter: the other splits' partial sums fenced before their counts) sums
+    the splits in split order, as the reduction pass does: every pressure the same
+    bit for bit, in both gate layouts and with two plane groups per bounce (the
+    counters reused by the second launch), over three bounces, and twice in a row
+    on one model (the counters back at zero)."""
+    name, sc = _reduce_cases()[case]
+    outs = {}
+    for inprod in (1, 0):
+        _banner(f"transfer form, {name}: reduction {'inside the product' if inprod else 'as its own pass'}")
+        runner.load_scene(sc, transfer_mode=1)
+        api.set_reduce_in_product(inprod)
+        runner.project_scene(sc)
+        first = runner.read_results(sc)
+        api.reset_pressures()
+        api.clear_vis_cache()
+        runner.project_scene(sc)
+        outs[inprod] = runner.read_results(sc)
+        api.TearDownCuda()
+        a, b = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]]) for o in (first, outs[inprod]))
+        assert a.tobytes() == b.tobytes()
+    a, b = (np.concatenate([o["points"].ravel()] + [t.ravel() for t in o["targets"]]) for o in (outs[1], outs[0]))
+    assert np.isfinite(a).all() and np.abs(a).max() > 0
+    assert a.tobytes() == b.tobytes()
