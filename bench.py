@@ -50,8 +50,8 @@ TOL = 1e-3
 
 # HBM bytes per launch of the dominant kernel from rocprofv3 PMC runs (calibrated
 # FETCH_SIZE and WRITE_SIZE, see profiles/transfer_gemm_hbm_bytes.md), per workload.
-PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 764.5e6,
-               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 95.8e6}
+PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 741.2e6,
+               ("trihedral_1m_20kHz_lambda4", "k_transfer_gemm"): 89.0e6}
 # Vector wave-instructions (SQ_INSTS_VALU, exact) of one scene's visibility words
 # (k_vis_mask: SELF, sources -> fragments, fragments -> field points; k_mask_transpose),
 # per workload and form (0 plain, 1 transfer: there the S build evaluates the SELF
