@@ -144,3 +144,24 @@ def test_contexts_without_gpu():
     with pytest.raises(api.SonarRenderError):
         api.context_select(h1)  # destroyed
     assert api.context_current() == 0
+
+
+def test_form_settings_checked_and_reset(api):
+    """The settings that choose between bit-identical ways of computing a form
+    take their documented values only (status -2 otherwise, the setting kept),
+    and TearDownCuda puts them back to their defaults. No GPU call is made."""
+    lib = api.cpp_lib
+    cases = [("sr_set_transfer_symmetric", "sr_transfer_symmetric", (0, 1, 2), (-1, 3), 2),
+             ("sr_set_inline_visibility", "sr_inline_visibility", (0, 1), (-1, 2), 1),
+             ("sr_set_reduce_in_product", "sr_reduce_in_product", (0, 1), (-1, 2), 0)]
+    for setter, getter, good, bad, default in cases:
+        set_, get = getattr(lib, setter), getattr(lib, getter)
+        assert get() == default, setter
+        for v in good:
+            assert set_(v) == 0 and get() == v, (setter, v)
+        for v in bad:
+            assert set_(v) == -2 and get() == good[-1], (setter, v)
+            assert lib.sr_last_error().decode(), setter
+    api.TearDownCuda()
+    for setter, getter, good, bad, default in cases:
+        assert getattr(lib, getter)() == default, getter
