@@ -61,7 +61,7 @@ PMC_TRAFFIC = {("trihedral_1m_20kHz", "k_transfer_gemm"): 741.2e6,
 # 1,024 SIMDs at 2.4 GHz (VALU_ISSUE_PEAK) is exceeded by simple integer instructions.
 VIS_VALU_INSTR = {("trihedral_1m_20kHz", 0): 220_974_056, ("trihedral_1m_20kHz", 1): 7_924_496,
                   ("trihedral_1m_20kHz_lambda4", 1): 28_157_461}
-BUILD_S_VALU_INSTR = {("trihedral_1m_20kHz", 1): 89_202_954, ("trihedral_1m_20kHz_lambda4", 1): 13_602_360}
+BUILD_S_VALU_INSTR = {("trihedral_1m_20kHz", 1): 89_531_691, ("trihedral_1m_20kHz_lambda4", 1): 13_602_360}
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9  # G wave-instructions/s
 
 
