@@ -162,7 +162,9 @@ def roofline(stats, mode, workload, stream_gbs=None):
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "algorithmic_bytes_per_launch": round(algo),
-            "algorithmic_bytes": "FP16 complex S of every ordered cross-facet pair (4 B) + FP16 complex input "
+            "algorithmic_bytes": "FP16 complex S of every ordered pair of fragments not on one plane (4 B; "
+                                 "pairs within a facet or a run of adjacent facets with identical normals are "
+                                 "zero and never stored) + FP16 complex input "
                                  "(4 B per cell and incidence) + FP32 complex P_out (8 B per cell and incidence)",
             "kernel": "k_transfer_gemm (FP16 MFMA 32x32x16, streams the FP16 transfer matrix once per bounce)",
             "mfma_tflops_executed": round(tf, 2), "mfma_frac": round(tf / FP16_PEAK_TFLOPS, 4),

@@ -937,6 +937,10 @@ static int ensure_transfer(Model &m, int i) {
             for (size_t k = q; k < e; k++) run[k] = make_int2(fr[q].x, fr[e - 1].y);
             q = e;
         }
+        // the pairs within one run: zero in S, never stored or read
+        t.zeroPairs = 0;
+        for (size_t q = 0; q < fr.size(); q++)
+            if (q == 0 || run[q].x != run[q - 1].x) t.zeroPairs += (double)(run[q].y - run[q].x) * (run[q].y - run[q].x);
         // Per 128-row group: its facet's range of K positions (single-facet groups
         // only) and that facet's run's, the K chunks wholly inside the run (zero:
         // never read) and, with the gate inside the product, the first K step of
@@ -1215,10 +1219,11 @@ static int bounce_mixed(Model &m, int i, bool first, bool more) {
             // flops of the GEMM as executed: 2 * rows * 128 columns * 2 * K (padded)
             const double fl = 2.0 * (double)t.nRt * 32 * 128 * (double)t.nKs * 16;
             // algorithmic HBM bytes of the bounce product: the FP16 complex S entry of every
-            // ordered cross-facet pair (4 B), the FP16 complex input of every active cell and
-            // plane (4 B) and the FP32 complex output P_out (8 B); partial sums, padding and
-            // re-reads are traffic, not work
-            const double by = ((double)n * n - o.sameFacetPairs) * 4.0 + (double)n * np * 4.0 + (double)n * np * 8.0;
+            // ordered pair of fragments not on one plane (4 B: pairs within a facet, or within
+            // a run of adjacent facets with identical normals, are zero, never stored or read),
+            // the FP16 complex input of every active cell and plane (4 B) and the FP32 complex
+            // output P_out (8 B); partial sums, padding and re-reads are traffic, not work
+            const double by = ((double)n * n - t.zeroPairs) * 4.0 + (double)n * np * 4.0 + (double)n * np * 8.0;
             ProfScope ps(m, PK_GEMM, pairs, fl, by);
             if (transfer_gemm(gm, m.stream)) return -1;
         }

@@ -65,6 +65,7 @@ struct Object {
         // symmetric S build (padded rows): the 64 x 64 blocks (row block, K chunk) on
         // or above the diagonal that are not skipped; none when the skipped chunks
         // are not symmetric
+        double zeroPairs = 0;      // ordered pairs within one run of facets with identical normals (zero in S)
         int2 *symItems = nullptr;
         int nSym = 0;
         int2 *nearQ = nullptr, *nearM = nullptr;  // near pairs: quadrature, learned
