@@ -134,61 +134,87 @@ __global__ void k_nf_samples(uint32_t seed, int n, float dMin, float dMax, float
 // ---------------------------------------------------------------------------
 // MLP: W1 [64][8], b1[64], W2 [64][64], b2[64], W3 [2][64], b3[2] packed in that order.
 
-__global__ void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X, int n, float *__restrict__ H1,
-                              float *__restrict__ H2, float *__restrict__ Out) {
+// A workgroup takes kMlpRows samples; thread (neuron o = t % 64, group t / 64)
+// computes neuron o of 8 of them, the layer's inputs shared through LDS. Every
+// sum runs in the order of the one-thread-per-sample form (the same results),
+// with 64 times the threads.
+constexpr int kMlpRows = 32;
+constexpr int kMlpPad = 65;  // LDS row stride of a sample's 64 activations
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X, int n,
+                                                     float *__restrict__ H1, float *__restrict__ H2,
+                                                     float *__restrict__ Out) {
     __shared__ float w[kMlpParams];
+    __shared__ float sx[kMlpRows * kMlpIn], sh1[kMlpRows * kMlpPad], sh2[kMlpRows * kMlpPad];
+    const int i0 = blockIdx.x * kMlpRows;
     for (int q = threadIdx.x; q < kMlpParams; q += blockDim.x) w[q] = P[q];
+    for (int q = threadIdx.x; q < kMlpRows * kMlpIn; q += blockDim.x)
+        sx[q] = i0 * kMlpIn + q < n * kMlpIn ? X[(size_t)i0 * kMlpIn + q] : 0.f;
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     const float *W1 = w, *b1 = w + kOffB1, *W2 = w + kOffW2, *b2 = w + kOffB2, *W3 = w + kOffW3, *b3 = w + kOffB3;
-    float x[kMlpIn], h1[64], h2[64];
-    for (int q = 0; q < kMlpIn; q++) x[q] = X[(size_t)i * kMlpIn + q];
-    for (int o = 0; o < 64; o++) {
+    const int o = threadIdx.x & 63, g = threadIdx.x >> 6;
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int sl = g * (kMlpRows / 4) + k, i = i0 + sl;
         float a = b1[o];
-        for (int q = 0; q < kMlpIn; q++) a = fmaf(W1[o * kMlpIn + q], x[q], a);
-        h1[o] = tanhf(a);
-        H1[(size_t)i * 64 + o] = h1[o];
+        for (int q = 0; q < kMlpIn; q++) a = fmaf(W1[o * kMlpIn + q], sx[sl * kMlpIn + q], a);
+        const float h = tanhf(a);
+        sh1[sl * kMlpPad + o] = h;
+        if (i < n) H1[(size_t)i * 64 + o] = h;
     }
-    for (int o = 0; o < 64; o++) {
+    __syncthreads();
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int sl = g * (kMlpRows / 4) + k, i = i0 + sl;
         float a = b2[o];
-        for (int q = 0; q < 64; q++) a = fmaf(W2[o * 64 + q], h1[q], a);
-        h2[o] = tanhf(a);
-        H2[(size_t)i * 64 + o] = h2[o];
+        for (int q = 0; q < 64; q++) a = fmaf(W2[o * 64 + q], sh1[sl * kMlpPad + q], a);
+        const float h = tanhf(a);
+        sh2[sl * kMlpPad + o] = h;
+        if (i < n) H2[(size_t)i * 64 + o] = h;
     }
-    for (int o = 0; o < 2; o++) {
-        float a = b3[o];
-        for (int q = 0; q < 64; q++) a = fmaf(W3[o * 64 + q], h2[q], a);
-        Out[(size_t)i * 2 + o] = a;
+    __syncthreads();
+    if (threadIdx.x < 2 * kMlpRows) {  // output c of sample sl
+        const int sl = threadIdx.x >> 1, c = threadIdx.x & 1, i = i0 + sl;
+        float a = b3[c];
+        for (int q = 0; q < 64; q++) a = fmaf(W3[c * 64 + q], sh2[sl * kMlpPad + q], a);
+        if (i < n) Out[(size_t)i * 2 + c] = a;
     }
 }
 
 // dL/dOut for L = mean over samples of |Out - Y|^2 (both components), then the
-// hidden deltas D2 = (W3^T dOut) (1 - h2^2), D1 = (W2^T D2) (1 - h1^2).
-__global__ void k_mlp_backward(const float *__restrict__ P, const float *__restrict__ Y, int n,
-                               const float *__restrict__ H1, const float *__restrict__ H2,
-                               const float *__restrict__ Out, float *__restrict__ G3, float *__restrict__ D2,
-                               float *__restrict__ D1) {
+// hidden deltas D2 = (W3^T dOut) (1 - h2^2), D1 = (W2^T D2) (1 - h1^2); the
+// same layout of threads over samples and neurons as the forward pass.
+__global__ __launch_bounds__(256) void k_mlp_backward(const float *__restrict__ P, const float *__restrict__ Y, int n,
+                                                      const float *__restrict__ H1, const float *__restrict__ H2,
+                                                      const float *__restrict__ Out, float *__restrict__ G3,
+                                                      float *__restrict__ D2, float *__restrict__ D1) {
     __shared__ float w[kMlpParams];
+    __shared__ float sg[kMlpRows * 2], sd2[kMlpRows * kMlpPad];
+    const int i0 = blockIdx.x * kMlpRows;
     for (int q = threadIdx.x; q < kMlpParams; q += blockDim.x) w[q] = P[q];
+    if (threadIdx.x < 2 * kMlpRows) {
+        const int sl = threadIdx.x >> 1, c = threadIdx.x & 1, i = i0 + sl;
+        float gv = 0.f;
+        if (i < n) {
+            gv = 2.0f * (Out[(size_t)i * 2 + c] - Y[(size_t)i * 2 + c]) / (float)n;
+            G3[(size_t)i * 2 + c] = gv;
+        }
+        sg[threadIdx.x] = gv;
+    }
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     const float *W2 = w + kOffW2, *W3 = w + kOffW3;
-    float g[2];
-    for (int o = 0; o < 2; o++) {
-        g[o] = 2.0f * (Out[(size_t)i * 2 + o] - Y[(size_t)i * 2 + o]) / (float)n;
-        G3[(size_t)i * 2 + o] = g[o];
+    const int q = threadIdx.x & 63, g = threadIdx.x >> 6;
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int sl = g * (kMlpRows / 4) + k, i = i0 + sl;
+        const float h = i < n ? H2[(size_t)i * 64 + q] : 0.f;
+        const float d2 = (W3[q] * sg[2 * sl] + W3[64 + q] * sg[2 * sl + 1]) * (1.0f - h * h);
+        sd2[sl * kMlpPad + q] = d2;
+        if (i < n) D2[(size_t)i * 64 + q] = d2;
     }
-    float d2[64];
-    for (int q = 0; q < 64; q++) {
-        const float h = H2[(size_t)i * 64 + q];
-        d2[q] = (W3[q] * g[0] + W3[64 + q] * g[1]) * (1.0f - h * h);
-        D2[(size_t)i * 64 + q] = d2[q];
-    }
-    for (int q = 0; q < 64; q++) {
+    __syncthreads();
+    for (int k = 0; k < kMlpRows / 4; k++) {
+        const int sl = g * (kMlpRows / 4) + k, i = i0 + sl;
+        if (i >= n) continue;
         float a = 0.f;
-        for (int o = 0; o < 64; o++) a = fmaf(W2[o * 64 + q], d2[o], a);
+        for (int o = 0; o < 64; o++) a = fmaf(W2[o * 64 + q], sd2[sl * kMlpPad + o], a);
         const float h = H1[(size_t)i * 64 + q];
         D1[(size_t)i * 64 + q] = a * (1.0f - h * h);
     }
@@ -210,21 +236,27 @@ __global__ void k_mlp_wgrad(const float *__restrict__ X, const float *__restrict
     double acc = 0.0;
     if (p < kOffB1) {  // W1[o][q]
         const int o = p / kMlpIn, q = p % kMlpIn;
+#pragma unroll 8
         for (int i = i0; i < i1; i++) acc += (double)D1[(size_t)i * 64 + o] * X[(size_t)i * kMlpIn + q];
     } else if (p < kOffW2) {  // b1
         const int o = p - kOffB1;
+#pragma unroll 8
         for (int i = i0; i < i1; i++) acc += D1[(size_t)i * 64 + o];
     } else if (p < kOffB2) {  // W2[o][q]
         const int o = (p - kOffW2) / 64, q = (p - kOffW2) % 64;
+#pragma unroll 8
         for (int i = i0; i < i1; i++) acc += (double)D2[(size_t)i * 64 + o] * H1[(size_t)i * 64 + q];
     } else if (p < kOffW3) {
         const int o = p - kOffB2;
+#pragma unroll 8
         for (int i = i0; i < i1; i++) acc += D2[(size_t)i * 64 + o];
     } else if (p < kOffB3) {  // W3[o][q]
         const int o = (p - kOffW3) / 64, q = (p - kOffW3) % 64;
+#pragma unroll 8
         for (int i = i0; i < i1; i++) acc += (double)G3[(size_t)i * 2 + o] * H2[(size_t)i * 64 + q];
     } else {
         const int o = p - kOffB3;
+#pragma unroll 8
         for (int i = i0; i < i1; i++) acc += G3[(size_t)i * 2 + o];
     }
     partial[(size_t)c * kMlpParams + p] = acc;
@@ -265,14 +297,14 @@ int nf_samples(uint32_t seed, int n, float dMin, float dMax, float rMax, float *
 }
 
 int mlp_forward(const float *P, const float *X, int n, float *H1, float *H2, float *Out, hipStream_t st) {
-    k_mlp_forward<<<(n + 127) / 128, 128, 0, st>>>(P, X, n, H1, H2, Out);
+    k_mlp_forward<<<(n + kMlpRows - 1) / kMlpRows, 256, 0, st>>>(P, X, n, H1, H2, Out);
     SR_HIP(hipGetLastError());
     return 0;
 }
 
 int mlp_backward(const float *P, const float *X, const float *Y, int n, const float *H1, const float *H2,
                  const float *Out, float *G3, float *D2, float *D1, double *partial, float *grad, hipStream_t st) {
-    k_mlp_backward<<<(n + 127) / 128, 128, 0, st>>>(P, Y, n, H1, H2, Out, G3, D2, D1);
+    k_mlp_backward<<<(n + kMlpRows - 1) / kMlpRows, 256, 0, st>>>(P, Y, n, H1, H2, Out, G3, D2, D1);
     SR_HIP(hipGetLastError());
     k_mlp_wgrad<<<dim3((kMlpParams + 127) / 128, kGradChunks), 128, 0, st>>>(X, H1, H2, G3, D2, D1, n, partial);
     SR_HIP(hipGetLastError());
