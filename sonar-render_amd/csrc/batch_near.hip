@@ -79,7 +79,11 @@ struct BatchNearSrc {
 }  // namespace
 
 // One wave per (scene, 64-row block); lane = row. The emitters are walked in
-// order, so each row's pairs land in ascending emitter order.
+// order, so each row's pairs land in ascending emitter order: 64 at a time,
+// lane k loading emitter e0 + k's visibility word and position (one coalesced
+// load each), then the emitters with a non-zero word in ascending order, their
+// word and position read from lane k (a walk that loaded each emitter in turn
+// waited a memory round trip per emitter).
 __global__ __launch_bounds__(256) void k_near_rows(const NearBatchTask *__restrict__ tasks,
                                                    const int2 *__restrict__ items, int nitems) {
     const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -93,15 +97,25 @@ __global__ __launch_bounds__(256) void k_near_rows(const NearBatchTask *__restri
     const int k0 = (fill && rv) ? t.rowStart[row] : 0;
     const uint64_t *mw = t.mask + (size_t)it.y * t.n;
     int cnt = 0;
-    for (int e = 0; e < t.n; e++) {
-        const uint64_t word = mw[e];
-        if (!word) continue;  // uniform
-        if (rv && ((word >> lane) & 1ull) && near_r2(rp, t.p[e]) < t.radius2) {
-            if (fill) {
-                t.rows[k0 + cnt] = row;
-                t.cols[k0 + cnt] = e;
+    for (int e0 = 0; e0 < t.n; e0 += 64) {
+        const int el = e0 + lane;
+        const uint64_t wl = el < t.n ? mw[el] : 0ull;
+        const float4 pl = wl ? t.p[el] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint64_t nz = __ballot(wl != 0ull); nz; nz &= nz - 1) {
+            const int k = __builtin_ctzll(nz);  // uniform
+            const uint64_t word = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(wl >> 32), k) << 32) |
+                                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wl, k);
+            const float4 ep = make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(pl.x), k)),
+                                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pl.y), k)),
+                                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pl.z), k)),
+                                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pl.w), k)));
+            if (rv && ((word >> lane) & 1ull) && near_r2(rp, ep) < t.radius2) {
+                if (fill) {
+                    t.rows[k0 + cnt] = row;
+                    t.cols[k0 + cnt] = e0 + k;
+                }
+                cnt++;
             }
-            cnt++;
         }
     }
     if (!fill && rv) t.rowCount[row] = cnt;
