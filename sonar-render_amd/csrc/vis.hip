@@ -547,6 +547,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask(MaskTask tk, Bvh bvh, V
 
 // Persistent form over many tasks (scenes): each wave pulls (task, receiver
 // block, emitter chunk) items from a shared counter until the list is empty.
+constexpr int kVisClaim = 8;  // items a wave claims at once from the work counter
 template <int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *__restrict__ tasks,
                                                                const Bvh *__restrict__ bvhs,
@@ -558,19 +559,25 @@ __global__ __launch_bounds__(64 * WAVES) void k_vis_mask_multi(const MaskTask *_
     __shared__ float4 sTriW[WAVES][4 * kBruteTris];  // few occluders: this wave's scene's records
     const int wave = threadIdx.x >> 6;
     for (;;) {
-        int it = 0;
-        if (__lane_id() == 0) it = atomicAdd(counter, 1);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= nitems) break;
-        const int4 w = items[it];
-        const Bvh &b = bvhs[w.x];
-        if (b.ntri > 0 && b.ntri <= kBruteTris) {
-            // staged by this wave alone: its LDS accesses run in order, so the
-            // records it reads next are the ones it wrote (after its own last reads)
-            if (__lane_id() < 4 * b.ntri) sTriW[wave][__lane_id()] = b.tris[__lane_id()];
-            vis_wave<true>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], sTriW[wave], sEmAll[wave]);
-        } else {
-            vis_wave<false>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], nullptr, sEmAll[wave]);
+        // kVisClaim consecutive items per claim: one atomic on the shared counter
+        // per item had queued every wave of the GPU on that one address
+        int it0 = 0;
+        if (__lane_id() == 0) it0 = atomicAdd(counter, kVisClaim);
+        it0 = __builtin_amdgcn_readfirstlane(it0);
+        if (it0 >= nitems) break;
+        const int it1 = min(nitems, it0 + kVisClaim);
+        for (int it = it0; it < it1; it++) {
+            const int4 w = items[it];
+            const Bvh &b = bvhs[w.x];
+            if (b.ntri > 0 && b.ntri <= kBruteTris) {
+                // staged by this wave alone: its LDS accesses run in order, so the
+                // records it reads next are the ones it wrote (after its own last reads)
+                if (__lane_id() < 4 * b.ntri) sTriW[wave][__lane_id()] = b.tris[__lane_id()];
+                vis_wave<true>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], sTriW[wave],
+                               sEmAll[wave]);
+            } else {
+                vis_wave<false>(tasks[w.x], b, vcs[w.x], w.y, w.z, emPerWave, stkAll[wave], nullptr, sEmAll[wave]);
+            }
         }
     }
 }
