@@ -171,6 +171,8 @@ __global__ __launch_bounds__(256) void k_gather(GatherTask t, Phys ph) {
     gather_block<KIND, NB, ATT>(t, ph, blockIdx.x, blockIdx.y, L);
 }
 
+constexpr int kGatherClaim = 4;  // items a workgroup claims at once from the work counter
+
 // Persistent form: workgroups pull (task, receiver-block group, emitter range)
 // items from a shared counter until the list is exhausted; every task may be a
 // different scene with its own sizes and wavenumber.
@@ -181,15 +183,19 @@ __global__ __launch_bounds__(256) void k_gather_multi(const GatherTask *__restri
     __shared__ GatherLds<NB> L;
     __shared__ int s_item;
     for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        // kGatherClaim consecutive items per claim (fewer atomics queued on the counter)
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, kGatherClaim);
         __syncthreads();
-        const int it = s_item;
+        const int it0 = s_item;
         __syncthreads();
-        if (it >= nitems) break;
-        const int4 w = items[it];
-        const GatherTask t = tasks[w.x];
-        const Phys ph = phys[w.x];
-        gather_block<KIND, NB, ATT>(t, ph, w.y, w.z, L);
+        if (it0 >= nitems) break;
+        const int it1 = min(nitems, it0 + kGatherClaim);
+        for (int it = it0; it < it1; it++) {
+            const int4 w = items[it];
+            const GatherTask t = tasks[w.x];
+            const Phys ph = phys[w.x];
+            gather_block<KIND, NB, ATT>(t, ph, w.y, w.z, L);
+        }
     }
 }
 
